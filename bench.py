@@ -1,0 +1,182 @@
+"""Throughput of the lattice-Boltzmann step on one MI355X.
+
+Headline: the plain FP32 collide-stream step (BGK + exact-difference forcing,
+halfway + interpolated bounce-back, k-omega off) on the 8192 x 1024 channel
+with a seeded 16-Gaussian fouling layer on each wall, in MLUPS (million
+fluid-node updates per second), with inputs resident in HBM.  Further forms
+(FP16 / BF16 population storage, the fused LB + k-omega step on the
+4096 x 512 serpentine channel) are reported beside it under "forms".
+
+Algorithmic bytes per lattice update (compulsory state traffic only):
+  P32  9*4 read + 9*4 write + 2 (mask)           = 74 B
+  P16  9*2 read + 9*2 write + 2                  = 38 B
+  K32  P32 + k, omega read/write (16) + WallD (4) = 94 B
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+B_LU = {"f32": 74.0, "f16": 38.0, "bf16": 38.0, "f64": 146.0, "k32": 94.0}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def setup(w, storage, komega=False, rtol=1e-4, maxit=1000):
+    from komegafoulingmodel_amd import lib
+
+    L = lib.Lattice(w.nX, w.nY, storage=storage, komega=komega)
+    L.set_params(w.tau0, w.Fval, w.nu, kappa_rel_tol=rtol, kappa_max_iters=maxit)
+    L.build_geometry(w.C0, w.BL, **w.geom_params())
+    nType = L.geometry("nType")
+    rho, ux, uy = w.initial_velocity(nType)
+    L.init_state(rho, ux, uy)
+    if komega:
+        L.init_komega(w.k_ini, w.omega_ini, w.Re)
+    fluid = int(np.count_nonzero((nType.astype(np.int32) & 1) == 0))
+    return L, fluid
+
+
+def measure(L, fluid, steps, warmup, blu, options=0):
+    L.step(warmup, options)
+    L.synchronize()
+    t0 = time.perf_counter()
+    ms_step, ms_kernel = L.time_steps(steps, options)
+    L.synchronize()
+    wall = time.perf_counter() - t0
+    mlups = fluid * steps / wall / 1e6
+    achieved = blu * fluid / (ms_kernel * 1e-3) / 1e9
+    return dict(mlups=mlups, ms_per_step=wall * 1e3 / steps, ms_per_step_events=ms_step,
+                ms_per_kernel=ms_kernel, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
+                fluid_nodes=fluid, steps=steps)
+
+
+def read_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(w, seconds=15.0, max_steps=40):
+    """The FP64 oracle (the reference algorithm restated, OpenMP over rows) on
+    the host cores, on a bounded number of steps of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    g = oracle.geometry(w.nX, w.nY, w.C0, w.BL, **w.geom_params())
+    threads = oracle.load().orc_max_threads()
+    orc = oracle.OracleLattice(g, tau0=w.tau0, Fval=w.Fval, nu=w.nu, threads=threads)
+    rho, ux, uy = w.initial_velocity(g["nType"])
+    orc.init_state(rho, ux, uy)
+    fluid = int(np.count_nonzero((g["nType"].astype(np.int32) & 1) == 0))
+    orc.step(1)
+    t0 = time.perf_counter()
+    orc.step(2)
+    per = (time.perf_counter() - t0) / 2
+    n = int(max(2, min(max_steps, seconds / max(per, 1e-9))))
+    t0 = time.perf_counter()
+    orc.step(n)
+    dt = time.perf_counter() - t0
+    return dict(value=fluid * n / dt / 1e6, unit="MLUPS", cores=int(threads), kind="port",
+                sample=f"{n} FP64 steps of the {w.nX}x{w.nY} fouled channel (oracle, OpenMP)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--forms", default="f16,bf16,k32")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the lattice step runs on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+
+    from komegafoulingmodel_amd import channel
+
+    w = channel.config("c3")
+    log(f"workload c3 {w.nX}x{w.nY} fouled, FP32, {args.steps} steps")
+    L, fluid = setup(w, "f32")
+    r = measure(L, fluid, args.steps, args.warmup, B_LU["f32"])
+    L.close()
+    log(f"p32: {r}")
+
+    forms = {"p32": r}
+    for form in [f for f in args.forms.split(",") if f]:
+        if form in ("f16", "bf16"):
+            Lf, nf = setup(w, form)
+            forms["p" + form[1:] if form == "f16" else "pbf16"] = measure(
+                Lf, nf, args.steps, args.warmup, B_LU[form])
+            Lf.close()
+        elif form == "k32":
+            wk = channel.config("c2")
+            log(f"k32 on c2 {wk.nX}x{wk.nY}")
+            Lk, nk = setup(wk, "f32", komega=True)
+            forms["k32"] = measure(Lk, nk, max(5, args.steps // 50), 3, B_LU["k32"])
+            st = Lk.solver_stats()
+            forms["k32"]["solver_sweeps_last"] = [st.kappa_iters, st.omega_iters]
+            Lk.close()
+        log(f"{form}: done")
+    for k, v in forms.items():
+        forms[k] = {a: (round(b, 6) if isinstance(b, float) else b) for a, b in v.items()}
+    if "p16" in forms:
+        forms["p16"]["speedup_vs_p32"] = round(forms["p16"]["mlups"] / r["mlups"], 3)
+    if "pbf16" in forms:
+        forms["pbf16"]["speedup_vs_p32"] = round(forms["pbf16"]["mlups"] / r["mlups"], 3)
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(w, seconds=args.cpu_seconds)
+
+    traffic = read_traffic("c3_p32")
+    out = {
+        "metric": "MLUPS (plain FP32 fused LB step, fouled 8192x1024 channel)",
+        "value": round(r["mlups"], 3),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded fouling layer, seeded +-1e-4 velocity noise)",
+        "config": {"workload": "c3: 8192x1024 channel, 16-Gaussian fouling per wall, tau=0.8, "
+                               "F=1e-6, k-omega off, FP32 populations",
+                   "nX": w.nX, "nY": w.nY, "fluid_nodes": fluid, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 3),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(r["frac"], 4),
+                     "traffic": traffic, "bytes_per_lu": B_LU["f32"],
+                     "kernel": "lbCollideStream<float,float,false>",
+                     "kernel_ms": round(r["ms_per_kernel"], 6)},
+        "cpu_baseline": cpu,
+        "forms": forms,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,255 @@
+/*
+ * kofm.h -- C ABI of the MI355X-native lattice-Boltzmann + k-omega time step.
+ *
+ * The reference (zmills04/kOmegaFoulingModel) has no FFI: its hot path is
+ * reached through the C++ classes clVariablesLB, kOmega, shearStress,
+ * clVariablesLS and Reducer, which drive OpenCL kernels.  Each entry point
+ * below replaces the reference interface cited beside it (paths relative to
+ * the reference checkout).  The C++ facades with the reference's class and
+ * method names (komegafoulingmodel_amd/csrc/facade.hpp) call only these
+ * functions, as does any other host language (ctypes, cgo, JNI, N-API).
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative error code (the
+ *    reference exit codes of ConstDef.h:588-611, negated); the process is
+ *    never terminated.  kofm_last_error() describes the last failure.
+ *  - Host buffers stay owned by the caller and are copied.  Host arrays use
+ *    the reference layout: x + XsizeFull*y (+ XsizeFull*nY*k for planes),
+ *    XsizeFull = ceil(nX/128)*128 (clProblem.cpp:469), directions in the
+ *    order C,E,W,N,S,NE,SW,SE,NW (Heading.cl:197-215), FP64 unless stated.
+ *  - A context owns all device memory and one HIP stream on the current
+ *    device.  A context is not thread-safe; separate contexts may run
+ *    concurrently.  All calls are asynchronous on that stream except the
+ *    ones that return host data (download, reduce, counts, timing).
+ */
+#ifndef KOFM_H
+#define KOFM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kofm_ctx kofm_ctx;
+
+/* error codes (ConstDef.h:588-611, negated, plus argument/device errors) */
+#define KOFM_OK 0
+#define KOFM_ERR_ARG (-1)
+#define KOFM_ERR_DEVICE (-2)
+#define KOFM_ERR_STATE (-3)
+#define KOFM_ERR_INITIALIZING_VLS (-20)
+#define KOFM_ERR_INITIALIZING_VLB (-21)
+#define KOFM_ERR_INITIALIZING_KOMEGA (-22)
+#define KOFM_ERR_SOLVER_NAN (-30)
+
+/* population storage in HBM (all forms collide in FP32 except F64) */
+#define KOFM_STORE_F32 0  /* FP32 deviations f_i - w_i */
+#define KOFM_STORE_F16 1  /* FP16 scaled deviations (f_i - w_i) * 1024 / w_i */
+#define KOFM_STORE_BF16 2 /* BF16 scaled deviations (f_i - w_i) * 1024 / w_i */
+#define KOFM_STORE_F64 3  /* FP64 deviations, FP64 arithmetic (GPU FP64 oracle) */
+
+/* options for kofm_step (Heading.cl:309) */
+#define KOFM_OPTION_SAVE_MACRO_FIELDS 0x2
+
+/* fields for kofm_download / kofm_reduce */
+#define KOFM_FIELD_F 0        /* 9*D populations, reference FA/FB layout       */
+#define KOFM_FIELD_RHO 1      /* D   Ro_array                                   */
+#define KOFM_FIELD_UX 2       /* D   Ux_array (u_x + Fval/2)                    */
+#define KOFM_FIELD_UY 3       /* D   Uy_array                                   */
+#define KOFM_FIELD_KAPPA 4    /* D   Kappa_array                                */
+#define KOFM_FIELD_OMEGA 5    /* D   Omega_array                                */
+#define KOFM_FIELD_NUT 6      /* D   Nut_array                                  */
+#define KOFM_FIELD_SXY 7      /* 3*D Sxy_array (Sxx, Sxy, Syy planes)           */
+#define KOFM_FIELD_WALLD 8    /* D   WallD                                      */
+#define KOFM_FIELD_TAU_NODE 9 /* nSS shearStress::Tau (per wall node)           */
+#define KOFM_FIELD_TAU_BL 10  /* nBLshear  BL.Tau over the shear link range     */
+
+/* reduction operators (ReduceGenerator.cpp:219-305) */
+#define KOFM_RED_SUM 1
+#define KOFM_RED_MAX 2
+#define KOFM_RED_MIN 4
+
+/* geometry arrays for kofm_geometry_array */
+#define KOFM_GEOM_NTYPE 0     /* int16 [D]       clVariablesLS::nType            */
+#define KOFM_GEOM_DX 1        /* double[4*D]     clVariablesLS::dXArr (E,W,N,S)  */
+#define KOFM_GEOM_IBB 2       /* int32 [nIbb]    ibbArr (node + D*dir)           */
+#define KOFM_GEOM_IBBQ 3      /* double[nIbb]    ibbDistArr                      */
+#define KOFM_GEOM_SS 4        /* int32 [nSS]     ssArr                           */
+#define KOFM_GEOM_SSMAP 5     /* int32 [D]       ssArrIndMap                     */
+#define KOFM_GEOM_SSINDS 6    /* int32 [nX]      ssArrInds                       */
+#define KOFM_GEOM_LSMAP 7     /* uint16[2*XF]    lsMap                           */
+#define KOFM_GEOM_WALLD 8     /* double[D]       kOmega::WallD                   */
+#define KOFM_GEOM_CFSS 9      /* double[16*nSS]  shearStress::shearCoeffs        */
+#define KOFM_GEOM_SIND 10     /* int32 [4*nBLs]  shearStress::sInds              */
+#define KOFM_GEOM_WEIGHTS 11  /* double[4*nBLs]  shearStress::ssWeights          */
+#define KOFM_GEOM_BOUNDS 12   /* int32 [4]       BL_BOT_START, NUM_BL_BOT, BL_TOP_START, NUM_BL_TOP */
+#define KOFM_GEOM_MASSES 13   /* double[2]       clVariablesLS::Masses           */
+
+typedef struct kofm_grid {
+  int32_t nX;         /* "nX" */
+  int32_t nY;         /* "nY" */
+  int32_t storage;    /* KOFM_STORE_* */
+  int32_t use_komega; /* "Use Turb Model" */
+} kofm_grid;
+
+typedef struct kofm_params {
+  double tau0;          /* relaxation frequency 1/(3 nu + 0.5) (reference 'tau', #define tau0) */
+  double Fval;          /* body-force velocity increment per step (FTERM_VAL) */
+  double nu;            /* kinematic viscosity (MU_NUMBER) */
+  double rho0;          /* "Rho" */
+  double max_turb_visc; /* "Max Turb Visc" (MAX_TURB_VISC) */
+  double ko_dt;         /* k-omega time step (kOmega::timeStep, DTFD) */
+  double kappa_rel_tol; /* "Kappa Max Rel Tol" */
+  double kappa_abs_tol; /* "Kappa Max Abs Tol" */
+  int32_t kappa_max_iters; /* "Kappa Max Iterations" */
+  double omega_rel_tol; /* "Omega Max Rel Tol" */
+  double omega_abs_tol; /* "Omega Max Abs Tol" */
+  int32_t omega_max_iters; /* "Omega Max Iterations" */
+} kofm_params;
+
+typedef struct kofm_geom_params {
+  double pipe_radius;          /* "Pipe Radius" (Channel_Height = 2*(int)R) */
+  double eps;                  /* clProblem::eps (1e-5) */
+  double min_dx;               /* "Min dX Dist" (MINIMUM_DX_DIST) */
+  int32_t wall_d_search_rad;   /* "Wall D Search Rad" */
+  double x_release_pos;        /* "X Release Pos" */
+  double x_min_val;            /* "X Min Val" (reference default X Release Pos - 1.25) */
+  double x_stop_pos;           /* "X Stop Pos" */
+  double cutoff_radius;        /* "Cutoff Radius" (SHEAR_CUTOFF_RADIUS) */
+  int32_t index_radius_search; /* "Index Radius Search" */
+} kofm_geom_params;
+
+typedef struct kofm_geom_counts {
+  int32_t nX, nY, XsizeFull;
+  int64_t D;          /* XsizeFull * nY */
+  int32_t nIbb;       /* boundary links (ibbArr entries) */
+  int32_t nSS;        /* wall nodes (ssArr entries) */
+  int32_t nBLshear;   /* boundary links in the shear range (NUM_BL_TOTAL) */
+  int32_t nIbbNodes;  /* distinct nodes carrying interpolated links */
+} kofm_geom_counts;
+
+typedef struct kofm_solver_stats {
+  int32_t kappa_iters;  /* sweeps of the last k solve */
+  int32_t omega_iters;  /* sweeps of the last omega solve */
+  double kappa_res0, kappa_res;
+  double omega_res0, omega_res;
+  int32_t nan_flag;     /* a non-finite residual was seen since the last query */
+} kofm_solver_stats;
+
+/* --- lifecycle ------------------------------------------------------------ */
+
+/* Allocates device storage for an nX x nY lattice on the current HIP device.
+ * Replaces clVariablesLB::allocateArrays/allocateBuffers
+ * (clVariablesLB.cpp:47-73) and kOmega::allocateArrays (kOmega.cpp:76-108). */
+int kofm_create(const kofm_grid* grid, kofm_ctx** out);
+
+/* A context without device state, for geometry preparation on machines
+ * without a GPU: kofm_build_geometry, kofm_geometry_counts,
+ * kofm_geometry_array and kofm_set_params work; every other call returns
+ * KOFM_ERR_STATE.  No counterpart in the reference, whose geometry build
+ * needs the OpenCL context (clVariablesLS::ini runs after clEnv::iniOpenCL). */
+int kofm_create_host(const kofm_grid* grid, kofm_ctx** out);
+
+/* Releases the context and all its device memory. */
+void kofm_destroy(kofm_ctx* ctx);
+
+/* Last error text of this context (or of the last failed kofm_create when
+ * ctx is NULL). */
+const char* kofm_last_error(const kofm_ctx* ctx);
+
+/* Relaxation, forcing and solver constants.  Replaces
+ * clVariablesLB::loadParams/setSourceDefines (clVariablesLB.cpp:553-639,
+ * 860-877), kOmega::loadParams/setSourceDefines (kOmega.cpp:352-378,
+ * 660-680) and the MU_NUMBER define (clVariablesTR.cpp:1908). */
+int kofm_set_params(kofm_ctx* ctx, const kofm_params* p);
+
+/* --- geometry ------------------------------------------------------------- */
+
+/* Builds the node-type map, wall distances dXArr, interpolated
+ * bounce-back link list, wall-node list, lsMap, WallD and the wall-shear
+ * coefficients from the wall polyline (load/lsc0.txt, nN points, x y
+ * interleaved) and its links (load/lsbl.txt, nBL pairs), and uploads them.
+ * Replaces clVariablesLS::ini (clVariablesLS.cpp:569-620),
+ * clVariablesLS::iniIBBArrays (:1085-1164, lsKernels.cl:429-505),
+ * kOmega::iniWallD (kOmega.cpp:330-349), clVariablesTR::iniBLandNodI
+ * (clVariablesTR.cpp:476-658) and shearStress::iniShearCoeffs
+ * (shearStress.cpp:135-139, trKernelsShear.cl:332-496). */
+int kofm_build_geometry(kofm_ctx* ctx, const double* C0, int32_t nN, const int32_t* BL,
+                        int32_t nBL, const kofm_geom_params* gp);
+
+int kofm_geometry_counts(const kofm_ctx* ctx, kofm_geom_counts* out);
+
+/* Copies one host-side geometry array (KOFM_GEOM_*) into out. */
+int kofm_geometry_array(const kofm_ctx* ctx, int32_t which, void* out);
+
+/* --- state ---------------------------------------------------------------- */
+
+/* Populations set to the product-form equilibrium of (rho, ux, uy) on
+ * non-solid nodes (clVariablesLB::iniDistsFromVel, clVariablesLB.cpp:367-405,
+ * feq of :77-117); NULL arrays give the rest state at rho0
+ * (clVariablesLB::iniDists, :407-425).  Macro arrays are set as the
+ * reference sets them (rho0 on fluid nodes, zero velocity) and the strain
+ * tensor to zero. */
+int kofm_init_state(kofm_ctx* ctx, const double* rho, const double* ux, const double* uy);
+
+/* k, omega and nu_t = k/omega on fluid nodes (kOmega::setInitialValues,
+ * kOmega.cpp:530-570, fillByNodeType Array.h:903-918).  Values <= 0 select
+ * the reference's turbulence-intensity estimate with Re and "Geometric
+ * Penalty Factor" = 1: umean = Re nu / R / 4 when the current mean velocity
+ * is zero. */
+int kofm_init_komega(kofm_ctx* ctx, double k_ini, double omega_ini, double Re);
+
+/* Uploads populations (9*D, the reference FA layout as loaded from
+ * load/lbf.bin, clVariablesLB.cpp:883-902) and, when non-NULL, k, omega
+ * (load/lbkappa.bin, load/lbomega.bin, kOmega.cpp:685-710), nu_t and S (3*D).
+ * Macro fields are rebuilt as Calc_Ro_J does on restart (lbKernels.cl:137-164). */
+int kofm_upload_state(kofm_ctx* ctx, const double* f, const double* k, const double* omega,
+                      const double* nut, const double* S);
+
+/* --- time stepping -------------------------------------------------------- */
+
+/* nsteps reference time steps (clVariablesLB::Solve, clVariablesLB.h:375-441:
+ * collide-stream with bounce-back, interpolated bounce-back, and with k-omega
+ * on the diffusivity/coefficient kernels and implicit omega and k solves of
+ * kOmega::Solve, kOmega.cpp:31-73).  options: KOFM_OPTION_SAVE_MACRO_FIELDS
+ * writes rho, u_x + F/2 and u_y every step (always on with k-omega). */
+int kofm_step(kofm_ctx* ctx, int32_t nsteps, uint32_t options);
+
+/* Runs nsteps as kofm_step and returns the mean device time per step in
+ * *ms_per_step and of the dominant (collide-stream) kernel in
+ * *ms_per_kernel, both measured with HIP events on the context stream. */
+int kofm_time_steps(kofm_ctx* ctx, int32_t nsteps, uint32_t options, double* ms_per_step,
+                    double* ms_per_kernel);
+
+/* Wall shear at wall nodes and boundary links from the newest populations
+ * and the stored macros (shearStress::calculateShear, shearStress.cpp:70-83:
+ * TR_shear_1 and TR_shear_2, trKernelsShear.cl:174-316). */
+int kofm_wall_shear(kofm_ctx* ctx);
+
+int kofm_get_solver_stats(kofm_ctx* ctx, kofm_solver_stats* out);
+
+/* --- output --------------------------------------------------------------- */
+
+/* Sum/min/max of one field over the zero-padded power-of-two domain,
+ * accumulated in FP64 (Reducer::reduceSingle, Reducer.h:240-273). */
+int kofm_reduce(kofm_ctx* ctx, int32_t field, int32_t op, double* out);
+
+/* Copies one field (KOFM_FIELD_*) to host FP64 in the reference layout. */
+int kofm_download(kofm_ctx* ctx, int32_t field, double* host);
+
+/* Writes lbf.bin / lbkappa.bin / lbomega.bin (raw FP64, Array.h:1178-1194,
+ * clVariablesLB.cpp:752-765, GMRESSolver::saveCheckPoint) and the save2file
+ * text dumps lbux.txt, lbuy.txt, lbro.txt (+ lbkappa.txt, lbomega.txt) in
+ * the reference format (Array.h:2528-2546: one row per x, tab-separated
+ * values per y, 6 significant digits) into dir. */
+int kofm_save2file(kofm_ctx* ctx, const char* dir);
+
+/* Blocks until all work on the context stream is done. */
+int kofm_synchronize(kofm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KOFM_H */

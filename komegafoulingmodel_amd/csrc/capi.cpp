@@ -1,0 +1,305 @@
+// capi.cpp -- the extern "C" boundary declared in include/kofm.h.
+#include <hip/hip_runtime_api.h>
+
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/kofm.h"
+#include "engine.hpp"
+#include "geometry.hpp"
+#include "io.hpp"
+
+struct kofm_ctx {
+  int nX = 0, nY = 0, storage = 0;
+  bool komega = false;
+  std::unique_ptr<kofm::Engine> engine;
+  kofm::Geometry geo;
+  kofm::Params params;
+  double pipeRadius = 0.;
+  bool haveGeo = false;
+  std::string err;
+};
+
+namespace {
+thread_local std::string g_createError;
+
+#define NEED_ENGINE(ctx)                                                              \
+  do {                                                                                \
+    if (!(ctx)->engine) {                                                             \
+      (ctx)->err = "host-only context (kofm_create_host) has no device state";        \
+      return KOFM_ERR_STATE;                                                          \
+    }                                                                                 \
+  } while (0)
+
+int setErr(kofm_ctx* c, int rc, const std::string& msg) {
+  if (c) c->err = msg;
+  return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int kofm_create(const kofm_grid* grid, kofm_ctx** out) {
+  if (!grid || !out) {
+    g_createError = "null argument";
+    return KOFM_ERR_ARG;
+  }
+  *out = nullptr;
+  if (grid->nX < 4 || grid->nY < 4) {
+    g_createError = "lattice must be at least 4 x 4";
+    return KOFM_ERR_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    g_createError = "no HIP device available";
+    return KOFM_ERR_DEVICE;
+  }
+  auto* c = new kofm_ctx;
+  c->nX = grid->nX;
+  c->nY = grid->nY;
+  c->storage = grid->storage;
+  c->komega = grid->use_komega != 0;
+  std::string err;
+  c->engine = kofm::makeEngine(c->nX, c->nY, c->storage, c->komega, err);
+  if (!c->engine) {
+    g_createError = err.empty() ? "engine creation failed" : err;
+    delete c;
+    return KOFM_ERR_DEVICE;
+  }
+  c->engine->setParams(c->params);
+  *out = c;
+  return KOFM_OK;
+}
+
+int kofm_create_host(const kofm_grid* grid, kofm_ctx** out) {
+  if (!grid || !out) {
+    g_createError = "null argument";
+    return KOFM_ERR_ARG;
+  }
+  *out = nullptr;
+  if (grid->nX < 4 || grid->nY < 4) {
+    g_createError = "lattice must be at least 4 x 4";
+    return KOFM_ERR_ARG;
+  }
+  auto* c = new kofm_ctx;
+  c->nX = grid->nX;
+  c->nY = grid->nY;
+  c->storage = grid->storage;
+  c->komega = grid->use_komega != 0;
+  *out = c;
+  return KOFM_OK;
+}
+
+void kofm_destroy(kofm_ctx* ctx) { delete ctx; }
+
+const char* kofm_last_error(const kofm_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_createError.c_str();
+}
+
+int kofm_set_params(kofm_ctx* ctx, const kofm_params* p) {
+  if (!ctx || !p) return KOFM_ERR_ARG;
+  if (!(p->tau0 > 0. && p->tau0 < 2.))
+    return setErr(ctx, KOFM_ERR_ARG, "tau0 (relaxation frequency) must lie in (0, 2)");
+  kofm::Params& q = ctx->params;
+  q.tau0 = p->tau0;
+  q.Fval = p->Fval;
+  q.nu = p->nu;
+  q.rho0 = p->rho0;
+  q.maxNut = p->max_turb_visc;
+  q.koDt = p->ko_dt;
+  q.relTol[0] = p->kappa_rel_tol;
+  q.absTol[0] = p->kappa_abs_tol;
+  q.maxIters[0] = p->kappa_max_iters;
+  q.relTol[1] = p->omega_rel_tol;
+  q.absTol[1] = p->omega_abs_tol;
+  q.maxIters[1] = p->omega_max_iters;
+  if (q.maxIters[0] < 1 || q.maxIters[1] < 1)
+    return setErr(ctx, KOFM_ERR_ARG, "solver iteration limits must be >= 1");
+  if (ctx->engine) ctx->engine->setParams(q);
+  return KOFM_OK;
+}
+
+int kofm_build_geometry(kofm_ctx* ctx, const double* C0, int32_t nN, const int32_t* BL,
+                        int32_t nBL, const kofm_geom_params* gp) {
+  if (!ctx || !C0 || !BL || !gp || nN < 2 || nBL < 1) return KOFM_ERR_ARG;
+  kofm::GeometryInput in;
+  in.nX = ctx->nX;
+  in.nY = ctx->nY;
+  in.C0.assign(C0, C0 + 2 * size_t(nN));
+  in.BL.assign(BL, BL + 2 * size_t(nBL));
+  in.pipeRadius = gp->pipe_radius;
+  in.eps = gp->eps;
+  in.minDx = gp->min_dx;
+  in.wallDSearchRad = gp->wall_d_search_rad;
+  in.xRelease = gp->x_release_pos;
+  in.xMin = gp->x_min_val;
+  in.xStop = gp->x_stop_pos;
+  in.cutoff = gp->cutoff_radius;
+  in.indexRadiusSearch = gp->index_radius_search;
+  std::string err;
+  int rc = kofm::buildGeometry(in, ctx->geo, err);
+  if (rc) return setErr(ctx, rc, err);
+  if (ctx->engine) {
+    rc = ctx->engine->uploadGeometry(ctx->geo, err);
+    if (rc) return setErr(ctx, rc, err);
+  }
+  ctx->pipeRadius = gp->pipe_radius;
+  ctx->haveGeo = true;
+  return KOFM_OK;
+}
+
+int kofm_geometry_counts(const kofm_ctx* ctx, kofm_geom_counts* out) {
+  if (!ctx || !out) return KOFM_ERR_ARG;
+  const kofm::Geometry& g = ctx->geo;
+  out->nX = ctx->nX;
+  out->nY = ctx->nY;
+  out->XsizeFull = ((ctx->nX + 127) / 128) * 128;
+  out->D = int64_t(out->XsizeFull) * ctx->nY;
+  out->nIbb = int32_t(g.ibb.size());
+  out->nSS = int32_t(g.ss.size());
+  out->nBLshear = g.nBLshear();
+  int nodes = 0;
+  {
+    std::vector<char> seen(size_t(out->D), 0);
+    for (int32_t l : g.ibb) {
+      size_t node = size_t(l % out->D);
+      if (!seen[node]) {
+        seen[node] = 1;
+        nodes++;
+      }
+    }
+  }
+  out->nIbbNodes = nodes;
+  return KOFM_OK;
+}
+
+int kofm_geometry_array(const kofm_ctx* ctx, int32_t which, void* out) {
+  if (!ctx || !out) return KOFM_ERR_ARG;
+  if (!ctx->haveGeo) return KOFM_ERR_STATE;
+  const kofm::Geometry& g = ctx->geo;
+  auto put = [&](const auto& v) {
+    if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(v[0]));
+    return KOFM_OK;
+  };
+  switch (which) {
+    case KOFM_GEOM_NTYPE: return put(g.nType);
+    case KOFM_GEOM_DX: return put(g.dX);
+    case KOFM_GEOM_IBB: return put(g.ibb);
+    case KOFM_GEOM_IBBQ: return put(g.ibbQ);
+    case KOFM_GEOM_SS: return put(g.ss);
+    case KOFM_GEOM_SSMAP: return put(g.ssMap);
+    case KOFM_GEOM_SSINDS: return put(g.ssInds);
+    case KOFM_GEOM_LSMAP: return put(g.lsMap);
+    case KOFM_GEOM_WALLD: return put(g.wallD);
+    case KOFM_GEOM_CFSS: return put(g.cfss);
+    case KOFM_GEOM_SIND: return put(g.sInd);
+    case KOFM_GEOM_WEIGHTS: return put(g.weights);
+    case KOFM_GEOM_BOUNDS: std::memcpy(out, g.bounds, sizeof(g.bounds)); return KOFM_OK;
+    case KOFM_GEOM_MASSES: std::memcpy(out, g.masses, sizeof(g.masses)); return KOFM_OK;
+    default: return KOFM_ERR_ARG;
+  }
+}
+
+int kofm_init_state(kofm_ctx* ctx, const double* rho, const double* ux, const double* uy) {
+  if (!ctx) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  if (!ctx->haveGeo) return setErr(ctx, KOFM_ERR_STATE, "build the geometry first");
+  std::string err;
+  int rc = ctx->engine->initState(rho, ux, uy, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_init_komega(kofm_ctx* ctx, double k_ini, double omega_ini, double Re) {
+  if (!ctx) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int rc = ctx->engine->initKOmega(k_ini, omega_ini, Re, ctx->pipeRadius, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_upload_state(kofm_ctx* ctx, const double* f, const double* k, const double* omega,
+                      const double* nut, const double* S) {
+  if (!ctx || !f) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int rc = ctx->engine->uploadState(f, k, omega, nut, S, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_step(kofm_ctx* ctx, int32_t nsteps, uint32_t options) {
+  if (!ctx || nsteps < 0) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int rc = ctx->engine->step(nsteps, options, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_time_steps(kofm_ctx* ctx, int32_t nsteps, uint32_t options, double* ms_per_step,
+                    double* ms_per_kernel) {
+  if (!ctx || !ms_per_step || !ms_per_kernel) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int rc = ctx->engine->timeSteps(nsteps, options, *ms_per_step, *ms_per_kernel, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_wall_shear(kofm_ctx* ctx) {
+  if (!ctx) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int rc = ctx->engine->wallShear(err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_get_solver_stats(kofm_ctx* ctx, kofm_solver_stats* out) {
+  if (!ctx || !out) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  kofm::SolverStats s;
+  std::string err;
+  int rc = ctx->engine->solverStats(s, err);
+  if (rc) return setErr(ctx, rc, err);
+  out->kappa_iters = s.iters[0];
+  out->omega_iters = s.iters[1];
+  out->kappa_res0 = s.res0[0];
+  out->kappa_res = s.res[0];
+  out->omega_res0 = s.res0[1];
+  out->omega_res = s.res[1];
+  out->nan_flag = s.nan;
+  return KOFM_OK;
+}
+
+int kofm_reduce(kofm_ctx* ctx, int32_t field, int32_t op, double* out) {
+  if (!ctx || !out) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int rc = ctx->engine->reduce(field, op, *out, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_download(kofm_ctx* ctx, int32_t field, double* host) {
+  if (!ctx || !host) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int rc = ctx->engine->download(field, host, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_save2file(kofm_ctx* ctx, const char* dir) {
+  if (!ctx || !dir) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int rc = kofm::saveFields(*ctx->engine, ctx->nX, ctx->nY, ctx->komega, dir, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_synchronize(kofm_ctx* ctx) {
+  if (!ctx) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int rc = ctx->engine->synchronize(err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+}  // extern "C"

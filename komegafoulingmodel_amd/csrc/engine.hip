@@ -1,0 +1,617 @@
+// engine.hip -- device memory layout and launch sequencing of the step.
+//
+// HBM layout per lattice (D = XsizeFull * nY nodes, every plane starts on a
+// 256-byte boundary because XsizeFull is a multiple of 128):
+//   fA, fB      9 planes of stored deviations (ping-pong, push streaming)
+//   mask        uint16 per node (solid, wall-node, bounce-back bits)
+//   ibbIdx      int32 per node, read only at wall nodes (q-table row)
+//   rho/ux/uy   macro planes (written when asked, always with k-omega)
+//   k-omega     K, O (+ Jacobi temporaries), Nut, 3 strain planes, WallD,
+//               4 dX planes, Dk, Do, F1, 2 cross-diffusion planes,
+//               2 x 4 matrix planes and 2 right-hand sides
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/kofm.h"
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace kofm {
+namespace {
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      err = std::string(#expr) + ": " + hipGetErrorString(e_);                          \
+      return KOFM_ERR_DEVICE;                                                           \
+    }                                                                                   \
+  } while (0)
+
+class Buffer {
+ public:
+  Buffer() = default;
+  Buffer(const Buffer&) = delete;
+  Buffer& operator=(const Buffer&) = delete;
+  ~Buffer() { release(); }
+  hipError_t alloc(size_t bytes) {
+    release();
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(&p_, bytes);
+    if (e == hipSuccess) {
+      bytes_ = bytes;
+      e = hipMemset(p_, 0, bytes);
+    } else {
+      p_ = nullptr;
+    }
+    return e;
+  }
+  void release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    bytes_ = 0;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p_);
+  }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  void* p_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+template <class S, class R>
+class EngineT final : public Engine {
+ public:
+  EngineT(int nX, int nY, bool komega) : nX_(nX), nY_(nY), ko_(komega) {
+    XF_ = ((nX + 127) / 128) * 128;
+    D_ = int64_t(XF_) * nY;
+  }
+  ~EngineT() override {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto& e : evs_) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+
+  int create(std::string& err) {
+    HIP_TRY(hipStreamCreate(&stream_));
+    const size_t D = size_t(D_);
+    HIP_TRY(fA_.alloc(9 * D * sizeof(S)));
+    HIP_TRY(fB_.alloc(9 * D * sizeof(S)));
+    HIP_TRY(mask_.alloc(D * sizeof(uint16_t)));
+    HIP_TRY(ibbIdx_.alloc(D * sizeof(int32_t)));
+    HIP_TRY(rho_.alloc(D * sizeof(R)));
+    HIP_TRY(ux_.alloc(D * sizeof(R)));
+    HIP_TRY(uy_.alloc(D * sizeof(R)));
+    if (ko_) {
+      for (Buffer* b : {&K_, &O_, &nut_, &wallD_, &Dk_, &Do_, &F1_, &bk_, &bo_})
+        HIP_TRY(b->alloc(D * sizeof(R)));
+      HIP_TRY(S_.alloc(3 * D * sizeof(R)));
+      HIP_TRY(dkdo_.alloc(2 * D * sizeof(R)));
+      HIP_TRY(dX_.alloc(4 * D * sizeof(R)));
+      HIP_TRY(Ak_.alloc(4 * D * sizeof(R)));
+      HIP_TRY(Ao_.alloc(4 * D * sizeof(R)));
+    }
+    HIP_TRY(status_.alloc(sizeof(dev::BicgState)));
+    if (ko_)
+      for (int v = 0; v < 2; v++)
+        for (Buffer* b : {&kx_[v], &kr_[v], &krh_[v], &kp_[v][0], &kp_[v][1], &kv_[v][0], &kv_[v][1], &ks_[v], &kt_[v]})
+          HIP_TRY(b->alloc(D * sizeof(double)));
+    const size_t nTiles = size_t((nX_ + dev::kBlockX - 1) / dev::kBlockX) * nY_;
+    const size_t nblk = 8 * ((nTiles + 7) / 8);
+    HIP_TRY(partial_.alloc(4 * nblk * sizeof(double)));
+    HIP_TRY(redPartial_.alloc(1024 * sizeof(double)));
+    HIP_TRY(redOut_.alloc(sizeof(double)));
+    return 0;
+  }
+
+  int64_t fieldSize(int field) const override {
+    switch (field) {
+      case KOFM_FIELD_F: return 9 * D_;
+      case KOFM_FIELD_SXY: return 3 * D_;
+      case KOFM_FIELD_TAU_NODE: return int64_t(geo_.ss.size());
+      case KOFM_FIELD_TAU_BL: return geo_.nBLshear();
+      default: return D_;
+    }
+  }
+
+  int uploadGeometry(const Geometry& g, std::string& err) override {
+    geo_ = g;
+    const size_t D = size_t(D_);
+    // device mask and wall-node q table
+    std::vector<uint16_t> m(D, 0);
+    std::vector<int32_t> idx(D, -1);
+    std::vector<R> q;
+    for (size_t i = 0; i < D; i++) {
+      const uint16_t t = uint16_t(g.nType[i]);
+      uint16_t v = t & 0xFF00;
+      if (t & kMSolid) v |= dev::kSolid;
+      if (t & kSolidBoundary) v |= dev::kSolidBoundary;
+      m[i] = v;
+    }
+    for (int x = nX_; x < XF_; x++)
+      for (int y = 0; y < nY_; y++) m[x + size_t(XF_) * y] = dev::kSolid;
+    for (size_t l = 0; l < g.ibb.size(); l++) {
+      const int64_t loc = g.ibb[l];
+      const int dir = int(loc / D_);
+      const size_t node = size_t(loc - int64_t(dir) * D_);
+      if (idx[node] < 0) {
+        idx[node] = int32_t(q.size() / 8);
+        q.insert(q.end(), 8, R(-1));
+        m[node] |= dev::kIbbNode;
+      }
+      q[8 * size_t(idx[node]) + (dir - 1)] = R(g.ibbQ[l]);  // a repeated link keeps the last q
+    }
+    nIbbNodes_ = int(q.size() / 8);
+    HIP_TRY(hipMemcpy(mask_.as<void>(), m.data(), D * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ibbIdx_.as<void>(), idx.data(), D * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(ibbQ_.alloc(std::max<size_t>(q.size(), 8) * sizeof(R)));
+    if (!q.empty())
+      HIP_TRY(hipMemcpy(ibbQ_.as<void>(), q.data(), q.size() * sizeof(R), hipMemcpyHostToDevice));
+    // wall-shear tables
+    const size_t nSS = g.ss.size(), nb = size_t(g.nBLshear());
+    HIP_TRY(ss_.alloc(std::max<size_t>(nSS, 1) * sizeof(int32_t)));
+    HIP_TRY(cfss_.alloc(std::max<size_t>(16 * nSS, 1) * sizeof(R)));
+    HIP_TRY(tau_.alloc(std::max<size_t>(nSS, 1) * sizeof(R)));
+    HIP_TRY(sInd_.alloc(std::max<size_t>(4 * nb, 1) * sizeof(int32_t)));
+    HIP_TRY(wts_.alloc(std::max<size_t>(4 * nb, 1) * sizeof(R)));
+    HIP_TRY(tauBL_.alloc(std::max<size_t>(nb, 1) * sizeof(R)));
+    if (nSS) {
+      HIP_TRY(hipMemcpy(ss_.as<void>(), g.ss.data(), nSS * sizeof(int32_t), hipMemcpyHostToDevice));
+      HIP_TRY(upload<R>(cfss_, g.cfss));
+    }
+    if (nb) {
+      HIP_TRY(hipMemcpy(sInd_.as<void>(), g.sInd.data(), 4 * nb * sizeof(int32_t), hipMemcpyHostToDevice));
+      HIP_TRY(upload<R>(wts_, g.weights));
+    }
+    if (ko_) {
+      HIP_TRY(upload<R>(wallD_, g.wallD));
+      HIP_TRY(upload<R>(dX_, g.dX));
+    }
+    haveGeo_ = true;
+    return 0;
+  }
+
+  void setParams(const Params& p) override { par_ = p; }
+
+  int initState(const double* rho, const double* ux, const double* uy, std::string& err) override {
+    if (!haveGeo_) return fail(err, "geometry must be built before the state");
+    const size_t D = size_t(D_);
+    std::vector<double> f(9 * D, 0.);
+    std::vector<R> r(D, R(0)), u(D, R(0)), v(D, R(0));
+    for (int y = 0; y < nY_; y++)
+      for (int x = 0; x < nX_; x++) {
+        const size_t g = x + size_t(XF_) * y;
+        if (geo_.nType[g] & kMSolid) continue;
+        const double rr = rho ? rho[g] : par_.rho0, a = ux ? ux[g] : 0., b = uy ? uy[g] : 0.;
+        for (int i = 0; i < 9; i++) {
+          const double px = Lattice::cx[i] == 0 ? 2. / 3. - a * a : (1. / 3. + Lattice::cx[i] * a + a * a) * 0.5;
+          const double py = Lattice::cy[i] == 0 ? 2. / 3. - b * b : (1. / 3. + Lattice::cy[i] * b + b * b) * 0.5;
+          f[g + D * i] = rr * px * py;
+        }
+        r[g] = R(rr);
+        u[g] = R(a);
+        v[g] = R(b);
+      }
+    int rc = importF(f.data(), err);
+    if (rc) return rc;
+    HIP_TRY(upload<R>(rho_, r));
+    HIP_TRY(upload<R>(ux_, u));
+    HIP_TRY(upload<R>(uy_, v));
+    if (ko_) HIP_TRY(hipMemsetAsync(S_.as<void>(), 0, S_.bytes(), stream_));
+    preIbb_ = false;
+    haveState_ = true;
+    return 0;
+  }
+
+  int initKOmega(double k, double w, double Re, double pipeRadius, std::string& err) override {
+    if (!ko_) return fail(err, "k-omega is off for this context");
+    if (!haveState_) return fail(err, "populations must be initialised before k and omega");
+    if (k <= 0. || w <= 0.) {
+      // kOmega::setInitialValues (kOmega.cpp:530-570)
+      double sux = 0., sro = 0.;
+      int rc = reduce(KOFM_FIELD_UX, KOFM_RED_SUM, sux, err);
+      if (!rc) rc = reduce(KOFM_FIELD_RHO, KOFM_RED_SUM, sro, err);
+      if (rc) return rc;
+      double um = sux / sro;
+      if (um == 0.) um = Re * par_.nu / pipeRadius / 4.;
+      const double reIni = pipeRadius * 4. * um / par_.nu;
+      const double ti = 0.16 * std::pow(reIni, -1. / 8.);
+      const double lInv = 1. / (0.036 * pipeRadius);
+      if (k <= 0.) k = 1.5 * (ti * um) * (ti * um);
+      if (w <= 0.) w = std::sqrt(k) / 0.5477225575 * lInv;
+    }
+    if (k <= 0. || w <= 0.) {
+      err = "Initial value for initial tKE or omega <= 0.";
+      return KOFM_ERR_INITIALIZING_KOMEGA;
+    }
+    const double nt = k / w;
+    if (nt > 1.0) {
+      err = "Initial Turbulent viscosity greater than set maximum value";
+      return KOFM_ERR_INITIALIZING_KOMEGA;
+    }
+    const size_t D = size_t(D_);
+    std::vector<R> a(D, R(0)), b(D, R(0)), c(D, R(0));
+    for (size_t i = 0; i < D; i++)
+      if (geo_.nType[i] & kMFluid) {
+        a[i] = R(k);
+        b[i] = R(w);
+        c[i] = R(nt);
+      }
+    HIP_TRY(upload<R>(K_, a));
+    HIP_TRY(upload<R>(O_, b));
+    HIP_TRY(upload<R>(nut_, c));
+    HIP_TRY(hipMemsetAsync(S_.as<void>(), 0, S_.bytes(), stream_));
+    return 0;
+  }
+
+  int uploadState(const double* f, const double* k, const double* w, const double* nut,
+                  const double* strain, std::string& err) override {
+    if (!haveGeo_) return fail(err, "geometry must be built before the state");
+    const size_t D = size_t(D_);
+    int rc = importF(f, err);
+    if (rc) return rc;
+    if (ko_) {
+      if (k) HIP_TRY(upload<R>(K_, std::vector<double>(k, k + D)));
+      if (w) HIP_TRY(upload<R>(O_, std::vector<double>(w, w + D)));
+      if (nut) HIP_TRY(upload<R>(nut_, std::vector<double>(nut, nut + D)));
+      if (strain) HIP_TRY(upload<R>(S_, std::vector<double>(strain, strain + 3 * D)));
+    }
+    auto p = lbParams(0);
+    hipLaunchKernelGGL((dev::lbMacros<S, R>), gridXY(nX_), dim3(dev::kBlockX), 0, stream_,
+                       cur(), mask_.as<uint16_t>(), rho_.as<R>(), ux_.as<R>(), uy_.as<R>(), p);
+    HIP_TRY(hipGetLastError());
+    preIbb_ = false;
+    haveState_ = true;
+    return 0;
+  }
+
+  int step(int n, unsigned options, std::string& err) override {
+    if (!haveState_) return fail(err, "state not initialised");
+    for (int s = 0; s < n; s++) {
+      int rc = launchLB(options);
+      if (!rc && ko_) rc = koStep(err);
+      if (rc) return rc ? rc : KOFM_ERR_DEVICE;
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+  }
+
+  int timeSteps(int n, unsigned options, double& msStep, double& msKernel, std::string& err) override {
+    if (!haveState_) return fail(err, "state not initialised");
+    if (n <= 0) return fail(err, "nsteps must be positive");
+    while (int(evs_.size()) < 2 * n + 2) {
+      hipEvent_t e;
+      HIP_TRY(hipEventCreate(&e));
+      evs_.push_back(e);
+    }
+    HIP_TRY(hipEventRecord(evs_[0], stream_));
+    for (int s = 0; s < n; s++) {
+      HIP_TRY(hipEventRecord(evs_[2 + 2 * s], stream_));
+      int rc = launchLB(options);
+      HIP_TRY(hipEventRecord(evs_[3 + 2 * s], stream_));
+      if (!rc && ko_) rc = koStep(err);
+      if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(evs_[1], stream_));
+    HIP_TRY(hipEventSynchronize(evs_[1]));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, evs_[0], evs_[1]));
+    msStep = double(t) / n;
+    double acc = 0.;
+    for (int s = 0; s < n; s++) {
+      HIP_TRY(hipEventElapsedTime(&t, evs_[2 + 2 * s], evs_[3 + 2 * s]));
+      acc += t;
+    }
+    msKernel = acc / n;
+    HIP_TRY(hipGetLastError());
+    return 0;
+  }
+
+  int wallShear(std::string& err) override {
+    if (!haveState_) return fail(err, "state not initialised");
+    const int nSS = int(geo_.ss.size()), nb = geo_.nBLshear();
+    auto p = lbParams(0);
+    if (nSS > 0) {
+      hipLaunchKernelGGL((dev::shearNodes<S, R>), dim3((nSS + 63) / 64), dim3(64), 0, stream_,
+                         cur(), mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(), ibbQ_.as<R>(),
+                         ss_.as<int32_t>(), nSS, cfss_.as<R>(), rho_.as<R>(), ux_.as<R>(),
+                         uy_.as<R>(), ko_ ? nut_.as<R>() : nullptr, tau_.as<R>(), p);
+      HIP_TRY(hipGetLastError());
+    }
+    if (nb > 0) {
+      hipLaunchKernelGGL((dev::shearLinks<R>), dim3((nb + 63) / 64), dim3(64), 0, stream_,
+                         wts_.as<R>(), sInd_.as<int32_t>(), tau_.as<R>(), nb, tauBL_.as<R>());
+      HIP_TRY(hipGetLastError());
+    }
+    return 0;
+  }
+
+  int reduce(int field, int op, double& out, std::string& err) override {
+    const R* a = fieldPtr(field);
+    if (!a) return fail(err, "field cannot be reduced");
+    if (op != KOFM_RED_SUM && op != KOFM_RED_MAX && op != KOFM_RED_MIN) return fail(err, "bad op");
+    const int64_t n = fieldSize(field);
+    int64_t full = 2;
+    while (full < n) full *= 2;
+    const int blocks = int(std::min<int64_t>(1024, (n + 255) / 256));
+    hipLaunchKernelGGL((dev::reducePartial<R>), dim3(blocks), dim3(256), 0, stream_, a, n, op,
+                       redPartial_.as<double>());
+    hipLaunchKernelGGL(dev::reduceFinal, dim3(1), dim3(256), 0, stream_,
+                       (const double*)redPartial_.as<double>(), blocks, op, full > n ? 1 : 0,
+                       redOut_.as<double>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(&out, redOut_.as<void>(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    return 0;
+  }
+
+  int download(int field, double* host, std::string& err) override {
+    if (field == KOFM_FIELD_F) {
+      Buffer tmp;
+      HIP_TRY(tmp.alloc(9 * size_t(D_) * sizeof(double)));
+      auto p = lbParams(0);
+      hipLaunchKernelGGL((dev::lbExportF<S, R>), gridXY(XF_), dim3(dev::kBlockX), 0, stream_,
+                         cur(), mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(), ibbQ_.as<R>(),
+                         tmp.as<double>(), p);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(host, tmp.as<void>(), 9 * size_t(D_) * sizeof(double),
+                             hipMemcpyDeviceToHost, stream_));
+      HIP_TRY(hipStreamSynchronize(stream_));
+      return 0;
+    }
+    const R* a = fieldPtr(field);
+    if (!a) return fail(err, "field not available in this context");
+    const size_t n = size_t(fieldSize(field));
+    std::vector<R> h(n);
+    HIP_TRY(hipMemcpyAsync(h.data(), a, n * sizeof(R), hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    for (size_t i = 0; i < n; i++) host[i] = double(h[i]);
+    return 0;
+  }
+
+  int solverStats(SolverStats& s, std::string& err) override {
+    dev::BicgState st;
+    HIP_TRY(hipMemcpyAsync(&st, status_.as<void>(), sizeof(st), hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    for (int v = 0; v < 2; v++) {
+      s.iters[v] = st.iters[v];
+      s.res0[v] = st.r0[v];
+      s.res[v] = st.rn[v];
+    }
+    s.nan = nanSeen_ || st.nan;
+    nanSeen_ = false;
+    return 0;
+  }
+
+  int synchronize(std::string& err) override {
+    HIP_TRY(hipStreamSynchronize(stream_));
+    return 0;
+  }
+
+ private:
+  int nX_, nY_, XF_;
+  int64_t D_;
+  bool ko_;
+  hipStream_t stream_ = nullptr;
+  Buffer fA_, fB_, mask_, ibbIdx_, ibbQ_, rho_, ux_, uy_;
+  Buffer K_, O_, nut_, S_, wallD_, Dk_, Do_, F1_, dkdo_, dX_, Ak_, Ao_, bk_, bo_;
+  Buffer kx_[2], kr_[2], krh_[2], kp_[2][2], kv_[2][2], ks_[2], kt_[2];  // FP64 BiCGStab vectors, [0] = k
+  Buffer ss_, cfss_, tau_, sInd_, wts_, tauBL_;
+  Buffer status_, partial_, redPartial_, redOut_;
+  std::vector<hipEvent_t> evs_;
+  Geometry geo_;
+  Params par_;
+  int nIbbNodes_ = 0;
+  int curBuf_ = 0;       // 0: fA holds the newest populations
+  bool preIbb_ = false;  // newest populations still need the wall-node correction
+  bool haveGeo_ = false, haveState_ = false, nanSeen_ = false;
+
+  static int fail(std::string& err, const char* msg) {
+    err = msg;
+    return KOFM_ERR_STATE;
+  }
+
+  template <class T, class V>
+  hipError_t upload(Buffer& b, const std::vector<V>& v) {
+    std::vector<T> t(v.begin(), v.end());
+    return hipMemcpy(b.as<void>(), t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice);
+  }
+
+  S* cur() const { return curBuf_ == 0 ? fA_.as<S>() : fB_.as<S>(); }
+  S* other() const { return curBuf_ == 0 ? fB_.as<S>() : fA_.as<S>(); }
+
+  dim3 gridXY(int width) const { return dim3((width + dev::kBlockX - 1) / dev::kBlockX, nY_); }
+
+  dev::LBParams<R> lbParams(unsigned options) const {
+    dev::LBParams<R> p;
+    p.nX = nX_;
+    p.nY = nY_;
+    p.XF = XF_;
+    p.D = D_;
+    p.omega0 = R(par_.tau0);
+    p.Fval = R(par_.Fval);
+    p.nu = R(par_.nu);
+    p.maxNut = R(par_.maxNut);
+    p.saveMacros = (options & KOFM_OPTION_SAVE_MACRO_FIELDS) || ko_ ? 1 : 0;
+    p.applyIbb = preIbb_ ? 1 : 0;
+    return p;
+  }
+
+  dev::KOParams<R> koParams() const {
+    dev::KOParams<R> p;
+    p.nX = nX_;
+    p.nY = nY_;
+    p.XF = XF_;
+    p.D = D_;
+    p.nu = R(par_.nu);
+    p.dt = R(par_.koDt);
+    p.tiles.tilesX = (nX_ + dev::kBlockX - 1) / dev::kBlockX;
+    p.tiles.nTiles = p.tiles.tilesX * nY_;
+    p.tiles.per = (p.tiles.nTiles + 7) / 8;
+    return p;
+  }
+
+  const R* fieldPtr(int field) const {
+    switch (field) {
+      case KOFM_FIELD_RHO: return rho_.as<R>();
+      case KOFM_FIELD_UX: return ux_.as<R>();
+      case KOFM_FIELD_UY: return uy_.as<R>();
+      case KOFM_FIELD_KAPPA: return ko_ ? K_.as<R>() : nullptr;
+      case KOFM_FIELD_OMEGA: return ko_ ? O_.as<R>() : nullptr;
+      case KOFM_FIELD_NUT: return ko_ ? nut_.as<R>() : nullptr;
+      case KOFM_FIELD_SXY: return ko_ ? S_.as<R>() : nullptr;
+      case KOFM_FIELD_WALLD: return ko_ ? wallD_.as<R>() : nullptr;
+      case KOFM_FIELD_TAU_NODE: return tau_.as<R>();
+      case KOFM_FIELD_TAU_BL: return tauBL_.as<R>();
+      default: return nullptr;
+    }
+  }
+
+  int importF(const double* f, std::string& err) {
+    const size_t D = size_t(D_);
+    Buffer tmp;
+    HIP_TRY(tmp.alloc(9 * D * sizeof(double)));
+    HIP_TRY(hipMemcpy(tmp.as<void>(), f, 9 * D * sizeof(double), hipMemcpyHostToDevice));
+    curBuf_ = 0;
+    auto p = lbParams(0);
+    hipLaunchKernelGGL((dev::lbImportF<S, R>), gridXY(XF_), dim3(dev::kBlockX), 0, stream_,
+                       (const double*)tmp.as<double>(), fA_.as<S>(), p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream_));
+    return 0;
+  }
+
+  int launchLB(unsigned options) {
+    auto p = lbParams(options);
+    dev::KOFields kf{K_.as<void>(), O_.as<void>(), wallD_.as<void>(), nut_.as<void>(), S_.as<void>()};
+    if (ko_)
+      hipLaunchKernelGGL((dev::lbCollideStream<S, R, true>), gridXY(nX_), dim3(dev::kBlockX), 0,
+                         stream_, cur(), other(), mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(),
+                         ibbQ_.as<R>(), rho_.as<R>(), ux_.as<R>(), uy_.as<R>(), kf, p);
+    else
+      hipLaunchKernelGGL((dev::lbCollideStream<S, R, false>), gridXY(nX_), dim3(dev::kBlockX), 0,
+                         stream_, cur(), other(), mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(),
+                         ibbQ_.as<R>(), rho_.as<R>(), ux_.as<R>(), uy_.as<R>(), kf, p);
+    curBuf_ ^= 1;
+    preIbb_ = true;
+    return 0;
+  }
+
+  // kOmega::Solve (kOmega.cpp:31-73) with one sub-step: diffusivities,
+  // coefficients, then both implicit systems by Jacobi sweeps with
+  // device-side convergence checks, launched in chunks.
+  int koStep(std::string& err) {
+    auto kp = koParams();
+    const dim3 grid(8 * kp.tiles.per), blk(dev::kBlockX);
+    hipLaunchKernelGGL((dev::koDiffusivities<R>), grid, blk, 0, stream_, mask_.as<uint16_t>(),
+                       K_.as<R>(), O_.as<R>(), nut_.as<R>(), wallD_.as<R>(), Dk_.as<R>(),
+                       Do_.as<R>(), F1_.as<R>(), dkdo_.as<R>(), rho_.as<R>(), dX_.as<R>(), kp);
+    hipLaunchKernelGGL((dev::koCoeffs<R>), grid, blk, 0, stream_, mask_.as<uint16_t>(),
+                       K_.as<R>(), O_.as<R>(), nut_.as<R>(), Dk_.as<R>(), Do_.as<R>(),
+                       F1_.as<R>(), dkdo_.as<R>(), S_.as<R>(), rho_.as<R>(), ux_.as<R>(),
+                       uy_.as<R>(), dX_.as<R>(), wallD_.as<R>(), Ak_.as<R>(), Ao_.as<R>(),
+                       bk_.as<R>(), bo_.as<R>(), kp);
+    using Q = double;
+    dev::BicgVecs<R, Q> V;
+    R* X[2] = {K_.as<R>(), O_.as<R>()};
+    const R* A[2] = {Ak_.as<R>(), Ao_.as<R>()};
+    const R* B[2] = {bk_.as<R>(), bo_.as<R>()};
+    for (int v = 0; v < 2; v++) {
+      V.x[v] = X[v];
+      V.A[v] = A[v];
+      V.b[v] = B[v];
+      V.xq[v] = kx_[v].as<Q>();
+      V.r[v] = kr_[v].as<Q>();
+      V.rh[v] = krh_[v].as<Q>();
+      V.s[v] = ks_[v].as<Q>();
+      V.t[v] = kt_[v].as<Q>();
+      for (int c = 0; c < 2; c++) {
+        V.p[v][c] = kp_[v][c].as<Q>();
+        V.v[v][c] = kv_[v][c].as<Q>();
+      }
+    }
+    auto* st = status_.as<dev::BicgState>();
+    double* part = partial_.as<double>();
+    const int nblk = int(grid.x);
+    hipLaunchKernelGGL(dev::bicgReset, dim3(1), dim3(64), 0, stream_, st, par_.relTol[0],
+                       par_.absTol[0], par_.maxIters[0], par_.relTol[1], par_.absTol[1],
+                       par_.maxIters[1]);
+    hipLaunchKernelGGL((dev::bicgInit<R, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V, part, kp);
+    hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, (const double*)part, nblk, st, 0);
+    const int maxIt = std::max(par_.maxIters[0], par_.maxIters[1]);
+    const int chunk = 4;
+    int it = 0;
+    while (it < maxIt) {
+      for (int c = 0; c < chunk && it < maxIt; c++, it++) {
+        hipLaunchKernelGGL((dev::bicgPV<R, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V,
+                           (const dev::BicgState*)st, part, kp);
+        hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, (const double*)part, nblk, st, 1);
+        hipLaunchKernelGGL((dev::bicgST<R, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V,
+                           (const dev::BicgState*)st, part, kp);
+        hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, (const double*)part, nblk, st, 2);
+        hipLaunchKernelGGL((dev::bicgXR<R, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V,
+                           (const dev::BicgState*)st, part, kp);
+        hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, (const double*)part, nblk, st, 3);
+      }
+      dev::BicgState hs;
+      HIP_TRY(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, stream_));
+      HIP_TRY(hipStreamSynchronize(stream_));
+      if (hs.nan) nanSeen_ = true;
+      if (hs.done[0] && hs.done[1]) break;
+    }
+    hipLaunchKernelGGL((dev::bicgFinish<R, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V, kp);
+    HIP_TRY(hipGetLastError());
+    if (nanSeen_) {
+      err = "non-finite residual in the k-omega solve";
+      return KOFM_ERR_SOLVER_NAN;
+    }
+    return 0;
+  }
+};
+
+}  // namespace
+
+std::unique_ptr<Engine> makeEngine(int nX, int nY, int storage, bool komega, std::string& err) {
+  std::unique_ptr<Engine> e;
+  int rc = 0;
+  switch (storage) {
+    case KOFM_STORE_F32: {
+      auto* p = new EngineT<float, float>(nX, nY, komega);
+      e.reset(p);
+      rc = p->create(err);
+      break;
+    }
+    case KOFM_STORE_F16: {
+      auto* p = new EngineT<_Float16, float>(nX, nY, komega);
+      e.reset(p);
+      rc = p->create(err);
+      break;
+    }
+    case KOFM_STORE_BF16: {
+      auto* p = new EngineT<__bf16, float>(nX, nY, komega);
+      e.reset(p);
+      rc = p->create(err);
+      break;
+    }
+    case KOFM_STORE_F64: {
+      auto* p = new EngineT<double, double>(nX, nY, komega);
+      e.reset(p);
+      rc = p->create(err);
+      break;
+    }
+    default:
+      err = "unknown storage type";
+      return nullptr;
+  }
+  if (rc) return nullptr;
+  return e;
+}
+
+}  // namespace kofm

@@ -1,0 +1,51 @@
+// engine.hpp -- device-side state and launch sequencing for one lattice.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "geometry.hpp"
+
+namespace kofm {
+
+struct Params {
+  double tau0 = 1.25, Fval = 0., nu = 0.1, rho0 = 1., maxNut = 10., koDt = 1.;
+  double relTol[2] = {1e-4, 1e-4};  // k, omega
+  double absTol[2] = {1e-7, 1e-7};
+  int maxIters[2] = {1000, 1000};
+};
+
+struct SolverStats {
+  int iters[2] = {0, 0};
+  double res0[2] = {0., 0.}, res[2] = {0., 0.};
+  int nan = 0;
+};
+
+// Type-erased engine; the implementation is templated on the population
+// storage type and the arithmetic type.
+class Engine {
+ public:
+  virtual ~Engine() = default;
+  virtual int uploadGeometry(const Geometry& g, std::string& err) = 0;
+  virtual void setParams(const Params& p) = 0;
+  virtual int initState(const double* rho, const double* ux, const double* uy,
+                        std::string& err) = 0;
+  virtual int initKOmega(double k, double w, double Re, double pipeRadius, std::string& err) = 0;
+  virtual int uploadState(const double* f, const double* k, const double* w, const double* nut,
+                          const double* S, std::string& err) = 0;
+  virtual int step(int n, unsigned options, std::string& err) = 0;
+  virtual int timeSteps(int n, unsigned options, double& msStep, double& msKernel,
+                        std::string& err) = 0;
+  virtual int wallShear(std::string& err) = 0;
+  virtual int reduce(int field, int op, double& out, std::string& err) = 0;
+  virtual int download(int field, double* host, std::string& err) = 0;
+  virtual int solverStats(SolverStats& s, std::string& err) = 0;
+  virtual int synchronize(std::string& err) = 0;
+  virtual int64_t fieldSize(int field) const = 0;
+};
+
+// storage: KOFM_STORE_*
+std::unique_ptr<Engine> makeEngine(int nX, int nY, int storage, bool komega, std::string& err);
+
+}  // namespace kofm

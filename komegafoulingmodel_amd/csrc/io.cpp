@@ -1,0 +1,85 @@
+// io.cpp -- reference-compatible output files.
+#include "io.hpp"
+
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+
+#include "../../include/kofm.h"
+#include "engine.hpp"
+
+namespace kofm {
+
+int writeBin(const std::string& path, const std::vector<double>& a, std::string& err) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) {
+    err = "cannot open " + path;
+    return KOFM_ERR_ARG;
+  }
+  size_t n = std::fwrite(a.data(), sizeof(double), a.size(), f);
+  std::fclose(f);
+  if (n != a.size()) {
+    err = "short write to " + path;
+    return KOFM_ERR_ARG;
+  }
+  return 0;
+}
+
+int readBin(const std::string& path, std::vector<double>& a, size_t expect, std::string& err) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) {
+    err = "cannot open " + path;
+    return KOFM_ERR_ARG;
+  }
+  a.assign(expect, 0.);
+  size_t n = std::fread(a.data(), sizeof(double), expect, f);
+  std::fclose(f);
+  if (n != expect) {
+    err = path + " is shorter than the lattice";
+    return KOFM_ERR_ARG;
+  }
+  return 0;
+}
+
+int writeTxt2D(const std::string& path, const std::vector<double>& a, int nX, int XF, int nY,
+               std::string& err) {
+  std::ofstream out(path);
+  if (!out) {
+    err = "cannot open " + path;
+    return KOFM_ERR_ARG;
+  }
+  for (int i = 0; i < nX; i++) {
+    for (int j = 0; j < nY; j++) out << a[i + size_t(XF) * j] << "\t";
+    out << "\n";
+  }
+  return out ? 0 : KOFM_ERR_ARG;
+}
+
+int saveFields(Engine& e, int nX, int nY, bool komega, const std::string& dir, std::string& err) {
+  const int XF = ((nX + 127) / 128) * 128;
+  const size_t D = size_t(XF) * nY;
+  std::vector<double> f(9 * D);
+  int rc = e.download(KOFM_FIELD_F, f.data(), err);
+  if (!rc) rc = writeBin(dir + "/lbf.bin", f, err);
+  std::vector<double> a(D);
+  struct Item {
+    int field;
+    const char* txt;
+    const char* bin;
+  };
+  const Item items[] = {{KOFM_FIELD_UX, "lbux", nullptr},
+                        {KOFM_FIELD_UY, "lbuy", nullptr},
+                        {KOFM_FIELD_RHO, "lbro", nullptr},
+                        {KOFM_FIELD_KAPPA, "lbkappa", "lbkappa"},
+                        {KOFM_FIELD_OMEGA, "lbomega", "lbomega"}};
+  for (const Item& it : items) {
+    if (rc) break;
+    if (!komega && (it.field == KOFM_FIELD_KAPPA || it.field == KOFM_FIELD_OMEGA)) continue;
+    rc = e.download(it.field, a.data(), err);
+    if (!rc) rc = writeTxt2D(dir + "/" + it.txt + ".txt", a, nX, XF, nY, err);
+    if (!rc && it.bin) rc = writeBin(dir + "/" + it.bin + ".bin", a, err);
+  }
+  return rc;
+}
+
+}  // namespace kofm

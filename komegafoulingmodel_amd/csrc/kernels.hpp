@@ -1,0 +1,914 @@
+// kernels.hpp -- HIP device kernels of the lattice-Boltzmann + k-omega step
+// for gfx950 (CDNA4).  Everything here is bandwidth bound: SoA planes, one
+// lattice node per lane, 64-wide wavefronts along x, no MFMA.
+//
+// Storage.  Populations are kept as deviations from the lattice weights,
+// g_i = f_i - w_i, in the storage type S (float, _Float16, __bf16 or
+// double).  16-bit forms scale by 1024 / w_i before rounding so that every
+// direction uses the same part of the 16-bit range.  Arithmetic is done in
+// R (float, or double for the FP64 form).
+//
+// Time stepping follows the reference's push scheme (collide at the node,
+// halfway bounce-back into the node's own reversed slot, stream to the
+// neighbour; lbKernels.cl:175-335, 345-633), so the array in HBM after a
+// step is the reference's post-stream array.  The reference then runs a
+// separate Bouzidi interpolated bounce-back pass over the link list
+// (lbKernels.cl:640-668); here that correction is applied lazily, in
+// registers, by whichever kernel next reads a wall node (the next step, the
+// shear kernel, the download), using the same pre-correction inputs.  Wall
+// nodes carry the bit kIbbNode in the device mask and an index into an
+// 8-entry q table.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace kofm {
+namespace dev {
+
+// device mask bits (derived from the reference nType at upload)
+enum : uint32_t {
+  kSolid = 0x0001,         // M_SOLID_NODE
+  kIbbNode = 0x0002,       // node owns at least one interpolated link
+  kSolidBoundary = 0x0004, // SOLID_BOUNDARY_NODE (k, omega pinned to 0)
+  kBoundMask = 0xFF00,     // E_BOUND .. NW_BOUND, reference positions
+  kNesw = 0x0F00,          // NESW_BOUNDARY_NODE
+};
+
+constexpr int kBlockX = 256;
+
+__host__ __device__ constexpr int cx(int i) {
+  return (i == 1 || i == 5 || i == 7) ? 1 : ((i == 2 || i == 6 || i == 8) ? -1 : 0);
+}
+__host__ __device__ constexpr int cy(int i) {
+  return (i == 3 || i == 5 || i == 8) ? 1 : ((i == 4 || i == 6 || i == 7) ? -1 : 0);
+}
+__host__ __device__ constexpr int rev(int i) { return i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); }
+__host__ __device__ constexpr uint32_t boundBit(int i) { return i == 0 ? 0u : (0x80u << i); }
+__host__ __device__ constexpr double wgt(int i) {
+  return i == 0 ? 4. / 9. : (i < 5 ? 1. / 9. : 1. / 36.);
+}
+// 1-D weights: 2/3 at rest, 1/6 moving
+template <class R>
+__device__ constexpr R w1(int c) {
+  return c == 0 ? R(2. / 3.) : R(1. / 6.);
+}
+
+template <class S>
+struct Scaled {
+  static constexpr bool value = false;
+};
+template <>
+struct Scaled<_Float16> {
+  static constexpr bool value = true;
+};
+template <>
+struct Scaled<__bf16> {
+  static constexpr bool value = true;
+};
+
+// stored value <-> deviation g = f - w
+template <class S, class R>
+__device__ __forceinline__ R loadDev(const S* __restrict__ p, size_t idx, int dir) {
+  R v = static_cast<R>(p[idx]);
+  if constexpr (Scaled<S>::value) v *= R(wgt(dir) / 1024.);
+  return v;
+}
+template <class S, class R>
+__device__ __forceinline__ void storeDev(S* __restrict__ p, size_t idx, int dir, R g) {
+  if constexpr (Scaled<S>::value) g *= R(1024. / wgt(dir));
+  p[idx] = static_cast<S>(g);
+}
+
+template <class R>
+struct LBParams {
+  int nX, nY, XF;
+  int64_t D;
+  R omega0; // laminar relaxation frequency (reference tau0)
+  R Fval;   // forcing velocity increment (FTERM_VAL)
+  R nu;     // MU_NUMBER
+  R maxNut; // MAX_TURB_VISC
+  int saveMacros;
+  int applyIbb;
+};
+
+// 1-D equilibrium deviation: phi(c,u) - w1(c)
+template <class R>
+__device__ __forceinline__ R dphi(int c, R u) {
+  return c == 0 ? -u * u : R(0.5) * (R(c) * u + u * u);
+}
+// dphi(c, u + F) - dphi(c, u), without cancellation
+template <class R>
+__device__ __forceinline__ R dphiShift(int c, R u, R F) {
+  R t = F * (R(2) * u + F);
+  return c == 0 ? -t : R(0.5) * (R(c) * F + t);
+}
+
+// equilibrium deviation geq_i = feq_i(rho,u) - w_i, product form
+// (feq of clVariablesLB.cpp:77-117 written as rho*phi(cx,ux)*phi(cy,uy))
+template <class R>
+__device__ __forceinline__ R geq(int i, R drho, R rho, R ux, R uy) {
+  const int a = cx(i), b = cy(i);
+  R px = dphi(a, ux), py = dphi(b, uy);
+  return drho * R(wgt(i)) + rho * (w1<R>(a) * py + w1<R>(b) * px + px * py);
+}
+
+// Lazy Bouzidi correction of a wall node's freshly streamed populations
+// (lbIBB lbKernels.cl:640-668: q <= 1/2 interpolates between the bounced
+// value and the node's own incoming value, q > 1/2 between the bounced
+// value and the reversed value streamed to the upstream neighbour).
+template <class S, class R>
+__device__ __forceinline__ void ibbCorrect(R (&g)[9], const S* __restrict__ src, int x, int y,
+                                           const R* __restrict__ q8, const LBParams<R>& p) {
+  R nv[9];
+#pragma unroll
+  for (int j = 0; j < 9; j++) nv[j] = g[j];
+#pragma unroll
+  for (int d = 1; d < 9; d++) {
+    const R q = q8[d - 1];
+    if (q < R(0)) continue;
+    const int j = rev(d);
+    const R tq = R(2) * q;
+    if (tq <= R(1)) {
+      nv[j] = tq * g[j] + (R(1) - tq) * g[d];
+    } else {
+      int sx = x - cx(d);
+      sx = sx >= p.nX ? sx - p.nX : (sx < 0 ? sx + p.nX : sx);
+      const int sy = y - cy(d);
+      const R up = loadDev<S, R>(src, size_t(sx) + size_t(p.XF) * sy + size_t(p.D) * j, j);
+      nv[j] = R(1) / tq * (g[j] + (tq - R(1)) * up);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 9; j++) g[j] = nv[j];
+}
+
+template <class S, class R>
+__device__ __forceinline__ void loadNode(R (&g)[9], const S* __restrict__ src, size_t gi, int x,
+                                         int y, uint32_t m, const int32_t* __restrict__ ibbIdx,
+                                         const R* __restrict__ ibbQ, const LBParams<R>& p) {
+#pragma unroll
+  for (int i = 0; i < 9; i++) g[i] = loadDev<S, R>(src, gi + size_t(p.D) * i, i);
+  if (p.applyIbb && (m & kIbbNode)) ibbCorrect<S, R>(g, src, x, y, ibbQ + 8 * size_t(ibbIdx[gi]), p);
+}
+
+struct KOFields {
+  // pointers are typed R* at the call site; void* keeps this POD for launch
+  void* K;
+  void* O;
+  const void* wallD;
+  void* nut;
+  void* S; // 3 planes: Sxx, Sxy, Syy
+};
+
+// One step of LB_collision_SRT_Fluid (lbKernels.cl:175-335) or, with KO,
+// LB_collision_SRT_Fluid_w_kOmega (:345-633): read + lazy IBB, moments,
+// optional macros, EDM-forced BGK, halfway bounce-back, push streaming.
+template <class S, class R, bool KO>
+__global__ __launch_bounds__(kBlockX) void lbCollideStream(
+    const S* __restrict__ src, S* __restrict__ dst, const uint16_t* __restrict__ mask,
+    const int32_t* __restrict__ ibbIdx, const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
+    R* __restrict__ uxOut, R* __restrict__ uyOut, KOFields ko, LBParams<R> p) {
+  const int x = blockIdx.x * kBlockX + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= p.nX) return;
+  const size_t gi = size_t(x) + size_t(p.XF) * y;
+  const uint32_t m = mask[gi];
+  if (m & kSolid) return;
+
+  R g[9];
+  loadNode<S, R>(g, src, gi, x, y, m, ibbIdx, ibbQ, p);
+
+  const R drho = ((g[0] + g[1]) + (g[2] + g[3])) + ((g[4] + g[5]) + (g[6] + g[7])) + g[8];
+  const R rho = R(1) + drho;
+  const R inv = R(1) / rho;
+  const R ux = inv * (((g[1] - g[2]) + (g[5] - g[6])) + (g[7] - g[8]));
+  const R uy = inv * (((g[3] - g[4]) + (g[5] - g[6])) - (g[7] - g[8]));
+
+  R omega = p.omega0;
+  if constexpr (KO) {
+    R* K = static_cast<R*>(ko.K);
+    R* O = static_cast<R*>(ko.O);
+    const R* Wd = static_cast<const R*>(ko.wallD);
+    R* Nut = static_cast<R*>(ko.nut);
+    R* Sx = static_cast<R*>(ko.S);
+    R om = O[gi], k = K[gi];
+    const R yw = Wd[gi];
+    if (om < R(0)) {  // clamps written back (lbKernels.cl:401-410)
+      om = -om;
+      O[gi] = om;
+    }
+    if (k < R(0)) {
+      k = R(1e-16);
+      K[gi] = k;
+    }
+    const R a = R(2) * sqrt(k) / (R(0.09) * om * yw);
+    const R b = R(500) * p.nu / (yw * yw * om);
+    const R arg2 = a > b ? a : b;
+    R F2 = tanh(arg2 * arg2);
+    const R sxx = Sx[gi], sxy = Sx[gi + p.D], syy = Sx[gi + 2 * p.D];
+    F2 *= sqrt(R(2) * (sxx * sxx + R(2) * sxy * sxy + syy * syy));
+    const R den = R(0.31) * om > F2 ? R(0.31) * om : F2;
+    R nt = R(0.31) * k / den;
+    nt = nt < p.maxNut ? nt : p.maxNut;
+    nt = nt > R(1e-16) ? nt : R(1e-16);
+    Nut[gi] = nt;
+    omega = R(1) / (R(3) * (nt + p.nu) + R(0.5));
+    // strain rate from the pre-collision non-equilibrium with the laminar
+    // relaxation frequency (lbKernels.cl:518-583)
+    R ne[9];
+#pragma unroll
+    for (int i = 1; i < 9; i++) ne[i] = g[i] - geq<R>(i, drho, rho, ux, uy);
+    const R c = R(-1.5) * p.omega0 * inv;
+    const R exx = ((ne[1] + ne[2]) + (ne[5] + ne[6])) + (ne[7] + ne[8]);
+    const R eyy = ((ne[3] + ne[4]) + (ne[5] + ne[6])) + (ne[7] + ne[8]);
+    const R exy = (ne[5] + ne[6]) - (ne[7] + ne[8]);
+    Sx[gi] = c * exx;
+    Sx[gi + p.D] = c * exy;
+    Sx[gi + 2 * p.D] = c * eyy;
+  }
+
+  if (p.saveMacros) {
+    rhoOut[gi] = rho;
+    uxOut[gi] = ux + p.Fval * R(0.5);
+    uyOut[gi] = uy;
+  }
+
+  // exact-difference forcing: g += -w (g - geq(u)) + feq(u+F) - feq(u)
+  const int xe = x < p.nX - 1 ? x + 1 : 0;
+  const int xw = x > 0 ? x - 1 : p.nX - 1;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const int a = cx(i), b = cy(i);
+    const R gq = geq<R>(i, drho, rho, ux, uy);
+    const R force = rho * (w1<R>(b) + dphi(b, uy)) * dphiShift(a, ux, p.Fval);
+    const R gn = g[i] - omega * (g[i] - gq) + force;
+    if (i == 0) {
+      storeDev<S, R>(dst, gi, 0, gn);
+    } else if (m & boundBit(i)) {
+      storeDev<S, R>(dst, gi + size_t(p.D) * rev(i), rev(i), gn);
+    } else {
+      const int tx = a > 0 ? xe : (a < 0 ? xw : x);
+      storeDev<S, R>(dst, size_t(tx) + size_t(p.XF) * (y + b) + size_t(p.D) * i, i, gn);
+    }
+  }
+}
+
+// Reference-layout populations f = w + g (post interpolated bounce-back)
+// as FP64; solid nodes and the padding columns read 0 like the reference's
+// never-written entries.
+template <class S, class R>
+__global__ __launch_bounds__(kBlockX) void lbExportF(const S* __restrict__ src,
+                                                     const uint16_t* __restrict__ mask,
+                                                     const int32_t* __restrict__ ibbIdx,
+                                                     const R* __restrict__ ibbQ,
+                                                     double* __restrict__ out, LBParams<R> p) {
+  const int x = blockIdx.x * kBlockX + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= p.XF) return;
+  const size_t gi = size_t(x) + size_t(p.XF) * y;
+  const uint32_t m = x < p.nX ? mask[gi] : kSolid;
+  if (m & kSolid) {
+#pragma unroll
+    for (int i = 0; i < 9; i++) out[gi + size_t(p.D) * i] = 0.;
+    return;
+  }
+  R g[9];
+  loadNode<S, R>(g, src, gi, x, y, m, ibbIdx, ibbQ, p);
+#pragma unroll
+  for (int i = 0; i < 9; i++) out[gi + size_t(p.D) * i] = wgt(i) + double(g[i]);
+}
+
+// FP64 reference-layout populations -> stored deviations.
+template <class S, class R>
+__global__ __launch_bounds__(kBlockX) void lbImportF(const double* __restrict__ in,
+                                                     S* __restrict__ dst, LBParams<R> p) {
+  const int x = blockIdx.x * kBlockX + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= p.XF) return;
+  const size_t gi = size_t(x) + size_t(p.XF) * y;
+#pragma unroll
+  for (int i = 0; i < 9; i++) storeDev<S, R>(dst, gi + size_t(p.D) * i, i, R(in[gi + size_t(p.D) * i] - wgt(i)));
+}
+
+// Macros from populations (Calc_Ro_J lbKernels.cl:137-164, used on upload).
+template <class S, class R>
+__global__ __launch_bounds__(kBlockX) void lbMacros(const S* __restrict__ src,
+                                                    const uint16_t* __restrict__ mask,
+                                                    R* __restrict__ rho, R* __restrict__ ux,
+                                                    R* __restrict__ uy, LBParams<R> p) {
+  const int x = blockIdx.x * kBlockX + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= p.nX) return;
+  const size_t gi = size_t(x) + size_t(p.XF) * y;
+  if (mask[gi] & kSolid) return;
+  R g[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) g[i] = loadDev<S, R>(src, gi + size_t(p.D) * i, i);
+  const R drho = ((g[0] + g[1]) + (g[2] + g[3])) + ((g[4] + g[5]) + (g[6] + g[7])) + g[8];
+  const R r = R(1) + drho;
+  rho[gi] = r;
+  ux[gi] = (((g[1] - g[2]) + (g[5] - g[6])) + (g[7] - g[8])) / r;
+  uy[gi] = (((g[3] - g[4]) + (g[5] - g[6])) - (g[7] - g[8])) / r;
+}
+
+// ---------------------------------------------------------------------------
+// k-omega
+// ---------------------------------------------------------------------------
+
+// XCD-aware tile order for the stencil kernels.  Workgroups are dealt
+// round-robin over the 8 XCDs by linear id, so linear id L runs on XCD L % 8.
+// Tile T (row-major over tilesX x nY tiles of kBlockX nodes) is given to
+// XCD T / per, which therefore owns a contiguous band of rows and finds the
+// rows above and below a tile in its own L2.  The grid has 8 * per blocks;
+// ids past the last tile do no work.
+struct TileMap {
+  int tilesX, nTiles, per;
+};
+
+__device__ __forceinline__ bool tileOf(const TileMap& t, int& bx, int& by) {
+  const int L = blockIdx.x;
+  const int T = (L & 7) * t.per + (L >> 3);
+  if (T >= t.nTiles) return false;
+  bx = T % t.tilesX;
+  by = T / t.tilesX;
+  return true;
+}
+
+template <class R>
+__device__ __forceinline__ void fdCoeffs(R dp, R dm, R (&c)[6]) {
+  // kOmegaKernels.cl:37-55, 235-257
+  dp = dp > R(0.1) ? dp : R(0.1);
+  dm = dm > R(0.1) ? dm : R(0.1);
+  R dc = dp + dm;
+  dc = dc > R(0.1) ? dc : R(0.1);
+  c[0] = dm / (dp * dc);
+  c[1] = -dp / (dm * dc);
+  c[2] = (dp - dm) / (dp * dm);
+  if (fabs(c[2]) > R(10)) {
+    c[0] = R(1) / dc;
+    c[1] = R(-1) / dc;
+    c[2] = R(0);
+  }
+  c[3] = R(2) / (dp * dc);
+  c[4] = R(2) / (dm * dc);
+  c[5] = R(-2) / (dp * dm);
+}
+
+template <class R>
+struct KOParams {
+  int nX, nY, XF;
+  int64_t D;
+  R nu, dt;
+  TileMap tiles;
+};
+
+// Update_kOmega_Diffusivities (kOmegaKernels.cl:7-95).
+template <class R>
+__global__ __launch_bounds__(kBlockX) void koDiffusivities(
+    const uint16_t* __restrict__ mask, const R* __restrict__ K, const R* __restrict__ O,
+    const R* __restrict__ Nut, const R* __restrict__ Wd, R* __restrict__ Dk, R* __restrict__ Do,
+    R* __restrict__ F1a, R* __restrict__ dkdo, const R* __restrict__ rho,
+    const R* __restrict__ dX, KOParams<R> p) {
+  int bx, y;
+  if (!tileOf(p.tiles, bx, y)) return;
+  const int x = bx * kBlockX + threadIdx.x;
+  if (x >= p.nX) return;
+  const size_t g = size_t(x) + size_t(p.XF) * y;
+  if (mask[g] & kSolid) return;
+  R ax[6], ay[6];
+  fdCoeffs<R>(dX[g], dX[g + p.D], ax);
+  fdCoeffs<R>(dX[g + 2 * p.D], dX[g + 3 * p.D], ay);
+  const size_t gw = x > 0 ? g - 1 : g + (p.nX - 1);
+  const size_t ge = x < p.nX - 1 ? g + 1 : g - x;
+  const size_t gn = g + p.XF, gs = g - p.XF;
+  const R om = O[g], k = K[g], yw = Wd[g];
+  const R dOdx = ax[0] * O[ge] + ax[1] * O[gw] + ax[2] * om;
+  const R dKdx = ax[0] * K[ge] + ax[1] * K[gw] + ax[2] * k;
+  const R dOdy = ay[0] * O[gn] + ay[1] * O[gs] + ay[2] * om;
+  const R dKdy = ay[0] * K[gn] + ay[1] * K[gs] + ay[2] * k;
+  const R mlt = R(2 * 0.856) / om;
+  const R mx = mlt * dKdx, my = mlt * dKdy;
+  dkdo[g] = mx;
+  dkdo[g + p.D] = my;
+  R cd = rho[g] * (mx * dOdx + my * dOdy);
+  cd = cd > R(1.0e-10) ? cd : R(1.0e-10);
+  const R a = sqrt(k) / (R(0.09) * om * yw), b = R(500) * p.nu / (yw * yw * om);
+  R arg = a > b ? a : b;
+  const R c = R(4 * 0.856) * k / (cd * yw * yw);
+  arg = arg < c ? arg : c;
+  const R a2 = arg * arg;
+  const R F1 = tanh(a2 * a2);
+  F1a[g] = F1;
+  const R nt = Nut[g];
+  Dk[g] = p.nu + (R(0.85) * F1 + R(1) * (R(1) - F1)) * nt;
+  Do[g] = p.nu + (R(0.5) * F1 + R(0.856) * (R(1) - F1)) * nt;
+}
+
+// Update_kOmega_Coeffs_Implicit (kOmegaKernels.cl:187-480): normalised
+// five-point rows; A holds E, W, N, S planes (diagonal 1).
+template <class R>
+__global__ __launch_bounds__(kBlockX) void koCoeffs(
+    const uint16_t* __restrict__ mask, const R* __restrict__ K, const R* __restrict__ O,
+    const R* __restrict__ Nut, const R* __restrict__ Dk, const R* __restrict__ Do,
+    const R* __restrict__ F1a, const R* __restrict__ dkdo, const R* __restrict__ Sx,
+    const R* __restrict__ rho, const R* __restrict__ uxa, const R* __restrict__ uya,
+    const R* __restrict__ dX, const R* __restrict__ Wd, R* __restrict__ Ak, R* __restrict__ Ao,
+    R* __restrict__ bk, R* __restrict__ bo, KOParams<R> p) {
+  int bx, y;
+  if (!tileOf(p.tiles, bx, y)) return;
+  const int x = bx * kBlockX + threadIdx.x;
+  if (x >= p.nX) return;
+  const size_t g = size_t(x) + size_t(p.XF) * y;
+  const uint32_t m = mask[g];
+  if (m & kSolid) return;
+  R ax[6], ay[6];
+  fdCoeffs<R>(dX[g], dX[g + p.D], ax);
+  fdCoeffs<R>(dX[g + 2 * p.D], dX[g + 3 * p.D], ay);
+  const size_t gw = x > 0 ? g - 1 : g + (p.nX - 1);
+  const size_t ge = x < p.nX - 1 ? g + 1 : g - x;
+  const size_t gn = g + p.XF, gs = g - p.XF;
+  const R dk = Dk[g], dw = Do[g];
+  const R dkdx = ax[0] * Dk[ge] + ax[1] * Dk[gw] + ax[2] * dk;
+  const R dwdx = ax[0] * Do[ge] + ax[1] * Do[gw] + ax[2] * dw;
+  const R dkdy = ay[0] * Dk[gn] + ay[1] * Dk[gs] + ay[2] * dk;
+  const R dwdy = ay[0] * Do[gn] + ay[1] * Do[gs] + ay[2] * dw;
+  const R Ux = uxa[g], Uy = uya[g], Rh = rho[g];
+  const R Jxw = Ux - dwdx - dkdo[g], Jyw = Uy - dwdy - dkdo[g + p.D];
+  const R Jxk = Ux - dkdx, Jyk = Uy - dkdy;
+  R om = O[g] > R(0) ? O[g] : R(0);
+  const R k = K[g] > R(0) ? K[g] : R(0);
+  const R F1 = F1a[g], nt = Nut[g];
+  const R sxx = Sx[g], sxy = Sx[g + p.D], syy = Sx[g + 2 * p.D];
+  R Pk = R(2) * nt * (sxx * sxx + syy * syy + R(2) * sxy * sxy) -
+         R(2. / 3.) * k * Rh * (sxx + syy);
+  const R lim = R(10 * 0.09) * k * om;
+  Pk = Pk < lim ? Pk : lim;
+  Pk = Pk > R(0) ? Pk : R(0);
+  const R gam = (F1 * R(5. / 9.) + (R(1) - F1) * R(0.44)) / nt;
+  R Sc = R(-0.09) * om;
+  R dt = p.dt;
+  const R Kc = R(1) + dt * (Jxk * ax[2] + Jyk * ay[2] - dk * (ax[5] + ay[5])) - Sc;
+  R iv = R(1) / Kc;
+  Ak[g] = dt * (Jxk * ax[0] - dk * ax[3]) * iv;
+  Ak[g + p.D] = dt * (Jxk * ax[1] - dk * ax[4]) * iv;
+  Ak[g + 2 * p.D] = dt * (Jyk * ay[0] - dk * ay[3]) * iv;
+  Ak[g + 3 * p.D] = dt * (Jyk * ay[1] - dk * ay[4]) * iv;
+  bk[g] = (k + dt * Pk) * iv;
+  if (m & kNesw) {  // omega wall row (kOmegaKernels.cl:389-403)
+    dt = R(0);
+    Sc = R(0);
+    const R yp = Wd[g];
+    const R ov = (R(6) * p.nu / R(0.075)) / (yp * yp);
+    const R ol = sqrt(k) / (R(0.5477225575 * 0.41) * yp);
+    const R ow = sqrt(ov * ov + ol * ol);
+    om = ow < R(1000) ? ow : R(1000);
+  }
+  const R Wc = R(1) + dt * (Jxw * ax[2] + Jyw * ay[2] - dw * (ax[5] + ay[5])) - Sc;
+  iv = R(1) / Wc;
+  Ao[g] = dt * (Jxw * ax[0] - dw * ax[3]) * iv;
+  Ao[g + p.D] = dt * (Jxw * ax[1] - dw * ax[4]) * iv;
+  Ao[g + 2 * p.D] = dt * (Jyw * ay[0] - dw * ay[3]) * iv;
+  Ao[g + 3 * p.D] = dt * (Jyw * ay[1] - dw * ay[4]) * iv;
+  bo[g] = (om + dt * (gam * Pk)) * iv;
+}
+
+// ---------------------------------------------------------------------------
+// implicit k / omega solve: BiCGStab for both systems at once
+// ---------------------------------------------------------------------------
+//
+// The assembled rows (diagonal 1, four neighbours) are not diagonally
+// dominant next to curved walls, so a stationary iteration can diverge; the
+// reference uses restarted GMRES (GMRESSolver.cpp:227-355).  Here both systems
+// run unpreconditioned BiCGStab (van der Vorst 1992) in lock step: three
+// streaming kernels per iteration with the neighbour values of p and s
+// recomputed in registers (so p, v are double-buffered and s needs no
+// separate pass), and a one-block kernel after each that turns the per-block
+// FP64 partial dot products into the scalars.  All scalars stay in device
+// memory; the host only polls the done flags between chunks of iterations.
+
+struct BicgState {
+  double rho[2], rhoNew[2], alpha[2], omega[2], beta[2];
+  double r0[2], rn[2];
+  double rtol[2], atol[2];
+  int maxit[2];
+  int done[2];
+  int iters[2];
+  int restart[2];
+  int nan;
+  int cur;  // which p/v buffer holds the current iterate
+};
+
+// Solver vectors are FP64 (Q = double) for every storage form: the omega
+// field spans about seven decades (wall rows near 1000, core values near
+// 1e-4) and FP32 Krylov vectors cannot resolve the core next to the wall.
+template <class R, class Q>
+struct BicgVecs {
+  R* x[2];        // K, O in HBM
+  Q* xq[2];       // iterate
+  const R* A[2];  // 4 planes each: E, W, N, S
+  const R* b[2];
+  Q* r[2];
+  Q* rh[2];
+  Q* p[2][2];  // [variable][buffer]
+  Q* v[2][2];
+  Q* s[2];
+  Q* t[2];
+};
+
+template <class R>
+__device__ __forceinline__ void nbIndex(const KOParams<R>& p, int x, size_t g, size_t (&nb)[4]) {
+  nb[0] = x < p.nX - 1 ? g + 1 : g - x;        // E
+  nb[1] = x > 0 ? g - 1 : g + (p.nX - 1);      // W
+  nb[2] = g + p.XF;                            // N
+  nb[3] = g - p.XF;                            // S
+}
+
+// per-block sums of up to 4 doubles per variable -> partial[blk*8 + v*4 + k]
+__device__ __forceinline__ void blockPartials(double (&a)[2][2], double* __restrict__ partial) {
+  __shared__ double red[2][2][kBlockX / 64];
+#pragma unroll
+  for (int v = 0; v < 2; v++)
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      double x = a[v][k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+      if ((threadIdx.x & 63) == 0) red[v][k][threadIdx.x >> 6] = x;
+    }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int v = threadIdx.x >> 1, k = threadIdx.x & 1;
+    double sum = 0.;
+#pragma unroll
+    for (int w = 0; w < kBlockX / 64; w++) sum += red[v][k][w];
+    partial[size_t(blockIdx.x) * 4 + threadIdx.x] = sum;
+    (void)v;
+    (void)k;
+  }
+}
+
+// r = b - A x, rh = r, p = v = 0; partials (r, r).
+template <class R, class Q>
+__global__ __launch_bounds__(kBlockX) void bicgInit(const uint16_t* __restrict__ mask,
+                                                    BicgVecs<R, Q> V, double* __restrict__ partial,
+                                                    KOParams<R> p) {
+  int bx = 0, y = 0;
+  const bool live = tileOf(p.tiles, bx, y);
+  const int x = bx * kBlockX + threadIdx.x;
+  double a[2][2] = {{0., 0.}, {0., 0.}};
+  if (live && x < p.nX) {
+    const size_t g = size_t(x) + size_t(p.XF) * y;
+    if (!(mask[g] & kSolid)) {
+      size_t nb[4];
+      nbIndex(p, x, g, nb);
+#pragma unroll
+      for (int v = 0; v < 2; v++) {
+        const R* A = V.A[v];
+        const R* X = V.x[v];
+        const Q ax = Q(X[g]) + (Q(A[g]) * Q(X[nb[0]]) + Q(A[g + p.D]) * Q(X[nb[1]]) +
+                                Q(A[g + 2 * p.D]) * Q(X[nb[2]]) + Q(A[g + 3 * p.D]) * Q(X[nb[3]]));
+        const Q r = Q(V.b[v][g]) - ax;
+        V.xq[v][g] = Q(X[g]);
+        V.r[v][g] = r;
+        V.rh[v][g] = r;
+        V.p[v][0][g] = Q(0);
+        V.v[v][0][g] = Q(0);
+        a[v][0] = double(r) * double(r);
+      }
+    }
+  }
+  blockPartials(a, partial);
+}
+
+__device__ __forceinline__ void sumPartials(const double* __restrict__ partial, int nblk,
+                                            double (&out)[4]) {
+  __shared__ double red[4][16];
+  double a[4] = {0., 0., 0., 0.};
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < 4; k++) a[k] += partial[size_t(i) * 4 + k];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a[k] += __shfl_xor(a[k], off, 64);
+    if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = a[k];
+  }
+  __syncthreads();
+  for (int k = 0; k < 4; k++) {
+    double s = 0.;
+    for (int w = 0; w < int(blockDim.x + 63) / 64; w++) s += red[k][w];
+    out[k] = s;
+  }
+}
+
+// Scalar updates between the streaming kernels (one block of 1024 threads).
+// stage 0: after bicgInit, 1: after bicgPV (alpha), 2: after bicgST (omega),
+// 3: after bicgXR (rho, beta, convergence).
+__global__ __launch_bounds__(1024) void bicgScalars(const double* __restrict__ partial, int nblk,
+                                                    BicgState* __restrict__ st, int stage) {
+  double s[4];
+  sumPartials(partial, nblk, s);
+  if (threadIdx.x != 0) return;
+  for (int v = 0; v < 2; v++) {
+    if (st->done[v]) continue;
+    const double a = s[2 * v], b = s[2 * v + 1];
+    if (stage == 0) {
+      const double r0 = sqrt(a);
+      st->r0[v] = r0;
+      st->rn[v] = r0;
+      st->iters[v] = 0;
+      st->rho[v] = st->alpha[v] = st->omega[v] = 1.;
+      st->rhoNew[v] = a;  // (rh, r) with rh = r
+      st->beta[v] = a;
+      st->restart[v] = 0;
+      if (!isfinite(r0)) {
+        st->nan = 1;
+        st->done[v] = 1;
+      } else if (r0 == 0. || r0 <= st->atol[v]) {
+        st->done[v] = 1;
+      }
+    } else if (stage == 1) {
+      // a = (rh, v); a restart requested by the last stage 3 has now been
+      // consumed by bicgPV
+      st->restart[v] = 0;
+      if (a == 0. || !isfinite(a)) {
+        st->alpha[v] = 0.;
+        st->restart[v] = 1;
+      } else {
+        st->alpha[v] = st->rhoNew[v] / a;
+      }
+    } else if (stage == 2) {
+      // a = (t, s), b = (t, t)
+      st->omega[v] = (b > 0. && isfinite(b)) ? a / b : 0.;
+    } else {
+      // a = (rh, r), b = (r, r)
+      const double rn = sqrt(b);
+      st->iters[v] += 1;
+      st->rn[v] = rn;
+      if (!isfinite(rn)) {
+        st->nan = 1;
+        st->done[v] = 1;
+        continue;
+      }
+      if (rn <= st->rtol[v] * st->r0[v] || rn <= st->atol[v] || st->iters[v] >= st->maxit[v]) {
+        st->done[v] = 1;
+        continue;
+      }
+      st->rho[v] = st->rhoNew[v];
+      st->rhoNew[v] = a;
+      if (a == 0. || st->omega[v] == 0. || st->restart[v]) {
+        // breakdown or stagnation: restart with rh = r, p = v = 0
+        st->restart[v] = 1;
+        st->rho[v] = st->alpha[v] = st->omega[v] = 1.;
+        st->rhoNew[v] = b;
+        st->beta[v] = b;
+      } else {
+        st->beta[v] = (a / st->rho[v]) * (st->alpha[v] / st->omega[v]);
+      }
+    }
+  }
+  if (stage == 3) st->cur ^= 1;
+}
+
+// p' = r + beta (p - omega v) (neighbour values recomputed), v' = A p';
+// partials (rh, v').  On a restart p' = r and rh is reset to r.
+template <class R, class Q>
+__global__ __launch_bounds__(kBlockX) void bicgPV(const uint16_t* __restrict__ mask,
+                                                  BicgVecs<R, Q> V, const BicgState* __restrict__ st,
+                                                  double* __restrict__ partial, KOParams<R> p) {
+  const int cur = st->cur;
+  int bx = 0, y = 0;
+  const bool live = tileOf(p.tiles, bx, y);
+  const int x = bx * kBlockX + threadIdx.x;
+  double a[2][2] = {{0., 0.}, {0., 0.}};
+  if (live && x < p.nX) {
+    const size_t g = size_t(x) + size_t(p.XF) * y;
+    if (!(mask[g] & kSolid)) {
+      size_t nb[4];
+      nbIndex(p, x, g, nb);
+#pragma unroll
+      for (int v = 0; v < 2; v++) {
+        if (st->done[v]) continue;
+        const Q* r = V.r[v];
+        const Q* po = V.p[v][cur];
+        const Q* vo = V.v[v][cur];
+        const R* A = V.A[v];
+        const bool rs = st->restart[v] != 0;
+        const Q beta = Q(st->beta[v]), om = Q(st->omega[v]);
+        auto pn = [&](size_t i) { return rs ? r[i] : r[i] + beta * (po[i] - om * vo[i]); };
+        const Q pc = pn(g);
+        const Q vc = pc + (Q(A[g]) * pn(nb[0]) + Q(A[g + p.D]) * pn(nb[1]) +
+                           Q(A[g + 2 * p.D]) * pn(nb[2]) + Q(A[g + 3 * p.D]) * pn(nb[3]));
+        V.p[v][cur ^ 1][g] = pc;
+        V.v[v][cur ^ 1][g] = vc;
+        Q rh = V.rh[v][g];
+        if (rs) {
+          rh = r[g];
+          V.rh[v][g] = rh;
+        }
+        a[v][0] = double(rh) * double(vc);
+      }
+    }
+  }
+  blockPartials(a, partial);
+}
+
+// s = r - alpha v' (neighbours recomputed), t = A s; partials (t, s), (t, t).
+template <class R, class Q>
+__global__ __launch_bounds__(kBlockX) void bicgST(const uint16_t* __restrict__ mask,
+                                                  BicgVecs<R, Q> V, const BicgState* __restrict__ st,
+                                                  double* __restrict__ partial, KOParams<R> p) {
+  const int nxt = st->cur ^ 1;
+  int bx = 0, y = 0;
+  const bool live = tileOf(p.tiles, bx, y);
+  const int x = bx * kBlockX + threadIdx.x;
+  double a[2][2] = {{0., 0.}, {0., 0.}};
+  if (live && x < p.nX) {
+    const size_t g = size_t(x) + size_t(p.XF) * y;
+    if (!(mask[g] & kSolid)) {
+      size_t nb[4];
+      nbIndex(p, x, g, nb);
+#pragma unroll
+      for (int v = 0; v < 2; v++) {
+        if (st->done[v]) continue;
+        const Q* r = V.r[v];
+        const Q* vv = V.v[v][nxt];
+        const R* A = V.A[v];
+        const Q al = Q(st->alpha[v]);
+        auto sn = [&](size_t i) { return r[i] - al * vv[i]; };
+        const Q sc = sn(g);
+        const Q tc = sc + (Q(A[g]) * sn(nb[0]) + Q(A[g + p.D]) * sn(nb[1]) +
+                           Q(A[g + 2 * p.D]) * sn(nb[2]) + Q(A[g + 3 * p.D]) * sn(nb[3]));
+        V.s[v][g] = sc;
+        V.t[v][g] = tc;
+        a[v][0] = double(tc) * double(sc);
+        a[v][1] = double(tc) * double(tc);
+      }
+    }
+  }
+  blockPartials(a, partial);
+}
+
+// x += alpha p' + omega s, r = s - omega t; partials (rh, r), (r, r).
+template <class R, class Q>
+__global__ __launch_bounds__(kBlockX) void bicgXR(const uint16_t* __restrict__ mask,
+                                                  BicgVecs<R, Q> V, const BicgState* __restrict__ st,
+                                                  double* __restrict__ partial, KOParams<R> p) {
+  const int nxt = st->cur ^ 1;
+  int bx = 0, y = 0;
+  const bool live = tileOf(p.tiles, bx, y);
+  const int x = bx * kBlockX + threadIdx.x;
+  double a[2][2] = {{0., 0.}, {0., 0.}};
+  if (live && x < p.nX) {
+    const size_t g = size_t(x) + size_t(p.XF) * y;
+    if (!(mask[g] & kSolid)) {
+#pragma unroll
+      for (int v = 0; v < 2; v++) {
+        if (st->done[v]) continue;
+        const Q al = Q(st->alpha[v]), om = Q(st->omega[v]);
+        const Q sc = V.s[v][g];
+        V.xq[v][g] += al * V.p[v][nxt][g] + om * sc;
+        const Q r = sc - om * V.t[v][g];
+        V.r[v][g] = r;
+        a[v][0] = double(V.rh[v][g]) * double(r);
+        a[v][1] = double(r) * double(r);
+      }
+    }
+  }
+  blockPartials(a, partial);
+}
+
+// Writes the iterate back to K / O.
+template <class R, class Q>
+__global__ __launch_bounds__(kBlockX) void bicgFinish(const uint16_t* __restrict__ mask,
+                                                      BicgVecs<R, Q> V, KOParams<R> p) {
+  int bx = 0, y = 0;
+  if (!tileOf(p.tiles, bx, y)) return;
+  const int x = bx * kBlockX + threadIdx.x;
+  if (x >= p.nX) return;
+  const size_t g = size_t(x) + size_t(p.XF) * y;
+  if (mask[g] & kSolid) return;
+  V.x[0][g] = R(V.xq[0][g]);
+  V.x[1][g] = R(V.xq[1][g]);
+}
+
+// Resets the solver state for a new solve (a kernel, not a memset, so the
+// step can be captured into a graph).
+__global__ void bicgReset(BicgState* __restrict__ st, double relK, double absK, int maxK,
+                          double relO, double absO, int maxO) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  BicgState z = {};
+  z.rtol[0] = relK;
+  z.atol[0] = absK;
+  z.maxit[0] = maxK;
+  z.rtol[1] = relO;
+  z.atol[1] = absO;
+  z.maxit[1] = maxO;
+  *st = z;
+}
+
+// ---------------------------------------------------------------------------
+// wall shear
+// ---------------------------------------------------------------------------
+
+// TR_shear_1 (trKernelsShear.cl:174-222): traction at wall nodes from the
+// newest populations (with the pending interpolated bounce-back) and the
+// stored macros.
+template <class S, class R>
+__global__ __launch_bounds__(64) void shearNodes(
+    const S* __restrict__ src, const uint16_t* __restrict__ mask,
+    const int32_t* __restrict__ ibbIdx, const R* __restrict__ ibbQ,
+    const int32_t* __restrict__ ss, int nSS, const R* __restrict__ cfss,
+    const R* __restrict__ rho, const R* __restrict__ uxa, const R* __restrict__ uya,
+    const R* __restrict__ nut, R* __restrict__ tau, LBParams<R> p) {
+  const int n = blockIdx.x * 64 + threadIdx.x;
+  if (n >= nSS) return;
+  const int gi = ss[n];
+  const int x = gi % p.XF, y = gi / p.XF;
+  R g[9];
+  loadNode<S, R>(g, src, size_t(gi), x, y, mask[gi], ibbIdx, ibbQ, p);
+  const R r = rho[gi], ux = uxa[gi], uy = uya[gi];
+  R ve = (nut ? nut[gi] : R(0)) + p.nu;
+  ve *= R(1) / (R(3) * ve + R(0.5));
+  const R dr = r - R(1);
+  R tx = R(0), ty = R(0);
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    const R fn = g[a + 1] - geq<R>(a + 1, dr, r, ux, uy);
+    tx += fn * (cfss[16 * size_t(n) + a] * ve);
+    ty += fn * (cfss[16 * size_t(n) + 8 + a] * ve);
+  }
+  const R t = sqrt(tx * tx + ty * ty) / r;
+  tau[n] = tx < R(0) ? -t : t;
+}
+
+// TR_shear_2 (trKernelsShear.cl:300-316).
+template <class R>
+__global__ __launch_bounds__(64) void shearLinks(const R* __restrict__ w,
+                                                 const int32_t* __restrict__ sInd,
+                                                 const R* __restrict__ tau, int nb,
+                                                 R* __restrict__ blTau) {
+  const int n = blockIdx.x * 64 + threadIdx.x;
+  if (n >= nb) return;
+  R s = R(0);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int i = sInd[4 * n + k];
+    s += w[4 * n + k] * (i >= 0 ? tau[i] : R(0));
+  }
+  blTau[n] = s;
+}
+
+// ---------------------------------------------------------------------------
+// reductions (FP64 accumulation, deterministic two-pass)
+// ---------------------------------------------------------------------------
+
+template <class R>
+__global__ __launch_bounds__(256) void reducePartial(const R* __restrict__ a, int64_t n, int op,
+                                                     double* __restrict__ partial) {
+  __shared__ double red[4];
+  double acc = op == 1 ? 0. : (op == 2 ? -INFINITY : INFINITY);
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+    const double v = double(a[i]);
+    acc = op == 1 ? acc + v : (op == 2 ? fmax(acc, v) : fmin(acc, v));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(acc, off, 64);
+    acc = op == 1 ? acc + o : (op == 2 ? fmax(acc, o) : fmin(acc, o));
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int w = 1; w < 4; w++) r = op == 1 ? r + red[w] : (op == 2 ? fmax(r, red[w]) : fmin(r, red[w]));
+    partial[blockIdx.x] = r;
+  }
+}
+
+__global__ __launch_bounds__(256) void reduceFinal(const double* __restrict__ partial, int n,
+                                                   int op, int padded, double* __restrict__ out) {
+  __shared__ double red[4];
+  double acc = op == 1 ? 0. : (op == 2 ? -INFINITY : INFINITY);
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double v = partial[i];
+    acc = op == 1 ? acc + v : (op == 2 ? fmax(acc, v) : fmin(acc, v));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(acc, off, 64);
+    acc = op == 1 ? acc + o : (op == 2 ? fmax(acc, o) : fmin(acc, o));
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int w = 1; w < 4; w++) r = op == 1 ? r + red[w] : (op == 2 ? fmax(r, red[w]) : fmin(r, red[w]));
+    if (padded) r = op == 2 ? fmax(r, 0.) : (op == 4 ? fmin(r, 0.) : r);
+    *out = r;
+  }
+}
+
+}  // namespace dev
+}  // namespace kofm

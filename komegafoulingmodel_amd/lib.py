@@ -1,0 +1,259 @@
+"""ctypes binding of libkofm.so (the C ABI declared in include/kofm.h).
+
+The HIP library is required: importing works without it, but every call that
+needs it raises ``KofmLibraryMissing``.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkofm.so")
+
+STORE_F32, STORE_F16, STORE_BF16, STORE_F64 = 0, 1, 2, 3
+STORAGE_NAMES = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16, "f64": STORE_F64}
+OPTION_SAVE_MACRO_FIELDS = 0x2
+
+FIELD_F, FIELD_RHO, FIELD_UX, FIELD_UY = 0, 1, 2, 3
+FIELD_KAPPA, FIELD_OMEGA, FIELD_NUT, FIELD_SXY, FIELD_WALLD = 4, 5, 6, 7, 8
+FIELD_TAU_NODE, FIELD_TAU_BL = 9, 10
+FIELD_NAMES = {
+    "f": FIELD_F, "rho": FIELD_RHO, "ux": FIELD_UX, "uy": FIELD_UY, "k": FIELD_KAPPA,
+    "omega": FIELD_OMEGA, "nut": FIELD_NUT, "S": FIELD_SXY, "wallD": FIELD_WALLD,
+    "tau_node": FIELD_TAU_NODE, "tau_bl": FIELD_TAU_BL,
+}
+RED_SUM, RED_MAX, RED_MIN = 1, 2, 4
+
+GEOM = {
+    "nType": (0, np.int16), "dX": (1, np.float64), "ibb": (2, np.int32), "ibbQ": (3, np.float64),
+    "ss": (4, np.int32), "ssMap": (5, np.int32), "ssInds": (6, np.int32),
+    "lsMap": (7, np.uint16), "wallD": (8, np.float64), "cfss": (9, np.float64),
+    "sInd": (10, np.int32), "weights": (11, np.float64), "bounds": (12, np.int32),
+    "masses": (13, np.float64),
+}
+
+# Every symbol the header declares; tests check the library exports them all.
+EXPORTS = [
+    "kofm_create", "kofm_create_host", "kofm_destroy", "kofm_last_error", "kofm_set_params", "kofm_build_geometry",
+    "kofm_geometry_counts", "kofm_geometry_array", "kofm_init_state", "kofm_init_komega",
+    "kofm_upload_state", "kofm_step", "kofm_time_steps", "kofm_wall_shear",
+    "kofm_get_solver_stats", "kofm_reduce", "kofm_download", "kofm_save2file",
+    "kofm_synchronize",
+]
+
+
+class KofmLibraryMissing(RuntimeError):
+    pass
+
+
+class KofmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"kofm error {code}: {msg}")
+        self.code = code
+
+
+class Grid(C.Structure):
+    _fields_ = [("nX", C.c_int32), ("nY", C.c_int32), ("storage", C.c_int32),
+                ("use_komega", C.c_int32)]
+
+
+class Params(C.Structure):
+    _fields_ = [("tau0", C.c_double), ("Fval", C.c_double), ("nu", C.c_double),
+                ("rho0", C.c_double), ("max_turb_visc", C.c_double), ("ko_dt", C.c_double),
+                ("kappa_rel_tol", C.c_double), ("kappa_abs_tol", C.c_double),
+                ("kappa_max_iters", C.c_int32), ("omega_rel_tol", C.c_double),
+                ("omega_abs_tol", C.c_double), ("omega_max_iters", C.c_int32)]
+
+
+class GeomParams(C.Structure):
+    _fields_ = [("pipe_radius", C.c_double), ("eps", C.c_double), ("min_dx", C.c_double),
+                ("wall_d_search_rad", C.c_int32), ("x_release_pos", C.c_double),
+                ("x_min_val", C.c_double), ("x_stop_pos", C.c_double),
+                ("cutoff_radius", C.c_double), ("index_radius_search", C.c_int32)]
+
+
+class GeomCounts(C.Structure):
+    _fields_ = [("nX", C.c_int32), ("nY", C.c_int32), ("XsizeFull", C.c_int32), ("D", C.c_int64),
+                ("nIbb", C.c_int32), ("nSS", C.c_int32), ("nBLshear", C.c_int32),
+                ("nIbbNodes", C.c_int32)]
+
+
+class SolverStats(C.Structure):
+    _fields_ = [("kappa_iters", C.c_int32), ("omega_iters", C.c_int32),
+                ("kappa_res0", C.c_double), ("kappa_res", C.c_double),
+                ("omega_res0", C.c_double), ("omega_res", C.c_double), ("nan_flag", C.c_int32)]
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Loads libkofm.so (once) and declares the C signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise KofmLibraryMissing(
+            f"{path} is not built; run komegafoulingmodel_amd/build.py (or __graft_entry__.build())")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    sig = {
+        "kofm_create": (C.c_int, [C.POINTER(Grid), C.POINTER(P)]),
+        "kofm_create_host": (C.c_int, [C.POINTER(Grid), C.POINTER(P)]),
+        "kofm_destroy": (None, [P]),
+        "kofm_last_error": (C.c_char_p, [P]),
+        "kofm_set_params": (C.c_int, [P, C.POINTER(Params)]),
+        "kofm_build_geometry": (C.c_int, [P, dp, C.c_int32, C.POINTER(C.c_int32), C.c_int32,
+                                          C.POINTER(GeomParams)]),
+        "kofm_geometry_counts": (C.c_int, [P, C.POINTER(GeomCounts)]),
+        "kofm_geometry_array": (C.c_int, [P, C.c_int32, P]),
+        "kofm_init_state": (C.c_int, [P, dp, dp, dp]),
+        "kofm_init_komega": (C.c_int, [P, C.c_double, C.c_double, C.c_double]),
+        "kofm_upload_state": (C.c_int, [P, dp, dp, dp, dp, dp]),
+        "kofm_step": (C.c_int, [P, C.c_int32, C.c_uint32]),
+        "kofm_time_steps": (C.c_int, [P, C.c_int32, C.c_uint32, dp, dp]),
+        "kofm_wall_shear": (C.c_int, [P]),
+        "kofm_get_solver_stats": (C.c_int, [P, C.POINTER(SolverStats)]),
+        "kofm_reduce": (C.c_int, [P, C.c_int32, C.c_int32, dp]),
+        "kofm_download": (C.c_int, [P, C.c_int32, dp]),
+        "kofm_save2file": (C.c_int, [P, C.c_char_p]),
+        "kofm_synchronize": (C.c_int, [P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Lattice:
+    """One lattice on the GPU behind the C ABI.
+
+    Host arrays are float64 in the reference layout (x fastest, XsizeFull
+    columns, planes of XsizeFull*nY).
+    """
+
+    def __init__(self, nX: int, nY: int, storage: str | int = "f32", komega: bool = False,
+                 host_only: bool = False):
+        self.lib = load()
+        st = STORAGE_NAMES[storage] if isinstance(storage, str) else int(storage)
+        g = Grid(nX, nY, st, 1 if komega else 0)
+        h = C.c_void_p()
+        create = self.lib.kofm_create_host if host_only else self.lib.kofm_create
+        rc = create(C.byref(g), C.byref(h))
+        if rc != 0:
+            raise KofmError(rc, self.lib.kofm_last_error(None).decode())
+        self.h = h
+        self.nX, self.nY, self.komega, self.storage = nX, nY, komega, st
+        self.XF = ((nX + 127) // 128) * 128
+        self.D = self.XF * nY
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise KofmError(rc, self.lib.kofm_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.kofm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, tau0, Fval, nu, rho0=1.0, max_turb_visc=10.0, ko_dt=1.0,
+                   kappa_rel_tol=1e-4, kappa_abs_tol=1e-7, kappa_max_iters=1000,
+                   omega_rel_tol=None, omega_abs_tol=None, omega_max_iters=None):
+        p = Params(tau0, Fval, nu, rho0, max_turb_visc, ko_dt, kappa_rel_tol, kappa_abs_tol,
+                   kappa_max_iters,
+                   kappa_rel_tol if omega_rel_tol is None else omega_rel_tol,
+                   kappa_abs_tol if omega_abs_tol is None else omega_abs_tol,
+                   kappa_max_iters if omega_max_iters is None else omega_max_iters)
+        self._check(self.lib.kofm_set_params(self.h, C.byref(p)))
+
+    def build_geometry(self, C0: np.ndarray, BL: np.ndarray, **gp):
+        C0 = np.ascontiguousarray(C0, dtype=np.float64)
+        BL = np.ascontiguousarray(BL, dtype=np.int32)
+        p = GeomParams(gp.get("pipe_radius", 0.5 * (self.nY - 4)), gp.get("eps", 1e-5),
+                       gp.get("min_dx", 0.001), gp.get("wall_d_search_rad", 50),
+                       gp["x_release_pos"], gp.get("x_min_val", gp["x_release_pos"] - 1.25),
+                       gp["x_stop_pos"], gp.get("cutoff_radius", 5.0),
+                       gp.get("index_radius_search", 30))
+        self._check(self.lib.kofm_build_geometry(
+            self.h, _dptr(C0), C0.shape[0], BL.ctypes.data_as(C.POINTER(C.c_int32)), BL.shape[0],
+            C.byref(p)))
+
+    def counts(self) -> GeomCounts:
+        c = GeomCounts()
+        self._check(self.lib.kofm_geometry_counts(self.h, C.byref(c)))
+        return c
+
+    def geometry(self, name: str) -> np.ndarray:
+        which, dt = GEOM[name]
+        c = self.counts()
+        n = {"nType": c.D, "dX": 4 * c.D, "ibb": c.nIbb, "ibbQ": c.nIbb, "ss": c.nSS,
+             "ssMap": c.D, "ssInds": c.nX, "lsMap": 2 * c.XsizeFull, "wallD": c.D,
+             "cfss": 16 * c.nSS, "sInd": 4 * c.nBLshear, "weights": 4 * c.nBLshear,
+             "bounds": 4, "masses": 2}[name]
+        out = np.zeros(max(n, 1), dtype=dt)
+        self._check(self.lib.kofm_geometry_array(self.h, which, out.ctypes.data_as(C.c_void_p)))
+        return out[:n]
+
+    def init_state(self, rho=None, ux=None, uy=None):
+        a = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (rho, ux, uy)]
+        self._check(self.lib.kofm_init_state(self.h, *[_dptr(v) for v in a]))
+
+    def init_komega(self, k_ini=-1.0, omega_ini=-1.0, Re=0.0):
+        self._check(self.lib.kofm_init_komega(self.h, k_ini, omega_ini, Re))
+
+    def upload_state(self, f, k=None, omega=None, nut=None, S=None):
+        arrs = [None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+                for v in (f, k, omega, nut, S)]
+        self._check(self.lib.kofm_upload_state(self.h, *[_dptr(v) for v in arrs]))
+
+    def step(self, n: int = 1, options: int = 0):
+        self._check(self.lib.kofm_step(self.h, n, options))
+
+    def time_steps(self, n: int, options: int = 0):
+        a, b = C.c_double(), C.c_double()
+        self._check(self.lib.kofm_time_steps(self.h, n, options, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def wall_shear(self):
+        self._check(self.lib.kofm_wall_shear(self.h))
+
+    def solver_stats(self) -> SolverStats:
+        s = SolverStats()
+        self._check(self.lib.kofm_get_solver_stats(self.h, C.byref(s)))
+        return s
+
+    def reduce(self, field: str, op: int = RED_SUM) -> float:
+        out = C.c_double()
+        self._check(self.lib.kofm_reduce(self.h, FIELD_NAMES[field], op, C.byref(out)))
+        return out.value
+
+    def download(self, field: str) -> np.ndarray:
+        c = self.counts()
+        n = {"f": 9 * self.D, "S": 3 * self.D, "tau_node": c.nSS, "tau_bl": c.nBLshear}.get(field, self.D)
+        out = np.zeros(max(n, 1), dtype=np.float64)
+        self._check(self.lib.kofm_download(self.h, FIELD_NAMES[field], _dptr(out)))
+        return out[:n]
+
+    def save2file(self, directory: str):
+        self._check(self.lib.kofm_save2file(self.h, directory.encode()))
+
+    def synchronize(self):
+        self._check(self.lib.kofm_synchronize(self.h))
