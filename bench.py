@@ -35,12 +35,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def setup(w, storage, komega=False, rtol=1e-4, maxit=1000):
+def setup(w, storage, komega=False, rtol=1e-4, maxit=1000, tune=True):
     from komegafoulingmodel_amd import lib
 
     L = lib.Lattice(w.nX, w.nY, storage=storage, komega=komega)
     L.set_params(w.tau0, w.Fval, w.nu, kappa_rel_tol=rtol, kappa_max_iters=maxit)
     L.build_geometry(w.C0, w.BL, **w.geom_params())
+    L.launch = L.autotune(40) if tune else None
     nType = L.geometry("nType")
     rho, ux, uy = w.initial_velocity(nType)
     L.init_state(rho, ux, uy)
@@ -59,9 +60,13 @@ def measure(L, fluid, steps, warmup, blu, options=0):
     wall = time.perf_counter() - t0
     mlups = fluid * steps / wall / 1e6
     achieved = blu * fluid / (ms_kernel * 1e-3) / 1e9
-    return dict(mlups=mlups, ms_per_step=wall * 1e3 / steps, ms_per_step_events=ms_step,
-                ms_per_kernel=ms_kernel, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
-                fluid_nodes=fluid, steps=steps)
+    out = dict(mlups=mlups, ms_per_step=wall * 1e3 / steps, ms_per_step_events=ms_step,
+               ms_per_kernel=ms_kernel, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
+               fluid_nodes=fluid, steps=steps)
+    if getattr(L, "launch", None):
+        out["launch"] = {"block_x": L.launch[0], "nodes_per_lane": L.launch[1],
+                         "plane_pad": L.launch[2]}
+    return out
 
 
 def read_traffic(name):
@@ -99,6 +104,28 @@ def cpu_baseline(w, seconds=15.0, max_steps=40):
                 sample=f"{n} FP64 steps of the {w.nX}x{w.nY} fouled channel (oracle, OpenMP)")
 
 
+def copy_calibration(nbytes=2 << 30, reps=20):
+    """A plain device-to-device copy of known size (torch's copy kernel), the
+    reference for scaling the FETCH_SIZE / WRITE_SIZE counters and for the
+    achievable copy bandwidth."""
+    import torch
+
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "ms": ms,
+                      "gbs": 2 * n * 4 / (ms * 1e-3) / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,7 +134,11 @@ def main():
     ap.add_argument("--forms", default="f16,bf16,k32")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               "(the lattice step runs on a single MI355X)", file=sys.stderr)

@@ -245,8 +245,52 @@ int kofm_download(kofm_ctx* ctx, int32_t field, double* host);
  * values per y, 6 significant digits) into dir. */
 int kofm_save2file(kofm_ctx* ctx, const char* dir);
 
+/* --- launch shape ---------------------------------------------------------- */
+
+/* Work-group width of the collide-stream kernel (64, 128, 256, 512 or
+ * 1024), lattice nodes per lane (1, or 2 neighbouring nodes when nX is even)
+ * and padding (elements, multiple of 64) between the nine population planes
+ * in HBM.  The state is preserved.  No counterpart in the reference, whose
+ * work-group is fixed at 128 x 1 (ConstDef.h:70-79). */
+int kofm_set_launch(kofm_ctx* ctx, int32_t block_x, int32_t nodes_per_lane, int64_t plane_pad);
+
+/* Times the collide-stream kernel over the candidate widths, nodes per lane
+ * and paddings (steps launches each, on scratch populations at rest) and
+ * keeps the fastest; returns the choice and its time per step.  No
+ * counterpart in the reference. */
+int kofm_autotune(kofm_ctx* ctx, int32_t steps, int32_t* block_x, int32_t* nodes_per_lane,
+                  int64_t* plane_pad, double* ms_per_step);
+
+/* With on != 0 (the default) kofm_step replays a captured HIP graph of 32
+ * collide-stream launches for each whole block of 32 plain (k-omega off)
+ * steps; off launches every step separately.  Results are identical.  No
+ * counterpart in the reference. */
+int kofm_set_graphs(kofm_ctx* ctx, int32_t on);
+
 /* Blocks until all work on the context stream is done. */
 int kofm_synchronize(kofm_ctx* ctx);
+
+/* --- ensembles ------------------------------------------------------------- */
+
+/* Several independent channels (contexts with geometry and state, one
+ * population storage type, k-omega off) stepped together: one collide-stream
+ * launch per step covers every member's tiles, and whole blocks of 32 steps
+ * replay a captured HIP graph.  Members keep their own arrays and are read
+ * with kofm_download as usual; calls on a member are ordered after the
+ * ensemble's steps.  The reference runs one channel per process; this has no
+ * counterpart there. */
+typedef struct kofm_ensemble kofm_ensemble;
+
+int kofm_ensemble_create(kofm_ctx* const* members, int32_t n, kofm_ensemble** out);
+int kofm_ensemble_step(kofm_ensemble* e, int32_t nsteps, uint32_t options);
+/* Mean device time per ensemble step over nsteps (HIP events) and the total
+ * number of fluid nodes updated per step. */
+int kofm_ensemble_time_steps(kofm_ensemble* e, int32_t nsteps, uint32_t options,
+                             double* ms_per_step, int64_t* fluid_nodes);
+/* Work-group width of the ensemble launch (128, 256, 512 or 1024). */
+int kofm_ensemble_set_block(kofm_ensemble* e, int32_t block_x);
+const char* kofm_ensemble_last_error(const kofm_ensemble* e);
+void kofm_ensemble_destroy(kofm_ensemble* e);
 
 #ifdef __cplusplus
 }

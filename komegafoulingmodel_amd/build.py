@@ -14,6 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkofm.so")
+DRIVER = os.path.join(HERE, "kofm_run")
+CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("KOFM_ARCH", "gfx950")
 
@@ -60,8 +62,16 @@ def build(verbose: bool = False) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    # the reference-style driver (host C++ over the C ABI)
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(HERE, '..', 'include')}",
+           os.path.join(CSRC, "driver.cpp"), "-o", DRIVER + ".tmp", f"-L{HERE}", "-lkofm",
+           "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {DRIVER}")
     return LIB
 
 

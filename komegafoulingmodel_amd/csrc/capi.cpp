@@ -294,6 +294,38 @@ int kofm_save2file(kofm_ctx* ctx, const char* dir) {
   return rc ? setErr(ctx, rc, err) : KOFM_OK;
 }
 
+int kofm_set_launch(kofm_ctx* ctx, int32_t block_x, int32_t nodes_per_lane, int64_t plane_pad) {
+  if (!ctx) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int rc = ctx->engine->setLaunch(block_x, nodes_per_lane, plane_pad, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_autotune(kofm_ctx* ctx, int32_t steps, int32_t* block_x, int32_t* nodes_per_lane,
+                  int64_t* plane_pad, double* ms_per_step) {
+  if (!ctx || !block_x || !nodes_per_lane || !plane_pad || !ms_per_step) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int b = 0, npt = 1;
+  int64_t pad = 0;
+  double ms = 0.;
+  int rc = ctx->engine->autotune(steps, b, npt, pad, ms, err);
+  if (rc) return setErr(ctx, rc, err);
+  *block_x = b;
+  *nodes_per_lane = npt;
+  *plane_pad = pad;
+  *ms_per_step = ms;
+  return KOFM_OK;
+}
+
+int kofm_set_graphs(kofm_ctx* ctx, int32_t on) {
+  if (!ctx) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  ctx->engine->setGraphs(on != 0);
+  return KOFM_OK;
+}
+
 int kofm_synchronize(kofm_ctx* ctx) {
   if (!ctx) return KOFM_ERR_ARG;
   NEED_ENGINE(ctx);
@@ -301,5 +333,59 @@ int kofm_synchronize(kofm_ctx* ctx) {
   int rc = ctx->engine->synchronize(err);
   return rc ? setErr(ctx, rc, err) : KOFM_OK;
 }
+
+struct kofm_ensemble {
+  std::unique_ptr<kofm::Ensemble> impl;
+  std::string err;
+};
+
+int kofm_ensemble_create(kofm_ctx* const* members, int32_t n, kofm_ensemble** out) {
+  if (!members || n < 1 || !out) return KOFM_ERR_ARG;
+  *out = nullptr;
+  std::vector<kofm::Engine*> es;
+  for (int i = 0; i < n; i++) {
+    if (!members[i] || !members[i]->engine) {
+      g_createError = "ensemble members must be device contexts";
+      return KOFM_ERR_ARG;
+    }
+    es.push_back(members[i]->engine.get());
+  }
+  std::string err;
+  auto impl = kofm::makeEnsemble(es, err);
+  if (!impl) {
+    g_createError = err;
+    return KOFM_ERR_STATE;
+  }
+  auto* e = new kofm_ensemble;
+  e->impl = std::move(impl);
+  *out = e;
+  return KOFM_OK;
+}
+
+int kofm_ensemble_step(kofm_ensemble* e, int32_t nsteps, uint32_t options) {
+  if (!e || nsteps < 0) return KOFM_ERR_ARG;
+  int rc = e->impl->step(nsteps, options, e->err);
+  return rc ? rc : KOFM_OK;
+}
+
+int kofm_ensemble_time_steps(kofm_ensemble* e, int32_t nsteps, uint32_t options,
+                             double* ms_per_step, int64_t* fluid_nodes) {
+  if (!e || nsteps < 1 || !ms_per_step) return KOFM_ERR_ARG;
+  int rc = e->impl->timeSteps(nsteps, options, *ms_per_step, e->err);
+  if (fluid_nodes) *fluid_nodes = e->impl->fluidNodes();
+  return rc ? rc : KOFM_OK;
+}
+
+int kofm_ensemble_set_block(kofm_ensemble* e, int32_t block_x) {
+  if (!e) return KOFM_ERR_ARG;
+  int rc = e->impl->setBlock(block_x, e->err);
+  return rc ? rc : KOFM_OK;
+}
+
+const char* kofm_ensemble_last_error(const kofm_ensemble* e) {
+  return e ? e->err.c_str() : g_createError.c_str();
+}
+
+void kofm_ensemble_destroy(kofm_ensemble* e) { delete e; }
 
 }  // extern "C"

@@ -1,0 +1,89 @@
+// driver.cpp -- kofm_run: the reference's ini/run sequence (LBLS.cpp:65-381)
+// for the lattice-Boltzmann step, through the C++ facades.
+//
+//   kofm_run [load_dir] [out_dir]
+//
+// reads <load_dir>/RunParam.yaml, lsc0.txt, lsbl.txt (and lbf.bin,
+// lbkappa.bin, lbomega.bin for a restart), runs "Ini LB Steps" warm-up steps
+// and then steps to "Stop Time", computing the wall shear every
+// "LB Steps Per TR Wall" steps, appending the reductions every "Save Avgs"
+// interval and writing the save2file dumps every "Dump" interval.
+#include <algorithm>
+#include <cstdio>
+#include <exception>
+#include <string>
+
+#include "facade.hpp"
+
+// "kofm_run --params <load_dir>" prints the parameters derived from
+// RunParam.yaml and exits without touching a GPU.
+static int printParams(const std::string& loadDir) {
+  using namespace kofm::ref;
+  clProblem p;
+  p.loadRunParameters(loadDir + "/RunParam.yaml");
+  clVariablesLB vlb;
+  vlb.loadParams(p);
+  std::printf("nX %d\nnY %d\nXsizeFull %d\nPipe_radius %.17g\nChannel_Height %d\n", p.nX, p.nY,
+              p.XsizeFull, p.Pipe_radius, p.Channel_Height);
+  std::printf("tau0 %.17g\nMuVal %.17g\nRe %.17g\nFval %.17g\nRho %.17g\n", vlb.tau, vlb.MuVal,
+              vlb.Re, vlb.Fval, vlb.RhoVal);
+  std::printf("UseTurbModel %d\nStopTime %u\ntrStepsWall %u\n",
+              vlb.kOmegaClass.kOmegaSolverFlag ? 1 : 0, p.StopTime, p.trSteps_wall);
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  using namespace kofm::ref;
+  if (argc > 2 && std::string(argv[1]) == "--params") {
+    try {
+      return printParams(argv[2]);
+    } catch (const std::exception& e) {
+      std::fprintf(stderr, "kofm_run: %s\n", e.what());
+      return 1;
+    }
+  }
+  const std::string loadDir = argc > 1 ? argv[1] : "load";
+  const std::string outDir = argc > 2 ? argv[2] : ".";
+  try {
+    clProblem p;
+    p.loadRunParameters(loadDir + "/RunParam.yaml");
+    clVariablesLS vls;
+    vls.loadParams(p, loadDir);
+    clVariablesLB vlb;
+    vlb.loadParams(p);
+    vlb.ini(p, vls, loadDir);
+    std::printf("nX %d nY %d tau0 %.6f nu %.6g Fval %.6g k-omega %s\n", p.nX, p.nY, vlb.tau,
+                vlb.MuVal, vlb.Fval, vlb.kOmegaClass.kOmegaSolverFlag ? "on" : "off");
+    if (p.RestartTime == 0 && vlb.tlbmNumIniSteps_LB > 0) vlb.getInitialFlow(p, outDir);
+    shearStress ss;
+    unsigned nextAvg = p.saveAvgStartTime + p.saveTimeStepAvg;
+    unsigned nextDump = p.dumpStartTime + p.dumpTimeStep;
+    while (p.Time < p.StopTime) {
+      unsigned n = std::min(p.StopTime - p.Time, p.trSteps_wall);
+      n = std::min(n, nextAvg - p.Time);
+      n = std::min(n, nextDump - p.Time);
+      vlb.Solve(int(n));
+      p.Time += n;
+      if (p.Time % p.trSteps_wall == 0) ss.calculateShear(vlb.ctx);
+      if (p.Time >= nextAvg) {
+        vlb.updateTimeData(p.Time);
+        vlb.saveTimeData(outDir);
+        ss.updateTimeData(vlb.ctx, p.Time, outDir);
+        nextAvg += p.saveTimeStepAvg;
+      }
+      if (p.Time >= nextDump) {
+        vlb.save2file(outDir);
+        nextDump += p.dumpTimeStep;
+      }
+    }
+    vlb.save2file(outDir);
+    p.params.setInt(kofm::RunParams::kSystem, "Current Time", int(p.Time));
+    p.params.setBool(kofm::RunParams::kSystem, "Restart Run", true);
+    p.params.save(outDir + "/RunParam.yaml");
+    std::printf("done at time %u, Umean %g\n", p.Time, vlb.calcUmean());
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "kofm_run: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

@@ -66,15 +66,25 @@ class Buffer {
   size_t bytes_ = 0;
 };
 
+// Default padding of the population planes (elements).
+constexpr int64_t kDefaultPad = 1088;
+
+template <class S, class R>
+class EnsembleT;
+
 template <class S, class R>
 class EngineT final : public Engine {
+  friend class EnsembleT<S, R>;
+
  public:
   EngineT(int nX, int nY, bool komega) : nX_(nX), nY_(nY), ko_(komega) {
     XF_ = ((nX + 127) / 128) * 128;
     D_ = int64_t(XF_) * nY;
+    P_ = D_ + kDefaultPad;
   }
   ~EngineT() override {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    dropGraphs();
     for (auto& e : evs_) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
@@ -82,8 +92,8 @@ class EngineT final : public Engine {
   int create(std::string& err) {
     HIP_TRY(hipStreamCreate(&stream_));
     const size_t D = size_t(D_);
-    HIP_TRY(fA_.alloc(9 * D * sizeof(S)));
-    HIP_TRY(fB_.alloc(9 * D * sizeof(S)));
+    HIP_TRY(fA_.alloc(9 * size_t(P_) * sizeof(S)));
+    HIP_TRY(fB_.alloc(9 * size_t(P_) * sizeof(S)));
     HIP_TRY(mask_.alloc(D * sizeof(uint16_t)));
     HIP_TRY(ibbIdx_.alloc(D * sizeof(int32_t)));
     HIP_TRY(rho_.alloc(D * sizeof(R)));
@@ -91,12 +101,12 @@ class EngineT final : public Engine {
     HIP_TRY(uy_.alloc(D * sizeof(R)));
     if (ko_) {
       for (Buffer* b : {&K_, &O_, &nut_, &wallD_, &Dk_, &Do_, &F1_, &bk_, &bo_})
-        HIP_TRY(b->alloc(D * sizeof(R)));
-      HIP_TRY(S_.alloc(3 * D * sizeof(R)));
-      HIP_TRY(dkdo_.alloc(2 * D * sizeof(R)));
-      HIP_TRY(dX_.alloc(4 * D * sizeof(R)));
-      HIP_TRY(Ak_.alloc(4 * D * sizeof(R)));
-      HIP_TRY(Ao_.alloc(4 * D * sizeof(R)));
+        HIP_TRY(b->alloc(D * sizeof(double)));
+      HIP_TRY(S_.alloc(3 * D * sizeof(double)));
+      HIP_TRY(dkdo_.alloc(2 * D * sizeof(double)));
+      HIP_TRY(dX_.alloc(4 * D * sizeof(double)));
+      HIP_TRY(Ak_.alloc(4 * D * sizeof(double)));
+      HIP_TRY(Ao_.alloc(4 * D * sizeof(double)));
     }
     HIP_TRY(status_.alloc(sizeof(dev::BicgState)));
     if (ko_)
@@ -111,6 +121,8 @@ class EngineT final : public Engine {
     return 0;
   }
 
+  bool komega() const { return ko_; }
+
   int64_t fieldSize(int field) const override {
     switch (field) {
       case KOFM_FIELD_F: return 9 * D_;
@@ -123,6 +135,7 @@ class EngineT final : public Engine {
 
   int uploadGeometry(const Geometry& g, std::string& err) override {
     geo_ = g;
+    dropGraphs();
     const size_t D = size_t(D_);
     // device mask and wall-node q table
     std::vector<uint16_t> m(D, 0);
@@ -171,14 +184,17 @@ class EngineT final : public Engine {
       HIP_TRY(upload<R>(wts_, g.weights));
     }
     if (ko_) {
-      HIP_TRY(upload<R>(wallD_, g.wallD));
-      HIP_TRY(upload<R>(dX_, g.dX));
+      HIP_TRY(upload<double>(wallD_, g.wallD));
+      HIP_TRY(upload<double>(dX_, g.dX));
     }
     haveGeo_ = true;
     return 0;
   }
 
-  void setParams(const Params& p) override { par_ = p; }
+  void setParams(const Params& p) override {
+    par_ = p;
+    dropGraphs();
+  }
 
   int initState(const double* rho, const double* ux, const double* uy, std::string& err) override {
     if (!haveGeo_) return fail(err, "geometry must be built before the state");
@@ -237,16 +253,16 @@ class EngineT final : public Engine {
       return KOFM_ERR_INITIALIZING_KOMEGA;
     }
     const size_t D = size_t(D_);
-    std::vector<R> a(D, R(0)), b(D, R(0)), c(D, R(0));
+    std::vector<double> a(D, 0.), b(D, 0.), c(D, 0.);
     for (size_t i = 0; i < D; i++)
       if (geo_.nType[i] & kMFluid) {
-        a[i] = R(k);
-        b[i] = R(w);
-        c[i] = R(nt);
+        a[i] = k;
+        b[i] = w;
+        c[i] = nt;
       }
-    HIP_TRY(upload<R>(K_, a));
-    HIP_TRY(upload<R>(O_, b));
-    HIP_TRY(upload<R>(nut_, c));
+    HIP_TRY(upload<double>(K_, a));
+    HIP_TRY(upload<double>(O_, b));
+    HIP_TRY(upload<double>(nut_, c));
     HIP_TRY(hipMemsetAsync(S_.as<void>(), 0, S_.bytes(), stream_));
     return 0;
   }
@@ -258,10 +274,10 @@ class EngineT final : public Engine {
     int rc = importF(f, err);
     if (rc) return rc;
     if (ko_) {
-      if (k) HIP_TRY(upload<R>(K_, std::vector<double>(k, k + D)));
-      if (w) HIP_TRY(upload<R>(O_, std::vector<double>(w, w + D)));
-      if (nut) HIP_TRY(upload<R>(nut_, std::vector<double>(nut, nut + D)));
-      if (strain) HIP_TRY(upload<R>(S_, std::vector<double>(strain, strain + 3 * D)));
+      if (k) HIP_TRY(upload<double>(K_, std::vector<double>(k, k + D)));
+      if (w) HIP_TRY(upload<double>(O_, std::vector<double>(w, w + D)));
+      if (nut) HIP_TRY(upload<double>(nut_, std::vector<double>(nut, nut + D)));
+      if (strain) HIP_TRY(upload<double>(S_, std::vector<double>(strain, strain + 3 * D)));
     }
     auto p = lbParams(0);
     hipLaunchKernelGGL((dev::lbMacros<S, R>), gridXY(nX_), dim3(dev::kBlockX), 0, stream_,
@@ -274,13 +290,34 @@ class EngineT final : public Engine {
 
   int step(int n, unsigned options, std::string& err) override {
     if (!haveState_) return fail(err, "state not initialised");
-    for (int s = 0; s < n; s++) {
+    int s = 0;
+    if (!ko_ && useGraphs_) {
+      // the first step after an upload runs eagerly (no pending wall
+      // correction); whole blocks of kGraphSteps then replay a captured graph
+      if (!preIbb_ && s < n) {
+        launchLB(options);
+        s++;
+      }
+      while (n - s >= kGraphSteps) {
+        hipGraphExec_t g = nullptr;
+        int rc = graphFor(options, g, err);
+        if (rc) return rc;
+        HIP_TRY(hipGraphLaunch(g, stream_));
+        s += kGraphSteps;  // kGraphSteps is even: the buffer parity is unchanged
+      }
+    }
+    for (; s < n; s++) {
       int rc = launchLB(options);
       if (!rc && ko_) rc = koStep(err);
-      if (rc) return rc ? rc : KOFM_ERR_DEVICE;
+      if (rc) return rc;
     }
     HIP_TRY(hipGetLastError());
     return 0;
+  }
+
+  void setGraphs(bool on) override {
+    useGraphs_ = on;
+    dropGraphs();
   }
 
   int timeSteps(int n, unsigned options, double& msStep, double& msKernel, std::string& err) override {
@@ -322,7 +359,7 @@ class EngineT final : public Engine {
       hipLaunchKernelGGL((dev::shearNodes<S, R>), dim3((nSS + 63) / 64), dim3(64), 0, stream_,
                          cur(), mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(), ibbQ_.as<R>(),
                          ss_.as<int32_t>(), nSS, cfss_.as<R>(), rho_.as<R>(), ux_.as<R>(),
-                         uy_.as<R>(), ko_ ? nut_.as<R>() : nullptr, tau_.as<R>(), p);
+                         uy_.as<R>(), ko_ ? (const double*)nut_.as<double>() : nullptr, tau_.as<R>(), p);
       HIP_TRY(hipGetLastError());
     }
     if (nb > 0) {
@@ -334,15 +371,19 @@ class EngineT final : public Engine {
   }
 
   int reduce(int field, int op, double& out, std::string& err) override {
-    const R* a = fieldPtr(field);
-    if (!a) return fail(err, "field cannot be reduced");
+    const FieldRef f = fieldPtr(field);
+    if (!f.p) return fail(err, "field cannot be reduced");
     if (op != KOFM_RED_SUM && op != KOFM_RED_MAX && op != KOFM_RED_MIN) return fail(err, "bad op");
     const int64_t n = fieldSize(field);
     int64_t full = 2;
     while (full < n) full *= 2;
     const int blocks = int(std::min<int64_t>(1024, (n + 255) / 256));
-    hipLaunchKernelGGL((dev::reducePartial<R>), dim3(blocks), dim3(256), 0, stream_, a, n, op,
-                       redPartial_.as<double>());
+    if (f.dbl)
+      hipLaunchKernelGGL((dev::reducePartial<double>), dim3(blocks), dim3(256), 0, stream_,
+                         (const double*)f.p, n, op, redPartial_.as<double>());
+    else
+      hipLaunchKernelGGL((dev::reducePartial<float>), dim3(blocks), dim3(256), 0, stream_,
+                         (const float*)f.p, n, op, redPartial_.as<double>());
     hipLaunchKernelGGL(dev::reduceFinal, dim3(1), dim3(256), 0, stream_,
                        (const double*)redPartial_.as<double>(), blocks, op, full > n ? 1 : 0,
                        redOut_.as<double>());
@@ -366,11 +407,16 @@ class EngineT final : public Engine {
       HIP_TRY(hipStreamSynchronize(stream_));
       return 0;
     }
-    const R* a = fieldPtr(field);
-    if (!a) return fail(err, "field not available in this context");
+    const FieldRef f = fieldPtr(field);
+    if (!f.p) return fail(err, "field not available in this context");
     const size_t n = size_t(fieldSize(field));
-    std::vector<R> h(n);
-    HIP_TRY(hipMemcpyAsync(h.data(), a, n * sizeof(R), hipMemcpyDeviceToHost, stream_));
+    if (f.dbl) {
+      HIP_TRY(hipMemcpyAsync(host, f.p, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+      HIP_TRY(hipStreamSynchronize(stream_));
+      return 0;
+    }
+    std::vector<float> h(n);
+    HIP_TRY(hipMemcpyAsync(h.data(), f.p, n * sizeof(float), hipMemcpyDeviceToHost, stream_));
     HIP_TRY(hipStreamSynchronize(stream_));
     for (size_t i = 0; i < n; i++) host[i] = double(h[i]);
     return 0;
@@ -398,6 +444,11 @@ class EngineT final : public Engine {
  private:
   int nX_, nY_, XF_;
   int64_t D_;
+  int64_t P_;           // population plane stride: D_ + pad (the pad breaks power-of-two
+                        // plane distances, which otherwise stack all 18 streams on the
+                        // same HBM channels)
+  int lbBlock_ = 512;   // work-group width of the collide-stream kernel
+  int lbNpt_ = 1;       // lattice nodes per lane (1, or 2 neighbouring nodes)
   bool ko_;
   hipStream_t stream_ = nullptr;
   Buffer fA_, fB_, mask_, ibbIdx_, ibbQ_, rho_, ux_, uy_;
@@ -412,6 +463,42 @@ class EngineT final : public Engine {
   int curBuf_ = 0;       // 0: fA holds the newest populations
   bool preIbb_ = false;  // newest populations still need the wall-node correction
   bool haveGeo_ = false, haveState_ = false, nanSeen_ = false;
+  // captured blocks of plain LB steps, keyed by start buffer and options
+  static constexpr int kGraphSteps = 32;
+  bool useGraphs_ = true;
+  struct GraphKey {
+    int buf;
+    unsigned options;
+  };
+  std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs_;
+
+  void dropGraphs() {
+    for (auto& g : graphs_) (void)hipGraphExecDestroy(g.second);
+    graphs_.clear();
+  }
+
+  // Instantiates (once) the graph of kGraphSteps collide-stream launches for
+  // the current buffer parity, options and launch shape.
+  int graphFor(unsigned options, hipGraphExec_t& out, std::string& err) {
+    for (auto& g : graphs_)
+      if (g.first.buf == curBuf_ && g.first.options == options) {
+        out = g.second;
+        return 0;
+      }
+    const int buf0 = curBuf_;
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < kGraphSteps; k++) launchLB(options);
+    HIP_TRY(hipStreamEndCapture(stream_, &graph));
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_TRY(e);
+    curBuf_ = buf0;  // capture recorded the launches without running them
+    graphs_.push_back({{buf0, options}, exec});
+    out = exec;
+    return 0;
+  }
 
   static int fail(std::string& err, const char* msg) {
     err = msg;
@@ -428,6 +515,7 @@ class EngineT final : public Engine {
   S* other() const { return curBuf_ == 0 ? fB_.as<S>() : fA_.as<S>(); }
 
   dim3 gridXY(int width) const { return dim3((width + dev::kBlockX - 1) / dev::kBlockX, nY_); }
+  dim3 gridLB(int block) const { return dim3((nX_ + block - 1) / block, nY_); }
 
   dev::LBParams<R> lbParams(unsigned options) const {
     dev::LBParams<R> p;
@@ -435,6 +523,7 @@ class EngineT final : public Engine {
     p.nY = nY_;
     p.XF = XF_;
     p.D = D_;
+    p.P = P_;
     p.omega0 = R(par_.tau0);
     p.Fval = R(par_.Fval);
     p.nu = R(par_.nu);
@@ -444,33 +533,39 @@ class EngineT final : public Engine {
     return p;
   }
 
-  dev::KOParams<R> koParams() const {
-    dev::KOParams<R> p;
+  dev::KOParams koParams() const {
+    dev::KOParams p;
     p.nX = nX_;
     p.nY = nY_;
     p.XF = XF_;
     p.D = D_;
-    p.nu = R(par_.nu);
-    p.dt = R(par_.koDt);
+    p.nu = par_.nu;
+    p.dt = par_.koDt;
     p.tiles.tilesX = (nX_ + dev::kBlockX - 1) / dev::kBlockX;
     p.tiles.nTiles = p.tiles.tilesX * nY_;
     p.tiles.per = (p.tiles.nTiles + 7) / 8;
     return p;
   }
 
-  const R* fieldPtr(int field) const {
+  struct FieldRef {
+    const void* p;
+    bool dbl;
+  };
+
+  FieldRef fieldPtr(int field) const {
+    const bool Rd = sizeof(R) == sizeof(double);
     switch (field) {
-      case KOFM_FIELD_RHO: return rho_.as<R>();
-      case KOFM_FIELD_UX: return ux_.as<R>();
-      case KOFM_FIELD_UY: return uy_.as<R>();
-      case KOFM_FIELD_KAPPA: return ko_ ? K_.as<R>() : nullptr;
-      case KOFM_FIELD_OMEGA: return ko_ ? O_.as<R>() : nullptr;
-      case KOFM_FIELD_NUT: return ko_ ? nut_.as<R>() : nullptr;
-      case KOFM_FIELD_SXY: return ko_ ? S_.as<R>() : nullptr;
-      case KOFM_FIELD_WALLD: return ko_ ? wallD_.as<R>() : nullptr;
-      case KOFM_FIELD_TAU_NODE: return tau_.as<R>();
-      case KOFM_FIELD_TAU_BL: return tauBL_.as<R>();
-      default: return nullptr;
+      case KOFM_FIELD_RHO: return {rho_.as<void>(), Rd};
+      case KOFM_FIELD_UX: return {ux_.as<void>(), Rd};
+      case KOFM_FIELD_UY: return {uy_.as<void>(), Rd};
+      case KOFM_FIELD_KAPPA: return {ko_ ? K_.as<void>() : nullptr, true};
+      case KOFM_FIELD_OMEGA: return {ko_ ? O_.as<void>() : nullptr, true};
+      case KOFM_FIELD_NUT: return {ko_ ? nut_.as<void>() : nullptr, true};
+      case KOFM_FIELD_SXY: return {ko_ ? S_.as<void>() : nullptr, true};
+      case KOFM_FIELD_WALLD: return {ko_ ? wallD_.as<void>() : nullptr, true};
+      case KOFM_FIELD_TAU_NODE: return {tau_.as<void>(), Rd};
+      case KOFM_FIELD_TAU_BL: return {tauBL_.as<void>(), Rd};
+      default: return {nullptr, false};
     }
   }
 
@@ -488,41 +583,172 @@ class EngineT final : public Engine {
     return 0;
   }
 
+  template <bool KO, int BX, int NPT>
+  void launchLBT(const dev::LBParams<R>& p, const S* src, S* dst) {
+    dev::KOFields kf{K_.as<void>(), O_.as<void>(), wallD_.as<void>(), nut_.as<void>(), S_.as<void>()};
+    const dim3 grid((nX_ + BX * NPT - 1) / (BX * NPT), nY_);
+    if constexpr (NPT == 2)
+      hipLaunchKernelGGL((dev::lbCollideStream2<S, R, KO, BX>), grid, dim3(BX), 0, stream_, src,
+                         dst, mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(), ibbQ_.as<R>(),
+                         rho_.as<R>(), ux_.as<R>(), uy_.as<R>(), kf, p);
+    else
+      hipLaunchKernelGGL((dev::lbCollideStream<S, R, KO, BX>), grid, dim3(BX), 0, stream_, src,
+                         dst, mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(), ibbQ_.as<R>(),
+                         rho_.as<R>(), ux_.as<R>(), uy_.as<R>(), kf, p);
+  }
+
+  template <bool KO, int NPT>
+  void launchLBBlockN(int block, const dev::LBParams<R>& p, const S* src, S* dst) {
+    switch (block) {
+      case 64: launchLBT<KO, 64, NPT>(p, src, dst); break;
+      case 128: launchLBT<KO, 128, NPT>(p, src, dst); break;
+      case 512: launchLBT<KO, 512, NPT>(p, src, dst); break;
+      case 1024: launchLBT<KO, 1024, NPT>(p, src, dst); break;
+      default: launchLBT<KO, 256, NPT>(p, src, dst); break;
+    }
+  }
+
+  template <bool KO>
+  void launchLBBlock(int block, int npt, const dev::LBParams<R>& p, const S* src, S* dst) {
+    if (npt == 2 && (nX_ % 2) == 0)
+      launchLBBlockN<KO, 2>(block, p, src, dst);
+    else
+      launchLBBlockN<KO, 1>(block, p, src, dst);
+  }
+
   int launchLB(unsigned options) {
     auto p = lbParams(options);
-    dev::KOFields kf{K_.as<void>(), O_.as<void>(), wallD_.as<void>(), nut_.as<void>(), S_.as<void>()};
     if (ko_)
-      hipLaunchKernelGGL((dev::lbCollideStream<S, R, true>), gridXY(nX_), dim3(dev::kBlockX), 0,
-                         stream_, cur(), other(), mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(),
-                         ibbQ_.as<R>(), rho_.as<R>(), ux_.as<R>(), uy_.as<R>(), kf, p);
+      launchLBBlock<true>(lbBlock_, lbNpt_, p, cur(), other());
     else
-      hipLaunchKernelGGL((dev::lbCollideStream<S, R, false>), gridXY(nX_), dim3(dev::kBlockX), 0,
-                         stream_, cur(), other(), mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(),
-                         ibbQ_.as<R>(), rho_.as<R>(), ux_.as<R>(), uy_.as<R>(), kf, p);
+      launchLBBlock<false>(lbBlock_, lbNpt_, p, cur(), other());
     curBuf_ ^= 1;
     preIbb_ = true;
     return 0;
   }
 
+  // Re-lays the populations out with a new plane stride (state preserved
+  // through an FP64 copy in device memory).
+  int relayout(int64_t pad, std::string& err) {
+    const int64_t P = D_ + pad;
+    if (P == P_) return 0;
+    Buffer tmp;
+    if (haveState_) {
+      HIP_TRY(tmp.alloc(9 * size_t(D_) * sizeof(double)));
+      auto p = lbParams(0);
+      hipLaunchKernelGGL((dev::lbExportF<S, R>), gridXY(XF_), dim3(dev::kBlockX), 0, stream_,
+                         cur(), mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(), ibbQ_.as<R>(),
+                         tmp.as<double>(), p);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipStreamSynchronize(stream_));
+    }
+    P_ = P;
+    dropGraphs();
+    HIP_TRY(fA_.alloc(9 * size_t(P_) * sizeof(S)));
+    HIP_TRY(fB_.alloc(9 * size_t(P_) * sizeof(S)));
+    curBuf_ = 0;
+    if (haveState_) {
+      auto p = lbParams(0);
+      hipLaunchKernelGGL((dev::lbImportF<S, R>), gridXY(XF_), dim3(dev::kBlockX), 0, stream_,
+                         (const double*)tmp.as<double>(), fA_.as<S>(), p);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipStreamSynchronize(stream_));
+      preIbb_ = false;
+    }
+    return 0;
+  }
+
+ public:
+  int setLaunch(int block, int npt, int64_t pad, std::string& err) override {
+    if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
+      return fail(err, "block width must be 64, 128, 256, 512 or 1024");
+    if (npt != 1 && npt != 2) return fail(err, "nodes per lane must be 1 or 2");
+    if (npt == 2 && nX_ % 2) return fail(err, "two nodes per lane need an even nX");
+    if (pad < 0 || pad % 64) return fail(err, "plane padding must be a non-negative multiple of 64");
+    int rc = relayout(pad, err);
+    if (rc) return rc;
+    lbBlock_ = block;
+    lbNpt_ = npt;
+    dropGraphs();
+    return 0;
+  }
+
+  // Times the collide-stream kernel (wall corrections on, k-omega and macro
+  // output off) for each candidate work-group width and plane padding on
+  // scratch populations at rest, keeps the fastest.
+  int autotune(int steps, int& bestBlock, int& bestNpt, int64_t& bestPad, double& bestMs,
+               std::string& err) override {
+    if (!haveGeo_) return fail(err, "build the geometry before tuning");
+    if (steps < 1) return fail(err, "steps must be positive");
+    static const int blocks[] = {128, 256, 512, 1024};
+    static const int64_t pads[] = {0, 64, 1088, 17408, 263232};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    bestMs = 1e30;
+    bestBlock = lbBlock_;
+    bestNpt = lbNpt_;
+    bestPad = P_ - D_;
+    int rc = 0;
+    for (int64_t pad : pads) {
+      Buffer a, b;
+      if (hipSuccess != a.alloc(9 * size_t(D_ + pad) * sizeof(S)) ||
+          hipSuccess != b.alloc(9 * size_t(D_ + pad) * sizeof(S))) {
+        rc = fail(err, "out of device memory while tuning");
+        break;
+      }
+      auto p = lbParams(0);
+      p.P = D_ + pad;
+      p.saveMacros = 0;
+      p.applyIbb = 1;
+      for (int npt = 1; npt <= (nX_ % 2 ? 1 : 2); npt++)
+        for (int block : blocks) {
+          launchLBBlock<false>(block, npt, p, a.as<S>(), b.as<S>());  // warm-up
+          (void)hipEventRecord(e0, stream_);
+          for (int s = 0; s < steps; s++)
+            launchLBBlock<false>(block, npt, p, (s & 1) ? b.as<S>() : a.as<S>(),
+                                 (s & 1) ? a.as<S>() : b.as<S>());
+          (void)hipEventRecord(e1, stream_);
+          (void)hipEventSynchronize(e1);
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          const double per = double(ms) / steps;
+          if (per < bestMs) {
+            bestMs = per;
+            bestBlock = block;
+            bestNpt = npt;
+            bestPad = pad;
+          }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_TRY(hipGetLastError());
+    if (rc) return rc;
+    return setLaunch(bestBlock, bestNpt, bestPad, err);
+  }
+
+ private:
   // kOmega::Solve (kOmega.cpp:31-73) with one sub-step: diffusivities,
   // coefficients, then both implicit systems by Jacobi sweeps with
   // device-side convergence checks, launched in chunks.
   int koStep(std::string& err) {
     auto kp = koParams();
     const dim3 grid(8 * kp.tiles.per), blk(dev::kBlockX);
+    using K = double;
     hipLaunchKernelGGL((dev::koDiffusivities<R>), grid, blk, 0, stream_, mask_.as<uint16_t>(),
-                       K_.as<R>(), O_.as<R>(), nut_.as<R>(), wallD_.as<R>(), Dk_.as<R>(),
-                       Do_.as<R>(), F1_.as<R>(), dkdo_.as<R>(), rho_.as<R>(), dX_.as<R>(), kp);
+                       K_.as<K>(), O_.as<K>(), nut_.as<K>(), wallD_.as<K>(), Dk_.as<K>(),
+                       Do_.as<K>(), F1_.as<K>(), dkdo_.as<K>(), rho_.as<R>(), dX_.as<K>(), kp);
     hipLaunchKernelGGL((dev::koCoeffs<R>), grid, blk, 0, stream_, mask_.as<uint16_t>(),
-                       K_.as<R>(), O_.as<R>(), nut_.as<R>(), Dk_.as<R>(), Do_.as<R>(),
-                       F1_.as<R>(), dkdo_.as<R>(), S_.as<R>(), rho_.as<R>(), ux_.as<R>(),
-                       uy_.as<R>(), dX_.as<R>(), wallD_.as<R>(), Ak_.as<R>(), Ao_.as<R>(),
-                       bk_.as<R>(), bo_.as<R>(), kp);
+                       K_.as<K>(), O_.as<K>(), nut_.as<K>(), Dk_.as<K>(), Do_.as<K>(),
+                       F1_.as<K>(), dkdo_.as<K>(), S_.as<K>(), rho_.as<R>(), ux_.as<R>(),
+                       uy_.as<R>(), dX_.as<K>(), wallD_.as<K>(), Ak_.as<K>(), Ao_.as<K>(),
+                       bk_.as<K>(), bo_.as<K>(), kp);
     using Q = double;
-    dev::BicgVecs<R, Q> V;
-    R* X[2] = {K_.as<R>(), O_.as<R>()};
-    const R* A[2] = {Ak_.as<R>(), Ao_.as<R>()};
-    const R* B[2] = {bk_.as<R>(), bo_.as<R>()};
+    dev::BicgVecs<K, Q> V;
+    K* X[2] = {K_.as<K>(), O_.as<K>()};
+    const K* A[2] = {Ak_.as<K>(), Ao_.as<K>()};
+    const K* B[2] = {bk_.as<K>(), bo_.as<K>()};
     for (int v = 0; v < 2; v++) {
       V.x[v] = X[v];
       V.A[v] = A[v];
@@ -543,20 +769,20 @@ class EngineT final : public Engine {
     hipLaunchKernelGGL(dev::bicgReset, dim3(1), dim3(64), 0, stream_, st, par_.relTol[0],
                        par_.absTol[0], par_.maxIters[0], par_.relTol[1], par_.absTol[1],
                        par_.maxIters[1]);
-    hipLaunchKernelGGL((dev::bicgInit<R, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V, part, kp);
+    hipLaunchKernelGGL((dev::bicgInit<K, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V, part, kp);
     hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, (const double*)part, nblk, st, 0);
     const int maxIt = std::max(par_.maxIters[0], par_.maxIters[1]);
     const int chunk = 4;
     int it = 0;
     while (it < maxIt) {
       for (int c = 0; c < chunk && it < maxIt; c++, it++) {
-        hipLaunchKernelGGL((dev::bicgPV<R, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V,
+        hipLaunchKernelGGL((dev::bicgPV<K, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V,
                            (const dev::BicgState*)st, part, kp);
         hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, (const double*)part, nblk, st, 1);
-        hipLaunchKernelGGL((dev::bicgST<R, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V,
+        hipLaunchKernelGGL((dev::bicgST<K, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V,
                            (const dev::BicgState*)st, part, kp);
         hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, (const double*)part, nblk, st, 2);
-        hipLaunchKernelGGL((dev::bicgXR<R, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V,
+        hipLaunchKernelGGL((dev::bicgXR<K, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V,
                            (const dev::BicgState*)st, part, kp);
         hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, (const double*)part, nblk, st, 3);
       }
@@ -566,7 +792,7 @@ class EngineT final : public Engine {
       if (hs.nan) nanSeen_ = true;
       if (hs.done[0] && hs.done[1]) break;
     }
-    hipLaunchKernelGGL((dev::bicgFinish<R, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V, kp);
+    hipLaunchKernelGGL((dev::bicgFinish<K, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V, kp);
     HIP_TRY(hipGetLastError());
     if (nanSeen_) {
       err = "non-finite residual in the k-omega solve";
@@ -576,7 +802,223 @@ class EngineT final : public Engine {
   }
 };
 
+// Steps several independent lattices (same storage, k-omega off) with one
+// collide-stream launch per step: every member keeps its own arrays, the
+// launch walks a table of per-member launch descriptions.
+template <class S, class R>
+class EnsembleT final : public Ensemble {
+ public:
+  explicit EnsembleT(std::vector<EngineT<S, R>*> m) : m_(std::move(m)) {}
+  ~EnsembleT() override {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    dropGraphs();
+    if (done_) (void)hipEventDestroy(done_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+
+  int create(std::string& err) {
+    HIP_TRY(hipStreamCreate(&stream_));
+    HIP_TRY(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+    for (auto& b : ents_) HIP_TRY(b.alloc(m_.size() * sizeof(dev::BatchEntry<S, R>)));
+    HIP_TRY(eager_.alloc(m_.size() * sizeof(dev::BatchEntry<S, R>)));
+    return 0;
+  }
+
+  int64_t fluidNodes() const override {
+    int64_t n = 0;
+    for (auto* e : m_)
+      for (size_t i = 0; i < e->geo_.nType.size(); i++)
+        if (!(e->geo_.nType[i] & kMSolid) && int(i % e->XF_) < e->nX_) n++;
+    return n;
+  }
+
+  int setBlock(int block, std::string& err) override {
+    if (block != 128 && block != 256 && block != 512 && block != 1024)
+      return fail(err, "ensemble block width must be 128, 256, 512 or 1024");
+    block_ = block;
+    dropGraphs();
+    built_ = false;
+    return 0;
+  }
+
+  int step(int n, unsigned options, std::string& err) override {
+    for (auto* e : m_) {
+      if (!e->haveState_) return fail(err, "every member needs an initialised state");
+      HIP_TRY(hipStreamSynchronize(e->stream_));
+    }
+    int s = 0;
+    // first step after an upload runs without the pending wall correction
+    bool fresh = false;
+    for (auto* e : m_) fresh |= !e->preIbb_;
+    if (fresh && s < n) {
+      int rc = fill(eager_, options, 0, true, err);
+      if (rc) return rc;
+      launch(eager_);
+      advance(1);
+      s++;
+    }
+    if (!built_ || builtOptions_ != options || builtParity_ != parity()) {
+      int rc = fill(ents_[0], options, 0, false, err);
+      if (!rc) rc = fill(ents_[1], options, 1, false, err);
+      if (rc) return rc;
+      dropGraphs();
+      built_ = true;
+      builtOptions_ = options;
+      builtParity_ = parity();
+    }
+    int k = 0;  // steps since the entries were built (selects ents_[k & 1])
+    while (n - s >= kGraphSteps) {
+      if (!graph_) {
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        for (int j = 0; j < kGraphSteps; j++) launch(ents_[j & 1]);
+        HIP_TRY(hipStreamEndCapture(stream_, &g));
+        hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_TRY(e);
+      }
+      HIP_TRY(hipGraphLaunch(graph_, stream_));
+      s += kGraphSteps;
+    }
+    for (; s < n; s++, k++) launch(ents_[k & 1]);
+    advance(k);
+    // the built entries assume every member's current buffer; an odd tail
+    // flips them, so rebuild next time
+    if (k & 1) built_ = false;
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(done_, stream_));
+    for (auto* e : m_) HIP_TRY(hipStreamWaitEvent(e->stream_, done_, 0));
+    return 0;
+  }
+
+  int timeSteps(int n, unsigned options, double& ms, std::string& err) override {
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    int rc = step(1, options, err);  // settles entries and graphs
+    if (!rc) {
+      (void)hipEventRecord(a, stream_);
+      rc = step(n, options, err);
+      (void)hipEventRecord(b, stream_);
+      (void)hipEventSynchronize(b);
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, a, b);
+      ms = double(t) / n;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+  }
+
+  int synchronize(std::string& err) override {
+    HIP_TRY(hipStreamSynchronize(stream_));
+    return 0;
+  }
+
+ private:
+  static constexpr int kGraphSteps = 32;
+  std::vector<EngineT<S, R>*> m_;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t done_ = nullptr;
+  Buffer ents_[2], eager_;
+  hipGraphExec_t graph_ = nullptr;
+  int block_ = 512;
+  int totalTiles_ = 0;
+  bool built_ = false;
+  unsigned builtOptions_ = 0;
+  std::vector<int> builtParity_;
+
+  static int fail(std::string& err, const char* msg) {
+    err = msg;
+    return KOFM_ERR_STATE;
+  }
+  void dropGraphs() {
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+    graph_ = nullptr;
+  }
+  std::vector<int> parity() const {
+    std::vector<int> v;
+    for (auto* e : m_) v.push_back(e->curBuf_);
+    return v;
+  }
+  // entries for a step taken `shift` steps after now
+  int fill(Buffer& buf, unsigned options, int shift, bool eager, std::string& err) {
+    std::vector<dev::BatchEntry<S, R>> h(m_.size());
+    int tiles = 0;
+    for (size_t i = 0; i < m_.size(); i++) {
+      EngineT<S, R>* e = m_[i];
+      const bool flip = (shift & 1) != 0;
+      dev::BatchEntry<S, R>& b = h[i];
+      b.src = flip ? e->other() : e->cur();
+      b.dst = flip ? e->cur() : e->other();
+      b.mask = e->mask_.template as<uint16_t>();
+      b.ibbIdx = e->ibbIdx_.template as<int32_t>();
+      b.ibbQ = e->ibbQ_.template as<R>();
+      b.rho = e->rho_.template as<R>();
+      b.ux = e->ux_.template as<R>();
+      b.uy = e->uy_.template as<R>();
+      b.p = e->lbParams(options);
+      b.p.applyIbb = eager ? (e->preIbb_ ? 1 : 0) : 1;
+      b.tilesX = (e->nX_ + block_ - 1) / block_;
+      b.tileStart = tiles;
+      tiles += b.tilesX * e->nY_;
+    }
+    totalTiles_ = tiles;
+    HIP_TRY(hipMemcpy(buf.as<void>(), h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice));
+    return 0;
+  }
+  void launch(const Buffer& buf) {
+    const auto* ents = buf.as<const dev::BatchEntry<S, R>>();
+    const int n = int(m_.size());
+    switch (block_) {
+      case 128: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 128>), dim3(totalTiles_), dim3(128), 0, stream_, ents, n); break;
+      case 256: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 256>), dim3(totalTiles_), dim3(256), 0, stream_, ents, n); break;
+      case 1024: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 1024>), dim3(totalTiles_), dim3(1024), 0, stream_, ents, n); break;
+      default: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 512>), dim3(totalTiles_), dim3(512), 0, stream_, ents, n); break;
+    }
+  }
+  void advance(int k) {
+    for (auto* e : m_) {
+      if (k & 1) e->curBuf_ ^= 1;
+      if (k > 0) e->preIbb_ = true;
+      e->dropGraphs();
+    }
+  }
+};
+
+template <class S, class R>
+std::unique_ptr<Ensemble> tryEnsemble(const std::vector<Engine*>& members, std::string& err) {
+  std::vector<EngineT<S, R>*> m;
+  for (Engine* e : members) {
+    auto* t = dynamic_cast<EngineT<S, R>*>(e);
+    if (!t) return nullptr;
+    if (t->komega()) {
+      err = "ensemble members must have k-omega off";
+      return nullptr;
+    }
+    m.push_back(t);
+  }
+  auto* p = new EnsembleT<S, R>(std::move(m));
+  std::unique_ptr<Ensemble> out(p);
+  if (p->create(err)) return nullptr;
+  return out;
+}
+
 }  // namespace
+
+std::unique_ptr<Ensemble> makeEnsemble(const std::vector<Engine*>& members, std::string& err) {
+  if (members.empty()) {
+    err = "empty ensemble";
+    return nullptr;
+  }
+  std::unique_ptr<Ensemble> e;
+  if (!(e = tryEnsemble<float, float>(members, err)) && err.empty() &&
+      !(e = tryEnsemble<_Float16, float>(members, err)) && err.empty() &&
+      !(e = tryEnsemble<__bf16, float>(members, err)) && err.empty() &&
+      !(e = tryEnsemble<double, double>(members, err)) && err.empty())
+    err = "ensemble members must share one population storage type";
+  return e;
+}
 
 std::unique_ptr<Engine> makeEngine(int nX, int nY, int storage, bool komega, std::string& err) {
   std::unique_ptr<Engine> e;

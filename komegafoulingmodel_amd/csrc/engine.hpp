@@ -43,9 +43,24 @@ class Engine {
   virtual int solverStats(SolverStats& s, std::string& err) = 0;
   virtual int synchronize(std::string& err) = 0;
   virtual int64_t fieldSize(int field) const = 0;
+  virtual int setLaunch(int block, int npt, int64_t pad, std::string& err) = 0;
+  virtual void setGraphs(bool on) = 0;
+  virtual int autotune(int steps, int& block, int& npt, int64_t& pad, double& ms,
+                       std::string& err) = 0;
+};
+
+class Ensemble {
+ public:
+  virtual ~Ensemble() = default;
+  virtual int step(int n, unsigned options, std::string& err) = 0;
+  virtual int timeSteps(int n, unsigned options, double& ms, std::string& err) = 0;
+  virtual int setBlock(int block, std::string& err) = 0;
+  virtual int synchronize(std::string& err) = 0;
+  virtual int64_t fluidNodes() const = 0;
 };
 
 // storage: KOFM_STORE_*
 std::unique_ptr<Engine> makeEngine(int nX, int nY, int storage, bool komega, std::string& err);
+std::unique_ptr<Ensemble> makeEnsemble(const std::vector<Engine*>& members, std::string& err);
 
 }  // namespace kofm

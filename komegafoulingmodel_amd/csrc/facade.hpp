@@ -1,0 +1,355 @@
+// facade.hpp -- the reference's host classes for the lattice-Boltzmann step
+// (clProblem, clVariablesLS, clVariablesLB with its kOmega member,
+// shearStress, Reducer), re-expressed over the C ABI of include/kofm.h.  A
+// host program written against the reference's call sequence
+// (LBLS.cpp:65-160 ini, 185-381 run) drives the MI355X path through these
+// classes; they own no device code and call nothing but kofm_*.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/kofm.h"
+#include "runparams.hpp"
+
+namespace kofm {
+namespace ref {
+
+inline void check(kofm_ctx* c, int rc, const char* what) {
+  if (rc != KOFM_OK)
+    throw std::runtime_error(std::string(what) + ": " + kofm_last_error(c) + " (code " +
+                             std::to_string(rc) + ")");
+}
+
+// clProblem::loadParams (clProblem.cpp:456-554): sizes and cadences.
+class clProblem {
+ public:
+  int nX = 0, nY = 0, XsizeFull = 0, Channel_Height = 0, DeviceID = 0;
+  double Pipe_radius = 0.;
+  unsigned StopTime = 0, Time = 0, RestartTime = 0;
+  unsigned trSteps_wall = 3;
+  unsigned saveAvgStartTime = 0, saveTimeStepAvg = 1000, dumpStartTime = 0, dumpTimeStep = 1000;
+  RunParams params;
+
+  void loadRunParameters(const std::string& path) {
+    if (!params.load(path)) throw std::runtime_error("cannot read " + path);
+    const int d = RunParams::kSystem;
+    nX = params.getInt(d, "nX", 0);
+    nY = params.getInt(d, "nY", 0);
+    if (nX <= 0 || nY <= 0) throw std::runtime_error("nX and nY are required");
+    XsizeFull = int(std::ceil(double(nX) / 128.)) * 128;
+    Pipe_radius = params.getDouble(d, "Pipe Radius", 0.5 * (nY - 4));
+    Channel_Height = 2 * int(Pipe_radius);
+    DeviceID = params.getInt(d, "Device ID", 0);
+    RestartTime = unsigned(params.getInt(d, "Current Time", 0));
+    Time = RestartTime;
+    StopTime = unsigned(params.getInt(d, "Stop Time", 200000000));
+    trSteps_wall = unsigned(params.getInt(d, "LB Steps Per TR Wall", 3));
+    cadence("Save Avgs", 2000, saveAvgStartTime, saveTimeStepAvg);
+    cadence("Dump", 1000, dumpStartTime, dumpTimeStep);
+  }
+
+ private:
+  // getSaveStepInfo (clProblem.cpp:430-454)
+  void cadence(const std::string& name, unsigned defNum, unsigned& start, unsigned& step) {
+    const int d = RunParams::kSystem;
+    start = unsigned(params.getInt(d, name + " Start Time", 0));
+    double num = 0., ts = 0.;
+    const int which = params.either(d, name + " Num Steps", name + " Timestep", num, ts);
+    if (which == 2)
+      step = unsigned(ts);
+    else
+      step = (StopTime - start) / unsigned(which == 1 ? num : defNum);
+    if (step == 0) step = 1;
+  }
+};
+
+// Reducer<...>::reduceSingle / reduce / appendToFileFromDevice
+// (Reducer.h:88-290) for one field and operator, with the TimeData text
+// file "<name>.txt" (time, value rows; Array.h:2854-2904).
+class Reducer {
+ public:
+  Reducer() = default;
+  void ini(kofm_ctx* ctx, int field, int op, const std::string& name) {
+    ctx_ = ctx;
+    field_ = field;
+    op_ = op;
+    name_ = name;
+  }
+  double reduceSingle() {
+    double v = 0.;
+    check(ctx_, kofm_reduce(ctx_, field_, op_, &v), "reduce");
+    return v;
+  }
+  void reduce(unsigned time) { rows_.push_back({double(time), reduceSingle()}); }
+  void appendToFileFromDevice(const std::string& dir = ".") {
+    std::ofstream out(dir + "/" + name_ + ".txt", std::ios::app);
+    for (auto& r : rows_) out << r[0] << "\t" << r[1] << "\n";
+    rows_.clear();
+  }
+
+ private:
+  kofm_ctx* ctx_ = nullptr;
+  int field_ = 0, op_ = KOFM_RED_SUM;
+  std::string name_;
+  std::vector<std::vector<double>> rows_;
+};
+
+// clVariablesLS (clVariablesLS.cpp:487-620): wall polyline in, node types
+// and boundary lists built (here by kofm_build_geometry).
+class clVariablesLS {
+ public:
+  int nN = 0, nBL = 0;
+  std::vector<double> C0;
+  std::vector<int32_t> BL;
+  double xReleasePos = 90., xMinVal = 88.75, xStopPos = 1900., cutoffRadius = 5.;
+  int indRadiusSearch = 30, wallDSearchRad = 50;
+  double minDxDist = 0.001;
+
+  void loadParams(const clProblem& p, const std::string& loadDir = "load") {
+    const RunParams& rp = p.params;
+    nN = rp.getInt(RunParams::kLevelSet, "nN", 0);
+    nBL = nN - 2;
+    // geometry inputs that the reference reads in other classes
+    xReleasePos = rp.getDouble(RunParams::kParticles, "X Release Pos", 90.);
+    xMinVal = rp.getDouble(RunParams::kParticles, "X Min Val", xReleasePos - 1.25);
+    xStopPos = rp.getDouble(RunParams::kParticles, "X Stop Pos", 1900.);
+    cutoffRadius = rp.getDouble(RunParams::kParticles, "Cutoff Radius", 5.);
+    indRadiusSearch = rp.getInt(RunParams::kParticles, "Index Radius Search", 30);
+    wallDSearchRad = rp.getInt(RunParams::kFluid, "Wall D Search Rad", 50);
+    minDxDist = rp.getDouble(RunParams::kFouling, "Min dX Dist", 0.001);
+    // load/lsc0.txt: "x y" rows; load/lsbl.txt: index pairs (allocateArrays, :24-51)
+    C0 = readTable<double>(loadDir + "/lsc0.txt", 2);
+    BL = readTable<int32_t>(loadDir + "/lsbl.txt", 2);
+    if (nN == 0) nN = int(C0.size() / 2);
+    if (int(C0.size() / 2) != nN) throw std::runtime_error("lsc0.txt does not hold nN points");
+    nBL = int(BL.size() / 2);
+  }
+
+  void ini(kofm_ctx* ctx, const clProblem& p) {
+    kofm_geom_params gp{};
+    gp.pipe_radius = p.Pipe_radius;
+    gp.eps = 1e-5;
+    gp.min_dx = minDxDist;
+    gp.wall_d_search_rad = wallDSearchRad;
+    gp.x_release_pos = xReleasePos;
+    gp.x_min_val = xMinVal;
+    gp.x_stop_pos = xStopPos;
+    gp.cutoff_radius = cutoffRadius;
+    gp.index_radius_search = indRadiusSearch;
+    check(ctx, kofm_build_geometry(ctx, C0.data(), nN, BL.data(), nBL, &gp), "geometry");
+  }
+
+  template <class T>
+  static std::vector<T> readTable(const std::string& path, int cols) {
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("cannot read " + path);
+    std::vector<T> v;
+    double x;
+    while (in >> x) v.push_back(T(x));
+    if (v.size() % cols) throw std::runtime_error(path + " has a ragged last row");
+    return v;
+  }
+};
+
+// kOmega (kOmega.h:223-330): parameters and initial values; its transport
+// update runs inside kofm_step.
+class kOmega {
+ public:
+  bool kOmegaSolverFlag = false;
+  double kappaMaxRelTol = 1e-4, kappaMaxAbsTol = 1e-7, omegaMaxRelTol = 1e-4, omegaMaxAbsTol = 1e-7;
+  int kappaMaxIters = 1000, omegaMaxIters = 1000;
+  double maxTurbVisc = 10., kIniVal = -1., omegaIniVal = -1.;
+
+  // kOmega::loadParams (kOmega.cpp:352-378).  The reference reads the omega
+  // initial value from the key "K Initial Val" (kOmega.cpp:376); here it is
+  // read from "Omega Initial Val", the key kOmega::saveParams writes.
+  void loadParams(const RunParams& rp) {
+    const int d = RunParams::kFluid;
+    kOmegaSolverFlag = rp.getBool(d, "Use Turb Model", true);
+    kappaMaxRelTol = rp.getDouble(d, "Kappa Max Rel Tol", 1e-4);
+    kappaMaxAbsTol = rp.getDouble(d, "Kappa Max Abs Tol", 1e-7);
+    kappaMaxIters = rp.getInt(d, "Kappa Max Iterations", 1000);
+    omegaMaxRelTol = rp.getDouble(d, "Omega Max Rel Tol", kappaMaxRelTol);
+    omegaMaxAbsTol = rp.getDouble(d, "Omega Max Abs Tol", kappaMaxAbsTol);
+    omegaMaxIters = rp.getInt(d, "Omega Max Iterations", kappaMaxIters);
+    maxTurbVisc = rp.getDouble(d, "Max Turb Visc", 10.);
+    kIniVal = rp.getDouble(d, "K Initial Val", -1.);
+    omegaIniVal = rp.getDouble(d, "Omega Initial Val", -1.);
+  }
+  void ini(kofm_ctx* ctx, double Re) {
+    if (kOmegaSolverFlag) check(ctx, kofm_init_komega(ctx, kIniVal, omegaIniVal, Re), "k-omega init");
+  }
+  void Solve() {}  // fused into clVariablesLB::Solve
+};
+
+// clVariablesLB (clVariablesLB.h:40-458): fluid parameters, initial state,
+// the time step, output.
+class clVariablesLB {
+ public:
+  double tau = 1.99, MuVal = 0., Re = 10800., Fval = 0., RhoVal = 1.;
+  bool tauDefined = true, reDefined = true;
+  int tlbmIniDumpStep = 100000, tlbmNumIniSteps_LB = 1000000;
+  int storage = KOFM_STORE_F32;
+  kOmega kOmegaClass;
+  Reducer sumUx, sumUy, sumRo, sumKappa, sumOmega, minKappa, minOmega;
+  kofm_ctx* ctx = nullptr;
+
+  ~clVariablesLB() {
+    if (ctx) kofm_destroy(ctx);
+  }
+
+  // clVariablesLB::loadParams (clVariablesLB.cpp:553-639)
+  void loadParams(const clProblem& p) {
+    const RunParams& rp = p.params;
+    const int d = RunParams::kFluid;
+    double mu = 0., t = 0.;
+    switch (rp.either(d, "Mu", "Tau", mu, t)) {
+      case 1: MuVal = mu; tau = 1. / (MuVal * 3. + 0.5); tauDefined = false; break;
+      case 2: tau = t; MuVal = (1. / tau - 0.5) / 3.; break;
+      default: tau = 1.99; MuVal = (1. / tau - 0.5) / 3.; break;
+    }
+    double re = 0., fv = 0.;
+    const double R = p.Pipe_radius;
+    switch (rp.either(d, "Re", "Fval", re, fv)) {
+      case 1: Re = re; Fval = Re * 3. * MuVal * MuVal / R / R / R; break;
+      case 2: Fval = fv; Re = Fval * R * R * R / 3. / MuVal / MuVal; reDefined = false; break;
+      default: Re = 10800.; Fval = Re * 3. * MuVal * MuVal / R / R / R; break;
+    }
+    RhoVal = rp.getDouble(d, "Rho", 1.);
+    tlbmIniDumpStep = rp.getInt(d, "Ini Dump Step", 100000);
+    tlbmNumIniSteps_LB = rp.getInt(d, "Ini LB Steps", 1000000);
+    const std::string st = rp.getString(d, "Population Storage", "f32");
+    storage = st == "f16" ? KOFM_STORE_F16 : st == "bf16" ? KOFM_STORE_BF16 : st == "f64" ? KOFM_STORE_F64 : KOFM_STORE_F32;
+    kOmegaClass.loadParams(rp);
+  }
+
+  // allocation + geometry + initial state (clVariablesLB::ini, :307-365);
+  // restart from load/lbf.bin (+ lbkappa.bin, lbomega.bin) when present
+  void ini(const clProblem& p, clVariablesLS& ls, const std::string& loadDir = "load") {
+    kofm_grid g{p.nX, p.nY, storage, kOmegaClass.kOmegaSolverFlag ? 1 : 0};
+    if (kofm_create(&g, &ctx) != KOFM_OK) throw std::runtime_error(kofm_last_error(nullptr));
+    kofm_params q{};
+    q.tau0 = tau;
+    q.Fval = Fval;
+    q.nu = MuVal;
+    q.rho0 = RhoVal;
+    q.max_turb_visc = kOmegaClass.maxTurbVisc;
+    q.ko_dt = 1.;
+    q.kappa_rel_tol = kOmegaClass.kappaMaxRelTol;
+    q.kappa_abs_tol = kOmegaClass.kappaMaxAbsTol;
+    q.kappa_max_iters = kOmegaClass.kappaMaxIters;
+    q.omega_rel_tol = kOmegaClass.omegaMaxRelTol;
+    q.omega_abs_tol = kOmegaClass.omegaMaxAbsTol;
+    q.omega_max_iters = kOmegaClass.omegaMaxIters;
+    check(ctx, kofm_set_params(ctx, &q), "params");
+    ls.ini(ctx, p);
+    const size_t D = size_t(p.XsizeFull) * p.nY;
+    std::vector<double> f;
+    if (readBin(loadDir + "/lbf.bin", 9 * D, f)) {
+      std::vector<double> k, w;
+      const bool ko = kOmegaClass.kOmegaSolverFlag && readBin(loadDir + "/lbkappa.bin", D, k) &&
+                      readBin(loadDir + "/lbomega.bin", D, w);
+      check(ctx, kofm_upload_state(ctx, f.data(), ko ? k.data() : nullptr, ko ? w.data() : nullptr,
+                                   nullptr, nullptr), "restart");
+    } else {
+      check(ctx, kofm_init_state(ctx, nullptr, nullptr, nullptr), "iniDists");
+      kOmegaClass.ini(ctx, Re);
+    }
+    sumUx.ini(ctx, KOFM_FIELD_UX, KOFM_RED_SUM, "redUx");
+    sumUy.ini(ctx, KOFM_FIELD_UY, KOFM_RED_SUM, "redUy");
+    sumRo.ini(ctx, KOFM_FIELD_RHO, KOFM_RED_SUM, "redRo");
+    if (kOmegaClass.kOmegaSolverFlag) {
+      sumKappa.ini(ctx, KOFM_FIELD_KAPPA, KOFM_RED_SUM, "redKappa");
+      sumOmega.ini(ctx, KOFM_FIELD_OMEGA, KOFM_RED_SUM, "redOmega");
+      minKappa.ini(ctx, KOFM_FIELD_KAPPA, KOFM_RED_MIN, "minKappa");
+      minOmega.ini(ctx, KOFM_FIELD_OMEGA, KOFM_RED_MIN, "minOmega");
+    }
+  }
+
+  // clVariablesLB::Solve (clVariablesLB.h:375-441)
+  void Solve(int n = 1) { check(ctx, kofm_step(ctx, n, KOFM_OPTION_SAVE_MACRO_FIELDS), "step"); }
+
+  double calcUmean() { return sumUx.reduceSingle() / sumRo.reduceSingle(); }
+
+  // warm-up loop (clVariablesLB::getInitialFlow, :170-232)
+  void getInitialFlow(const clProblem& p, const std::string& outDir) {
+    int time = 0;
+    const double sizedom = double(p.nX) * double(p.Channel_Height);
+    while (time < tlbmNumIniSteps_LB) {
+      const int n = std::min(1000 - time % 1000, tlbmNumIniSteps_LB - time);
+      Solve(n);
+      time += n;
+      if (time % tlbmIniDumpStep == 0) save2file(outDir);
+      if (time % 1000 == 0) {
+        const double ro = sumRo.reduceSingle();
+        const double ux = sumUx.reduceSingle() / ro;
+        if (kOmegaClass.kOmegaSolverFlag)
+          std::printf("Time = %d, Umean = %g, K = %g, Omega = %g, Re = %g\n", time, ux,
+                      sumKappa.reduceSingle() / ro, sumOmega.reduceSingle() / ro,
+                      ux * p.Pipe_radius / MuVal);
+        else
+          std::printf("Time = %d, Umean = %g, Re = %g\n", time, ux, ux * p.Pipe_radius / MuVal);
+        (void)sizedom;
+      }
+    }
+    save2file(outDir);
+  }
+
+  void save2file(const std::string& dir) { check(ctx, kofm_save2file(ctx, dir.c_str()), "save2file"); }
+  void saveRestartFiles(const std::string& dir) { save2file(dir); }
+
+  void updateTimeData(unsigned time) {
+    sumUx.reduce(time);
+    sumUy.reduce(time);
+    sumRo.reduce(time);
+    if (kOmegaClass.kOmegaSolverFlag) {
+      sumKappa.reduce(time);
+      sumOmega.reduce(time);
+    }
+  }
+  void saveTimeData(const std::string& dir) {
+    for (Reducer* r : {&sumUx, &sumUy, &sumRo}) r->appendToFileFromDevice(dir);
+    if (kOmegaClass.kOmegaSolverFlag)
+      for (Reducer* r : {&sumKappa, &sumOmega}) r->appendToFileFromDevice(dir);
+  }
+
+ private:
+  static bool readBin(const std::string& path, size_t n, std::vector<double>& v) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    v.assign(n, 0.);
+    const size_t got = std::fread(v.data(), sizeof(double), n, f);
+    std::fclose(f);
+    return got == n;
+  }
+};
+
+// shearStress (shearStress.h:189-290): node and boundary-link wall shear.
+class shearStress {
+ public:
+  void calculateShear(kofm_ctx* ctx) { check(ctx, kofm_wall_shear(ctx), "shear"); }
+  std::vector<double> blTau(kofm_ctx* ctx) {
+    kofm_geom_counts c;
+    check(ctx, kofm_geometry_counts(ctx, &c), "counts");
+    std::vector<double> t(size_t(std::max(c.nBLshear, 1)));
+    check(ctx, kofm_download(ctx, KOFM_FIELD_TAU_BL, t.data()), "shear download");
+    t.resize(size_t(c.nBLshear));
+    return t;
+  }
+  // TimeData row "time\ttau_0\ttau_1...\n" in ShearStress.txt
+  void updateTimeData(kofm_ctx* ctx, unsigned time, const std::string& dir) {
+    std::ofstream out(dir + "/ShearStress.txt", std::ios::app);
+    out << time;
+    for (double v : blTau(ctx)) out << "\t" << v;
+    out << "\n";
+  }
+};
+
+}  // namespace ref
+}  // namespace kofm

@@ -83,7 +83,8 @@ __device__ __forceinline__ void storeDev(S* __restrict__ p, size_t idx, int dir,
 template <class R>
 struct LBParams {
   int nX, nY, XF;
-  int64_t D;
+  int64_t D;  // nodes per plane of the macro / k-omega fields (XF * nY)
+  int64_t P;  // population plane stride (D + padding, see engine.hip)
   R omega0; // laminar relaxation frequency (reference tau0)
   R Fval;   // forcing velocity increment (FTERM_VAL)
   R nu;     // MU_NUMBER
@@ -135,7 +136,7 @@ __device__ __forceinline__ void ibbCorrect(R (&g)[9], const S* __restrict__ src,
       int sx = x - cx(d);
       sx = sx >= p.nX ? sx - p.nX : (sx < 0 ? sx + p.nX : sx);
       const int sy = y - cy(d);
-      const R up = loadDev<S, R>(src, size_t(sx) + size_t(p.XF) * sy + size_t(p.D) * j, j);
+      const R up = loadDev<S, R>(src, size_t(sx) + size_t(p.XF) * sy + size_t(p.P) * j, j);
       nv[j] = R(1) / tq * (g[j] + (tq - R(1)) * up);
     }
   }
@@ -148,7 +149,7 @@ __device__ __forceinline__ void loadNode(R (&g)[9], const S* __restrict__ src, s
                                          int y, uint32_t m, const int32_t* __restrict__ ibbIdx,
                                          const R* __restrict__ ibbQ, const LBParams<R>& p) {
 #pragma unroll
-  for (int i = 0; i < 9; i++) g[i] = loadDev<S, R>(src, gi + size_t(p.D) * i, i);
+  for (int i = 0; i < 9; i++) g[i] = loadDev<S, R>(src, gi + size_t(p.P) * i, i);
   if (p.applyIbb && (m & kIbbNode)) ibbCorrect<S, R>(g, src, x, y, ibbQ + 8 * size_t(ibbIdx[gi]), p);
 }
 
@@ -161,24 +162,14 @@ struct KOFields {
   void* S; // 3 planes: Sxx, Sxy, Syy
 };
 
-// One step of LB_collision_SRT_Fluid (lbKernels.cl:175-335) or, with KO,
-// LB_collision_SRT_Fluid_w_kOmega (:345-633): read + lazy IBB, moments,
-// optional macros, EDM-forced BGK, halfway bounce-back, push streaming.
-template <class S, class R, bool KO>
-__global__ __launch_bounds__(kBlockX) void lbCollideStream(
-    const S* __restrict__ src, S* __restrict__ dst, const uint16_t* __restrict__ mask,
-    const int32_t* __restrict__ ibbIdx, const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
-    R* __restrict__ uxOut, R* __restrict__ uyOut, KOFields ko, LBParams<R> p) {
-  const int x = blockIdx.x * kBlockX + threadIdx.x;
-  const int y = blockIdx.y;
-  if (x >= p.nX) return;
-  const size_t gi = size_t(x) + size_t(p.XF) * y;
-  const uint32_t m = mask[gi];
-  if (m & kSolid) return;
-
-  R g[9];
-  loadNode<S, R>(g, src, gi, x, y, m, ibbIdx, ibbQ, p);
-
+// Collision of one node in place (pre-collision deviations in, post-
+// collision deviations out): moments, optional macro output, with KO the
+// SST eddy viscosity and strain, then EDM-forced BGK.  Shared by the one-
+// and two-node-per-lane kernels.
+template <class R, bool KO>
+__device__ __forceinline__ void collideNode(R (&g)[9], size_t gi, R* __restrict__ rhoOut,
+                                            R* __restrict__ uxOut, R* __restrict__ uyOut,
+                                            const KOFields& ko, const LBParams<R>& p) {
   const R drho = ((g[0] + g[1]) + (g[2] + g[3])) + ((g[4] + g[5]) + (g[6] + g[7])) + g[8];
   const R rho = R(1) + drho;
   const R inv = R(1) / rho;
@@ -187,42 +178,53 @@ __global__ __launch_bounds__(kBlockX) void lbCollideStream(
 
   R omega = p.omega0;
   if constexpr (KO) {
-    R* K = static_cast<R*>(ko.K);
-    R* O = static_cast<R*>(ko.O);
-    const R* Wd = static_cast<const R*>(ko.wallD);
-    R* Nut = static_cast<R*>(ko.nut);
-    R* Sx = static_cast<R*>(ko.S);
-    R om = O[gi], k = K[gi];
-    const R yw = Wd[gi];
-    if (om < R(0)) {  // clamps written back (lbKernels.cl:401-410)
+    // k-omega fields and arithmetic are FP64 in every form: the reference's
+    // near-wall omega rows are ill-conditioned and amplify FP32 rounding
+    using K = double;
+    K* Kf = static_cast<K*>(ko.K);
+    K* O = static_cast<K*>(ko.O);
+    const K* Wd = static_cast<const K*>(ko.wallD);
+    K* Nut = static_cast<K*>(ko.nut);
+    K* Sx = static_cast<K*>(ko.S);
+    K om = O[gi], k = Kf[gi];
+    const K yw = Wd[gi];
+    if (om < K(0)) {  // clamps written back (lbKernels.cl:401-410)
       om = -om;
       O[gi] = om;
     }
-    if (k < R(0)) {
-      k = R(1e-16);
-      K[gi] = k;
+    if (k < K(0)) {
+      k = K(1e-16);
+      Kf[gi] = k;
     }
-    const R a = R(2) * sqrt(k) / (R(0.09) * om * yw);
-    const R b = R(500) * p.nu / (yw * yw * om);
-    const R arg2 = a > b ? a : b;
-    R F2 = tanh(arg2 * arg2);
-    const R sxx = Sx[gi], sxy = Sx[gi + p.D], syy = Sx[gi + 2 * p.D];
-    F2 *= sqrt(R(2) * (sxx * sxx + R(2) * sxy * sxy + syy * syy));
-    const R den = R(0.31) * om > F2 ? R(0.31) * om : F2;
-    R nt = R(0.31) * k / den;
-    nt = nt < p.maxNut ? nt : p.maxNut;
-    nt = nt > R(1e-16) ? nt : R(1e-16);
+    const K nu = K(p.nu);
+    const K a = K(2) * sqrt(k) / (K(0.09) * om * yw);
+    const K b = K(500) * nu / (yw * yw * om);
+    const K arg2 = a > b ? a : b;
+    K F2 = tanh(arg2 * arg2);
+    const K sxx = Sx[gi], sxy = Sx[gi + p.D], syy = Sx[gi + 2 * p.D];
+    F2 *= sqrt(K(2) * (sxx * sxx + K(2) * sxy * sxy + syy * syy));
+    const K den = K(0.31) * om > F2 ? K(0.31) * om : F2;
+    K nt = K(0.31) * k / den;
+    nt = nt < K(p.maxNut) ? nt : K(p.maxNut);
+    nt = nt > K(1e-16) ? nt : K(1e-16);
     Nut[gi] = nt;
-    omega = R(1) / (R(3) * (nt + p.nu) + R(0.5));
+    omega = R(K(1) / (K(3) * (nt + nu) + K(0.5)));
     // strain rate from the pre-collision non-equilibrium with the laminar
-    // relaxation frequency (lbKernels.cl:518-583)
-    R ne[9];
+    // relaxation frequency (lbKernels.cl:518-583), in FP64
+    K gd[9];
 #pragma unroll
-    for (int i = 1; i < 9; i++) ne[i] = g[i] - geq<R>(i, drho, rho, ux, uy);
-    const R c = R(-1.5) * p.omega0 * inv;
-    const R exx = ((ne[1] + ne[2]) + (ne[5] + ne[6])) + (ne[7] + ne[8]);
-    const R eyy = ((ne[3] + ne[4]) + (ne[5] + ne[6])) + (ne[7] + ne[8]);
-    const R exy = (ne[5] + ne[6]) - (ne[7] + ne[8]);
+    for (int i = 0; i < 9; i++) gd[i] = K(g[i]);
+    const K dr = ((gd[0] + gd[1]) + (gd[2] + gd[3])) + ((gd[4] + gd[5]) + (gd[6] + gd[7])) + gd[8];
+    const K rd = K(1) + dr;
+    const K uxd = (((gd[1] - gd[2]) + (gd[5] - gd[6])) + (gd[7] - gd[8])) / rd;
+    const K uyd = (((gd[3] - gd[4]) + (gd[5] - gd[6])) - (gd[7] - gd[8])) / rd;
+    K ne[9];
+#pragma unroll
+    for (int i = 1; i < 9; i++) ne[i] = gd[i] - geq<K>(i, dr, rd, uxd, uyd);
+    const K c = K(-1.5) * K(p.omega0) / rd;
+    const K exx = ((ne[1] + ne[2]) + (ne[5] + ne[6])) + (ne[7] + ne[8]);
+    const K eyy = ((ne[3] + ne[4]) + (ne[5] + ne[6])) + (ne[7] + ne[8]);
+    const K exy = (ne[5] + ne[6]) - (ne[7] + ne[8]);
     Sx[gi] = c * exx;
     Sx[gi + p.D] = c * exy;
     Sx[gi + 2 * p.D] = c * eyy;
@@ -235,21 +237,185 @@ __global__ __launch_bounds__(kBlockX) void lbCollideStream(
   }
 
   // exact-difference forcing: g += -w (g - geq(u)) + feq(u+F) - feq(u)
-  const int xe = x < p.nX - 1 ? x + 1 : 0;
-  const int xw = x > 0 ? x - 1 : p.nX - 1;
 #pragma unroll
   for (int i = 0; i < 9; i++) {
     const int a = cx(i), b = cy(i);
     const R gq = geq<R>(i, drho, rho, ux, uy);
     const R force = rho * (w1<R>(b) + dphi(b, uy)) * dphiShift(a, ux, p.Fval);
-    const R gn = g[i] - omega * (g[i] - gq) + force;
+    g[i] = g[i] - omega * (g[i] - gq) + force;
+  }
+}
+
+// One step of LB_collision_SRT_Fluid (lbKernels.cl:175-335) or, with KO,
+// LB_collision_SRT_Fluid_w_kOmega (:345-633): read + lazy IBB, collision,
+// halfway bounce-back into the node's own reversed slot, push streaming.
+template <class S, class R, bool KO, int BX>
+__global__ __launch_bounds__(BX) void lbCollideStream(
+    const S* __restrict__ src, S* __restrict__ dst, const uint16_t* __restrict__ mask,
+    const int32_t* __restrict__ ibbIdx, const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
+    R* __restrict__ uxOut, R* __restrict__ uyOut, KOFields ko, LBParams<R> p) {
+  const int x = blockIdx.x * BX + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= p.nX) return;
+  const size_t gi = size_t(x) + size_t(p.XF) * y;
+  const uint32_t m = mask[gi];
+  if (m & kSolid) return;
+
+  R g[9];
+  loadNode<S, R>(g, src, gi, x, y, m, ibbIdx, ibbQ, p);
+  collideNode<R, KO>(g, gi, rhoOut, uxOut, uyOut, ko, p);
+
+  const int xe = x < p.nX - 1 ? x + 1 : 0;
+  const int xw = x > 0 ? x - 1 : p.nX - 1;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const int a = cx(i), b = cy(i);
     if (i == 0) {
-      storeDev<S, R>(dst, gi, 0, gn);
+      storeDev<S, R>(dst, gi, 0, g[0]);
     } else if (m & boundBit(i)) {
-      storeDev<S, R>(dst, gi + size_t(p.D) * rev(i), rev(i), gn);
+      storeDev<S, R>(dst, gi + size_t(p.P) * rev(i), rev(i), g[i]);
     } else {
       const int tx = a > 0 ? xe : (a < 0 ? xw : x);
-      storeDev<S, R>(dst, size_t(tx) + size_t(p.XF) * (y + b) + size_t(p.D) * i, i, gn);
+      storeDev<S, R>(dst, size_t(tx) + size_t(p.XF) * (y + b) + size_t(p.P) * i, i, g[i]);
+    }
+  }
+}
+
+template <class S>
+struct alignas(2 * sizeof(S)) Pair {
+  S a, b;
+};
+
+// Two neighbouring nodes (x, x+1) per lane, x even, nX even: the nine
+// population planes are read as two-element vectors, and a store is paired
+// whenever both nodes' targets are adjacent and aligned (rest, N and S
+// pushes, and bounce-back taken by both nodes).
+template <class S, class R, bool KO, int BX>
+__global__ __launch_bounds__(BX) void lbCollideStream2(
+    const S* __restrict__ src, S* __restrict__ dst, const uint16_t* __restrict__ mask,
+    const int32_t* __restrict__ ibbIdx, const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
+    R* __restrict__ uxOut, R* __restrict__ uyOut, KOFields ko, LBParams<R> p) {
+  const int x0 = 2 * (blockIdx.x * BX + threadIdx.x);
+  const int y = blockIdx.y;
+  if (x0 >= p.nX) return;
+  const size_t g0 = size_t(x0) + size_t(p.XF) * y;
+  const uint32_t mm = *reinterpret_cast<const uint32_t*>(mask + g0);
+  const uint32_t m[2] = {mm & 0xFFFFu, mm >> 16};
+  const bool live[2] = {!(m[0] & kSolid), !(m[1] & kSolid)};
+  if (!live[0] && !live[1]) return;
+
+  R g[2][9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const Pair<S> v = *reinterpret_cast<const Pair<S>*>(src + g0 + size_t(p.P) * i);
+    R a = static_cast<R>(v.a), b = static_cast<R>(v.b);
+    if constexpr (Scaled<S>::value) {
+      a *= R(wgt(i) / 1024.);
+      b *= R(wgt(i) / 1024.);
+    }
+    g[0][i] = a;
+    g[1][i] = b;
+  }
+#pragma unroll
+  for (int n = 0; n < 2; n++) {
+    if (!live[n]) continue;
+    if (p.applyIbb && (m[n] & kIbbNode))
+      ibbCorrect<S, R>(g[n], src, x0 + n, y, ibbQ + 8 * size_t(ibbIdx[g0 + n]), p);
+    collideNode<R, KO>(g[n], g0 + n, rhoOut, uxOut, uyOut, ko, p);
+  }
+
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const int a = cx(i), b = cy(i);
+    size_t t[2];
+    int slot[2];
+#pragma unroll
+    for (int n = 0; n < 2; n++) {
+      const int x = x0 + n;
+      if (i != 0 && (m[n] & boundBit(i))) {
+        t[n] = g0 + n + size_t(p.P) * rev(i);
+        slot[n] = rev(i);
+      } else {
+        int tx = x + a;
+        tx = tx >= p.nX ? tx - p.nX : (tx < 0 ? tx + p.nX : tx);
+        t[n] = size_t(tx) + size_t(p.XF) * (y + b) + size_t(p.P) * i;
+        slot[n] = i;
+      }
+    }
+    if (live[0] && live[1] && t[1] == t[0] + 1 && !(t[0] & 1)) {
+      R va = g[0][i], vb = g[1][i];
+      if constexpr (Scaled<S>::value) {
+        va *= R(1024. / wgt(slot[0]));
+        vb *= R(1024. / wgt(slot[1]));
+      }
+      Pair<S> v;
+      v.a = static_cast<S>(va);
+      v.b = static_cast<S>(vb);
+      *reinterpret_cast<Pair<S>*>(dst + t[0]) = v;
+    } else {
+#pragma unroll
+      for (int n = 0; n < 2; n++)
+        if (live[n]) storeDev<S, R>(dst, t[n], slot[n], g[n][i]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ensemble of independent channels in one launch
+// ---------------------------------------------------------------------------
+
+// Everything one channel's collide-stream launch needs.  Tiles of BX nodes
+// (one row segment each) are numbered channel after channel; tileStart[c]
+// is the first tile of channel c.
+template <class S, class R>
+struct BatchEntry {
+  const S* src;
+  S* dst;
+  const uint16_t* mask;
+  const int32_t* ibbIdx;
+  const R* ibbQ;
+  R* rho;
+  R* ux;
+  R* uy;
+  LBParams<R> p;
+  int tilesX;
+  int tileStart;
+};
+
+template <class S, class R, int BX>
+__global__ __launch_bounds__(BX) void lbCollideStreamBatch(const BatchEntry<S, R>* __restrict__ ents,
+                                                           int n) {
+  const int T = blockIdx.x;
+  int lo = 0, hi = n - 1;  // last channel whose first tile is <= T
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ents[mid].tileStart <= T) lo = mid; else hi = mid - 1;
+  }
+  const BatchEntry<S, R>& e = ents[lo];
+  const LBParams<R>& p = e.p;
+  const int t = T - e.tileStart;
+  const int y = t / e.tilesX;
+  const int x = (t - y * e.tilesX) * BX + threadIdx.x;
+  if (x >= p.nX || y >= p.nY) return;
+  const size_t gi = size_t(x) + size_t(p.XF) * y;
+  const uint32_t m = e.mask[gi];
+  if (m & kSolid) return;
+  R g[9];
+  loadNode<S, R>(g, e.src, gi, x, y, m, e.ibbIdx, e.ibbQ, p);
+  KOFields none{nullptr, nullptr, nullptr, nullptr, nullptr};
+  collideNode<R, false>(g, gi, e.rho, e.ux, e.uy, none, p);
+  const int xe = x < p.nX - 1 ? x + 1 : 0;
+  const int xw = x > 0 ? x - 1 : p.nX - 1;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const int a = cx(i), b = cy(i);
+    if (i == 0) {
+      storeDev<S, R>(e.dst, gi, 0, g[0]);
+    } else if (m & boundBit(i)) {
+      storeDev<S, R>(e.dst, gi + size_t(p.P) * rev(i), rev(i), g[i]);
+    } else {
+      const int tx = a > 0 ? xe : (a < 0 ? xw : x);
+      storeDev<S, R>(e.dst, size_t(tx) + size_t(p.XF) * (y + b) + size_t(p.P) * i, i, g[i]);
     }
   }
 }
@@ -288,7 +454,7 @@ __global__ __launch_bounds__(kBlockX) void lbImportF(const double* __restrict__ 
   if (x >= p.XF) return;
   const size_t gi = size_t(x) + size_t(p.XF) * y;
 #pragma unroll
-  for (int i = 0; i < 9; i++) storeDev<S, R>(dst, gi + size_t(p.D) * i, i, R(in[gi + size_t(p.D) * i] - wgt(i)));
+  for (int i = 0; i < 9; i++) storeDev<S, R>(dst, gi + size_t(p.P) * i, i, R(in[gi + size_t(p.D) * i] - wgt(i)));
 }
 
 // Macros from populations (Calc_Ro_J lbKernels.cl:137-164, used on upload).
@@ -304,7 +470,7 @@ __global__ __launch_bounds__(kBlockX) void lbMacros(const S* __restrict__ src,
   if (mask[gi] & kSolid) return;
   R g[9];
 #pragma unroll
-  for (int i = 0; i < 9; i++) g[i] = loadDev<S, R>(src, gi + size_t(p.D) * i, i);
+  for (int i = 0; i < 9; i++) g[i] = loadDev<S, R>(src, gi + size_t(p.P) * i, i);
   const R drho = ((g[0] + g[1]) + (g[2] + g[3])) + ((g[4] + g[5]) + (g[6] + g[7])) + g[8];
   const R r = R(1) + drho;
   rho[gi] = r;
@@ -355,21 +521,21 @@ __device__ __forceinline__ void fdCoeffs(R dp, R dm, R (&c)[6]) {
   c[5] = R(-2) / (dp * dm);
 }
 
-template <class R>
 struct KOParams {
   int nX, nY, XF;
   int64_t D;
-  R nu, dt;
+  double nu, dt;
   TileMap tiles;
 };
 
 // Update_kOmega_Diffusivities (kOmegaKernels.cl:7-95).
-template <class R>
+template <class M>
 __global__ __launch_bounds__(kBlockX) void koDiffusivities(
-    const uint16_t* __restrict__ mask, const R* __restrict__ K, const R* __restrict__ O,
-    const R* __restrict__ Nut, const R* __restrict__ Wd, R* __restrict__ Dk, R* __restrict__ Do,
-    R* __restrict__ F1a, R* __restrict__ dkdo, const R* __restrict__ rho,
-    const R* __restrict__ dX, KOParams<R> p) {
+    const uint16_t* __restrict__ mask, const double* __restrict__ K, const double* __restrict__ O,
+    const double* __restrict__ Nut, const double* __restrict__ Wd, double* __restrict__ Dk,
+    double* __restrict__ Do, double* __restrict__ F1a, double* __restrict__ dkdo,
+    const M* __restrict__ rho, const double* __restrict__ dX, KOParams p) {
+  using R = double;
   int bx, y;
   if (!tileOf(p.tiles, bx, y)) return;
   const int x = bx * kBlockX + threadIdx.x;
@@ -407,14 +573,15 @@ __global__ __launch_bounds__(kBlockX) void koDiffusivities(
 
 // Update_kOmega_Coeffs_Implicit (kOmegaKernels.cl:187-480): normalised
 // five-point rows; A holds E, W, N, S planes (diagonal 1).
-template <class R>
+template <class M>
 __global__ __launch_bounds__(kBlockX) void koCoeffs(
-    const uint16_t* __restrict__ mask, const R* __restrict__ K, const R* __restrict__ O,
-    const R* __restrict__ Nut, const R* __restrict__ Dk, const R* __restrict__ Do,
-    const R* __restrict__ F1a, const R* __restrict__ dkdo, const R* __restrict__ Sx,
-    const R* __restrict__ rho, const R* __restrict__ uxa, const R* __restrict__ uya,
-    const R* __restrict__ dX, const R* __restrict__ Wd, R* __restrict__ Ak, R* __restrict__ Ao,
-    R* __restrict__ bk, R* __restrict__ bo, KOParams<R> p) {
+    const uint16_t* __restrict__ mask, const double* __restrict__ K, const double* __restrict__ O,
+    const double* __restrict__ Nut, const double* __restrict__ Dk, const double* __restrict__ Do,
+    const double* __restrict__ F1a, const double* __restrict__ dkdo, const double* __restrict__ Sx,
+    const M* __restrict__ rho, const M* __restrict__ uxa, const M* __restrict__ uya,
+    const double* __restrict__ dX, const double* __restrict__ Wd, double* __restrict__ Ak,
+    double* __restrict__ Ao, double* __restrict__ bk, double* __restrict__ bo, KOParams p) {
+  using R = double;
   int bx, y;
   if (!tileOf(p.tiles, bx, y)) return;
   const int x = bx * kBlockX + threadIdx.x;
@@ -516,8 +683,7 @@ struct BicgVecs {
   Q* t[2];
 };
 
-template <class R>
-__device__ __forceinline__ void nbIndex(const KOParams<R>& p, int x, size_t g, size_t (&nb)[4]) {
+__device__ __forceinline__ void nbIndex(const KOParams& p, int x, size_t g, size_t (&nb)[4]) {
   nb[0] = x < p.nX - 1 ? g + 1 : g - x;        // E
   nb[1] = x > 0 ? g - 1 : g + (p.nX - 1);      // W
   nb[2] = g + p.XF;                            // N
@@ -552,7 +718,7 @@ __device__ __forceinline__ void blockPartials(double (&a)[2][2], double* __restr
 template <class R, class Q>
 __global__ __launch_bounds__(kBlockX) void bicgInit(const uint16_t* __restrict__ mask,
                                                     BicgVecs<R, Q> V, double* __restrict__ partial,
-                                                    KOParams<R> p) {
+                                                    KOParams p) {
   int bx = 0, y = 0;
   const bool live = tileOf(p.tiles, bx, y);
   const int x = bx * kBlockX + threadIdx.x;
@@ -676,7 +842,7 @@ __global__ __launch_bounds__(1024) void bicgScalars(const double* __restrict__ p
 template <class R, class Q>
 __global__ __launch_bounds__(kBlockX) void bicgPV(const uint16_t* __restrict__ mask,
                                                   BicgVecs<R, Q> V, const BicgState* __restrict__ st,
-                                                  double* __restrict__ partial, KOParams<R> p) {
+                                                  double* __restrict__ partial, KOParams p) {
   const int cur = st->cur;
   int bx = 0, y = 0;
   const bool live = tileOf(p.tiles, bx, y);
@@ -718,7 +884,7 @@ __global__ __launch_bounds__(kBlockX) void bicgPV(const uint16_t* __restrict__ m
 template <class R, class Q>
 __global__ __launch_bounds__(kBlockX) void bicgST(const uint16_t* __restrict__ mask,
                                                   BicgVecs<R, Q> V, const BicgState* __restrict__ st,
-                                                  double* __restrict__ partial, KOParams<R> p) {
+                                                  double* __restrict__ partial, KOParams p) {
   const int nxt = st->cur ^ 1;
   int bx = 0, y = 0;
   const bool live = tileOf(p.tiles, bx, y);
@@ -754,7 +920,7 @@ __global__ __launch_bounds__(kBlockX) void bicgST(const uint16_t* __restrict__ m
 template <class R, class Q>
 __global__ __launch_bounds__(kBlockX) void bicgXR(const uint16_t* __restrict__ mask,
                                                   BicgVecs<R, Q> V, const BicgState* __restrict__ st,
-                                                  double* __restrict__ partial, KOParams<R> p) {
+                                                  double* __restrict__ partial, KOParams p) {
   const int nxt = st->cur ^ 1;
   int bx = 0, y = 0;
   const bool live = tileOf(p.tiles, bx, y);
@@ -782,7 +948,7 @@ __global__ __launch_bounds__(kBlockX) void bicgXR(const uint16_t* __restrict__ m
 // Writes the iterate back to K / O.
 template <class R, class Q>
 __global__ __launch_bounds__(kBlockX) void bicgFinish(const uint16_t* __restrict__ mask,
-                                                      BicgVecs<R, Q> V, KOParams<R> p) {
+                                                      BicgVecs<R, Q> V, KOParams p) {
   int bx = 0, y = 0;
   if (!tileOf(p.tiles, bx, y)) return;
   const int x = bx * kBlockX + threadIdx.x;
@@ -821,7 +987,7 @@ __global__ __launch_bounds__(64) void shearNodes(
     const int32_t* __restrict__ ibbIdx, const R* __restrict__ ibbQ,
     const int32_t* __restrict__ ss, int nSS, const R* __restrict__ cfss,
     const R* __restrict__ rho, const R* __restrict__ uxa, const R* __restrict__ uya,
-    const R* __restrict__ nut, R* __restrict__ tau, LBParams<R> p) {
+    const double* __restrict__ nut, R* __restrict__ tau, LBParams<R> p) {
   const int n = blockIdx.x * 64 + threadIdx.x;
   if (n >= nSS) return;
   const int gi = ss[n];
@@ -829,7 +995,7 @@ __global__ __launch_bounds__(64) void shearNodes(
   R g[9];
   loadNode<S, R>(g, src, size_t(gi), x, y, mask[gi], ibbIdx, ibbQ, p);
   const R r = rho[gi], ux = uxa[gi], uy = uya[gi];
-  R ve = (nut ? nut[gi] : R(0)) + p.nu;
+  R ve = (nut ? R(nut[gi]) : R(0)) + p.nu;
   ve *= R(1) / (R(3) * ve + R(0.5));
   const R dr = r - R(1);
   R tx = R(0), ty = R(0);

@@ -41,7 +41,9 @@ EXPORTS = [
     "kofm_geometry_counts", "kofm_geometry_array", "kofm_init_state", "kofm_init_komega",
     "kofm_upload_state", "kofm_step", "kofm_time_steps", "kofm_wall_shear",
     "kofm_get_solver_stats", "kofm_reduce", "kofm_download", "kofm_save2file",
-    "kofm_synchronize",
+    "kofm_synchronize", "kofm_set_launch", "kofm_autotune", "kofm_set_graphs",
+    "kofm_ensemble_create", "kofm_ensemble_step", "kofm_ensemble_time_steps",
+    "kofm_ensemble_set_block", "kofm_ensemble_last_error", "kofm_ensemble_destroy",
 ]
 
 
@@ -122,6 +124,17 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "kofm_download": (C.c_int, [P, C.c_int32, dp]),
         "kofm_save2file": (C.c_int, [P, C.c_char_p]),
         "kofm_synchronize": (C.c_int, [P]),
+        "kofm_set_launch": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int64]),
+        "kofm_set_graphs": (C.c_int, [P, C.c_int32]),
+        "kofm_ensemble_create": (C.c_int, [C.POINTER(P), C.c_int32, C.POINTER(P)]),
+        "kofm_ensemble_step": (C.c_int, [P, C.c_int32, C.c_uint32]),
+        "kofm_ensemble_time_steps": (C.c_int, [P, C.c_int32, C.c_uint32, dp,
+                                               C.POINTER(C.c_int64)]),
+        "kofm_ensemble_set_block": (C.c_int, [P, C.c_int32]),
+        "kofm_ensemble_last_error": (C.c_char_p, [P]),
+        "kofm_ensemble_destroy": (None, [P]),
+        "kofm_autotune": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_int64), dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -255,5 +268,60 @@ class Lattice:
     def save2file(self, directory: str):
         self._check(self.lib.kofm_save2file(self.h, directory.encode()))
 
+    def set_launch(self, block_x: int, nodes_per_lane: int = 1, plane_pad: int = 1088):
+        self._check(self.lib.kofm_set_launch(self.h, block_x, nodes_per_lane, plane_pad))
+
+    def autotune(self, steps: int = 50):
+        """Picks the collide-stream work-group width, nodes per lane and plane
+        padding; returns (block_x, nodes_per_lane, plane_pad, ms_per_step)."""
+        b, n, pad, ms = C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()
+        self._check(self.lib.kofm_autotune(self.h, steps, C.byref(b), C.byref(n), C.byref(pad),
+                                           C.byref(ms)))
+        return b.value, n.value, pad.value, ms.value
+
+    def set_graphs(self, on: bool):
+        self._check(self.lib.kofm_set_graphs(self.h, 1 if on else 0))
+
     def synchronize(self):
         self._check(self.lib.kofm_synchronize(self.h))
+
+
+class Ensemble:
+    """Several Lattice objects (same storage, k-omega off) stepped together,
+    one launch per step."""
+
+    def __init__(self, members):
+        self.lib = load()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h for m in self.members])
+        h = C.c_void_p()
+        rc = self.lib.kofm_ensemble_create(arr, len(self.members), C.byref(h))
+        if rc != 0:
+            raise KofmError(rc, self.lib.kofm_ensemble_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise KofmError(rc, self.lib.kofm_ensemble_last_error(self.h).decode())
+
+    def step(self, n: int = 1, options: int = 0):
+        self._check(self.lib.kofm_ensemble_step(self.h, n, options))
+
+    def time_steps(self, n: int, options: int = 0):
+        ms, fl = C.c_double(), C.c_int64()
+        self._check(self.lib.kofm_ensemble_time_steps(self.h, n, options, C.byref(ms), C.byref(fl)))
+        return ms.value, fl.value
+
+    def set_block(self, block_x: int):
+        self._check(self.lib.kofm_ensemble_set_block(self.h, block_x))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.kofm_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

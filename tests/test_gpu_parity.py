@@ -30,8 +30,9 @@ def _mini_fouled():
 
 
 def _mini_wavy_ko():
-    nu, Re, H = 0.01, 2000.0, 40.0
-    return channel.wavy_channel(256, 56, H=H, amp=4.0, period=128.0, nu=nu, Re=Re,
+    # bulk velocity Re nu / H = 0.089 (lattice Mach 0.15)
+    nu, Re, H = 0.01, 500.0, 56.0
+    return channel.wavy_channel(256, 72, H=H, amp=4.0, period=128.0, nu=nu, Re=Re,
                                 Fval=channel.turbulent_forcing(H, nu, Re), seed=7)
 
 
@@ -112,7 +113,7 @@ def test_komega_fp32_matches_oracle():
     w = _mini_wavy_ko()
     L, orc = make_pair(w, "f32")
     m = fluid_mask(orc.nType)
-    for n in (1, 4):
+    for n in (1, 4, 15):
         L.step(n, 2)
         orc.step(n)
         st = L.solver_stats()
@@ -135,4 +136,57 @@ def test_komega_fp64_matches_oracle():
     for k, a in (("k", "K"), ("omega", "O"), ("nut", "nut"), ("ux", "ux")):
         e = rel_err(L.download(k)[m], getattr(orc, a)[m])
         print("ko f64", k, f"{e:.2e}", flush=True)
-        assert e < 1e-9, (k, e)
+        assert e < 1e-8, (k, e)  # FP64 solve floor against wall-dominated residuals
+
+
+def test_graph_replay_matches_eager_launches():
+    """Captured 32-step graphs give bit-identical populations to one launch
+    per step, replay after replay, with eager work and synchronisations in
+    between."""
+    w = _mini_fouled()
+    La, _ = make_pair(w, "f32")
+    Lb, _ = make_pair(w, "f32")
+    La.set_graphs(True)
+    Lb.set_graphs(False)
+    for rep in range(4):
+        La.step(64 + rep, 2)
+        Lb.step(64 + rep, 2)
+        La.synchronize()
+        fa, fb = La.download("f"), Lb.download("f")
+        assert np.array_equal(fa, fb), rep
+        assert np.array_equal(La.download("ux"), Lb.download("ux")), rep
+
+
+@pytest.mark.parametrize("block,npt,pad", [(128, 1, 0), (1024, 1, 64), (256, 2, 1088), (512, 2, 0)])
+def test_launch_shapes_give_identical_results(block, npt, pad):
+    """Every collide-stream launch shape (work-group width, one or two nodes
+    per lane, plane padding) produces the same populations bit for bit."""
+    w = _mini_fouled()
+    La, _ = make_pair(w, "f32")
+    Lb, _ = make_pair(w, "f32")
+    La.set_launch(512, 1, 1088)
+    Lb.set_launch(block, npt, pad)
+    La.step(40, 2)
+    Lb.step(40, 2)
+    assert np.array_equal(La.download("f"), Lb.download("f"))
+    assert np.array_equal(La.download("rho"), Lb.download("rho"))
+
+
+def test_ensemble_matches_individual_runs():
+    """Three channels of different sizes and fouling stepped by one ensemble
+    launch per step end bit-identical to the same channels stepped alone."""
+    from komegafoulingmodel_amd import lib
+    ws = [channel.fouled_channel(256, 48, Fval=1e-5, seed=31, amp=(0, 3), width=(10, 40)),
+          channel.fouled_channel(384, 64, Fval=2e-5, seed=32, amp=(0, 3), width=(10, 40)),
+          channel.flat_channel(128, 24, Fval=1e-5, seed=33)]
+    solo = [make_pair(w, "f32")[0] for w in ws]
+    group = [make_pair(w, "f32")[0] for w in ws]
+    ens = lib.Ensemble(group)
+    for n in (1, 70, 33):
+        ens.step(n, 2)
+        for L in solo:
+            L.step(n, 2)
+        for a, b in zip(group, solo):
+            assert np.array_equal(a.download("f"), b.download("f")), n
+            assert np.array_equal(a.download("ux"), b.download("ux")), n
+    ens.close()

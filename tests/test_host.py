@@ -105,3 +105,22 @@ def test_write_load_files(tmp_path):
     c = np.loadtxt(tmp_path / "lsc0.txt")
     b = np.loadtxt(tmp_path / "lsbl.txt", dtype=np.int64)
     assert np.array_equal(c, w.C0) and np.array_equal(b, w.BL)
+
+
+def test_runparam_yaml_through_the_facades():
+    """The shipped load/RunParam.yaml (kept as tests/golden/RunParam.yaml)
+    read by the C++ facades: Tau is the relaxation frequency and
+    Mu = (1/Tau - 0.5)/3, Fval = 3 Re Mu^2 / R^3 (clVariablesLB.cpp:566-600),
+    XsizeFull = ceil(nX/128)*128 (clProblem.cpp:469)."""
+    import subprocess
+    exe = os.path.join(ROOT, "komegafoulingmodel_amd", "kofm_run")
+    out = subprocess.run([exe, "--params", os.path.join(ROOT, "tests", "golden")],
+                         capture_output=True, text=True, check=True).stdout
+    v = {k: float(x) for k, x in (line.split() for line in out.strip().splitlines())}
+    tau, R, Re = 1.99445983, 100.0, 10800.0
+    mu = (1 / tau - 0.5) / 3
+    assert v["nX"] == 2000 and v["nY"] == 204 and v["XsizeFull"] == 2048
+    assert v["Channel_Height"] == 200 and v["Pipe_radius"] == R
+    assert v["tau0"] == tau and abs(v["MuVal"] - mu) < 1e-18
+    assert abs(v["Fval"] - Re * 3 * mu * mu / R ** 3) < 1e-22
+    assert v["UseTurbModel"] == 1 and v["trStepsWall"] == 3
