@@ -85,6 +85,9 @@ typedef struct kofm_ctx kofm_ctx;
 #define KOFM_GEOM_WEIGHTS 11  /* double[4*nBLs]  shearStress::ssWeights          */
 #define KOFM_GEOM_BOUNDS 12   /* int32 [4]       BL_BOT_START, NUM_BL_BOT, BL_TOP_START, NUM_BL_TOP */
 #define KOFM_GEOM_MASSES 13   /* double[2]       clVariablesLS::Masses           */
+/* no reference counterpart: sensitivities used by kofm_adjoint */
+#define KOFM_GEOM_IBB_SEG 14  /* int32 [nIbb]    wall segment (s, s+1) each link cuts */
+#define KOFM_GEOM_IBB_DQDY 15 /* double[2*nIbb]  dq/dy of wall points s and s+1     */
 
 typedef struct kofm_grid {
   int32_t nX;         /* "nX" */
@@ -269,6 +272,38 @@ int kofm_set_graphs(kofm_ctx* ctx, int32_t on);
 
 /* Blocks until all work on the context stream is done. */
 int kofm_synchronize(kofm_ctx* ctx);
+
+/* --- sensitivity to the wall shape ----------------------------------------- */
+
+/* J = mean over the last `window` steps of a run of
+ *       pbar(x_in) - pbar(x_out) + shear_weight * sum_BL tau_BL * |BL|
+ * with pbar(x) the mean of rho/3 over the fluid nodes of column x, tau_BL the
+ * boundary-link wall shear (TR_shear_2, trKernelsShear.cl:300-316, from the
+ * step's newest populations and its collision-time density and velocity)
+ * and |BL| the link length.  The reference has no adjoint; this is the
+ * objective of the wall-shape sensitivity. */
+typedef struct kofm_objective {
+  int32_t window;      /* 1 <= window <= nsteps */
+  int32_t x_in;        /* pressure column 1 (0 for the channel inlet) */
+  int32_t x_out;       /* pressure column 2 (nX-1 for the outlet) */
+  int32_t reserved;    /* 0 */
+  double shear_weight; /* weight of the wall-shear integral */
+} kofm_objective;
+
+/* Runs nsteps plain steps (k-omega off) from the current state, returns J
+ * and keeps the populations every checkpoint_interval steps (about
+ * sqrt(nsteps) keeps the checkpoints and the recomputed interval the same
+ * size).  The state advances exactly as with kofm_step. */
+int kofm_objective_run(kofm_ctx* ctx, int32_t nsteps, int32_t checkpoint_interval,
+                       const kofm_objective* obj, double* J);
+
+/* Reverse sweep over the last kofm_objective_run: each checkpoint interval
+ * is recomputed and the adjoint collide-stream runs backwards through it.
+ * dJ_dy[nN] receives dJ/dy of every wall point of kofm_build_geometry (the
+ * x coordinates held fixed; node types and link sets are held at the built
+ * geometry); dJ_dq[nIbb] (may be NULL) receives dJ/dq of every boundary link
+ * in KOFM_GEOM_IBB order (0 for a link repeated later in the list). */
+int kofm_adjoint(kofm_ctx* ctx, double* dJ_dy, double* dJ_dq);
 
 /* --- ensembles ------------------------------------------------------------- */
 

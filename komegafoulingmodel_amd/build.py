@@ -1,7 +1,12 @@
 """In-tree build of libkofm.so (HIP kernels + C ABI) for gfx950 with hipcc.
 
 Every HIP translation unit is compiled with packed-FP32 instruction selection
-disabled (``-Xclang -target-feature -Xclang -packed-fp32-ops``).
+disabled (``-Xclang -target-feature -Xclang -packed-fp32-ops``) and without
+automatic multiply-add fusion (``-ffp-contract=off``): rounding then follows
+the source's operation order, so every kernel variant that evaluates the same
+node update (launch widths, two nodes per lane, the ensemble launch) gives
+bit-identical results.  The step is bandwidth-bound; the unfused arithmetic
+costs no time.
 """
 from __future__ import annotations
 
@@ -38,6 +43,7 @@ DEVICE = [
     "-target-feature",
     "-Xclang",
     "-packed-fp32-ops",
+    "-ffp-contract=off",
     "-munsafe-fp-atomics",
 ]
 

@@ -161,6 +161,34 @@ def turbulent_forcing(H: float, nu: float, Re: float) -> float:
     return 0.5 * cf * Ub * Ub / (H / 2.0)
 
 
+def ensemble_members(seed: int = 5, n: int = 48) -> list:
+    """The batched ensemble: n plane channels, nX from {512, ..., 4096} in
+    steps of 512, nY from {64, ..., 512} in steps of 64, Re_H ~ U[2000, 20000],
+    each with its own seeded Gaussian fouling layer (channel i uses seed
+    seed + i).  The fouling amplitude is scaled by nY / 1024 (U[0, 8 nY/1024])
+    so the 16 Gaussians leave the narrowest channels open.  Each channel runs
+    laminar BGK at bulk velocity 0.05: nu = 0.05 H / Re_H and the Poiseuille
+    force F = 12 nu U / H^2."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n):
+        nX = int(rng.integers(1, 9)) * 512
+        nY = int(rng.integers(1, 9)) * 64
+        Re = float(rng.uniform(2000.0, 20000.0))
+        H = nY - 2.0
+        Ub = 0.05
+        nu = Ub * H / Re
+        r = np.random.Generator(np.random.PCG64(seed + i))
+        a = (0.0, 8.0 * nY / 1024.0)
+        hb = gaussian_fouling(nX, r, amp=a)
+        ht = gaussian_fouling(nX, r, amp=a)
+        C0, BL = wall_points(nX, lambda x: 0.5 + hb(x), lambda x: nY - 1.5 - ht(x))
+        out.append(Workload(f"ens{i}", nX, nY, C0, BL, pipe_radius=H / 2.0, tau0=_omega(nu),
+                            nu=nu, Fval=12.0 * nu * Ub / (H * H), noise=0.01, u_init=Ub,
+                            seed=seed + i, steps=10000))
+    return out
+
+
 def config(name: str) -> Workload:
     """The named workloads.
 

@@ -198,6 +198,8 @@ int kofm_geometry_array(const kofm_ctx* ctx, int32_t which, void* out) {
     case KOFM_GEOM_WEIGHTS: return put(g.weights);
     case KOFM_GEOM_BOUNDS: std::memcpy(out, g.bounds, sizeof(g.bounds)); return KOFM_OK;
     case KOFM_GEOM_MASSES: std::memcpy(out, g.masses, sizeof(g.masses)); return KOFM_OK;
+    case KOFM_GEOM_IBB_SEG: return put(g.ibbSeg);
+    case KOFM_GEOM_IBB_DQDY: return put(g.ibbDqDy);
     default: return KOFM_ERR_ARG;
   }
 }
@@ -317,6 +319,28 @@ int kofm_autotune(kofm_ctx* ctx, int32_t steps, int32_t* block_x, int32_t* nodes
   *plane_pad = pad;
   *ms_per_step = ms;
   return KOFM_OK;
+}
+
+int kofm_objective_run(kofm_ctx* ctx, int32_t nsteps, int32_t checkpoint_interval,
+                       const kofm_objective* obj, double* J) {
+  if (!ctx || !obj || !J) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  kofm::Objective o;
+  o.window = obj->window;
+  o.xIn = obj->x_in;
+  o.xOut = obj->x_out;
+  o.shearWeight = obj->shear_weight;
+  std::string err;
+  int rc = ctx->engine->objectiveRun(nsteps, checkpoint_interval, o, *J, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_adjoint(kofm_ctx* ctx, double* dJ_dy, double* dJ_dq) {
+  if (!ctx || !dJ_dy) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int rc = ctx->engine->adjoint(dJ_dy, dJ_dq, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
 }
 
 int kofm_set_graphs(kofm_ctx* ctx, int32_t on) {

@@ -18,6 +18,7 @@
 
 #include "../../include/kofm.h"
 #include "engine.hpp"
+#include "adjoint.hpp"
 #include "kernels.hpp"
 
 namespace kofm {
@@ -111,7 +112,7 @@ class EngineT final : public Engine {
     HIP_TRY(status_.alloc(sizeof(dev::BicgState)));
     if (ko_)
       for (int v = 0; v < 2; v++)
-        for (Buffer* b : {&kx_[v], &kr_[v], &krh_[v], &kp_[v][0], &kp_[v][1], &kv_[v][0], &kv_[v][1], &ks_[v], &kt_[v]})
+        for (Buffer* b : {&kr_[v], &krh_[v], &kp_[v][0], &kp_[v][1], &kv_[v][0], &kv_[v][1], &ks_[v], &kt_[v]})
           HIP_TRY(b->alloc(D * sizeof(double)));
     const size_t nTiles = size_t((nX_ + dev::kBlockX - 1) / dev::kBlockX) * nY_;
     const size_t nblk = 8 * ((nTiles + 7) / 8);
@@ -150,6 +151,8 @@ class EngineT final : public Engine {
     }
     for (int x = nX_; x < XF_; x++)
       for (int y = 0; y < nY_; y++) m[x + size_t(XF_) * y] = dev::kSolid;
+    ibbNodesH_.clear();
+    slotLink_.clear();
     for (size_t l = 0; l < g.ibb.size(); l++) {
       const int64_t loc = g.ibb[l];
       const int dir = int(loc / D_);
@@ -157,11 +160,20 @@ class EngineT final : public Engine {
       if (idx[node] < 0) {
         idx[node] = int32_t(q.size() / 8);
         q.insert(q.end(), 8, R(-1));
+        slotLink_.insert(slotLink_.end(), 8, -1);
+        ibbNodesH_.push_back(int32_t(node));
         m[node] |= dev::kIbbNode;
       }
-      q[8 * size_t(idx[node]) + (dir - 1)] = R(g.ibbQ[l]);  // a repeated link keeps the last q
+      // a repeated link keeps the last q
+      q[8 * size_t(idx[node]) + (dir - 1)] = R(g.ibbQ[l]);
+      slotLink_[8 * size_t(idx[node]) + (dir - 1)] = int32_t(l);
     }
     nIbbNodes_ = int(q.size() / 8);
+    HIP_TRY(ibbNodes_.alloc(std::max<size_t>(ibbNodesH_.size(), 1) * sizeof(int32_t)));
+    if (!ibbNodesH_.empty())
+      HIP_TRY(hipMemcpy(ibbNodes_.as<void>(), ibbNodesH_.data(), ibbNodesH_.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice));
+    run_.valid = false;
     HIP_TRY(hipMemcpy(mask_.as<void>(), m.data(), D * sizeof(uint16_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ibbIdx_.as<void>(), idx.data(), D * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(ibbQ_.alloc(std::max<size_t>(q.size(), 8) * sizeof(R)));
@@ -194,6 +206,7 @@ class EngineT final : public Engine {
   void setParams(const Params& p) override {
     par_ = p;
     dropGraphs();
+    run_.valid = false;
   }
 
   int initState(const double* rho, const double* ux, const double* uy, std::string& err) override {
@@ -441,6 +454,96 @@ class EngineT final : public Engine {
     return 0;
   }
 
+  int objectiveRun(int n, int interval, const Objective& o, double& J, std::string& err) override {
+    run_.valid = false;
+    if (ko_) return fail(err, "the adjoint covers the step without k-omega");
+    if (!haveState_) return fail(err, "state not initialised");
+    if (n < 1 || interval < 1 || o.window < 1 || o.window > n || o.xIn < 0 || o.xIn >= nX_ ||
+        o.xOut < 0 || o.xOut >= nX_) {
+      err = "objective run: need nsteps >= 1, interval >= 1, 1 <= window <= nsteps and "
+            "pressure columns inside the lattice";
+      return KOFM_ERR_ARG;
+    }
+    int rc = buildObjective(o, err);
+    if (rc) return rc;
+    const int nCk = (n + interval - 1) / interval;
+    const size_t state = 9 * size_t(P_) * sizeof(S);
+    rc = reserve(ckpt_, nCk, state, err);
+    if (rc) return rc;
+    ckIbb_.assign(size_t(nCk), 0);
+    dev::ObjectiveNodes on = objNodes();
+    zero<double>(objAccR_.as<double>(), on.n);
+    zero<double>(objAccT_.as<double>(), on.n);
+    // one captured graph per distinct segment shape (length, buffer parity,
+    // pending correction, first window step); the checkpoint copy runs eagerly
+    for (int c = 0; c < nCk; c++) {
+      const int n0 = c * interval, len = std::min(interval, n - n0);
+      HIP_TRY(hipMemcpyAsync(ckpt_[c]->template as<void>(), cur(), state, hipMemcpyDeviceToDevice,
+                             stream_));
+      ckIbb_[c] = preIbb_ ? 1 : 0;
+      const int w0 = std::min(len, std::max(0, n - o.window - n0));
+      const std::vector<int64_t> key = {0, len, curBuf_, ckIbb_[c], w0};
+      rc = replay(key, [&] { forwardSegment(len, w0, on); }, len, err);
+      if (rc) return rc;
+    }
+    HIP_TRY(hipGetLastError());
+    std::vector<double> accR(size_t(on.n)), accT(size_t(on.n));
+    HIP_TRY(hipMemcpyAsync(accR.data(), objAccR_.as<void>(), accR.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipMemcpyAsync(accT.data(), objAccT_.as<void>(), accT.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    J = 0.;
+    for (size_t k = 0; k < accR.size(); k++) J += objCpH_[k] * accR[k] + objCtH_[k] * accT[k];
+    objAccTH_ = accT;
+    run_.n = n;
+    run_.interval = interval;
+    run_.obj = o;
+    run_.valid = true;
+    return 0;
+  }
+
+  int adjoint(double* dJdy, double* dJdq, std::string& err) override {
+    if (!run_.valid) return fail(err, "run kofm_objective_run on this geometry first");
+    const int T = run_.n, K = run_.interval, W = run_.obj.window;
+    const size_t state = 9 * size_t(P_) * sizeof(S);
+    int rc = reserve(seg_, K + 1, state, err);
+    if (rc) return rc;
+    const size_t nl = std::max<size_t>(8 * size_t(nIbbNodes_), 8);
+    if (ab_[0].bytes() < 9 * size_t(P_) * sizeof(R) || upBar_.bytes() < nl * sizeof(R) ||
+        qBar_.bytes() < nl * sizeof(double)) {
+      dropSweeps();
+      for (Buffer* b : {&ab_[0], &ab_[1]}) HIP_TRY(b->alloc(9 * size_t(P_) * sizeof(R)));
+      HIP_TRY(upBar_.alloc(nl * sizeof(R)));
+      HIP_TRY(qBar_.alloc(nl * sizeof(double)));
+    }
+    zero<double>(qBar_.as<double>(), int64_t(nl));
+    zero<R>(ab_[0].as<R>(), 9 * P_);
+    dev::AdjointBuffers abuf{ibbNodes_.as<int32_t>(), upBar_.as<void>(), qBar_.as<double>(),
+                             nIbbNodes_};
+    dev::ObjectiveNodes on = objNodes();
+    const int nCk = (T + K - 1) / K;
+    int a = 0;  // ab_[a] holds the adjoint of the later level
+    for (int c = nCk - 1; c >= 0; c--) {
+      const int n0 = c * K, len = std::min(K, T - n0);
+      HIP_TRY(hipMemcpyAsync(seg_[0]->template as<void>(), ckpt_[c]->template as<void>(), state,
+                             hipMemcpyDeviceToDevice, stream_));
+      const int w0 = std::min(len, std::max(0, T - W - n0));
+      const std::vector<int64_t> key = {1, len, a, ckIbb_[c], w0};
+      rc = replay(key, [&] { adjointSegment(len, w0, ckIbb_[c], a, abuf, on); }, 0, err);
+      if (rc) return rc;
+      a ^= len & 1;
+    }
+    HIP_TRY(hipGetLastError());
+    std::vector<double> qb(8 * size_t(nIbbNodes_));
+    if (!qb.empty())
+      HIP_TRY(hipMemcpyAsync(qb.data(), qBar_.as<void>(), qb.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    wallGradient(qb, dJdy, dJdq);
+    return 0;
+  }
+
  private:
   int nX_, nY_, XF_;
   int64_t D_;
@@ -453,9 +556,23 @@ class EngineT final : public Engine {
   hipStream_t stream_ = nullptr;
   Buffer fA_, fB_, mask_, ibbIdx_, ibbQ_, rho_, ux_, uy_;
   Buffer K_, O_, nut_, S_, wallD_, Dk_, Do_, F1_, dkdo_, dX_, Ak_, Ao_, bk_, bo_;
-  Buffer kx_[2], kr_[2], krh_[2], kp_[2][2], kv_[2][2], ks_[2], kt_[2];  // FP64 BiCGStab vectors, [0] = k
+  Buffer kr_[2], krh_[2], kp_[2][2], kv_[2][2], ks_[2], kt_[2];  // FP64 BiCGStab vectors, [0] = k
   Buffer ss_, cfss_, tau_, sInd_, wts_, tauBL_;
   Buffer status_, partial_, redPartial_, redOut_;
+  int lastIters_ = 1;  // iterations the previous k-omega solve needed
+  // adjoint: checkpoints of the objective run, the recomputed segment, the
+  // two adjoint population levels, per-link buffers and the objective nodes
+  struct ObjectiveRun {
+    int n = 0, interval = 0;
+    Objective obj;
+    bool valid = false;
+  } run_;
+  std::vector<std::unique_ptr<Buffer>> ckpt_, seg_;
+  std::vector<int> ckIbb_;
+  Buffer ab_[2], upBar_, qBar_, ibbNodes_;
+  Buffer objNode_, objWall_, objCp_, objCt_, objAccR_, objAccT_, objBar_, objUp_;
+  std::vector<int32_t> ibbNodesH_, slotLink_, objNodeH_, objWallH_;
+  std::vector<double> objCpH_, objCtH_, objAccTH_;
   std::vector<hipEvent_t> evs_;
   Geometry geo_;
   Params par_;
@@ -472,9 +589,20 @@ class EngineT final : public Engine {
   };
   std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs_;
 
+  // captured objective-run and adjoint segments (their launches bake in the
+  // checkpoint-segment and adjoint buffers, the objective nodes and the
+  // parameters)
+  std::vector<std::pair<std::vector<int64_t>, hipGraphExec_t>> sweepGraphs_;
+
+  void dropSweeps() {
+    for (auto& g : sweepGraphs_) (void)hipGraphExecDestroy(g.second);
+    sweepGraphs_.clear();
+  }
+
   void dropGraphs() {
     for (auto& g : graphs_) (void)hipGraphExecDestroy(g.second);
     graphs_.clear();
+    dropSweeps();
   }
 
   // Instantiates (once) the graph of kGraphSteps collide-stream launches for
@@ -498,6 +626,258 @@ class EngineT final : public Engine {
     graphs_.push_back({{buf0, options}, exec});
     out = exec;
     return 0;
+  }
+
+  // len plain steps from the context state with the objective accumulated
+  // from step w0 of the segment on (the state advances as with kofm_step).
+  void forwardSegment(int len, int w0, const dev::ObjectiveNodes& on) {
+    const dim3 og((on.n + 63) / 64);
+    for (int m = 0; m < len; m++) {
+      const S* src = cur();
+      auto p = lbParams(0);
+      launchLB(0);
+      if (m >= w0)
+        hipLaunchKernelGGL((dev::objectiveForward<S, R>), og, dim3(64), 0, stream_, src,
+                           (const S*)cur(), mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(),
+                           ibbQ_.as<R>(), cfss_.as<R>(), on, p);
+    }
+  }
+
+  // Recomputes a segment from seg_[0] and runs the adjoint back through it;
+  // ab_[a] holds the adjoint after the segment on entry.
+  void adjointSegment(int len, int w0, int ap0, int a, const dev::AdjointBuffers& abuf,
+                      const dev::ObjectiveNodes& on) {
+    const dim3 og((on.n + 63) / 64), ig((nIbbNodes_ + 63) / 64);
+    for (int m = 0; m < len; m++)
+      launchLBFrom(seg_[m]->template as<S>(), seg_[m + 1]->template as<S>(), m == 0 ? ap0 : 1);
+    for (int m = len - 1; m >= 0; m--) {
+      const int apply = m == 0 ? ap0 : 1;
+      const bool win = m >= w0;
+      const S* sn = seg_[m]->template as<S>();
+      const S* sn1 = seg_[m + 1]->template as<S>();
+      R* next = ab_[a].as<R>();
+      R* curA = ab_[a ^ 1].as<R>();
+      auto p = lbParams(0);
+      p.applyIbb = apply;
+      p.saveMacros = 0;
+      if (win) {
+        hipLaunchKernelGGL((dev::objectiveGrad<S, R>), og, dim3(64), 0, stream_, sn, sn1,
+                           mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(), ibbQ_.as<R>(),
+                           cfss_.as<R>(), on, p);
+        objScatter(sn1, next, abuf, on, 0, 1, p);
+      }
+      hipLaunchKernelGGL((dev::lbAdjointStep<S, R, 256>), gridLB(256), dim3(256), 0, stream_, sn,
+                         (const R*)next, curA, mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(),
+                         ibbQ_.as<R>(), abuf, p);
+      if (apply && nIbbNodes_ > 0)
+        hipLaunchKernelGGL((dev::lbAdjointUpstream<R>), ig, dim3(64), 0, stream_, curA,
+                           ibbQ_.as<R>(), abuf, p);
+      if (win) objScatter(sn, curA, abuf, on, 1, apply, p);
+      a ^= 1;
+    }
+  }
+
+  // Launches the graph captured for key, capturing it from enqueue() the
+  // first time.  A capture leaves curBuf_/preIbb_ where the enqueued steps
+  // put them; a replay advances them by `steps` plain steps.
+  template <class F>
+  int replay(const std::vector<int64_t>& key, F&& enqueue, int steps, std::string& err) {
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : sweepGraphs_)
+      if (g.first == key) exec = g.second;
+    if (exec) {
+      curBuf_ ^= steps & 1;
+      if (steps > 0) preIbb_ = true;
+    } else {
+      hipGraph_t graph = nullptr;
+      HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      enqueue();
+      HIP_TRY(hipStreamEndCapture(stream_, &graph));
+      hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIP_TRY(e);
+      sweepGraphs_.push_back({key, exec});
+    }
+    HIP_TRY(hipGraphLaunch(exec, stream_));
+    return 0;
+  }
+
+  // Objective nodes: fluid nodes of the two pressure columns and the wall
+  // nodes, with their per-step coefficients dJ/drho and dJ/dtau.
+  int buildObjective(const Objective& o, std::string& err) {
+    const double W = double(o.window);
+    std::vector<int32_t> where(size_t(D_), -1);
+    objNodeH_.clear();
+    objWallH_.clear();
+    objCpH_.clear();
+    objCtH_.clear();
+    auto slot = [&](int gi) {
+      if (where[gi] < 0) {
+        where[gi] = int32_t(objNodeH_.size());
+        objNodeH_.push_back(gi);
+        objWallH_.push_back(-1);
+        objCpH_.push_back(0.);
+        objCtH_.push_back(0.);
+      }
+      return where[gi];
+    };
+    for (int col = 0; col < 2; col++) {
+      const int x = col == 0 ? o.xIn : o.xOut;
+      int nf = 0;
+      for (int y = 0; y < nY_; y++)
+        if (!(geo_.nType[x + size_t(XF_) * y] & kMSolid)) nf++;
+      if (nf == 0) {
+        err = "a pressure column of the objective has no fluid node";
+        return KOFM_ERR_ARG;
+      }
+      const double c = (col == 0 ? 1. : -1.) / (3. * nf * W);
+      for (int y = 0; y < nY_; y++) {
+        const int gi = x + XF_ * y;
+        if (!(geo_.nType[gi] & kMSolid)) objCpH_[slot(gi)] += c;
+      }
+    }
+    const int nSS = int(geo_.ss.size()), nb = geo_.nBLshear();
+    std::vector<double> ct(size_t(nSS), 0.);
+    if (o.shearWeight != 0.)
+      for (int k = 0; k < nb; k++) {
+        const double bl = linkLength(geo_.shearLink[k]);
+        for (int j = 0; j < 4; j++) {
+          const int id = geo_.sInd[4 * k + j];
+          if (id >= 0) ct[id] += o.shearWeight * bl * geo_.weights[4 * k + j] / W;
+        }
+      }
+    for (int w = 0; w < nSS; w++) {
+      const int k = slot(geo_.ss[w]);
+      objWallH_[k] = w;
+      objCtH_[k] = ct[w];
+    }
+    const size_t n = objNodeH_.size();
+    if (objNode_.bytes() != n * sizeof(int32_t)) {  // captured segments bake the pointers in
+      dropSweeps();
+      HIP_TRY(objNode_.alloc(n * sizeof(int32_t)));
+      HIP_TRY(objWall_.alloc(n * sizeof(int32_t)));
+      HIP_TRY(objCp_.alloc(n * sizeof(double)));
+      HIP_TRY(objCt_.alloc(n * sizeof(double)));
+      HIP_TRY(objAccR_.alloc(n * sizeof(double)));
+      HIP_TRY(objAccT_.alloc(n * sizeof(double)));
+      HIP_TRY(objBar_.alloc(18 * n * sizeof(R)));
+      HIP_TRY(objUp_.alloc(8 * n * sizeof(R)));
+    }
+    HIP_TRY(hipMemcpy(objNode_.as<void>(), objNodeH_.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(objWall_.as<void>(), objWallH_.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(objCp_.as<void>(), objCpH_.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(objCt_.as<void>(), objCtH_.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+  }
+
+  dev::ObjectiveNodes objNodes() const {
+    return {objNode_.as<int32_t>(), objWall_.as<int32_t>(), objCp_.as<double>(),
+            objCt_.as<double>(), objAccR_.as<double>(), objAccT_.as<double>(),
+            objBar_.as<void>(), objUp_.as<void>(), int(objNodeH_.size())};
+  }
+
+  void objScatter(const S* s, R* abv, const dev::AdjointBuffers& abuf, const dev::ObjectiveNodes& on,
+                  int level, int apply, const dev::LBParams<R>& p) {
+    const dim3 og((on.n + 63) / 64);
+    hipLaunchKernelGGL((dev::objectiveScatter<S, R>), og, dim3(64), 0, stream_, s, abv,
+                       mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(), ibbQ_.as<R>(), abuf, on, level,
+                       apply, p);
+    hipLaunchKernelGGL((dev::objectiveUpstream<R>), og, dim3(64), 0, stream_, abv,
+                       mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(), ibbQ_.as<R>(), on, apply, p);
+  }
+
+  double pointY(int i) const { return geo_.C0[2 * size_t(i) + 1]; }
+  double linkLength(int bl) const {
+    const int a = geo_.BL[2 * size_t(bl)], b = geo_.BL[2 * size_t(bl) + 1];
+    const double dx = geo_.C0[2 * size_t(b)] - geo_.C0[2 * size_t(a)];
+    const double dy = pointY(b) - pointY(a);
+    return std::sqrt(dx * dx + dy * dy);
+  }
+
+  // dJ/dy of the wall points from the per-link dJ/dq and, through the shear
+  // weights and link lengths, from the window-mean wall traction.
+  void wallGradient(const std::vector<double>& qb, double* dJdy, double* dJdq) const {
+    const size_t nN = geo_.C0.size() / 2, nL = geo_.ibb.size();
+    std::vector<double> dq(nL, 0.), dy(nN, 0.);
+    for (size_t r = 0; r < qb.size(); r++)
+      if (slotLink_[r] >= 0) dq[size_t(slotLink_[r])] = qb[r];
+    for (size_t l = 0; l < nL; l++) {
+      const int s = geo_.ibbSeg[l];
+      dy[size_t(s)] += dq[l] * geo_.ibbDqDy[2 * l];
+      dy[size_t(s) + 1] += dq[l] * geo_.ibbDqDy[2 * l + 1];
+    }
+    const double sw = run_.obj.shearWeight, Wn = double(run_.obj.window);
+    const int nb = geo_.nBLshear();
+    if (sw != 0.) {
+      std::vector<double> tauAvg(geo_.ss.size(), 0.);
+      for (size_t k = 0; k < objNodeH_.size(); k++)
+        if (objWallH_[k] >= 0) tauAvg[size_t(objWallH_[k])] = objAccTH_[k] / Wn;
+      const double c = geo_.cutoff, kNorm = 1. / std::sqrt(2. * 3.1415926535897931);
+      for (int k = 0; k < nb; k++) {
+        const int bl = geo_.shearLink[k];
+        const int ia = geo_.BL[2 * size_t(bl)], ib = geo_.BL[2 * size_t(bl) + 1];
+        const double ax = geo_.C0[2 * size_t(ia)], ay = pointY(ia);
+        const double bx = geo_.C0[2 * size_t(ib)], by = pointY(ib);
+        const double len = std::sqrt((bx - ax) * (bx - ax) + (by - ay) * (by - ay));
+        const double mx = 0.5 * (ax + bx), my = 0.5 * (ay + by);
+        double Wt[4], tv[4], wt[4], sum = 0.;
+        int id[4];
+        for (int j = 0; j < 4; j++) {
+          id[j] = geo_.sInd[4 * size_t(k) + j];
+          Wt[j] = geo_.weights[4 * size_t(k) + j];
+          tv[j] = id[j] >= 0 ? tauAvg[size_t(id[j])] : 0.;
+          double d = 100.;
+          if (id[j] >= 0) {
+            const double tx = double(geo_.ss[id[j]] % XF_) - mx, ty = double(geo_.ss[id[j]] / XF_) - my;
+            d = std::sqrt(tx * tx + ty * ty);
+          }
+          wt[j] = kNorm * std::exp(-0.5 * (d / c) * (d / c));
+          sum += wt[j];
+        }
+        double Wb[4], dot = 0., blb = 0.;
+        for (int j = 0; j < 4; j++) {
+          Wb[j] = sw * len * tv[j];
+          dot += Wb[j] * Wt[j];
+          blb += sw * Wt[j] * tv[j];
+        }
+        double myb = 0.;
+        for (int j = 0; j < 4; j++) {
+          if (id[j] < 0) continue;
+          const double wtb = (Wb[j] - dot) / sum;
+          const double ty = double(geo_.ss[id[j]] / XF_);
+          myb += wtb * (-wt[j] / (c * c)) * (my - ty);
+        }
+        const double dl = len > 0. ? (by - ay) / len : 0.;
+        dy[size_t(ia)] += 0.5 * myb - blb * dl;
+        dy[size_t(ib)] += 0.5 * myb + blb * dl;
+      }
+    }
+    if (dJdy) std::copy(dy.begin(), dy.end(), dJdy);
+    if (dJdq) std::copy(dq.begin(), dq.end(), dJdq);
+  }
+
+  // Grows a list of equally sized device buffers (existing ones keep their
+  // addresses unless the size changes).
+  int reserve(std::vector<std::unique_ptr<Buffer>>& v, int n, size_t bytes, std::string& err) {
+    if (!v.empty() && v[0]->bytes() != bytes) {
+      dropSweeps();
+      v.clear();
+    }
+    while (int(v.size()) < n) {
+      v.emplace_back(new Buffer);
+      hipError_t e = v.back()->alloc(bytes);
+      if (e != hipSuccess) {
+        v.pop_back();
+        err = std::string("adjoint storage: ") + hipGetErrorString(e);
+        return KOFM_ERR_DEVICE;
+      }
+    }
+    return 0;
+  }
+
+  template <class T>
+  void zero(T* a, int64_t n) {
+    hipLaunchKernelGGL((dev::fillZero<T>), dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream_, a, n);
   }
 
   static int fail(std::string& err, const char* msg) {
@@ -616,6 +996,15 @@ class EngineT final : public Engine {
       launchLBBlockN<KO, 1>(block, p, src, dst);
   }
 
+  // One plain step from src to dst with an explicit correction flag (the
+  // adjoint's recompute; the context's own state is not touched).
+  void launchLBFrom(const S* src, S* dst, int apply) {
+    auto p = lbParams(0);
+    p.applyIbb = apply;
+    p.saveMacros = 0;
+    launchLBBlock<false>(lbBlock_, lbNpt_, p, src, dst);
+  }
+
   int launchLB(unsigned options) {
     auto p = lbParams(options);
     if (ko_)
@@ -644,6 +1033,9 @@ class EngineT final : public Engine {
     }
     P_ = P;
     dropGraphs();
+    run_.valid = false;
+    ckpt_.clear();
+    seg_.clear();
     HIP_TRY(fA_.alloc(9 * size_t(P_) * sizeof(S)));
     HIP_TRY(fB_.alloc(9 * size_t(P_) * sizeof(S)));
     curBuf_ = 0;
@@ -729,70 +1121,72 @@ class EngineT final : public Engine {
   }
 
  private:
-  // kOmega::Solve (kOmega.cpp:31-73) with one sub-step: diffusivities,
-  // coefficients, then both implicit systems by Jacobi sweeps with
-  // device-side convergence checks, launched in chunks.
+  // kOmega::Solve (kOmega.cpp:31-73) with one sub-step: diffusivities, then
+  // the coefficients with the first residual, then both implicit systems by
+  // BiCGStab with device-side scalars and convergence checks.  The host
+  // launches as many iterations as the previous step needed (at most 8)
+  // before its first poll of the done flags, then chunks of four; launches
+  // past convergence return at once.
   int koStep(std::string& err) {
     auto kp = koParams();
     const dim3 grid(8 * kp.tiles.per), blk(dev::kBlockX);
+    // the solver kernels walk their XCD's tiles with a fixed block count
+    const int nsb = std::min(8 * kp.tiles.per, 2048);
+    const dim3 sgrid(nsb);
     using K = double;
     hipLaunchKernelGGL((dev::koDiffusivities<R>), grid, blk, 0, stream_, mask_.as<uint16_t>(),
                        K_.as<K>(), O_.as<K>(), nut_.as<K>(), wallD_.as<K>(), Dk_.as<K>(),
                        Do_.as<K>(), F1_.as<K>(), dkdo_.as<K>(), rho_.as<R>(), dX_.as<K>(), kp);
-    hipLaunchKernelGGL((dev::koCoeffs<R>), grid, blk, 0, stream_, mask_.as<uint16_t>(),
-                       K_.as<K>(), O_.as<K>(), nut_.as<K>(), Dk_.as<K>(), Do_.as<K>(),
-                       F1_.as<K>(), dkdo_.as<K>(), S_.as<K>(), rho_.as<R>(), ux_.as<R>(),
-                       uy_.as<R>(), dX_.as<K>(), wallD_.as<K>(), Ak_.as<K>(), Ao_.as<K>(),
-                       bk_.as<K>(), bo_.as<K>(), kp);
-    using Q = double;
-    dev::BicgVecs<K, Q> V;
-    K* X[2] = {K_.as<K>(), O_.as<K>()};
-    const K* A[2] = {Ak_.as<K>(), Ao_.as<K>()};
-    const K* B[2] = {bk_.as<K>(), bo_.as<K>()};
+    dev::BicgVecs V;
+    V.x[0] = K_.as<K>();
+    V.x[1] = O_.as<K>();
+    V.A[0] = Ak_.as<K>();
+    V.A[1] = Ao_.as<K>();
+    V.b[0] = bk_.as<K>();
+    V.b[1] = bo_.as<K>();
     for (int v = 0; v < 2; v++) {
-      V.x[v] = X[v];
-      V.A[v] = A[v];
-      V.b[v] = B[v];
-      V.xq[v] = kx_[v].as<Q>();
-      V.r[v] = kr_[v].as<Q>();
-      V.rh[v] = krh_[v].as<Q>();
-      V.s[v] = ks_[v].as<Q>();
-      V.t[v] = kt_[v].as<Q>();
+      V.r[v] = kr_[v].as<K>();
+      V.rh[v] = krh_[v].as<K>();
+      V.s[v] = ks_[v].as<K>();
+      V.t[v] = kt_[v].as<K>();
       for (int c = 0; c < 2; c++) {
-        V.p[v][c] = kp_[v][c].as<Q>();
-        V.v[v][c] = kv_[v][c].as<Q>();
+        V.p[v][c] = kp_[v][c].as<K>();
+        V.v[v][c] = kv_[v][c].as<K>();
       }
     }
+    V.partial = partial_.as<double>();
     auto* st = status_.as<dev::BicgState>();
-    double* part = partial_.as<double>();
-    const int nblk = int(grid.x);
+    const double* part = partial_.as<double>();
     hipLaunchKernelGGL(dev::bicgReset, dim3(1), dim3(64), 0, stream_, st, par_.relTol[0],
                        par_.absTol[0], par_.maxIters[0], par_.relTol[1], par_.absTol[1],
                        par_.maxIters[1]);
-    hipLaunchKernelGGL((dev::bicgInit<K, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V, part, kp);
-    hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, (const double*)part, nblk, st, 0);
+    hipLaunchKernelGGL((dev::koCoeffs<R>), sgrid, blk, 0, stream_, mask_.as<uint16_t>(),
+                       nut_.as<K>(), Dk_.as<K>(), Do_.as<K>(), F1_.as<K>(), dkdo_.as<K>(),
+                       S_.as<K>(), rho_.as<R>(), ux_.as<R>(), uy_.as<R>(), dX_.as<K>(),
+                       wallD_.as<K>(), V, kp);
+    hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, part, nsb, st, 0);
     const int maxIt = std::max(par_.maxIters[0], par_.maxIters[1]);
-    const int chunk = 4;
-    int it = 0;
-    while (it < maxIt) {
-      for (int c = 0; c < chunk && it < maxIt; c++, it++) {
-        hipLaunchKernelGGL((dev::bicgPV<K, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V,
-                           (const dev::BicgState*)st, part, kp);
-        hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, (const double*)part, nblk, st, 1);
-        hipLaunchKernelGGL((dev::bicgST<K, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V,
-                           (const dev::BicgState*)st, part, kp);
-        hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, (const double*)part, nblk, st, 2);
-        hipLaunchKernelGGL((dev::bicgXR<K, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V,
-                           (const dev::BicgState*)st, part, kp);
-        hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, (const double*)part, nblk, st, 3);
+    int it = 0, batch = std::min(std::max(1, lastIters_), 8);
+    dev::BicgState hs;
+    while (true) {
+      for (int c = 0; c < batch && it < maxIt; c++, it++) {
+        hipLaunchKernelGGL(dev::bicgPV, sgrid, blk, 0, stream_, mask_.as<uint16_t>(), V,
+                           (const dev::BicgState*)st, kp);
+        hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, part, nsb, st, 1);
+        hipLaunchKernelGGL(dev::bicgST, sgrid, blk, 0, stream_, mask_.as<uint16_t>(), V,
+                           (const dev::BicgState*)st, kp);
+        hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, part, nsb, st, 2);
+        hipLaunchKernelGGL(dev::bicgXR, sgrid, blk, 0, stream_, mask_.as<uint16_t>(), V,
+                           (const dev::BicgState*)st, kp);
+        hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, part, nsb, st, 3);
       }
-      dev::BicgState hs;
       HIP_TRY(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, stream_));
       HIP_TRY(hipStreamSynchronize(stream_));
       if (hs.nan) nanSeen_ = true;
-      if (hs.done[0] && hs.done[1]) break;
+      if ((hs.done[0] && hs.done[1]) || it >= maxIt) break;
+      batch = 4;
     }
-    hipLaunchKernelGGL((dev::bicgFinish<K, Q>), grid, blk, 0, stream_, mask_.as<uint16_t>(), V, kp);
+    lastIters_ = std::max(hs.iters[0], hs.iters[1]);
     HIP_TRY(hipGetLastError());
     if (nanSeen_) {
       err = "non-finite residual in the k-omega solve";

@@ -22,6 +22,14 @@ struct SolverStats {
   int nan = 0;
 };
 
+// J = mean over the last `window` steps of a run of
+//   pbar(xIn) - pbar(xOut) + shearWeight * sum over shear links of tau_BL * |link|
+// with pbar the mean of rho / 3 over the fluid nodes of a column.
+struct Objective {
+  int window = 1, xIn = 0, xOut = 0;
+  double shearWeight = 0.;
+};
+
 // Type-erased engine; the implementation is templated on the population
 // storage type and the arithmetic type.
 class Engine {
@@ -47,6 +55,12 @@ class Engine {
   virtual void setGraphs(bool on) = 0;
   virtual int autotune(int steps, int& block, int& npt, int64_t& pad, double& ms,
                        std::string& err) = 0;
+  // Runs n steps accumulating J and keeping a population checkpoint every
+  // `interval` steps; adjoint() then returns dJ/dy of the wall points (and
+  // dJ/dq of the boundary links, if dJdq is not null).
+  virtual int objectiveRun(int n, int interval, const Objective& o, double& J,
+                           std::string& err) = 0;
+  virtual int adjoint(double* dJdy, double* dJdq, std::string& err) = 0;
 };
 
 class Ensemble {

@@ -27,6 +27,9 @@ class Builder {
   int run(std::string& err) {
     g_.nX = in_.nX;
     g_.nY = in_.nY;
+    g_.C0 = in_.C0;
+    g_.BL = in_.BL;
+    g_.cutoff = in_.cutoff;
     g_.XF = ((in_.nX + 127) / 128) * 128;
     g_.D = int64_t(g_.XF) * g_.nY;
     nN_ = int(in_.C0.size() / 2);
@@ -320,6 +323,8 @@ class Builder {
     static const int flip8[8] = {1, 0, 3, 2, 5, 4, 7, 6};
     g_.ibb.clear();
     g_.ibbQ.clear();
+    g_.ibbSeg.clear();
+    g_.ibbDqDy.clear();
     for (int s = 0; s < nN_ - 1; s++) {
       if (s == nN_ / 2 - 1) continue;
       V2 v0 = pt(s), v1 = pt(s + 1);
@@ -354,6 +359,12 @@ class Builder {
             if (std::fabs(bl - len(sub(cut, v0)) - len(sub(cut, v1))) >= 1.0e-5) continue;
             g_.ibb.push_back(int32_t(i + int64_t(g_.XF) * j + g_.D * (dir + 1)));
             g_.ibbQ.push_back(dist);
+            // dist = n.(v0 - L0) / n.Ld with n = (-(y1 - y0), x1 - x0)
+            const double ex = v0.x - L0.x, dxs = v1.x - v0.x;
+            const double M = -(v1.y - v0.y) * Ld.x + dxs * Ld.y;
+            g_.ibbSeg.push_back(s);
+            g_.ibbDqDy.push_back((ex + dxs - dist * Ld.x) / M);
+            g_.ibbDqDy.push_back((-ex + dist * Ld.x) / M);
           }
         }
     }
@@ -418,11 +429,13 @@ class Builder {
     }
     // four nearest wall nodes to each link midpoint
     const int nb = g_.nBLshear();
+    g_.shearLink.assign(size_t(nb), 0);
     g_.sInd.assign(4 * size_t(nb), -1);
     g_.weights.assign(4 * size_t(nb), 0.);
     const double kNorm = 1. / std::sqrt(2. * 3.1415926535897931);
     for (int k = 0; k < nb; k++) {
       int b = k < g_.bounds[1] ? k + g_.bounds[0] : k + g_.bounds[2] - g_.bounds[1];
+      g_.shearLink[k] = b < half ? b : (nBL_ - 1 - b) + half;
       double h = bl[b] / 2.;
       V2 mid = {p0[b].x + h * nv[b].y, p0[b].y + h * (-nv[b].x)};
       int ix = int(mid.x);

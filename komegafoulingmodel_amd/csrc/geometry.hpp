@@ -73,6 +73,17 @@ struct Geometry {
   double masses[2] = {0., 0.};
   int ssMinX = 0, ssMaxX = 0;
 
+  // First-order sensitivity of the continuous geometry data to the y
+  // coordinates of the wall points (the x coordinates stay fixed), for the
+  // adjoint.  The node types, the link set and the four-nearest choices are
+  // piecewise constant in the points and are not differentiated.
+  std::vector<double> C0;         // 2*nN, the wall points the geometry was built from
+  std::vector<int32_t> BL;        // 2*nBL
+  double cutoff = 0.;             // Cutoff Radius of the shear weights
+  std::vector<int32_t> ibbSeg;    // nIbb: the link cuts the wall segment (s, s+1)
+  std::vector<double> ibbDqDy;    // 2*nIbb: dq/dy of points s and s+1
+  std::vector<int32_t> shearLink; // nBLshear: boundary link of each shear entry
+
   int nBLshear() const { return bounds[1] + bounds[3]; }
 };
 

@@ -166,15 +166,29 @@ struct KOFields {
 // collision deviations out): moments, optional macro output, with KO the
 // SST eddy viscosity and strain, then EDM-forced BGK.  Shared by the one-
 // and two-node-per-lane kernels.
+// Density deviation, density, its inverse and velocity of one node (the
+// same operation order wherever a kernel needs a node's moments).
+template <class R>
+struct Moments {
+  R drho, rho, inv, ux, uy;
+};
+template <class R>
+__device__ __forceinline__ Moments<R> moments(const R (&g)[9]) {
+  Moments<R> m;
+  m.drho = ((g[0] + g[1]) + (g[2] + g[3])) + ((g[4] + g[5]) + (g[6] + g[7])) + g[8];
+  m.rho = R(1) + m.drho;
+  m.inv = R(1) / m.rho;
+  m.ux = m.inv * (((g[1] - g[2]) + (g[5] - g[6])) + (g[7] - g[8]));
+  m.uy = m.inv * (((g[3] - g[4]) + (g[5] - g[6])) - (g[7] - g[8]));
+  return m;
+}
+
 template <class R, bool KO>
 __device__ __forceinline__ void collideNode(R (&g)[9], size_t gi, R* __restrict__ rhoOut,
                                             R* __restrict__ uxOut, R* __restrict__ uyOut,
                                             const KOFields& ko, const LBParams<R>& p) {
-  const R drho = ((g[0] + g[1]) + (g[2] + g[3])) + ((g[4] + g[5]) + (g[6] + g[7])) + g[8];
-  const R rho = R(1) + drho;
-  const R inv = R(1) / rho;
-  const R ux = inv * (((g[1] - g[2]) + (g[5] - g[6])) + (g[7] - g[8]));
-  const R uy = inv * (((g[3] - g[4]) + (g[5] - g[6])) - (g[7] - g[8]));
+  const Moments<R> mo = moments(g);
+  const R drho = mo.drho, rho = mo.rho, ux = mo.ux, uy = mo.uy;
 
   R omega = p.omega0;
   if constexpr (KO) {
@@ -573,22 +587,18 @@ __global__ __launch_bounds__(kBlockX) void koDiffusivities(
 
 // Update_kOmega_Coeffs_Implicit (kOmegaKernels.cl:187-480): normalised
 // five-point rows; A holds E, W, N, S planes (diagonal 1).
+struct KORows {
+  double Ak[4], Ao[4], bk, bo;
+};
+
 template <class M>
-__global__ __launch_bounds__(kBlockX) void koCoeffs(
-    const uint16_t* __restrict__ mask, const double* __restrict__ K, const double* __restrict__ O,
+__device__ __forceinline__ void koCoeffRow(
+    int x, size_t g, uint32_t m, const double* __restrict__ K, const double* __restrict__ O,
     const double* __restrict__ Nut, const double* __restrict__ Dk, const double* __restrict__ Do,
     const double* __restrict__ F1a, const double* __restrict__ dkdo, const double* __restrict__ Sx,
     const M* __restrict__ rho, const M* __restrict__ uxa, const M* __restrict__ uya,
-    const double* __restrict__ dX, const double* __restrict__ Wd, double* __restrict__ Ak,
-    double* __restrict__ Ao, double* __restrict__ bk, double* __restrict__ bo, KOParams p) {
+    const double* __restrict__ dX, const double* __restrict__ Wd, const KOParams& p, KORows& o) {
   using R = double;
-  int bx, y;
-  if (!tileOf(p.tiles, bx, y)) return;
-  const int x = bx * kBlockX + threadIdx.x;
-  if (x >= p.nX) return;
-  const size_t g = size_t(x) + size_t(p.XF) * y;
-  const uint32_t m = mask[g];
-  if (m & kSolid) return;
   R ax[6], ay[6];
   fdCoeffs<R>(dX[g], dX[g + p.D], ax);
   fdCoeffs<R>(dX[g + 2 * p.D], dX[g + 3 * p.D], ay);
@@ -617,11 +627,11 @@ __global__ __launch_bounds__(kBlockX) void koCoeffs(
   R dt = p.dt;
   const R Kc = R(1) + dt * (Jxk * ax[2] + Jyk * ay[2] - dk * (ax[5] + ay[5])) - Sc;
   R iv = R(1) / Kc;
-  Ak[g] = dt * (Jxk * ax[0] - dk * ax[3]) * iv;
-  Ak[g + p.D] = dt * (Jxk * ax[1] - dk * ax[4]) * iv;
-  Ak[g + 2 * p.D] = dt * (Jyk * ay[0] - dk * ay[3]) * iv;
-  Ak[g + 3 * p.D] = dt * (Jyk * ay[1] - dk * ay[4]) * iv;
-  bk[g] = (k + dt * Pk) * iv;
+  o.Ak[0] = dt * (Jxk * ax[0] - dk * ax[3]) * iv;
+  o.Ak[1] = dt * (Jxk * ax[1] - dk * ax[4]) * iv;
+  o.Ak[2] = dt * (Jyk * ay[0] - dk * ay[3]) * iv;
+  o.Ak[3] = dt * (Jyk * ay[1] - dk * ay[4]) * iv;
+  o.bk = (k + dt * Pk) * iv;
   if (m & kNesw) {  // omega wall row (kOmegaKernels.cl:389-403)
     dt = R(0);
     Sc = R(0);
@@ -633,11 +643,11 @@ __global__ __launch_bounds__(kBlockX) void koCoeffs(
   }
   const R Wc = R(1) + dt * (Jxw * ax[2] + Jyw * ay[2] - dw * (ax[5] + ay[5])) - Sc;
   iv = R(1) / Wc;
-  Ao[g] = dt * (Jxw * ax[0] - dw * ax[3]) * iv;
-  Ao[g + p.D] = dt * (Jxw * ax[1] - dw * ax[4]) * iv;
-  Ao[g + 2 * p.D] = dt * (Jyw * ay[0] - dw * ay[3]) * iv;
-  Ao[g + 3 * p.D] = dt * (Jyw * ay[1] - dw * ay[4]) * iv;
-  bo[g] = (om + dt * (gam * Pk)) * iv;
+  o.Ao[0] = dt * (Jxw * ax[0] - dw * ax[3]) * iv;
+  o.Ao[1] = dt * (Jxw * ax[1] - dw * ax[4]) * iv;
+  o.Ao[2] = dt * (Jyw * ay[0] - dw * ay[3]) * iv;
+  o.Ao[3] = dt * (Jyw * ay[1] - dw * ay[4]) * iv;
+  o.bo = (om + dt * (gam * Pk)) * iv;
 }
 
 // ---------------------------------------------------------------------------
@@ -647,12 +657,17 @@ __global__ __launch_bounds__(kBlockX) void koCoeffs(
 // The assembled rows (diagonal 1, four neighbours) are not diagonally
 // dominant next to curved walls, so a stationary iteration can diverge; the
 // reference uses restarted GMRES (GMRESSolver.cpp:227-355).  Here both systems
-// run unpreconditioned BiCGStab (van der Vorst 1992) in lock step: three
-// streaming kernels per iteration with the neighbour values of p and s
-// recomputed in registers (so p, v are double-buffered and s needs no
-// separate pass), and a one-block kernel after each that turns the per-block
-// FP64 partial dot products into the scalars.  All scalars stay in device
-// memory; the host only polls the done flags between chunks of iterations.
+// run unpreconditioned BiCGStab (van der Vorst 1992) in lock step, iterating
+// in place on K and O: three streaming kernels per iteration with the
+// neighbour values of p and s recomputed in registers (so p, v are double-
+// buffered and s needs no separate pass), each followed by a one-block
+// kernel that turns the per-block FP64 partial dot products into the
+// scalars.  The streaming kernels run a fixed number of blocks that walk
+// their XCD's band of row tiles, so there are few partials to sum.  The
+// first residual r = b - A x is formed by the coefficient kernel itself.
+// All scalars stay in device memory; the host only polls the done flags.
+// (Cross-block reduction inside one kernel would need agent-scope release
+// fences, which write back the L2 and cost far more than a launch here.)
 
 struct BicgState {
   double rho[2], rhoNew[2], alpha[2], omega[2], beta[2];
@@ -666,21 +681,20 @@ struct BicgState {
   int cur;  // which p/v buffer holds the current iterate
 };
 
-// Solver vectors are FP64 (Q = double) for every storage form: the omega
-// field spans about seven decades (wall rows near 1000, core values near
-// 1e-4) and FP32 Krylov vectors cannot resolve the core next to the wall.
-template <class R, class Q>
+// Solver vectors are FP64 for every storage form: the omega field spans about
+// seven decades (wall rows near 1000, core values near 1e-4) and FP32 Krylov
+// vectors cannot resolve the core next to the wall.
 struct BicgVecs {
-  R* x[2];        // K, O in HBM
-  Q* xq[2];       // iterate
-  const R* A[2];  // 4 planes each: E, W, N, S
-  const R* b[2];
-  Q* r[2];
-  Q* rh[2];
-  Q* p[2][2];  // [variable][buffer]
-  Q* v[2][2];
-  Q* s[2];
-  Q* t[2];
+  double* x[2];        // K, O in HBM (the iterate)
+  double* A[2];        // 4 planes each: E, W, N, S
+  double* b[2];
+  double* r[2];
+  double* rh[2];
+  double* p[2][2];     // [variable][buffer]
+  double* v[2][2];
+  double* s[2];
+  double* t[2];
+  double* partial;     // 4 doubles per block
 };
 
 __device__ __forceinline__ void nbIndex(const KOParams& p, int x, size_t g, size_t (&nb)[4]) {
@@ -690,7 +704,20 @@ __device__ __forceinline__ void nbIndex(const KOParams& p, int x, size_t g, size
   nb[3] = g - p.XF;                            // S
 }
 
-// per-block sums of up to 4 doubles per variable -> partial[blk*8 + v*4 + k]
+// Calls f(x, y) for every node of the row tiles this block owns: the
+// blocks of XCD k = blockIdx % 8 walk the k-th band of tiles (so a stencil's
+// neighbour rows stay in that XCD's L2) with a stride of gridDim / 8.
+template <class F>
+__device__ __forceinline__ void forOwnTiles(const KOParams& p, F&& f) {
+  const int xcd = int(blockIdx.x & 7), stride = int(gridDim.x >> 3);
+  const int end = min((xcd + 1) * p.tiles.per, p.tiles.nTiles);
+  for (int T = xcd * p.tiles.per + int(blockIdx.x >> 3); T < end; T += stride) {
+    const int x = (T % p.tiles.tilesX) * kBlockX + int(threadIdx.x);
+    if (x < p.nX) f(x, T / p.tiles.tilesX);
+  }
+}
+
+// per-block sums of two doubles per variable -> partial[blk*4 + v*2 + k]
 __device__ __forceinline__ void blockPartials(double (&a)[2][2], double* __restrict__ partial) {
   __shared__ double red[2][2][kBlockX / 64];
 #pragma unroll
@@ -709,42 +736,7 @@ __device__ __forceinline__ void blockPartials(double (&a)[2][2], double* __restr
 #pragma unroll
     for (int w = 0; w < kBlockX / 64; w++) sum += red[v][k][w];
     partial[size_t(blockIdx.x) * 4 + threadIdx.x] = sum;
-    (void)v;
-    (void)k;
   }
-}
-
-// r = b - A x, rh = r, p = v = 0; partials (r, r).
-template <class R, class Q>
-__global__ __launch_bounds__(kBlockX) void bicgInit(const uint16_t* __restrict__ mask,
-                                                    BicgVecs<R, Q> V, double* __restrict__ partial,
-                                                    KOParams p) {
-  int bx = 0, y = 0;
-  const bool live = tileOf(p.tiles, bx, y);
-  const int x = bx * kBlockX + threadIdx.x;
-  double a[2][2] = {{0., 0.}, {0., 0.}};
-  if (live && x < p.nX) {
-    const size_t g = size_t(x) + size_t(p.XF) * y;
-    if (!(mask[g] & kSolid)) {
-      size_t nb[4];
-      nbIndex(p, x, g, nb);
-#pragma unroll
-      for (int v = 0; v < 2; v++) {
-        const R* A = V.A[v];
-        const R* X = V.x[v];
-        const Q ax = Q(X[g]) + (Q(A[g]) * Q(X[nb[0]]) + Q(A[g + p.D]) * Q(X[nb[1]]) +
-                                Q(A[g + 2 * p.D]) * Q(X[nb[2]]) + Q(A[g + 3 * p.D]) * Q(X[nb[3]]));
-        const Q r = Q(V.b[v][g]) - ax;
-        V.xq[v][g] = Q(X[g]);
-        V.r[v][g] = r;
-        V.rh[v][g] = r;
-        V.p[v][0][g] = Q(0);
-        V.v[v][0][g] = Q(0);
-        a[v][0] = double(r) * double(r);
-      }
-    }
-  }
-  blockPartials(a, partial);
 }
 
 __device__ __forceinline__ void sumPartials(const double* __restrict__ partial, int nblk,
@@ -769,10 +761,11 @@ __device__ __forceinline__ void sumPartials(const double* __restrict__ partial, 
 }
 
 // Scalar updates between the streaming kernels (one block of 1024 threads).
-// stage 0: after bicgInit, 1: after bicgPV (alpha), 2: after bicgST (omega),
-// 3: after bicgXR (rho, beta, convergence).
+// stage 0: after the coefficients (r0), 1: after bicgPV (alpha), 2: after
+// bicgST (omega), 3: after bicgXR (rho, beta, convergence).
 __global__ __launch_bounds__(1024) void bicgScalars(const double* __restrict__ partial, int nblk,
                                                     BicgState* __restrict__ st, int stage) {
+  if (st->done[0] && st->done[1]) return;
   double s[4];
   sumPartials(partial, nblk, s);
   if (threadIdx.x != 0) return;
@@ -780,14 +773,15 @@ __global__ __launch_bounds__(1024) void bicgScalars(const double* __restrict__ p
     if (st->done[v]) continue;
     const double a = s[2 * v], b = s[2 * v + 1];
     if (stage == 0) {
+      // a = (r, r); the first bicgPV takes p = r (restart form), rh = r
       const double r0 = sqrt(a);
       st->r0[v] = r0;
       st->rn[v] = r0;
       st->iters[v] = 0;
       st->rho[v] = st->alpha[v] = st->omega[v] = 1.;
-      st->rhoNew[v] = a;  // (rh, r) with rh = r
+      st->rhoNew[v] = a;
       st->beta[v] = a;
-      st->restart[v] = 0;
+      st->restart[v] = 1;
       if (!isfinite(r0)) {
         st->nan = 1;
         st->done[v] = 1;
@@ -837,126 +831,137 @@ __global__ __launch_bounds__(1024) void bicgScalars(const double* __restrict__ p
   if (stage == 3) st->cur ^= 1;
 }
 
+// Update_kOmega_Coeffs_Implicit (kOmegaKernels.cl:187-480): the normalised
+// rows of both systems, then the first residual r = b - A x from the current
+// K and O (the solve's initial guess); partials (r, r).
+template <class M>
+__global__ __launch_bounds__(kBlockX) void koCoeffs(
+    const uint16_t* __restrict__ mask, const double* __restrict__ Nut, const double* __restrict__ Dk,
+    const double* __restrict__ Do, const double* __restrict__ F1a, const double* __restrict__ dkdo,
+    const double* __restrict__ Sx, const M* __restrict__ rho, const M* __restrict__ uxa,
+    const M* __restrict__ uya, const double* __restrict__ dX, const double* __restrict__ Wd,
+    BicgVecs V, KOParams p) {
+  double a[2][2] = {{0., 0.}, {0., 0.}};
+  forOwnTiles(p, [&](int x, int y) {
+    const size_t g = size_t(x) + size_t(p.XF) * y;
+    const uint32_t m = mask[g];
+    if (m & kSolid) return;
+    const double* K = V.x[0];
+    const double* O = V.x[1];
+    KORows o;
+    koCoeffRow<M>(x, g, m, K, O, Nut, Dk, Do, F1a, dkdo, Sx, rho, uxa, uya, dX, Wd, p, o);
+    size_t nb[4];
+    nbIndex(p, x, g, nb);
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      const double* A = v == 0 ? o.Ak : o.Ao;
+      const double* X = V.x[v];
+#pragma unroll
+      for (int d = 0; d < 4; d++) V.A[v][g + size_t(d) * p.D] = A[d];
+      const double b = v == 0 ? o.bk : o.bo;
+      V.b[v][g] = b;
+      const double ax = X[g] + (A[0] * X[nb[0]] + A[1] * X[nb[1]] + A[2] * X[nb[2]] +
+                                A[3] * X[nb[3]]);
+      const double r = b - ax;
+      V.r[v][g] = r;
+      V.rh[v][g] = r;
+      a[v][0] += r * r;
+    }
+  });
+  blockPartials(a, V.partial);
+}
+
 // p' = r + beta (p - omega v) (neighbour values recomputed), v' = A p';
 // partials (rh, v').  On a restart p' = r and rh is reset to r.
-template <class R, class Q>
-__global__ __launch_bounds__(kBlockX) void bicgPV(const uint16_t* __restrict__ mask,
-                                                  BicgVecs<R, Q> V, const BicgState* __restrict__ st,
-                                                  double* __restrict__ partial, KOParams p) {
+__global__ __launch_bounds__(kBlockX) void bicgPV(const uint16_t* __restrict__ mask, BicgVecs V,
+                                                  const BicgState* __restrict__ st, KOParams p) {
+  if (st->done[0] && st->done[1]) return;
   const int cur = st->cur;
-  int bx = 0, y = 0;
-  const bool live = tileOf(p.tiles, bx, y);
-  const int x = bx * kBlockX + threadIdx.x;
   double a[2][2] = {{0., 0.}, {0., 0.}};
-  if (live && x < p.nX) {
+  forOwnTiles(p, [&](int x, int y) {
     const size_t g = size_t(x) + size_t(p.XF) * y;
-    if (!(mask[g] & kSolid)) {
-      size_t nb[4];
-      nbIndex(p, x, g, nb);
+    if (mask[g] & kSolid) return;
+    size_t nb[4];
+    nbIndex(p, x, g, nb);
 #pragma unroll
-      for (int v = 0; v < 2; v++) {
-        if (st->done[v]) continue;
-        const Q* r = V.r[v];
-        const Q* po = V.p[v][cur];
-        const Q* vo = V.v[v][cur];
-        const R* A = V.A[v];
-        const bool rs = st->restart[v] != 0;
-        const Q beta = Q(st->beta[v]), om = Q(st->omega[v]);
-        auto pn = [&](size_t i) { return rs ? r[i] : r[i] + beta * (po[i] - om * vo[i]); };
-        const Q pc = pn(g);
-        const Q vc = pc + (Q(A[g]) * pn(nb[0]) + Q(A[g + p.D]) * pn(nb[1]) +
-                           Q(A[g + 2 * p.D]) * pn(nb[2]) + Q(A[g + 3 * p.D]) * pn(nb[3]));
-        V.p[v][cur ^ 1][g] = pc;
-        V.v[v][cur ^ 1][g] = vc;
-        Q rh = V.rh[v][g];
-        if (rs) {
-          rh = r[g];
-          V.rh[v][g] = rh;
-        }
-        a[v][0] = double(rh) * double(vc);
+    for (int v = 0; v < 2; v++) {
+      if (st->done[v]) continue;
+      const double* r = V.r[v];
+      const double* po = V.p[v][cur];
+      const double* vo = V.v[v][cur];
+      const double* A = V.A[v];
+      const bool rs = st->restart[v] != 0;
+      const double beta = st->beta[v], om = st->omega[v];
+      auto pn = [&](size_t i) { return rs ? r[i] : r[i] + beta * (po[i] - om * vo[i]); };
+      const double pc = pn(g);
+      const double vc = pc + (A[g] * pn(nb[0]) + A[g + p.D] * pn(nb[1]) +
+                              A[g + 2 * p.D] * pn(nb[2]) + A[g + 3 * p.D] * pn(nb[3]));
+      V.p[v][cur ^ 1][g] = pc;
+      V.v[v][cur ^ 1][g] = vc;
+      double rh = V.rh[v][g];
+      if (rs) {
+        rh = r[g];
+        V.rh[v][g] = rh;
       }
+      a[v][0] += rh * vc;
     }
-  }
-  blockPartials(a, partial);
+  });
+  blockPartials(a, V.partial);
 }
 
 // s = r - alpha v' (neighbours recomputed), t = A s; partials (t, s), (t, t).
-template <class R, class Q>
-__global__ __launch_bounds__(kBlockX) void bicgST(const uint16_t* __restrict__ mask,
-                                                  BicgVecs<R, Q> V, const BicgState* __restrict__ st,
-                                                  double* __restrict__ partial, KOParams p) {
+__global__ __launch_bounds__(kBlockX) void bicgST(const uint16_t* __restrict__ mask, BicgVecs V,
+                                                  const BicgState* __restrict__ st, KOParams p) {
+  if (st->done[0] && st->done[1]) return;
   const int nxt = st->cur ^ 1;
-  int bx = 0, y = 0;
-  const bool live = tileOf(p.tiles, bx, y);
-  const int x = bx * kBlockX + threadIdx.x;
   double a[2][2] = {{0., 0.}, {0., 0.}};
-  if (live && x < p.nX) {
+  forOwnTiles(p, [&](int x, int y) {
     const size_t g = size_t(x) + size_t(p.XF) * y;
-    if (!(mask[g] & kSolid)) {
-      size_t nb[4];
-      nbIndex(p, x, g, nb);
+    if (mask[g] & kSolid) return;
+    size_t nb[4];
+    nbIndex(p, x, g, nb);
 #pragma unroll
-      for (int v = 0; v < 2; v++) {
-        if (st->done[v]) continue;
-        const Q* r = V.r[v];
-        const Q* vv = V.v[v][nxt];
-        const R* A = V.A[v];
-        const Q al = Q(st->alpha[v]);
-        auto sn = [&](size_t i) { return r[i] - al * vv[i]; };
-        const Q sc = sn(g);
-        const Q tc = sc + (Q(A[g]) * sn(nb[0]) + Q(A[g + p.D]) * sn(nb[1]) +
-                           Q(A[g + 2 * p.D]) * sn(nb[2]) + Q(A[g + 3 * p.D]) * sn(nb[3]));
-        V.s[v][g] = sc;
-        V.t[v][g] = tc;
-        a[v][0] = double(tc) * double(sc);
-        a[v][1] = double(tc) * double(tc);
-      }
+    for (int v = 0; v < 2; v++) {
+      if (st->done[v]) continue;
+      const double* r = V.r[v];
+      const double* vv = V.v[v][nxt];
+      const double* A = V.A[v];
+      const double al = st->alpha[v];
+      auto sn = [&](size_t i) { return r[i] - al * vv[i]; };
+      const double sc = sn(g);
+      const double tc = sc + (A[g] * sn(nb[0]) + A[g + p.D] * sn(nb[1]) +
+                              A[g + 2 * p.D] * sn(nb[2]) + A[g + 3 * p.D] * sn(nb[3]));
+      V.s[v][g] = sc;
+      V.t[v][g] = tc;
+      a[v][0] += tc * sc;
+      a[v][1] += tc * tc;
     }
-  }
-  blockPartials(a, partial);
+  });
+  blockPartials(a, V.partial);
 }
 
 // x += alpha p' + omega s, r = s - omega t; partials (rh, r), (r, r).
-template <class R, class Q>
-__global__ __launch_bounds__(kBlockX) void bicgXR(const uint16_t* __restrict__ mask,
-                                                  BicgVecs<R, Q> V, const BicgState* __restrict__ st,
-                                                  double* __restrict__ partial, KOParams p) {
+__global__ __launch_bounds__(kBlockX) void bicgXR(const uint16_t* __restrict__ mask, BicgVecs V,
+                                                  const BicgState* __restrict__ st, KOParams p) {
+  if (st->done[0] && st->done[1]) return;
   const int nxt = st->cur ^ 1;
-  int bx = 0, y = 0;
-  const bool live = tileOf(p.tiles, bx, y);
-  const int x = bx * kBlockX + threadIdx.x;
   double a[2][2] = {{0., 0.}, {0., 0.}};
-  if (live && x < p.nX) {
+  forOwnTiles(p, [&](int x, int y) {
     const size_t g = size_t(x) + size_t(p.XF) * y;
-    if (!(mask[g] & kSolid)) {
+    if (mask[g] & kSolid) return;
 #pragma unroll
-      for (int v = 0; v < 2; v++) {
-        if (st->done[v]) continue;
-        const Q al = Q(st->alpha[v]), om = Q(st->omega[v]);
-        const Q sc = V.s[v][g];
-        V.xq[v][g] += al * V.p[v][nxt][g] + om * sc;
-        const Q r = sc - om * V.t[v][g];
-        V.r[v][g] = r;
-        a[v][0] = double(V.rh[v][g]) * double(r);
-        a[v][1] = double(r) * double(r);
-      }
+    for (int v = 0; v < 2; v++) {
+      if (st->done[v]) continue;
+      const double al = st->alpha[v], om = st->omega[v];
+      const double sc = V.s[v][g];
+      V.x[v][g] += al * V.p[v][nxt][g] + om * sc;
+      const double r = sc - om * V.t[v][g];
+      V.r[v][g] = r;
+      a[v][0] += V.rh[v][g] * r;
+      a[v][1] += r * r;
     }
-  }
-  blockPartials(a, partial);
-}
-
-// Writes the iterate back to K / O.
-template <class R, class Q>
-__global__ __launch_bounds__(kBlockX) void bicgFinish(const uint16_t* __restrict__ mask,
-                                                      BicgVecs<R, Q> V, KOParams p) {
-  int bx = 0, y = 0;
-  if (!tileOf(p.tiles, bx, y)) return;
-  const int x = bx * kBlockX + threadIdx.x;
-  if (x >= p.nX) return;
-  const size_t g = size_t(x) + size_t(p.XF) * y;
-  if (mask[g] & kSolid) return;
-  V.x[0][g] = R(V.xq[0][g]);
-  V.x[1][g] = R(V.xq[1][g]);
+  });
+  blockPartials(a, V.partial);
 }
 
 // Resets the solver state for a new solve (a kernel, not a memset, so the
@@ -978,6 +983,30 @@ __global__ void bicgReset(BicgState* __restrict__ st, double relK, double absK, 
 // wall shear
 // ---------------------------------------------------------------------------
 
+// nu_e / (3 nu_e + 1/2) with nu_e = nu_t + nu
+template <class R>
+__device__ __forceinline__ R shearScale(R nut, R nu) {
+  R ve = nut + nu;
+  return ve * (R(1) / (R(3) * ve + R(0.5)));
+}
+
+// Signed traction magnitude of one wall node from its newest populations g,
+// the stored density r and velocity (ux + F/2, uy), and its 16 coefficients.
+template <class R>
+__device__ __forceinline__ R wallTraction(const R (&g)[9], R r, R ux, R uy,
+                                          const R* __restrict__ cf, R ve) {
+  const R dr = r - R(1);
+  R tx = R(0), ty = R(0);
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    const R fn = g[a + 1] - geq<R>(a + 1, dr, r, ux, uy);
+    tx += fn * (cf[a] * ve);
+    ty += fn * (cf[8 + a] * ve);
+  }
+  const R t = sqrt(tx * tx + ty * ty) / r;
+  return tx < R(0) ? -t : t;
+}
+
 // TR_shear_1 (trKernelsShear.cl:174-222): traction at wall nodes from the
 // newest populations (with the pending interpolated bounce-back) and the
 // stored macros.
@@ -994,19 +1023,8 @@ __global__ __launch_bounds__(64) void shearNodes(
   const int x = gi % p.XF, y = gi / p.XF;
   R g[9];
   loadNode<S, R>(g, src, size_t(gi), x, y, mask[gi], ibbIdx, ibbQ, p);
-  const R r = rho[gi], ux = uxa[gi], uy = uya[gi];
-  R ve = (nut ? R(nut[gi]) : R(0)) + p.nu;
-  ve *= R(1) / (R(3) * ve + R(0.5));
-  const R dr = r - R(1);
-  R tx = R(0), ty = R(0);
-#pragma unroll
-  for (int a = 0; a < 8; a++) {
-    const R fn = g[a + 1] - geq<R>(a + 1, dr, r, ux, uy);
-    tx += fn * (cfss[16 * size_t(n) + a] * ve);
-    ty += fn * (cfss[16 * size_t(n) + 8 + a] * ve);
-  }
-  const R t = sqrt(tx * tx + ty * ty) / r;
-  tau[n] = tx < R(0) ? -t : t;
+  tau[n] = wallTraction<R>(g, rho[gi], uxa[gi], uya[gi], cfss + 16 * size_t(n),
+                           shearScale<R>(nut ? R(nut[gi]) : R(0), p.nu));
 }
 
 // TR_shear_2 (trKernelsShear.cl:300-316).

@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libkofm.so")
+# KOFM_LIBRARY names another build of the library (for example one compiled
+# with different code-generation flags); the default is the in-tree build.
+LIB_PATH = os.environ.get("KOFM_LIBRARY") or os.path.join(HERE, "libkofm.so")
 
 STORE_F32, STORE_F16, STORE_BF16, STORE_F64 = 0, 1, 2, 3
 STORAGE_NAMES = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16, "f64": STORE_F64}
@@ -32,7 +34,7 @@ GEOM = {
     "ss": (4, np.int32), "ssMap": (5, np.int32), "ssInds": (6, np.int32),
     "lsMap": (7, np.uint16), "wallD": (8, np.float64), "cfss": (9, np.float64),
     "sInd": (10, np.int32), "weights": (11, np.float64), "bounds": (12, np.int32),
-    "masses": (13, np.float64),
+    "masses": (13, np.float64), "ibbSeg": (14, np.int32), "ibbDqDy": (15, np.float64),
 }
 
 # Every symbol the header declares; tests check the library exports them all.
@@ -42,6 +44,7 @@ EXPORTS = [
     "kofm_upload_state", "kofm_step", "kofm_time_steps", "kofm_wall_shear",
     "kofm_get_solver_stats", "kofm_reduce", "kofm_download", "kofm_save2file",
     "kofm_synchronize", "kofm_set_launch", "kofm_autotune", "kofm_set_graphs",
+    "kofm_objective_run", "kofm_adjoint",
     "kofm_ensemble_create", "kofm_ensemble_step", "kofm_ensemble_time_steps",
     "kofm_ensemble_set_block", "kofm_ensemble_last_error", "kofm_ensemble_destroy",
 ]
@@ -81,6 +84,11 @@ class GeomCounts(C.Structure):
     _fields_ = [("nX", C.c_int32), ("nY", C.c_int32), ("XsizeFull", C.c_int32), ("D", C.c_int64),
                 ("nIbb", C.c_int32), ("nSS", C.c_int32), ("nBLshear", C.c_int32),
                 ("nIbbNodes", C.c_int32)]
+
+
+class Objective(C.Structure):
+    _fields_ = [("window", C.c_int32), ("x_in", C.c_int32), ("x_out", C.c_int32),
+                ("reserved", C.c_int32), ("shear_weight", C.c_double)]
 
 
 class SolverStats(C.Structure):
@@ -126,6 +134,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "kofm_synchronize": (C.c_int, [P]),
         "kofm_set_launch": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int64]),
         "kofm_set_graphs": (C.c_int, [P, C.c_int32]),
+        "kofm_objective_run": (C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(Objective), dp]),
+        "kofm_adjoint": (C.c_int, [P, dp, dp]),
         "kofm_ensemble_create": (C.c_int, [C.POINTER(P), C.c_int32, C.POINTER(P)]),
         "kofm_ensemble_step": (C.c_int, [P, C.c_int32, C.c_uint32]),
         "kofm_ensemble_time_steps": (C.c_int, [P, C.c_int32, C.c_uint32, dp,
@@ -208,6 +218,7 @@ class Lattice:
         self._check(self.lib.kofm_build_geometry(
             self.h, _dptr(C0), C0.shape[0], BL.ctypes.data_as(C.POINTER(C.c_int32)), BL.shape[0],
             C.byref(p)))
+        self._nN = C0.shape[0]
 
     def counts(self) -> GeomCounts:
         c = GeomCounts()
@@ -220,7 +231,7 @@ class Lattice:
         n = {"nType": c.D, "dX": 4 * c.D, "ibb": c.nIbb, "ibbQ": c.nIbb, "ss": c.nSS,
              "ssMap": c.D, "ssInds": c.nX, "lsMap": 2 * c.XsizeFull, "wallD": c.D,
              "cfss": 16 * c.nSS, "sInd": 4 * c.nBLshear, "weights": 4 * c.nBLshear,
-             "bounds": 4, "masses": 2}[name]
+             "bounds": 4, "masses": 2, "ibbSeg": c.nIbb, "ibbDqDy": 2 * c.nIbb}[name]
         out = np.zeros(max(n, 1), dtype=dt)
         self._check(self.lib.kofm_geometry_array(self.h, which, out.ctypes.data_as(C.c_void_p)))
         return out[:n]
@@ -281,6 +292,23 @@ class Lattice:
 
     def set_graphs(self, on: bool):
         self._check(self.lib.kofm_set_graphs(self.h, 1 if on else 0))
+
+    def objective_run(self, nsteps: int, interval: int, window: int, x_in: int = 0,
+                      x_out: int | None = None, shear_weight: float = 0.1) -> float:
+        """Runs nsteps accumulating the wall objective J over the last `window`
+        steps and keeping checkpoints every `interval` steps; returns J."""
+        o = Objective(window, x_in, self.nX - 1 if x_out is None else x_out, 0, shear_weight)
+        J = C.c_double()
+        self._check(self.lib.kofm_objective_run(self.h, nsteps, interval, C.byref(o), C.byref(J)))
+        return J.value
+
+    def adjoint(self):
+        """dJ/dy of every wall point and dJ/dq of every boundary link for the
+        last objective_run."""
+        dy = np.zeros(self._nN, dtype=np.float64)
+        dq = np.zeros(max(self.counts().nIbb, 1), dtype=np.float64)
+        self._check(self.lib.kofm_adjoint(self.h, _dptr(dy), _dptr(dq)))
+        return dy, dq[:self.counts().nIbb]
 
     def synchronize(self):
         self._check(self.lib.kofm_synchronize(self.h))

@@ -1,0 +1,155 @@
+"""GPU tests of the wall-shape adjoint (through the C ABI).
+
+* The objective J accumulated on the GPU equals J recomputed from the FP64
+  oracle's own trajectory (pressure columns and boundary-link wall shear).
+* The FP64 adjoint gradient matches central finite differences of J
+  (h = 1e-3 node on a control point) to 1e-4 relative on the components
+  whose two evaluations keep the node types, the link set, the wall nodes
+  and the shear neighbours of the base geometry (the gradient holds those
+  fixed, as such a difference does), move no link's q across 1/2 (where
+  the bounce-back interpolation changes formula) and give the same difference
+  with a 10x smaller step (J holds sign(tau_x)|tau| of the wall nodes, which
+  is not smooth where tau_x changes sign).
+* The FP32 adjoint agrees with the FP64 adjoint.
+* torch.autograd through sensitivity.wall_objective returns the adjoint.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from komegafoulingmodel_amd import sensitivity as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(storage, **kw):
+    p = dict(nX=192, nY=48, H=36.0, amp=3.6, period=96.0, nu=0.02, Re=200.0, n_ctrl=8,
+             steps=300, window=100, interval=16, storage=storage, seed=5)
+    p.update(kw)
+    return S.WallShapeProblem(**p)
+
+
+def _topology(P, theta):
+    """The piecewise-constant parts of the geometry the gradient holds fixed."""
+    L = P.lattice()
+    P.build(theta)
+    return tuple(L.geometry(k).copy() for k in ("nType", "ibb", "ss", "sInd", "cfss"))
+
+
+def _straddles_kink(P, theta, j, h):
+    """True if moving control point j by +-h carries some link's q across
+    1/2, where the interpolated bounce-back switches formula (the objective
+    is continuous there but its derivative jumps)."""
+    L = P.lattice()
+    P.build(theta)
+    C0, _ = P.points(theta)
+    q, seg = L.geometry("ibbQ"), L.geometry("ibbSeg")
+    dq = L.geometry("ibbDqDy").reshape(-1, 2)
+    dth = P.dy_dtheta(C0)[:, j]
+    slope = dq[:, 0] * dth[seg] + dq[:, 1] * dth[seg + 1]
+    return bool(np.any(np.abs(q - 0.5) <= 2.0 * h * np.abs(slope)))
+
+
+def test_objective_matches_oracle_trajectory():
+    P = _problem("f64")
+    th = P.initial_theta()
+    J = P.objective(th)
+    C0, BL = P.points(th)
+    g = oracle.geometry(P.nX, P.nY, C0, BL, **P.geom_params())
+    orc = oracle.OracleLattice(g, tau0=P.tau0, Fval=P.Fval, nu=P.nu)
+    L = P.lattice()
+    P.build(th)
+    nType = L.geometry("nType")
+    fluid = (nType.astype(np.int32) & 1) == 0
+    rng = np.random.Generator(np.random.PCG64(P.seed + 1))
+    ux = P.Ub * (1.0 + rng.uniform(-P.noise, P.noise, L.D))
+    uy = P.Ub * rng.uniform(-P.noise, P.noise, L.D)
+    orc.init_state(np.where(fluid, 1.0, 0.0), np.where(fluid, ux, 0.0), np.where(fluid, uy, 0.0))
+    XF = g["XF"]
+    half = len(BL) // 2
+    b = g["bounds"]
+    nb = int(b[1] + b[3])
+    links = [k + b[0] if k < b[1] else k + b[2] - b[1] for k in range(nb)]
+    links = [i if i < half else (len(BL) - 1 - i) + half for i in links]
+    blen = np.array([np.hypot(*(C0[BL[i][1]] - C0[BL[i][0]])) for i in links])
+    cols = [np.array([x + XF * y for y in range(P.nY) if fluid[x + XF * y]]) for x in (0, P.nX - 1)]
+    acc = 0.0
+    for n in range(P.steps):
+        orc.step(1)
+        if n >= P.steps - P.window:
+            _, tb = orc.shear()
+            acc += (orc.rho[cols[0]].mean() - orc.rho[cols[1]].mean()) / 3.0
+            acc += P.shear_weight * float(np.sum(tb * blen))
+    Jo = acc / P.window
+    print(f"J gpu {J:.12e} oracle {Jo:.12e}", flush=True)
+    assert abs(J - Jo) <= 1e-9 * abs(Jo)
+    P.close()
+
+
+def test_adjoint_fp64_matches_finite_differences():
+    P = _problem("f64")
+    th = P.initial_theta()
+    J0 = P.objective(th)
+    g = P.gradient()
+    base = _topology(P, th)
+    h = 1e-3
+    order = np.argsort(-np.abs(g))
+    checked = []
+    for j in order:
+        e = np.zeros_like(th)
+        e[j] = h
+        tp, tm = _topology(P, th + e), _topology(P, th - e)
+        if not all(np.array_equal(a, b) for t in (tp, tm) for a, b in zip(t, base)):
+            continue
+        if _straddles_kink(P, th, j, h):
+            print(f"theta[{j}] skipped: a link's q crosses 1/2 within +-h", flush=True)
+            continue
+        fd = (P.objective(th + e) - P.objective(th - e)) / (2 * h)
+        rel = abs(g[j] - fd) / abs(fd)
+        # the signed traction sign(tau_x) |tau| is not smooth where a wall
+        # node's tau_x changes sign during the window; a difference that
+        # spans such a point differs from its own 10x smaller-step value
+        fd4 = (P.objective(th + 0.1 * e) - P.objective(th - 0.1 * e)) / (0.2 * h)
+        rel4 = abs(g[j] - fd4) / abs(fd4)
+        print(f"theta[{j}] adjoint {g[j]:.10e} fd(1e-3) {fd:.10e} rel {rel:.2e} "
+              f"fd(1e-4) {fd4:.10e} rel {rel4:.2e}", flush=True)
+        if rel >= 1e-4 and abs(fd - fd4) > 10 * abs(g[j] - fd4):
+            print(f"theta[{j}] skipped: J is not smooth within +-h", flush=True)
+            continue
+        checked.append(rel)
+        if len(checked) == 6:
+            break
+    print("J0", J0, "checked", len(checked), flush=True)
+    assert len(checked) >= 4
+    assert max(checked) < 1e-4
+    P.close()
+
+
+def test_adjoint_fp32_agrees_with_fp64():
+    th = _problem("f64").initial_theta()
+    grads = {}
+    for st in ("f64", "f32"):
+        P = _problem(st)
+        P.objective(th)
+        grads[st] = P.gradient()
+        P.close()
+    err = np.max(np.abs(grads["f32"] - grads["f64"])) / np.max(np.abs(grads["f64"]))
+    print(f"fp32 vs fp64 adjoint gradient {err:.2e}", flush=True)
+    assert err < 1e-3
+
+
+def test_torch_autograd_returns_the_adjoint():
+    import torch
+    P = _problem("f64")
+    th = P.initial_theta()
+    dev = "cuda" if torch.cuda.is_available() else "cpu"
+    t = torch.tensor(th, dtype=torch.float64, device=dev, requires_grad=True)
+    J = S.wall_objective(t, P)
+    J.backward()
+    P2 = _problem("f64")
+    J2 = P2.objective(th)
+    g2 = P2.gradient()
+    assert abs(J.item() - J2) <= 1e-14 * abs(J2)
+    assert np.array_equal(t.grad.cpu().numpy(), g2)
+    P.close()
+    P2.close()

@@ -109,22 +109,46 @@ def test_reductions_match_oracle():
             assert abs(a - b) <= 1e-11 * max(1.0, abs(b)), (field, op, a, b)
 
 
+# nu_t is limited by F2 |S| near the walls, and S comes from the non-
+# equilibrium part of the stored populations: with FP32 storage that
+# difference loses about eps32 * |g| / |g - geq| ~ 1e-4 relative, so nu_t gets
+# its own bound; populations, velocity, k and omega keep the 1e-5 bound.
+TOL32_NUT = 1e-3
+
+
 def test_komega_fp32_matches_oracle():
+    """FP32 populations with the FP64 k-omega update against the FP64 oracle.
+
+    Steps 1 and 5 are held to 1e-5 (nu_t to TOL32_NUT).  Past that the start-
+    up of this k-omega system amplifies any rounding: the oracle itself, run
+    from its initial populations rounded once to FP32, differs from the
+    unrounded run by ~1e-2 in k after 20 steps.  At step 20 the GPU result
+    must therefore stay within three times that envelope of the oracle."""
     w = _mini_wavy_ko()
     L, orc = make_pair(w, "f32")
+    _, env = make_pair(w, "f32", oracle_only=True)
+    env.FA[:] = env.FA.astype(np.float32).astype(np.float64)
     m = fluid_mask(orc.nType)
-    for n in (1, 4, 15):
+    fields = (("k", "K"), ("omega", "O"), ("nut", "nut"))
+    for total, n in ((1, 1), (5, 4), (20, 15)):
         L.step(n, 2)
         orc.step(n)
+        env.step(n)
         st = L.solver_stats()
-        errs = {k: rel_err(L.download(k)[m], getattr(orc, a)[m])
-                for k, a in (("k", "K"), ("omega", "O"), ("nut", "nut"))}
+        errs = {k: rel_err(L.download(k)[m], getattr(orc, a)[m]) for k, a in fields}
         errs["u"] = vec_rel_err(L.download("ux")[m], L.download("uy")[m], orc.ux[m], orc.uy[m])
         errs["f"] = rel_err(L.download("f"), orc.FA)
-        print("ko", n, {k: f"{v:.2e}" for k, v in errs.items()},
+        envs = {k: rel_err(getattr(env, a)[m], getattr(orc, a)[m]) for k, a in fields}
+        envs["u"] = vec_rel_err(env.ux[m], env.uy[m], orc.ux[m], orc.uy[m])
+        envs["f"] = rel_err(env.FA, orc.FA)
+        print("ko step", total, {k: f"{v:.2e}" for k, v in errs.items()},
+              "envelope", {k: f"{v:.2e}" for k, v in envs.items()},
               "iters", st.kappa_iters, st.omega_iters, flush=True)
         for k, v in errs.items():
-            assert v < TOL32, (k, v)
+            bound = TOL32_NUT if k == "nut" else TOL32
+            if total == 20:
+                bound = max(bound, 3.0 * envs[k])
+            assert v < bound, (total, k, v, envs[k])
 
 
 def test_komega_fp64_matches_oracle():
