@@ -65,7 +65,7 @@ def measure(L, fluid, steps, warmup, blu, options=0):
                fluid_nodes=fluid, steps=steps)
     if getattr(L, "launch", None):
         out["launch"] = {"block_x": L.launch[0], "nodes_per_lane": L.launch[1],
-                         "plane_pad": L.launch[2]}
+                         "steps_per_pass": L.launch[4], "plane_pad": L.launch[2]}
     return out
 
 
@@ -107,9 +107,10 @@ def measure_adjoint(**kw):
 
 
 def measure_ensemble(storage, steps=300):
-    """48 channels of uneven size, Re and fouling in one launch per step
-    (graph-captured blocks of 32 steps) against the same channels stepped one
-    after another, each with its own captured graphs."""
+    """48 channels of uneven size, Re and fouling stepped by the ensemble
+    (graph-captured blocks of 32 steps; one launch per step over all members,
+    and members packed into groups that fit the last-level cache) against the
+    same channels stepped one after another, each with its own graphs."""
     from komegafoulingmodel_amd import channel, lib
 
     ws = channel.ensemble_members()
@@ -129,18 +130,25 @@ def measure_ensemble(storage, steps=300):
         L.synchronize()
     t_seq = time.perf_counter() - t0
     ens = lib.Ensemble(Ls)
-    ens.step(64)
-    ms, nodes = ens.time_steps(steps)
+    out = {"channels": len(ws), "fluid_nodes": fluid, "steps": steps,
+           "sequential_mlups": fluid * steps / t_seq / 1e6}
+    # one launch per step over all members, then cache-sized groups (default)
+    for label, budget in (("one_group", 0), ("grouped", 160 << 20)):
+        groups = ens.set_cache_budget(budget)
+        ens.step(64)
+        ms, nodes = ens.time_steps(steps)
+        out[label] = {"groups": groups, "mlups": fluid / (ms * 1e-3) / 1e6, "ms_per_step": ms,
+                      "speedup_vs_sequential": (t_seq / steps) / (ms * 1e-3)}
+    out["mlups"] = out["grouped"]["mlups"]
     ens.close()
     for L in Ls:
         L.close()
-    return {"channels": len(ws), "fluid_nodes": fluid, "steps": steps,
-            "mlups": fluid / (ms * 1e-3) / 1e6, "ms_per_step": ms,
-            "sequential_mlups": fluid * steps / t_seq / 1e6,
-            "speedup_vs_sequential": (t_seq / steps) / (ms * 1e-3)}
+    return out
 
 
 def kernel_name(launch):
+    if launch.get("steps_per_pass", 1) == 2:
+        return f"lbCollideStreamT2<float,float,64,{launch['block_x'] // 64}>"
     k = "lbCollideStream2" if launch.get("nodes_per_lane", 1) == 2 else "lbCollideStream"
     return f"{k}<float,float,false,{launch['block_x']}>"
 

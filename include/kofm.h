@@ -250,19 +250,22 @@ int kofm_save2file(kofm_ctx* ctx, const char* dir);
 
 /* --- launch shape ---------------------------------------------------------- */
 
-/* Work-group width of the collide-stream kernel (64, 128, 256, 512 or
- * 1024), lattice nodes per lane (1, or 2 neighbouring nodes when nX is even)
- * and padding (elements, multiple of 64) between the nine population planes
- * in HBM.  The state is preserved.  No counterpart in the reference, whose
- * work-group is fixed at 128 x 1 (ConstDef.h:70-79). */
-int kofm_set_launch(kofm_ctx* ctx, int32_t block_x, int32_t nodes_per_lane, int64_t plane_pad);
+/* Launch shape of the plain collide-stream step: work-group width (64, 128,
+ * 256, 512 or 1024), lattice nodes per lane (1, or 2 neighbouring nodes when
+ * nX is even), steps per pass (1, or 2 for the temporally blocked kernel:
+ * k-omega off, nX % 64 == 0, one node per lane, width 512 or 1024 for a
+ * 64 x 8 or 64 x 16 tile) and the padding (elements, multiple of 64)
+ * between the nine population planes in HBM.  The state is preserved and
+ * every shape gives bit-identical results.  No counterpart in the reference,
+ * whose work-group is fixed at 128 x 1 (ConstDef.h:70-79). */
+int kofm_set_launch(kofm_ctx* ctx, int32_t block_x, int32_t nodes_per_lane,
+                    int32_t steps_per_pass, int64_t plane_pad);
 
-/* Times the collide-stream kernel over the candidate widths, nodes per lane
- * and paddings (steps launches each, on scratch populations at rest) and
- * keeps the fastest; returns the choice and its time per step.  No
- * counterpart in the reference. */
+/* Times the collide-stream kernels over the candidate shapes (about `steps`
+ * steps each, on scratch populations at rest) and keeps the fastest; returns
+ * the choice and its time per step.  No counterpart in the reference. */
 int kofm_autotune(kofm_ctx* ctx, int32_t steps, int32_t* block_x, int32_t* nodes_per_lane,
-                  int64_t* plane_pad, double* ms_per_step);
+                  int32_t* steps_per_pass, int64_t* plane_pad, double* ms_per_step);
 
 /* With on != 0 (the default) kofm_step replays a captured HIP graph of 32
  * collide-stream launches for each whole block of 32 plain (k-omega off)
@@ -309,8 +312,9 @@ int kofm_adjoint(kofm_ctx* ctx, double* dJ_dy, double* dJ_dq);
 
 /* Several independent channels (contexts with geometry and state, one
  * population storage type, k-omega off) stepped together: one collide-stream
- * launch per step covers every member's tiles, and whole blocks of 32 steps
- * replay a captured HIP graph.  Members keep their own arrays and are read
+ * launch per step covers every tile of a group of members, and whole blocks
+ * of 32 steps replay a captured HIP graph (see
+ * kofm_ensemble_set_cache_budget for the grouping).  Members keep their own arrays and are read
  * with kofm_download as usual; calls on a member are ordered after the
  * ensemble's steps.  The reference runs one channel per process; this has no
  * counterpart there. */
@@ -324,6 +328,13 @@ int kofm_ensemble_time_steps(kofm_ensemble* e, int32_t nsteps, uint32_t options,
                              double* ms_per_step, int64_t* fluid_nodes);
 /* Work-group width of the ensemble launch (128, 256, 512 or 1024). */
 int kofm_ensemble_set_block(kofm_ensemble* e, int32_t block_x);
+/* Members are packed (largest first) into groups whose population arrays
+ * (both buffers) total at most `bytes`; kofm_ensemble_step advances each
+ * group through all its steps before the next, so a group that fits the
+ * last-level cache (256 MB on MI355X) runs from it.  0 puts every member in
+ * one launch per step.  The default is 160 MiB.  *groups (may be NULL)
+ * receives the number of groups.  Results do not depend on the grouping. */
+int kofm_ensemble_set_cache_budget(kofm_ensemble* e, int64_t bytes, int32_t* groups);
 const char* kofm_ensemble_last_error(const kofm_ensemble* e);
 void kofm_ensemble_destroy(kofm_ensemble* e);
 

@@ -296,26 +296,29 @@ int kofm_save2file(kofm_ctx* ctx, const char* dir) {
   return rc ? setErr(ctx, rc, err) : KOFM_OK;
 }
 
-int kofm_set_launch(kofm_ctx* ctx, int32_t block_x, int32_t nodes_per_lane, int64_t plane_pad) {
+int kofm_set_launch(kofm_ctx* ctx, int32_t block_x, int32_t nodes_per_lane,
+                    int32_t steps_per_pass, int64_t plane_pad) {
   if (!ctx) return KOFM_ERR_ARG;
   NEED_ENGINE(ctx);
   std::string err;
-  int rc = ctx->engine->setLaunch(block_x, nodes_per_lane, plane_pad, err);
+  int rc = ctx->engine->setLaunch(block_x, nodes_per_lane, steps_per_pass, plane_pad, err);
   return rc ? setErr(ctx, rc, err) : KOFM_OK;
 }
 
 int kofm_autotune(kofm_ctx* ctx, int32_t steps, int32_t* block_x, int32_t* nodes_per_lane,
-                  int64_t* plane_pad, double* ms_per_step) {
-  if (!ctx || !block_x || !nodes_per_lane || !plane_pad || !ms_per_step) return KOFM_ERR_ARG;
+                  int32_t* steps_per_pass, int64_t* plane_pad, double* ms_per_step) {
+  if (!ctx || !block_x || !nodes_per_lane || !steps_per_pass || !plane_pad || !ms_per_step)
+    return KOFM_ERR_ARG;
   NEED_ENGINE(ctx);
   std::string err;
-  int b = 0, npt = 1;
+  int b = 0, npt = 1, spp = 1;
   int64_t pad = 0;
   double ms = 0.;
-  int rc = ctx->engine->autotune(steps, b, npt, pad, ms, err);
+  int rc = ctx->engine->autotune(steps, b, npt, spp, pad, ms, err);
   if (rc) return setErr(ctx, rc, err);
   *block_x = b;
   *nodes_per_lane = npt;
+  *steps_per_pass = spp;
   *plane_pad = pad;
   *ms_per_step = ms;
   return KOFM_OK;
@@ -403,6 +406,13 @@ int kofm_ensemble_time_steps(kofm_ensemble* e, int32_t nsteps, uint32_t options,
 int kofm_ensemble_set_block(kofm_ensemble* e, int32_t block_x) {
   if (!e) return KOFM_ERR_ARG;
   int rc = e->impl->setBlock(block_x, e->err);
+  return rc ? rc : KOFM_OK;
+}
+
+int kofm_ensemble_set_cache_budget(kofm_ensemble* e, int64_t bytes, int32_t* groups) {
+  if (!e) return KOFM_ERR_ARG;
+  int rc = e->impl->setCacheBudget(bytes, e->err);
+  if (!rc && groups) *groups = e->impl->groups();
   return rc ? rc : KOFM_OK;
 }
 

@@ -319,10 +319,16 @@ class EngineT final : public Engine {
         s += kGraphSteps;  // kGraphSteps is even: the buffer parity is unchanged
       }
     }
-    for (; s < n; s++) {
+    while (s < n) {
+      if (!ko_ && lbSpp_ == 2 && n - s >= 2) {
+        launchLB2(options);
+        s += 2;
+        continue;
+      }
       int rc = launchLB(options);
       if (!rc && ko_) rc = koStep(err);
       if (rc) return rc;
+      s++;
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -342,10 +348,18 @@ class EngineT final : public Engine {
       evs_.push_back(e);
     }
     HIP_TRY(hipEventRecord(evs_[0], stream_));
-    for (int s = 0; s < n; s++) {
-      HIP_TRY(hipEventRecord(evs_[2 + 2 * s], stream_));
-      int rc = launchLB(options);
-      HIP_TRY(hipEventRecord(evs_[3 + 2 * s], stream_));
+    int launches = 0;
+    for (int s = 0; s < n; launches++) {
+      HIP_TRY(hipEventRecord(evs_[2 + 2 * launches], stream_));
+      int rc = 0;
+      if (!ko_ && lbSpp_ == 2 && n - s >= 2) {
+        launchLB2(options);
+        s += 2;
+      } else {
+        rc = launchLB(options);
+        s++;
+      }
+      HIP_TRY(hipEventRecord(evs_[3 + 2 * launches], stream_));
       if (!rc && ko_) rc = koStep(err);
       if (rc) return rc;
     }
@@ -355,11 +369,11 @@ class EngineT final : public Engine {
     HIP_TRY(hipEventElapsedTime(&t, evs_[0], evs_[1]));
     msStep = double(t) / n;
     double acc = 0.;
-    for (int s = 0; s < n; s++) {
-      HIP_TRY(hipEventElapsedTime(&t, evs_[2 + 2 * s], evs_[3 + 2 * s]));
+    for (int k = 0; k < launches; k++) {
+      HIP_TRY(hipEventElapsedTime(&t, evs_[2 + 2 * k], evs_[3 + 2 * k]));
       acc += t;
     }
-    msKernel = acc / n;
+    msKernel = acc / n;  // collide-stream time per step
     HIP_TRY(hipGetLastError());
     return 0;
   }
@@ -552,6 +566,7 @@ class EngineT final : public Engine {
                         // same HBM channels)
   int lbBlock_ = 512;   // work-group width of the collide-stream kernel
   int lbNpt_ = 1;       // lattice nodes per lane (1, or 2 neighbouring nodes)
+  int lbSpp_ = 1;       // plain steps per pass (2: the temporally blocked kernel)
   bool ko_;
   hipStream_t stream_ = nullptr;
   Buffer fA_, fB_, mask_, ibbIdx_, ibbQ_, rho_, ux_, uy_;
@@ -616,7 +631,10 @@ class EngineT final : public Engine {
     const int buf0 = curBuf_;
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < kGraphSteps; k++) launchLB(options);
+    if (lbSpp_ == 2)
+      for (int k = 0; k < kGraphSteps / 2; k++) launchLB2(options);
+    else
+      for (int k = 0; k < kGraphSteps; k++) launchLB(options);
     HIP_TRY(hipStreamEndCapture(stream_, &graph));
     hipGraphExec_t exec = nullptr;
     hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -1005,6 +1023,32 @@ class EngineT final : public Engine {
     launchLBBlock<false>(lbBlock_, lbNpt_, p, src, dst);
   }
 
+  void launchT2(int block, const dev::LBParams<R>& pA, const dev::LBParams<R>& pB, const S* src,
+                S* dst) {
+    if (block == 1024)
+      hipLaunchKernelGGL((dev::lbCollideStreamT2<S, R, 64, 16>), dim3(nX_ / 64, (nY_ + 15) / 16),
+                         dim3(1024), 0, stream_, src, dst, mask_.as<uint16_t>(),
+                         ibbIdx_.as<int32_t>(), ibbQ_.as<R>(), rho_.as<R>(), ux_.as<R>(),
+                         uy_.as<R>(), pA, pB);
+    else
+      hipLaunchKernelGGL((dev::lbCollideStreamT2<S, R, 64, 8>), dim3(nX_ / 64, (nY_ + 7) / 8),
+                         dim3(512), 0, stream_, src, dst, mask_.as<uint16_t>(),
+                         ibbIdx_.as<int32_t>(), ibbQ_.as<R>(), rho_.as<R>(), ux_.as<R>(),
+                         uy_.as<R>(), pA, pB);
+  }
+
+  // Two plain steps in one pass (k-omega off); the macros, if asked for,
+  // are the second step's.
+  void launchLB2(unsigned options) {
+    auto pA = lbParams(0);
+    pA.saveMacros = 0;
+    auto pB = lbParams(options);
+    pB.applyIbb = 1;
+    launchT2(lbBlock_, pA, pB, cur(), other());
+    curBuf_ ^= 1;
+    preIbb_ = true;
+  }
+
   int launchLB(unsigned options) {
     auto p = lbParams(options);
     if (ko_)
@@ -1051,16 +1095,21 @@ class EngineT final : public Engine {
   }
 
  public:
-  int setLaunch(int block, int npt, int64_t pad, std::string& err) override {
+  int setLaunch(int block, int npt, int spp, int64_t pad, std::string& err) override {
     if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
       return fail(err, "block width must be 64, 128, 256, 512 or 1024");
     if (npt != 1 && npt != 2) return fail(err, "nodes per lane must be 1 or 2");
     if (npt == 2 && nX_ % 2) return fail(err, "two nodes per lane need an even nX");
+    if (spp != 1 && spp != 2) return fail(err, "steps per pass must be 1 or 2");
+    if (spp == 2 && (ko_ || nX_ % 64 || npt != 1 || (block != 512 && block != 1024)))
+      return fail(err, "two steps per pass need k-omega off, nX % 64 == 0, one node per lane "
+                       "and a block of 512 or 1024 (a 64 x 8 or 64 x 16 tile)");
     if (pad < 0 || pad % 64) return fail(err, "plane padding must be a non-negative multiple of 64");
     int rc = relayout(pad, err);
     if (rc) return rc;
     lbBlock_ = block;
     lbNpt_ = npt;
+    lbSpp_ = spp;
     dropGraphs();
     return 0;
   }
@@ -1068,8 +1117,8 @@ class EngineT final : public Engine {
   // Times the collide-stream kernel (wall corrections on, k-omega and macro
   // output off) for each candidate work-group width and plane padding on
   // scratch populations at rest, keeps the fastest.
-  int autotune(int steps, int& bestBlock, int& bestNpt, int64_t& bestPad, double& bestMs,
-               std::string& err) override {
+  int autotune(int steps, int& bestBlock, int& bestNpt, int& bestSpp, int64_t& bestPad,
+               double& bestMs, std::string& err) override {
     if (!haveGeo_) return fail(err, "build the geometry before tuning");
     if (steps < 1) return fail(err, "steps must be positive");
     static const int blocks[] = {128, 256, 512, 1024};
@@ -1080,6 +1129,7 @@ class EngineT final : public Engine {
     bestMs = 1e30;
     bestBlock = lbBlock_;
     bestNpt = lbNpt_;
+    bestSpp = lbSpp_;
     bestPad = P_ - D_;
     int rc = 0;
     for (int64_t pad : pads) {
@@ -1109,6 +1159,29 @@ class EngineT final : public Engine {
             bestMs = per;
             bestBlock = block;
             bestNpt = npt;
+            bestSpp = 1;
+            bestPad = pad;
+          }
+        }
+      if (!ko_ && nX_ % 64 == 0)
+        for (int block : {512, 1024}) {
+          auto pA = p, pB = p;
+          pA.applyIbb = 1;
+          launchT2(block, pA, pB, a.as<S>(), b.as<S>());  // warm-up
+          const int pairs = std::max(1, steps / 2);
+          (void)hipEventRecord(e0, stream_);
+          for (int s = 0; s < pairs; s++)
+            launchT2(block, pA, pB, (s & 1) ? b.as<S>() : a.as<S>(), (s & 1) ? a.as<S>() : b.as<S>());
+          (void)hipEventRecord(e1, stream_);
+          (void)hipEventSynchronize(e1);
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          const double per = double(ms) / (2 * pairs);
+          if (per < bestMs) {
+            bestMs = per;
+            bestBlock = block;
+            bestNpt = 1;
+            bestSpp = 2;
             bestPad = pad;
           }
         }
@@ -1117,7 +1190,7 @@ class EngineT final : public Engine {
     (void)hipEventDestroy(e1);
     HIP_TRY(hipGetLastError());
     if (rc) return rc;
-    return setLaunch(bestBlock, bestNpt, bestPad, err);
+    return setLaunch(bestBlock, bestNpt, bestSpp, bestPad, err);
   }
 
  private:
@@ -1205,7 +1278,7 @@ class EnsembleT final : public Ensemble {
   explicit EnsembleT(std::vector<EngineT<S, R>*> m) : m_(std::move(m)) {}
   ~EnsembleT() override {
     if (stream_) (void)hipStreamSynchronize(stream_);
-    dropGraphs();
+    groups_.clear();
     if (done_) (void)hipEventDestroy(done_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
@@ -1213,9 +1286,7 @@ class EnsembleT final : public Ensemble {
   int create(std::string& err) {
     HIP_TRY(hipStreamCreate(&stream_));
     HIP_TRY(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
-    for (auto& b : ents_) HIP_TRY(b.alloc(m_.size() * sizeof(dev::BatchEntry<S, R>)));
-    HIP_TRY(eager_.alloc(m_.size() * sizeof(dev::BatchEntry<S, R>)));
-    return 0;
+    return regroup(err);
   }
 
   int64_t fluidNodes() const override {
@@ -1230,55 +1301,29 @@ class EnsembleT final : public Ensemble {
     if (block != 128 && block != 256 && block != 512 && block != 1024)
       return fail(err, "ensemble block width must be 128, 256, 512 or 1024");
     block_ = block;
-    dropGraphs();
-    built_ = false;
-    return 0;
+    return regroup(err);
   }
 
+  int setCacheBudget(int64_t bytes, std::string& err) override {
+    if (bytes < 0) return fail(err, "the cache budget must be >= 0");
+    budget_ = bytes;
+    return regroup(err);
+  }
+
+  int groups() const override { return int(groups_.size()); }
+
+  // Every group advances n steps before the next group starts: the members
+  // are independent, and a group whose populations fit the last-level cache
+  // then streams its steps from there.
   int step(int n, unsigned options, std::string& err) override {
     for (auto* e : m_) {
       if (!e->haveState_) return fail(err, "every member needs an initialised state");
       HIP_TRY(hipStreamSynchronize(e->stream_));
     }
-    int s = 0;
-    // first step after an upload runs without the pending wall correction
-    bool fresh = false;
-    for (auto* e : m_) fresh |= !e->preIbb_;
-    if (fresh && s < n) {
-      int rc = fill(eager_, options, 0, true, err);
+    for (auto& g : groups_) {
+      int rc = g->step(*this, n, options, err);
       if (rc) return rc;
-      launch(eager_);
-      advance(1);
-      s++;
     }
-    if (!built_ || builtOptions_ != options || builtParity_ != parity()) {
-      int rc = fill(ents_[0], options, 0, false, err);
-      if (!rc) rc = fill(ents_[1], options, 1, false, err);
-      if (rc) return rc;
-      dropGraphs();
-      built_ = true;
-      builtOptions_ = options;
-      builtParity_ = parity();
-    }
-    int k = 0;  // steps since the entries were built (selects ents_[k & 1])
-    while (n - s >= kGraphSteps) {
-      if (!graph_) {
-        hipGraph_t g = nullptr;
-        HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        for (int j = 0; j < kGraphSteps; j++) launch(ents_[j & 1]);
-        HIP_TRY(hipStreamEndCapture(stream_, &g));
-        hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        HIP_TRY(e);
-      }
-      HIP_TRY(hipGraphLaunch(graph_, stream_));
-      s += kGraphSteps;
-    }
-    for (; s < n; s++, k++) launch(ents_[k & 1]);
-    advance(k);
-    // the built entries assume every member's current buffer; an odd tail
-    // flips them, so rebuild next time
-    if (k & 1) built_ = false;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(done_, stream_));
     for (auto* e : m_) HIP_TRY(hipStreamWaitEvent(e->stream_, done_, 0));
@@ -1311,71 +1356,156 @@ class EnsembleT final : public Ensemble {
 
  private:
   static constexpr int kGraphSteps = 32;
+
+  // Members stepped by one launch per step, with their entry tables.
+  struct Group {
+    std::vector<EngineT<S, R>*> m;
+    Buffer ents[2], eager;
+    hipGraphExec_t graph = nullptr;
+    int totalTiles = 0;
+    bool built = false;
+    unsigned builtOptions = 0;
+    std::vector<int> builtParity;
+
+    ~Group() { dropGraph(); }
+    void dropGraph() {
+      if (graph) (void)hipGraphExecDestroy(graph);
+      graph = nullptr;
+    }
+    std::vector<int> parity() const {
+      std::vector<int> v;
+      for (auto* e : m) v.push_back(e->curBuf_);
+      return v;
+    }
+
+    int step(EnsembleT& en, int n, unsigned options, std::string& err) {
+      hipStream_t stream_ = en.stream_;
+      int s = 0;
+      // first step after an upload runs without the pending wall correction
+      bool fresh = false;
+      for (auto* e : m) fresh |= !e->preIbb_;
+      if (fresh && s < n) {
+        int rc = fill(en, eager, options, 0, true, err);
+        if (rc) return rc;
+        en.launch(eager, totalTiles, int(m.size()));
+        advance(1);
+        s++;
+      }
+      if (!built || builtOptions != options || builtParity != parity()) {
+        int rc = fill(en, ents[0], options, 0, false, err);
+        if (!rc) rc = fill(en, ents[1], options, 1, false, err);
+        if (rc) return rc;
+        dropGraph();
+        built = true;
+        builtOptions = options;
+        builtParity = parity();
+      }
+      int k = 0;  // steps since the entries were built (selects ents[k & 1])
+      while (n - s >= kGraphSteps) {
+        if (!graph) {
+          hipGraph_t g = nullptr;
+          HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+          for (int j = 0; j < kGraphSteps; j++) en.launch(ents[j & 1], totalTiles, int(m.size()));
+          HIP_TRY(hipStreamEndCapture(stream_, &g));
+          hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(g);
+          HIP_TRY(e);
+        }
+        HIP_TRY(hipGraphLaunch(graph, stream_));
+        s += kGraphSteps;
+      }
+      for (; s < n; s++, k++) en.launch(ents[k & 1], totalTiles, int(m.size()));
+      advance(k);
+      // the built entries assume every member's current buffer; an odd tail
+      // flips them, so rebuild next time
+      if (k & 1) built = false;
+      return 0;
+    }
+
+    // entries for a step taken `shift` steps after now
+    int fill(EnsembleT& en, Buffer& buf, unsigned options, int shift, bool eager, std::string& err) {
+      std::vector<dev::BatchEntry<S, R>> h(m.size());
+      int tiles = 0;
+      for (size_t i = 0; i < m.size(); i++) {
+        EngineT<S, R>* e = m[i];
+        const bool flip = (shift & 1) != 0;
+        dev::BatchEntry<S, R>& b = h[i];
+        b.src = flip ? e->other() : e->cur();
+        b.dst = flip ? e->cur() : e->other();
+        b.mask = e->mask_.template as<uint16_t>();
+        b.ibbIdx = e->ibbIdx_.template as<int32_t>();
+        b.ibbQ = e->ibbQ_.template as<R>();
+        b.rho = e->rho_.template as<R>();
+        b.ux = e->ux_.template as<R>();
+        b.uy = e->uy_.template as<R>();
+        b.p = e->lbParams(options);
+        b.p.applyIbb = eager ? (e->preIbb_ ? 1 : 0) : 1;
+        b.tilesX = (e->nX_ + en.block_ - 1) / en.block_;
+        b.tileStart = tiles;
+        tiles += b.tilesX * e->nY_;
+      }
+      totalTiles = tiles;
+      HIP_TRY(hipMemcpy(buf.as<void>(), h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice));
+      return 0;
+    }
+
+    void advance(int k) {
+      for (auto* e : m) {
+        if (k & 1) e->curBuf_ ^= 1;
+        if (k > 0) e->preIbb_ = true;
+        e->dropGraphs();
+      }
+    }
+  };
+
   std::vector<EngineT<S, R>*> m_;
+  std::vector<std::unique_ptr<Group>> groups_;
   hipStream_t stream_ = nullptr;
   hipEvent_t done_ = nullptr;
-  Buffer ents_[2], eager_;
-  hipGraphExec_t graph_ = nullptr;
   int block_ = 512;
-  int totalTiles_ = 0;
-  bool built_ = false;
-  unsigned builtOptions_ = 0;
-  std::vector<int> builtParity_;
+  // population bytes (both buffers) per group; 0 puts every member in one
+  // launch.  The default fits the 256 MB last-level cache with room to spare.
+  int64_t budget_ = int64_t(160) << 20;
 
   static int fail(std::string& err, const char* msg) {
     err = msg;
     return KOFM_ERR_STATE;
   }
-  void dropGraphs() {
-    if (graph_) (void)hipGraphExecDestroy(graph_);
-    graph_ = nullptr;
-  }
-  std::vector<int> parity() const {
-    std::vector<int> v;
-    for (auto* e : m_) v.push_back(e->curBuf_);
-    return v;
-  }
-  // entries for a step taken `shift` steps after now
-  int fill(Buffer& buf, unsigned options, int shift, bool eager, std::string& err) {
-    std::vector<dev::BatchEntry<S, R>> h(m_.size());
-    int tiles = 0;
-    for (size_t i = 0; i < m_.size(); i++) {
-      EngineT<S, R>* e = m_[i];
-      const bool flip = (shift & 1) != 0;
-      dev::BatchEntry<S, R>& b = h[i];
-      b.src = flip ? e->other() : e->cur();
-      b.dst = flip ? e->cur() : e->other();
-      b.mask = e->mask_.template as<uint16_t>();
-      b.ibbIdx = e->ibbIdx_.template as<int32_t>();
-      b.ibbQ = e->ibbQ_.template as<R>();
-      b.rho = e->rho_.template as<R>();
-      b.ux = e->ux_.template as<R>();
-      b.uy = e->uy_.template as<R>();
-      b.p = e->lbParams(options);
-      b.p.applyIbb = eager ? (e->preIbb_ ? 1 : 0) : 1;
-      b.tilesX = (e->nX_ + block_ - 1) / block_;
-      b.tileStart = tiles;
-      tiles += b.tilesX * e->nY_;
+
+  // First-fit decreasing packing of the members into groups under budget_.
+  int regroup(std::string& err) {
+    if (stream_) HIP_TRY(hipStreamSynchronize(stream_));
+    groups_.clear();
+    std::vector<int> order(m_.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = int(i);
+    auto bytes = [&](int i) { return 2 * 9 * m_[i]->P_ * int64_t(sizeof(S)); };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return bytes(a) > bytes(b); });
+    std::vector<int64_t> used;
+    for (int i : order) {
+      size_t g = 0;
+      if (budget_ > 0)
+        while (g < groups_.size() && used[g] + bytes(i) > budget_) g++;
+      if (g == groups_.size()) {
+        groups_.emplace_back(new Group);
+        used.push_back(0);
+      }
+      groups_[g]->m.push_back(m_[i]);
+      used[g] += bytes(i);
     }
-    totalTiles_ = tiles;
-    HIP_TRY(hipMemcpy(buf.as<void>(), h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice));
+    for (auto& g : groups_) {
+      for (Buffer* b : {&g->ents[0], &g->ents[1], &g->eager})
+        HIP_TRY(b->alloc(g->m.size() * sizeof(dev::BatchEntry<S, R>)));
+    }
     return 0;
   }
-  void launch(const Buffer& buf) {
+
+  void launch(const Buffer& buf, int totalTiles, int n) {
     const auto* ents = buf.as<const dev::BatchEntry<S, R>>();
-    const int n = int(m_.size());
     switch (block_) {
-      case 128: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 128>), dim3(totalTiles_), dim3(128), 0, stream_, ents, n); break;
-      case 256: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 256>), dim3(totalTiles_), dim3(256), 0, stream_, ents, n); break;
-      case 1024: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 1024>), dim3(totalTiles_), dim3(1024), 0, stream_, ents, n); break;
-      default: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 512>), dim3(totalTiles_), dim3(512), 0, stream_, ents, n); break;
-    }
-  }
-  void advance(int k) {
-    for (auto* e : m_) {
-      if (k & 1) e->curBuf_ ^= 1;
-      if (k > 0) e->preIbb_ = true;
-      e->dropGraphs();
+      case 128: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 128>), dim3(totalTiles), dim3(128), 0, stream_, ents, n); break;
+      case 256: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 256>), dim3(totalTiles), dim3(256), 0, stream_, ents, n); break;
+      case 1024: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 1024>), dim3(totalTiles), dim3(1024), 0, stream_, ents, n); break;
+      default: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 512>), dim3(totalTiles), dim3(512), 0, stream_, ents, n); break;
     }
   }
 };

@@ -51,9 +51,9 @@ class Engine {
   virtual int solverStats(SolverStats& s, std::string& err) = 0;
   virtual int synchronize(std::string& err) = 0;
   virtual int64_t fieldSize(int field) const = 0;
-  virtual int setLaunch(int block, int npt, int64_t pad, std::string& err) = 0;
+  virtual int setLaunch(int block, int npt, int spp, int64_t pad, std::string& err) = 0;
   virtual void setGraphs(bool on) = 0;
-  virtual int autotune(int steps, int& block, int& npt, int64_t& pad, double& ms,
+  virtual int autotune(int steps, int& block, int& npt, int& spp, int64_t& pad, double& ms,
                        std::string& err) = 0;
   // Runs n steps accumulating J and keeping a population checkpoint every
   // `interval` steps; adjoint() then returns dJ/dy of the wall points (and
@@ -69,6 +69,8 @@ class Ensemble {
   virtual int step(int n, unsigned options, std::string& err) = 0;
   virtual int timeSteps(int n, unsigned options, double& ms, std::string& err) = 0;
   virtual int setBlock(int block, std::string& err) = 0;
+  virtual int setCacheBudget(int64_t bytes, std::string& err) = 0;
+  virtual int groups() const = 0;
   virtual int synchronize(std::string& err) = 0;
   virtual int64_t fluidNodes() const = 0;
 };

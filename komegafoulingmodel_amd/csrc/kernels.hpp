@@ -118,9 +118,10 @@ __device__ __forceinline__ R geq(int i, R drho, R rho, R ux, R uy) {
 // (lbIBB lbKernels.cl:640-668: q <= 1/2 interpolates between the bounced
 // value and the node's own incoming value, q > 1/2 between the bounced
 // value and the reversed value streamed to the upstream neighbour).
-template <class S, class R>
-__device__ __forceinline__ void ibbCorrect(R (&g)[9], const S* __restrict__ src, int x, int y,
-                                           const R* __restrict__ q8, const LBParams<R>& p) {
+// up(d) returns the stored slot rev(d) of the node one step upstream
+// (x - c_d), read only for q > 1/2.
+template <class R, class Up>
+__device__ __forceinline__ void ibbCorrectWith(R (&g)[9], const R* __restrict__ q8, Up&& up) {
   R nv[9];
 #pragma unroll
   for (int j = 0; j < 9; j++) nv[j] = g[j];
@@ -130,18 +131,25 @@ __device__ __forceinline__ void ibbCorrect(R (&g)[9], const S* __restrict__ src,
     if (q < R(0)) continue;
     const int j = rev(d);
     const R tq = R(2) * q;
-    if (tq <= R(1)) {
+    if (tq <= R(1))
       nv[j] = tq * g[j] + (R(1) - tq) * g[d];
-    } else {
-      int sx = x - cx(d);
-      sx = sx >= p.nX ? sx - p.nX : (sx < 0 ? sx + p.nX : sx);
-      const int sy = y - cy(d);
-      const R up = loadDev<S, R>(src, size_t(sx) + size_t(p.XF) * sy + size_t(p.P) * j, j);
-      nv[j] = R(1) / tq * (g[j] + (tq - R(1)) * up);
-    }
+    else
+      nv[j] = R(1) / tq * (g[j] + (tq - R(1)) * up(d));
   }
 #pragma unroll
   for (int j = 0; j < 9; j++) g[j] = nv[j];
+}
+
+template <class S, class R>
+__device__ __forceinline__ void ibbCorrect(R (&g)[9], const S* __restrict__ src, int x, int y,
+                                           const R* __restrict__ q8, const LBParams<R>& p) {
+  ibbCorrectWith<R>(g, q8, [&](int d) {
+    int sx = x - cx(d);
+    sx = sx >= p.nX ? sx - p.nX : (sx < 0 ? sx + p.nX : sx);
+    const int sy = y - cy(d);
+    const int j = rev(d);
+    return loadDev<S, R>(src, size_t(sx) + size_t(p.XF) * sy + size_t(p.P) * j, j);
+  });
 }
 
 template <class S, class R>
@@ -370,6 +378,96 @@ __global__ __launch_bounds__(BX) void lbCollideStream2(
 #pragma unroll
       for (int n = 0; n < 2; n++)
         if (live[n]) storeDev<S, R>(dst, t[n], slot[n], g[n][i]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// two steps per pass (temporal blocking)
+// ---------------------------------------------------------------------------
+
+// Two plain steps in one pass over a TX x TY tile (nX % TX == 0).  The first
+// step runs on the tile plus a one-node ring, reading the stored
+// populations from HBM and pushing into LDS; the second runs on the tile,
+// reading LDS and pushing to HBM.  The intermediate populations never touch
+// HBM: per two lattice updates the pass reads the tile and its ring once and
+// writes the tile once.  Every value that passes through LDS is rounded to
+// the storage type first, and the node update is the same code as the
+// one-step kernel, so the result is bit-identical to two one-step launches.
+template <class S, class R, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void lbCollideStreamT2(
+    const S* __restrict__ src, S* __restrict__ dst, const uint16_t* __restrict__ mask,
+    const int32_t* __restrict__ ibbIdx, const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
+    R* __restrict__ uxOut, R* __restrict__ uyOut, LBParams<R> pA, LBParams<R> pB) {
+  constexpr int RX = TX + 2, RY = TY + 2, NR = RX * RY;
+  __shared__ S mid[9][NR];
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+  const KOFields none{nullptr, nullptr, nullptr, nullptr, nullptr};
+
+  // step 1 on the tile and its ring
+  for (int r = threadIdx.x; r < NR; r += TX * TY) {
+    const int rx = r % RX, ry = r / RX;
+    const int y = y0 - 1 + ry;
+    if (y < 0 || y >= pA.nY) continue;
+    int x = x0 - 1 + rx;
+    x = x < 0 ? x + pA.nX : (x >= pA.nX ? x - pA.nX : x);
+    const size_t gi = size_t(x) + size_t(pA.XF) * y;
+    const uint32_t m = mask[gi];
+    if (m & kSolid) continue;
+    R g[9];
+    loadNode<S, R>(g, src, gi, x, y, m, ibbIdx, ibbQ, pA);
+    collideNode<R, false>(g, gi, rhoOut, uxOut, uyOut, none, pA);
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      int tr, slot;
+      if (i != 0 && (m & boundBit(i))) {
+        tr = r;
+        slot = rev(i);
+      } else {
+        const int tx = rx + cx(i), ty = ry + cy(i);
+        if (tx < 0 || tx >= RX || ty < 0 || ty >= RY) continue;
+        tr = tx + RX * ty;
+        slot = i;
+      }
+      R v = g[i];
+      if constexpr (Scaled<S>::value) v *= R(1024. / wgt(slot));
+      mid[slot][tr] = static_cast<S>(v);
+    }
+  }
+  __syncthreads();
+
+  // step 2 on the tile
+  const int ix = threadIdx.x % TX, iy = threadIdx.x / TX;
+  const int x = x0 + ix, y = y0 + iy;
+  if (y >= pB.nY) return;
+  const size_t gi = size_t(x) + size_t(pB.XF) * y;
+  const uint32_t m = mask[gi];
+  if (m & kSolid) return;
+  const int r = (ix + 1) + RX * (iy + 1);
+  auto lds = [&](int slot, int rr) {
+    R v = static_cast<R>(mid[slot][rr]);
+    if constexpr (Scaled<S>::value) v *= R(wgt(slot) / 1024.);
+    return v;
+  };
+  R g[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) g[i] = lds(i, r);
+  if (m & kIbbNode)
+    ibbCorrectWith<R>(g, ibbQ + 8 * size_t(ibbIdx[gi]),
+                      [&](int d) { return lds(rev(d), r - cx(d) - RX * cy(d)); });
+  collideNode<R, false>(g, gi, rhoOut, uxOut, uyOut, none, pB);
+  const int xe = x < pB.nX - 1 ? x + 1 : 0;
+  const int xw = x > 0 ? x - 1 : pB.nX - 1;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const int a = cx(i), b = cy(i);
+    if (i == 0) {
+      storeDev<S, R>(dst, gi, 0, g[0]);
+    } else if (m & boundBit(i)) {
+      storeDev<S, R>(dst, gi + size_t(pB.P) * rev(i), rev(i), g[i]);
+    } else {
+      const int tx = a > 0 ? xe : (a < 0 ? xw : x);
+      storeDev<S, R>(dst, size_t(tx) + size_t(pB.XF) * (y + b) + size_t(pB.P) * i, i, g[i]);
     }
   }
 }

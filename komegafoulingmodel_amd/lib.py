@@ -46,7 +46,8 @@ EXPORTS = [
     "kofm_synchronize", "kofm_set_launch", "kofm_autotune", "kofm_set_graphs",
     "kofm_objective_run", "kofm_adjoint",
     "kofm_ensemble_create", "kofm_ensemble_step", "kofm_ensemble_time_steps",
-    "kofm_ensemble_set_block", "kofm_ensemble_last_error", "kofm_ensemble_destroy",
+    "kofm_ensemble_set_block", "kofm_ensemble_set_cache_budget", "kofm_ensemble_last_error",
+    "kofm_ensemble_destroy",
 ]
 
 
@@ -132,7 +133,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "kofm_download": (C.c_int, [P, C.c_int32, dp]),
         "kofm_save2file": (C.c_int, [P, C.c_char_p]),
         "kofm_synchronize": (C.c_int, [P]),
-        "kofm_set_launch": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int64]),
+        "kofm_set_launch": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int64]),
         "kofm_set_graphs": (C.c_int, [P, C.c_int32]),
         "kofm_objective_run": (C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(Objective), dp]),
         "kofm_adjoint": (C.c_int, [P, dp, dp]),
@@ -141,10 +142,11 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "kofm_ensemble_time_steps": (C.c_int, [P, C.c_int32, C.c_uint32, dp,
                                                C.POINTER(C.c_int64)]),
         "kofm_ensemble_set_block": (C.c_int, [P, C.c_int32]),
+        "kofm_ensemble_set_cache_budget": (C.c_int, [P, C.c_int64, C.POINTER(C.c_int32)]),
         "kofm_ensemble_last_error": (C.c_char_p, [P]),
         "kofm_ensemble_destroy": (None, [P]),
         "kofm_autotune": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
-                                    C.POINTER(C.c_int64), dp]),
+                                    C.POINTER(C.c_int32), C.POINTER(C.c_int64), dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -279,16 +281,19 @@ class Lattice:
     def save2file(self, directory: str):
         self._check(self.lib.kofm_save2file(self.h, directory.encode()))
 
-    def set_launch(self, block_x: int, nodes_per_lane: int = 1, plane_pad: int = 1088):
-        self._check(self.lib.kofm_set_launch(self.h, block_x, nodes_per_lane, plane_pad))
+    def set_launch(self, block_x: int, nodes_per_lane: int = 1, plane_pad: int = 1088,
+                   steps_per_pass: int = 1):
+        self._check(self.lib.kofm_set_launch(self.h, block_x, nodes_per_lane, steps_per_pass,
+                                             plane_pad))
 
     def autotune(self, steps: int = 50):
-        """Picks the collide-stream work-group width, nodes per lane and plane
-        padding; returns (block_x, nodes_per_lane, plane_pad, ms_per_step)."""
-        b, n, pad, ms = C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()
-        self._check(self.lib.kofm_autotune(self.h, steps, C.byref(b), C.byref(n), C.byref(pad),
-                                           C.byref(ms)))
-        return b.value, n.value, pad.value, ms.value
+        """Picks the collide-stream work-group width, nodes per lane, steps per
+        pass and plane padding; returns (block_x, nodes_per_lane, plane_pad,
+        ms_per_step, steps_per_pass)."""
+        b, n, sp, pad, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()
+        self._check(self.lib.kofm_autotune(self.h, steps, C.byref(b), C.byref(n), C.byref(sp),
+                                           C.byref(pad), C.byref(ms)))
+        return b.value, n.value, pad.value, ms.value, sp.value
 
     def set_graphs(self, on: bool):
         self._check(self.lib.kofm_set_graphs(self.h, 1 if on else 0))
@@ -342,6 +347,13 @@ class Ensemble:
 
     def set_block(self, block_x: int):
         self._check(self.lib.kofm_ensemble_set_block(self.h, block_x))
+
+    def set_cache_budget(self, nbytes: int) -> int:
+        """Groups the members under a population-byte budget (0: one group);
+        returns the number of groups."""
+        g = C.c_int32()
+        self._check(self.lib.kofm_ensemble_set_cache_budget(self.h, nbytes, C.byref(g)))
+        return g.value
 
     def close(self):
         if getattr(self, "h", None):

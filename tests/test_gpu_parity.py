@@ -181,24 +181,29 @@ def test_graph_replay_matches_eager_launches():
         assert np.array_equal(La.download("ux"), Lb.download("ux")), rep
 
 
-@pytest.mark.parametrize("block,npt,pad", [(128, 1, 0), (1024, 1, 64), (256, 2, 1088), (512, 2, 0)])
-def test_launch_shapes_give_identical_results(block, npt, pad):
+@pytest.mark.parametrize("block,npt,pad,spp", [(128, 1, 0, 1), (1024, 1, 64, 1), (256, 2, 1088, 1),
+                                               (512, 2, 0, 1), (512, 1, 1088, 2), (1024, 1, 0, 2)])
+def test_launch_shapes_give_identical_results(block, npt, pad, spp):
     """Every collide-stream launch shape (work-group width, one or two nodes
-    per lane, plane padding) produces the same populations bit for bit."""
+    per lane, one or two steps per pass, plane padding) produces the same
+    populations bit for bit.  41 steps: an eager first step, a captured
+    block of 32, then pairs and a single step."""
     w = _mini_fouled()
     La, _ = make_pair(w, "f32")
     Lb, _ = make_pair(w, "f32")
     La.set_launch(512, 1, 1088)
-    Lb.set_launch(block, npt, pad)
-    La.step(40, 2)
-    Lb.step(40, 2)
+    Lb.set_launch(block, npt, pad, spp)
+    La.step(41, 2)
+    Lb.step(41, 2)
     assert np.array_equal(La.download("f"), Lb.download("f"))
     assert np.array_equal(La.download("rho"), Lb.download("rho"))
 
 
-def test_ensemble_matches_individual_runs():
-    """Three channels of different sizes and fouling stepped by one ensemble
-    launch per step end bit-identical to the same channels stepped alone."""
+@pytest.mark.parametrize("budget,groups", [(0, 1), (2 << 20, 2)])
+def test_ensemble_matches_individual_runs(budget, groups):
+    """Three channels of different sizes and fouling stepped by ensemble
+    launches (all in one group, or packed into two cache-sized groups) end
+    bit-identical to the same channels stepped alone."""
     from komegafoulingmodel_amd import lib
     ws = [channel.fouled_channel(256, 48, Fval=1e-5, seed=31, amp=(0, 3), width=(10, 40)),
           channel.fouled_channel(384, 64, Fval=2e-5, seed=32, amp=(0, 3), width=(10, 40)),
@@ -206,6 +211,7 @@ def test_ensemble_matches_individual_runs():
     solo = [make_pair(w, "f32")[0] for w in ws]
     group = [make_pair(w, "f32")[0] for w in ws]
     ens = lib.Ensemble(group)
+    assert ens.set_cache_budget(budget) == groups
     for n in (1, 70, 33):
         ens.step(n, 2)
         for L in solo:
