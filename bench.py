@@ -59,7 +59,9 @@ def measure(L, fluid, steps, warmup, blu, options=0):
     L.synchronize()
     wall = time.perf_counter() - t0
     mlups = fluid * steps / wall / 1e6
-    achieved = blu * fluid / (ms_kernel * 1e-3) / 1e9
+    # with k-omega the captured step is timed as a whole (ms_kernel < 0): the
+    # algorithmic bytes are then set against the whole step
+    achieved = blu * fluid / ((ms_kernel if ms_kernel > 0 else ms_step) * 1e-3) / 1e9
     out = dict(mlups=mlups, ms_per_step=wall * 1e3 / steps, ms_per_step_events=ms_step,
                ms_per_kernel=ms_kernel, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
                fluid_nodes=fluid, steps=steps)
@@ -132,14 +134,19 @@ def measure_ensemble(storage, steps=300):
     ens = lib.Ensemble(Ls)
     out = {"channels": len(ws), "fluid_nodes": fluid, "steps": steps,
            "sequential_mlups": fluid * steps / t_seq / 1e6}
-    # one launch per step over all members, then cache-sized groups (default)
-    for label, budget in (("one_group", 0), ("grouped", 160 << 20)):
+    # one launch per step over all members, then groups sized for the L2
+    # (4 MB per XCD) and for the last-level cache; the best is reported
+    out["budgets"] = {}
+    for budget in (0, 16 << 20, 48 << 20, 160 << 20):
         groups = ens.set_cache_budget(budget)
         ens.step(64)
         ms, nodes = ens.time_steps(steps)
-        out[label] = {"groups": groups, "mlups": fluid / (ms * 1e-3) / 1e6, "ms_per_step": ms,
-                      "speedup_vs_sequential": (t_seq / steps) / (ms * 1e-3)}
-    out["mlups"] = out["grouped"]["mlups"]
+        out["budgets"][str(budget)] = {
+            "groups": groups, "mlups": fluid / (ms * 1e-3) / 1e6, "ms_per_step": ms,
+            "speedup_vs_sequential": (t_seq / steps) / (ms * 1e-3)}
+    best = max(out["budgets"].values(), key=lambda v: v["mlups"])
+    out["mlups"] = best["mlups"]
+    out["speedup_vs_sequential"] = best["speedup_vs_sequential"]
     ens.close()
     for L in Ls:
         L.close()
@@ -148,7 +155,8 @@ def measure_ensemble(storage, steps=300):
 
 def kernel_name(launch):
     if launch.get("steps_per_pass", 1) == 2:
-        return f"lbCollideStreamT2<float,float,64,{launch['block_x'] // 64}>"
+        tile = {512: (64, 8), 640: (128, 5), 832: (64, 13)}[launch["block_x"]]
+        return f"lbCollideStreamT2<float,float,{tile[0]},{tile[1]}>"
     k = "lbCollideStream2" if launch.get("nodes_per_lane", 1) == 2 else "lbCollideStream"
     return f"{k}<float,float,false,{launch['block_x']}>"
 

@@ -253,9 +253,10 @@ int kofm_save2file(kofm_ctx* ctx, const char* dir);
 /* Launch shape of the plain collide-stream step: work-group width (64, 128,
  * 256, 512 or 1024), lattice nodes per lane (1, or 2 neighbouring nodes when
  * nX is even), steps per pass (1, or 2 for the temporally blocked kernel:
- * k-omega off, nX % 64 == 0, one node per lane, width 512 or 1024 for a
- * 64 x 8 or 64 x 16 tile) and the padding (elements, multiple of 64)
- * between the nine population planes in HBM.  The state is preserved and
+ * k-omega off, one node per lane, and block_x is then the tile's node count,
+ * 512 for 64 x 8, 832 for 64 x 13 or 640 for 128 x 5, whose width must
+ * divide nX) and the padding (elements, multiple of 64) between the nine
+ * population planes in HBM.  The state is preserved and
  * every shape gives bit-identical results.  No counterpart in the reference,
  * whose work-group is fixed at 128 x 1 (ConstDef.h:70-79). */
 int kofm_set_launch(kofm_ctx* ctx, int32_t block_x, int32_t nodes_per_lane,

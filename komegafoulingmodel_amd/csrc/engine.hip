@@ -325,8 +325,7 @@ class EngineT final : public Engine {
         s += 2;
         continue;
       }
-      int rc = launchLB(options);
-      if (!rc && ko_) rc = koStep(err);
+      int rc = ko_ ? koFullStep(options, err) : launchLB(options);
       if (rc) return rc;
       s++;
     }
@@ -346,6 +345,21 @@ class EngineT final : public Engine {
       hipEvent_t e;
       HIP_TRY(hipEventCreate(&e));
       evs_.push_back(e);
+    }
+    if (ko_ && useGraphs_) {
+      // whole captured steps; the collide-stream is not timed on its own
+      HIP_TRY(hipEventRecord(evs_[0], stream_));
+      for (int s = 0; s < n; s++) {
+        int rc = koFullStep(options, err);
+        if (rc) return rc;
+      }
+      HIP_TRY(hipEventRecord(evs_[1], stream_));
+      HIP_TRY(hipEventSynchronize(evs_[1]));
+      float t = 0.f;
+      HIP_TRY(hipEventElapsedTime(&t, evs_[0], evs_[1]));
+      msStep = double(t) / n;
+      msKernel = -1.;
+      return 0;
     }
     HIP_TRY(hipEventRecord(evs_[0], stream_));
     int launches = 0;
@@ -1023,18 +1037,29 @@ class EngineT final : public Engine {
     launchLBBlock<false>(lbBlock_, lbNpt_, p, src, dst);
   }
 
-  void launchT2(int block, const dev::LBParams<R>& pA, const dev::LBParams<R>& pB, const S* src,
+  // The temporally blocked kernel on tiles of `nodes` nodes: 512 (64 x 8),
+  // 832 (64 x 13) or 640 (128 x 5); each work-group has one lane per node of
+  // the tile and its ring.
+  static int t2Width(int nodes) { return nodes == 640 ? 128 : 64; }
+  template <int TX, int TY>
+  void launchT2T(const dev::LBParams<R>& pA, const dev::LBParams<R>& pB, const S* src, S* dst) {
+    dev::TileMap tm;
+    tm.tilesX = nX_ / TX;
+    tm.nTiles = tm.tilesX * ((nY_ + TY - 1) / TY);
+    tm.per = (tm.nTiles + 7) / 8;
+    hipLaunchKernelGGL((dev::lbCollideStreamT2<S, R, TX, TY>), dim3(8 * tm.per),
+                       dim3(dev::t2Lanes<TX, TY>()), 0,
+                       stream_, src, dst, mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(), ibbQ_.as<R>(),
+                       rho_.as<R>(), ux_.as<R>(), uy_.as<R>(), pA, pB, tm);
+  }
+  void launchT2(int nodes, const dev::LBParams<R>& pA, const dev::LBParams<R>& pB, const S* src,
                 S* dst) {
-    if (block == 1024)
-      hipLaunchKernelGGL((dev::lbCollideStreamT2<S, R, 64, 16>), dim3(nX_ / 64, (nY_ + 15) / 16),
-                         dim3(1024), 0, stream_, src, dst, mask_.as<uint16_t>(),
-                         ibbIdx_.as<int32_t>(), ibbQ_.as<R>(), rho_.as<R>(), ux_.as<R>(),
-                         uy_.as<R>(), pA, pB);
+    if (nodes == 640)
+      launchT2T<128, 5>(pA, pB, src, dst);
+    else if (nodes == 832)
+      launchT2T<64, 13>(pA, pB, src, dst);
     else
-      hipLaunchKernelGGL((dev::lbCollideStreamT2<S, R, 64, 8>), dim3(nX_ / 64, (nY_ + 7) / 8),
-                         dim3(512), 0, stream_, src, dst, mask_.as<uint16_t>(),
-                         ibbIdx_.as<int32_t>(), ibbQ_.as<R>(), rho_.as<R>(), ux_.as<R>(),
-                         uy_.as<R>(), pA, pB);
+      launchT2T<64, 8>(pA, pB, src, dst);
   }
 
   // Two plain steps in one pass (k-omega off); the macros, if asked for,
@@ -1096,14 +1121,16 @@ class EngineT final : public Engine {
 
  public:
   int setLaunch(int block, int npt, int spp, int64_t pad, std::string& err) override {
-    if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
+    if (spp == 1 && block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
       return fail(err, "block width must be 64, 128, 256, 512 or 1024");
     if (npt != 1 && npt != 2) return fail(err, "nodes per lane must be 1 or 2");
     if (npt == 2 && nX_ % 2) return fail(err, "two nodes per lane need an even nX");
     if (spp != 1 && spp != 2) return fail(err, "steps per pass must be 1 or 2");
-    if (spp == 2 && (ko_ || nX_ % 64 || npt != 1 || (block != 512 && block != 1024)))
-      return fail(err, "two steps per pass need k-omega off, nX % 64 == 0, one node per lane "
-                       "and a block of 512 or 1024 (a 64 x 8 or 64 x 16 tile)");
+    if (spp == 2 && (ko_ || npt != 1 || (block != 512 && block != 640 && block != 832) ||
+                     nX_ % t2Width(block)))
+      return fail(err, "two steps per pass need k-omega off, one node per lane and a tile of "
+                       "512 (64 x 8), 832 (64 x 13) or 640 (128 x 5) nodes (block_x) whose "
+                       "width divides nX");
     if (pad < 0 || pad % 64) return fail(err, "plane padding must be a non-negative multiple of 64");
     int rc = relayout(pad, err);
     if (rc) return rc;
@@ -1163,8 +1190,9 @@ class EngineT final : public Engine {
             bestPad = pad;
           }
         }
-      if (!ko_ && nX_ % 64 == 0)
-        for (int block : {512, 1024}) {
+      if (!ko_)
+        for (int block : {512, 640, 832}) {
+          if (nX_ % t2Width(block)) continue;
           auto pA = p, pB = p;
           pA.applyIbb = 1;
           launchT2(block, pA, pB, a.as<S>(), b.as<S>());  // warm-up
@@ -1194,22 +1222,9 @@ class EngineT final : public Engine {
   }
 
  private:
-  // kOmega::Solve (kOmega.cpp:31-73) with one sub-step: diffusivities, then
-  // the coefficients with the first residual, then both implicit systems by
-  // BiCGStab with device-side scalars and convergence checks.  The host
-  // launches as many iterations as the previous step needed (at most 8)
-  // before its first poll of the done flags, then chunks of four; launches
-  // past convergence return at once.
-  int koStep(std::string& err) {
-    auto kp = koParams();
-    const dim3 grid(8 * kp.tiles.per), blk(dev::kBlockX);
-    // the solver kernels walk their XCD's tiles with a fixed block count
-    const int nsb = std::min(8 * kp.tiles.per, 2048);
-    const dim3 sgrid(nsb);
+  // The solver's device vectors.
+  dev::BicgVecs bicgVecs() const {
     using K = double;
-    hipLaunchKernelGGL((dev::koDiffusivities<R>), grid, blk, 0, stream_, mask_.as<uint16_t>(),
-                       K_.as<K>(), O_.as<K>(), nut_.as<K>(), wallD_.as<K>(), Dk_.as<K>(),
-                       Do_.as<K>(), F1_.as<K>(), dkdo_.as<K>(), rho_.as<R>(), dX_.as<K>(), kp);
     dev::BicgVecs V;
     V.x[0] = K_.as<K>();
     V.x[1] = O_.as<K>();
@@ -1228,36 +1243,100 @@ class EngineT final : public Engine {
       }
     }
     V.partial = partial_.as<double>();
+    return V;
+  }
+
+  // the solver kernels walk their XCD's tiles with a fixed block count
+  int solverBlocks() const { return std::min(8 * koParams().tiles.per, 2048); }
+
+  // kOmega::Solve (kOmega.cpp:31-73), first part: diffusivities, then the
+  // coefficients with the first residual and the solver's stage 0.
+  void enqueueKoHead() {
+    auto kp = koParams();
+    const dim3 grid(8 * kp.tiles.per), blk(dev::kBlockX);
+    const int nsb = solverBlocks();
+    using K = double;
+    hipLaunchKernelGGL((dev::koDiffusivities<R>), grid, blk, 0, stream_, mask_.as<uint16_t>(),
+                       K_.as<K>(), O_.as<K>(), nut_.as<K>(), wallD_.as<K>(), Dk_.as<K>(),
+                       Do_.as<K>(), F1_.as<K>(), dkdo_.as<K>(), rho_.as<R>(), dX_.as<K>(), kp);
     auto* st = status_.as<dev::BicgState>();
-    const double* part = partial_.as<double>();
     hipLaunchKernelGGL(dev::bicgReset, dim3(1), dim3(64), 0, stream_, st, par_.relTol[0],
                        par_.absTol[0], par_.maxIters[0], par_.relTol[1], par_.absTol[1],
                        par_.maxIters[1]);
-    hipLaunchKernelGGL((dev::koCoeffs<R>), sgrid, blk, 0, stream_, mask_.as<uint16_t>(),
+    hipLaunchKernelGGL((dev::koCoeffs<R>), dim3(nsb), blk, 0, stream_, mask_.as<uint16_t>(),
                        nut_.as<K>(), Dk_.as<K>(), Do_.as<K>(), F1_.as<K>(), dkdo_.as<K>(),
                        S_.as<K>(), rho_.as<R>(), ux_.as<R>(), uy_.as<R>(), dX_.as<K>(),
-                       wallD_.as<K>(), V, kp);
-    hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, part, nsb, st, 0);
+                       wallD_.as<K>(), bicgVecs(), kp);
+    hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_,
+                       (const double*)partial_.as<double>(), nsb, st, 0);
+  }
+
+  // k BiCGStab iterations of both systems (launches past convergence return
+  // at once).
+  void enqueueIters(int k) {
+    auto kp = koParams();
+    const dim3 sgrid(solverBlocks()), blk(dev::kBlockX);
+    const int nsb = solverBlocks();
+    const dev::BicgVecs V = bicgVecs();
+    auto* st = status_.as<dev::BicgState>();
+    const double* part = partial_.as<double>();
+    for (int c = 0; c < k; c++) {
+      hipLaunchKernelGGL(dev::bicgPV, sgrid, blk, 0, stream_, mask_.as<uint16_t>(), V,
+                         (const dev::BicgState*)st, kp);
+      hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, part, nsb, st, 1);
+      hipLaunchKernelGGL(dev::bicgST, sgrid, blk, 0, stream_, mask_.as<uint16_t>(), V,
+                         (const dev::BicgState*)st, kp);
+      hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, part, nsb, st, 2);
+      hipLaunchKernelGGL(dev::bicgXR, sgrid, blk, 0, stream_, mask_.as<uint16_t>(), V,
+                         (const dev::BicgState*)st, kp);
+      hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, part, nsb, st, 3);
+    }
+  }
+
+  // One full k-omega step.  The collide-stream, the coefficients and as many
+  // iterations as the previous step needed (at most 8) are replayed from a
+  // captured graph (or launched, with graphs off); then the host polls the
+  // done flags and adds chunks of four iterations until both systems have
+  // converged.
+  int koFullStep(unsigned options, std::string& err) {
+    const int batch = std::min(std::max(1, lastIters_), 8);
+    if (useGraphs_) {
+      const std::vector<int64_t> key = {2, curBuf_, preIbb_ ? 1 : 0, int64_t(options), batch};
+      int rc = replay(key, [&] {
+        launchLB(options);
+        enqueueKoHead();
+        enqueueIters(batch);
+      }, 1, err);
+      if (rc) return rc;
+    } else {
+      launchLB(options);
+      enqueueKoHead();
+      enqueueIters(batch);
+    }
+    return koFinish(batch, err);
+  }
+
+  // kOmega::Solve alone (after a collide-stream launched by the caller).
+  int koStep(std::string& err) {
+    const int batch = std::min(std::max(1, lastIters_), 8);
+    enqueueKoHead();
+    enqueueIters(batch);
+    return koFinish(batch, err);
+  }
+
+  int koFinish(int launched, std::string& err) {
     const int maxIt = std::max(par_.maxIters[0], par_.maxIters[1]);
-    int it = 0, batch = std::min(std::max(1, lastIters_), 8);
+    int it = launched;
+    auto* st = status_.as<dev::BicgState>();
     dev::BicgState hs;
     while (true) {
-      for (int c = 0; c < batch && it < maxIt; c++, it++) {
-        hipLaunchKernelGGL(dev::bicgPV, sgrid, blk, 0, stream_, mask_.as<uint16_t>(), V,
-                           (const dev::BicgState*)st, kp);
-        hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, part, nsb, st, 1);
-        hipLaunchKernelGGL(dev::bicgST, sgrid, blk, 0, stream_, mask_.as<uint16_t>(), V,
-                           (const dev::BicgState*)st, kp);
-        hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, part, nsb, st, 2);
-        hipLaunchKernelGGL(dev::bicgXR, sgrid, blk, 0, stream_, mask_.as<uint16_t>(), V,
-                           (const dev::BicgState*)st, kp);
-        hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, part, nsb, st, 3);
-      }
       HIP_TRY(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, stream_));
       HIP_TRY(hipStreamSynchronize(stream_));
       if (hs.nan) nanSeen_ = true;
       if ((hs.done[0] && hs.done[1]) || it >= maxIt) break;
-      batch = 4;
+      const int k = std::min(4, maxIt - it);
+      enqueueIters(k);
+      it += k;
     }
     lastIters_ = std::max(hs.iters[0], hs.iters[1]);
     HIP_TRY(hipGetLastError());

@@ -382,6 +382,24 @@ __global__ __launch_bounds__(BX) void lbCollideStream2(
   }
 }
 
+// XCD-aware tile order.  Workgroups are dealt round-robin over the 8 XCDs
+// by linear id, so linear id L runs on XCD L % 8.  Tile T (row-major) is
+// given to XCD T / per, which therefore owns a contiguous run of tiles and
+// finds a tile's neighbours in its own L2.  The grid has 8 * per blocks; ids
+// past the last tile do no work.
+struct TileMap {
+  int tilesX, nTiles, per;
+};
+
+__device__ __forceinline__ bool tileOf(const TileMap& t, int& bx, int& by) {
+  const int L = blockIdx.x;
+  const int T = (L & 7) * t.per + (L >> 3);
+  if (T >= t.nTiles) return false;
+  bx = T % t.tilesX;
+  by = T / t.tilesX;
+  return true;
+}
+
 // ---------------------------------------------------------------------------
 // two steps per pass (temporal blocking)
 // ---------------------------------------------------------------------------
@@ -394,18 +412,34 @@ __global__ __launch_bounds__(BX) void lbCollideStream2(
 // writes the tile once.  Every value that passes through LDS is rounded to
 // the storage type first, and the node update is the same code as the
 // one-step kernel, so the result is bit-identical to two one-step launches.
+//
+// Tiles are numbered row band by row band; the blocks of XCD k (blockIdx % 8)
+// take the k-th contiguous run of tiles, so horizontally neighbouring tiles
+// (which share their ring columns) and the bands above and below run on the
+// same XCD and meet in its L2.
+// The block has one lane per node of the tile and its ring (rounded up to
+// whole waves), so the first step takes a single pass; the second step runs
+// on the first TX * TY lanes.
+template <int TX, int TY>
+constexpr int t2Lanes() {
+  return ((TX + 2) * (TY + 2) + 63) / 64 * 64;
+}
+
 template <class S, class R, int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void lbCollideStreamT2(
+__global__ __launch_bounds__((t2Lanes<TX, TY>())) void lbCollideStreamT2(
     const S* __restrict__ src, S* __restrict__ dst, const uint16_t* __restrict__ mask,
     const int32_t* __restrict__ ibbIdx, const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
-    R* __restrict__ uxOut, R* __restrict__ uyOut, LBParams<R> pA, LBParams<R> pB) {
+    R* __restrict__ uxOut, R* __restrict__ uyOut, LBParams<R> pA, LBParams<R> pB, TileMap tm) {
   constexpr int RX = TX + 2, RY = TY + 2, NR = RX * RY;
+  static_assert(t2Lanes<TX, TY>() <= 1024, "tile and ring must fit one work-group");
   __shared__ S mid[9][NR];
-  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+  int tbx = 0, tby = 0;
+  if (!tileOf(tm, tbx, tby)) return;
+  const int x0 = tbx * TX, y0 = tby * TY;
   const KOFields none{nullptr, nullptr, nullptr, nullptr, nullptr};
 
   // step 1 on the tile and its ring
-  for (int r = threadIdx.x; r < NR; r += TX * TY) {
+  for (int r = threadIdx.x; r < NR; r += t2Lanes<TX, TY>()) {
     const int rx = r % RX, ry = r / RX;
     const int y = y0 - 1 + ry;
     if (y < 0 || y >= pA.nY) continue;
@@ -437,6 +471,7 @@ __global__ __launch_bounds__(TX * TY) void lbCollideStreamT2(
   __syncthreads();
 
   // step 2 on the tile
+  if (threadIdx.x >= TX * TY) return;
   const int ix = threadIdx.x % TX, iy = threadIdx.x / TX;
   const int x = x0 + ix, y = y0 + iy;
   if (y >= pB.nY) return;
@@ -594,24 +629,9 @@ __global__ __launch_bounds__(kBlockX) void lbMacros(const S* __restrict__ src,
 // k-omega
 // ---------------------------------------------------------------------------
 
-// XCD-aware tile order for the stencil kernels.  Workgroups are dealt
-// round-robin over the 8 XCDs by linear id, so linear id L runs on XCD L % 8.
-// Tile T (row-major over tilesX x nY tiles of kBlockX nodes) is given to
-// XCD T / per, which therefore owns a contiguous band of rows and finds the
-// rows above and below a tile in its own L2.  The grid has 8 * per blocks;
-// ids past the last tile do no work.
-struct TileMap {
-  int tilesX, nTiles, per;
-};
-
-__device__ __forceinline__ bool tileOf(const TileMap& t, int& bx, int& by) {
-  const int L = blockIdx.x;
-  const int T = (L & 7) * t.per + (L >> 3);
-  if (T >= t.nTiles) return false;
-  bx = T % t.tilesX;
-  by = T / t.tilesX;
-  return true;
-}
+// The stencil kernels use the XCD-aware tile order (TileMap) over row tiles
+// of kBlockX nodes: each XCD owns a contiguous band of rows and finds the
+// rows above and below a tile in its own L2.
 
 template <class R>
 __device__ __forceinline__ void fdCoeffs(R dp, R dm, R (&c)[6]) {

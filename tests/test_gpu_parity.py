@@ -181,8 +181,24 @@ def test_graph_replay_matches_eager_launches():
         assert np.array_equal(La.download("ux"), Lb.download("ux")), rep
 
 
+def test_komega_graph_steps_match_eager_launches():
+    """k-omega steps replayed from captured graphs (collide-stream, the
+    coefficients and the predicted number of solver iterations) end
+    bit-identical to the same steps launched one kernel at a time."""
+    w = _mini_wavy_ko()
+    La, _ = make_pair(w, "f32")
+    Lb, _ = make_pair(w, "f32")
+    Lb.set_graphs(False)
+    for n in (1, 6, 5):
+        La.step(n, 2)
+        Lb.step(n, 2)
+        for f in ("f", "k", "omega", "nut"):
+            assert np.array_equal(La.download(f), Lb.download(f)), (n, f)
+
+
 @pytest.mark.parametrize("block,npt,pad,spp", [(128, 1, 0, 1), (1024, 1, 64, 1), (256, 2, 1088, 1),
-                                               (512, 2, 0, 1), (512, 1, 1088, 2), (1024, 1, 0, 2)])
+                                               (512, 2, 0, 1), (512, 1, 1088, 2), (640, 1, 0, 2),
+                                               (832, 1, 64, 2)])
 def test_launch_shapes_give_identical_results(block, npt, pad, spp):
     """Every collide-stream launch shape (work-group width, one or two nodes
     per lane, one or two steps per pass, plane padding) produces the same
