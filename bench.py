@@ -110,15 +110,15 @@ def measure_adjoint(**kw):
 
 def measure_ensemble(storage, steps=300):
     """48 channels of uneven size, Re and fouling stepped by the ensemble
-    (graph-captured blocks of 32 steps; one launch per step over all members,
-    and members packed into groups that fit the last-level cache) against the
-    same channels stepped one after another, each with its own graphs."""
+    (graph-captured blocks of 32 steps) against the same channels stepped one
+    after another, each autotuned and with its own graphs."""
     from komegafoulingmodel_amd import channel, lib
 
     ws = channel.ensemble_members()
     Ls, fluid = [], 0
     for w in ws:
         L, nf = setup(w, storage, tune=False)
+        L.autotune(20)  # each channel alone gets its own best launch shape
         Ls.append(L)
         fluid += nf
     for L in Ls:  # sequential reference, warm (graphs captured)
@@ -134,29 +134,35 @@ def measure_ensemble(storage, steps=300):
     ens = lib.Ensemble(Ls)
     out = {"channels": len(ws), "fluid_nodes": fluid, "steps": steps,
            "sequential_mlups": fluid * steps / t_seq / 1e6}
-    # one launch per step over all members, then groups sized for the L2
-    # (4 MB per XCD) and for the last-level cache; the best is reported
-    out["budgets"] = {}
-    for budget in (0, 16 << 20, 48 << 20, 160 << 20):
-        groups = ens.set_cache_budget(budget)
-        ens.step(64)
-        ms, nodes = ens.time_steps(steps)
-        out["budgets"][str(budget)] = {
-            "groups": groups, "mlups": fluid / (ms * 1e-3) / 1e6, "ms_per_step": ms,
-            "speedup_vs_sequential": (t_seq / steps) / (ms * 1e-3)}
-    best = max(out["budgets"].values(), key=lambda v: v["mlups"])
-    out["mlups"] = best["mlups"]
-    out["speedup_vs_sequential"] = best["speedup_vs_sequential"]
+    # one launch per step over all members or members grouped to fit the
+    # last-level cache; one step per launch or two (64 x 13 tiles, or 64 x 26
+    # with two nodes per lane); the best is reported
+    out["shapes"] = {}
+    for budget in (0, 160 << 20):
+        for block, spp in ((512, 1), (832, 2), (1664, 2)):
+            groups = ens.set_cache_budget(budget)
+            ens.set_launch(block, spp)
+            ens.step(64)
+            ms, nodes = ens.time_steps(steps)
+            out["shapes"][f"budget{budget >> 20}MiB_block{block}_spp{spp}"] = {
+                "groups": groups, "mlups": fluid / (ms * 1e-3) / 1e6, "ms_per_step": ms,
+                "speedup_vs_sequential": (t_seq / steps) / (ms * 1e-3)}
+    best = max(out["shapes"].items(), key=lambda kv: kv[1]["mlups"])
+    out["best"] = best[0]
+    out["mlups"] = best[1]["mlups"]
+    out["speedup_vs_sequential"] = best[1]["speedup_vs_sequential"]
     ens.close()
     for L in Ls:
         L.close()
     return out
 
 
-def kernel_name(launch):
+def kernel_name(launch, storage="float"):
     if launch.get("steps_per_pass", 1) == 2:
-        tile = {512: (64, 8), 640: (128, 5), 832: (64, 13)}[launch["block_x"]]
-        return f"lbCollideStreamT2<float,float,{tile[0]},{tile[1]}>"
+        tile = {512: (64, 8), 640: (128, 5), 832: (64, 13), 1024: (64, 16), 1536: (128, 12),
+                1664: (64, 26)}[launch["block_x"]]
+        k = "lbCollideStreamT2P" if launch["block_x"] in (1024, 1536, 1664) else "lbCollideStreamT2"
+        return f"{k}<{storage},float,{tile[0]},{tile[1]}>"
     k = "lbCollideStream2" if launch.get("nodes_per_lane", 1) == 2 else "lbCollideStream"
     return f"{k}<float,float,false,{launch['block_x']}>"
 

@@ -252,11 +252,12 @@ int kofm_save2file(kofm_ctx* ctx, const char* dir);
 
 /* Launch shape of the plain collide-stream step: work-group width (64, 128,
  * 256, 512 or 1024), lattice nodes per lane (1, or 2 neighbouring nodes when
- * nX is even), steps per pass (1, or 2 for the temporally blocked kernel:
- * k-omega off, one node per lane, and block_x is then the tile's node count,
- * 512 for 64 x 8, 832 for 64 x 13 or 640 for 128 x 5, whose width must
- * divide nX) and the padding (elements, multiple of 64) between the nine
- * population planes in HBM.  The state is preserved and
+ * nX is even), steps per pass (1, or 2 for the temporally blocked kernels:
+ * k-omega off, nodes_per_lane 1, and block_x is then the tile's node count,
+ * 512 (64 x 8), 832 (64 x 13) or 640 (128 x 5) with one node per lane, or
+ * 1024 (64 x 16), 1664 (64 x 26) or 1536 (128 x 12) with two neighbouring
+ * nodes per lane; the tile width must divide nX) and the padding (elements,
+ * multiple of 64) between the nine population planes in HBM.  The state is preserved and
  * every shape gives bit-identical results.  No counterpart in the reference,
  * whose work-group is fixed at 128 x 1 (ConstDef.h:70-79). */
 int kofm_set_launch(kofm_ctx* ctx, int32_t block_x, int32_t nodes_per_lane,
@@ -329,6 +330,11 @@ int kofm_ensemble_time_steps(kofm_ensemble* e, int32_t nsteps, uint32_t options,
                              double* ms_per_step, int64_t* fluid_nodes);
 /* Work-group width of the ensemble launch (128, 256, 512 or 1024). */
 int kofm_ensemble_set_block(kofm_ensemble* e, int32_t block_x);
+/* Launch shape: steps_per_launch 1 with block_x 128, 256, 512 or 1024, or
+ * 2 (the temporally blocked kernels) with block_x the tile's node count,
+ * 832 (64 x 13) or 1664 (64 x 26, two nodes per lane), every member's nX a
+ * multiple of 64.  Results are bit-identical for every shape. */
+int kofm_ensemble_set_launch(kofm_ensemble* e, int32_t block_x, int32_t steps_per_launch);
 /* Members are packed (largest first) into groups whose population arrays
  * (both buffers) total at most `bytes`; kofm_ensemble_step advances each
  * group through all its steps before the next, so a group that fits the

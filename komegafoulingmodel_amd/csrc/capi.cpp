@@ -409,6 +409,12 @@ int kofm_ensemble_set_block(kofm_ensemble* e, int32_t block_x) {
   return rc ? rc : KOFM_OK;
 }
 
+int kofm_ensemble_set_launch(kofm_ensemble* e, int32_t block_x, int32_t steps_per_launch) {
+  if (!e) return KOFM_ERR_ARG;
+  int rc = e->impl->setLaunch(block_x, steps_per_launch, e->err);
+  return rc ? rc : KOFM_OK;
+}
+
 int kofm_ensemble_set_cache_budget(kofm_ensemble* e, int64_t bytes, int32_t* groups) {
   if (!e) return KOFM_ERR_ARG;
   int rc = e->impl->setCacheBudget(bytes, e->err);

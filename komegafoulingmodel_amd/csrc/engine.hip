@@ -1037,10 +1037,26 @@ class EngineT final : public Engine {
     launchLBBlock<false>(lbBlock_, lbNpt_, p, src, dst);
   }
 
-  // The temporally blocked kernel on tiles of `nodes` nodes: 512 (64 x 8),
-  // 832 (64 x 13) or 640 (128 x 5); each work-group has one lane per node of
-  // the tile and its ring.
-  static int t2Width(int nodes) { return nodes == 640 ? 128 : 64; }
+  // The temporally blocked kernels, selected by the tile's node count: one
+  // node per lane on 512 (64 x 8), 832 (64 x 13) or 640 (128 x 5) node
+  // tiles; two neighbouring nodes per lane (aligned two-element accesses)
+  // on 1024 (64 x 16), 1664 (64 x 26) or 1536 (128 x 12) node tiles.
+  static int t2Width(int nodes) { return (nodes == 640 || nodes == 1536) ? 128 : 64; }
+  static bool t2Tile(int nodes) {
+    return nodes == 512 || nodes == 640 || nodes == 832 || nodes == 1024 || nodes == 1536 ||
+           nodes == 1664;
+  }
+  template <int TX, int TY>
+  void launchT2PT(const dev::LBParams<R>& pA, const dev::LBParams<R>& pB, const S* src, S* dst) {
+    dev::TileMap tm;
+    tm.tilesX = nX_ / TX;
+    tm.nTiles = tm.tilesX * ((nY_ + TY - 1) / TY);
+    tm.per = (tm.nTiles + 7) / 8;
+    hipLaunchKernelGGL((dev::lbCollideStreamT2P<S, R, TX, TY>), dim3(8 * tm.per),
+                       dim3(dev::t2pLanes<TX, TY>()), 0, stream_, src, dst, mask_.as<uint16_t>(),
+                       ibbIdx_.as<int32_t>(), ibbQ_.as<R>(), rho_.as<R>(), ux_.as<R>(),
+                       uy_.as<R>(), pA, pB, tm);
+  }
   template <int TX, int TY>
   void launchT2T(const dev::LBParams<R>& pA, const dev::LBParams<R>& pB, const S* src, S* dst) {
     dev::TileMap tm;
@@ -1054,12 +1070,14 @@ class EngineT final : public Engine {
   }
   void launchT2(int nodes, const dev::LBParams<R>& pA, const dev::LBParams<R>& pB, const S* src,
                 S* dst) {
-    if (nodes == 640)
-      launchT2T<128, 5>(pA, pB, src, dst);
-    else if (nodes == 832)
-      launchT2T<64, 13>(pA, pB, src, dst);
-    else
-      launchT2T<64, 8>(pA, pB, src, dst);
+    switch (nodes) {
+      case 640: launchT2T<128, 5>(pA, pB, src, dst); break;
+      case 832: launchT2T<64, 13>(pA, pB, src, dst); break;
+      case 1024: launchT2PT<64, 16>(pA, pB, src, dst); break;
+      case 1536: launchT2PT<128, 12>(pA, pB, src, dst); break;
+      case 1664: launchT2PT<64, 26>(pA, pB, src, dst); break;
+      default: launchT2T<64, 8>(pA, pB, src, dst); break;
+    }
   }
 
   // Two plain steps in one pass (k-omega off); the macros, if asked for,
@@ -1126,11 +1144,10 @@ class EngineT final : public Engine {
     if (npt != 1 && npt != 2) return fail(err, "nodes per lane must be 1 or 2");
     if (npt == 2 && nX_ % 2) return fail(err, "two nodes per lane need an even nX");
     if (spp != 1 && spp != 2) return fail(err, "steps per pass must be 1 or 2");
-    if (spp == 2 && (ko_ || npt != 1 || (block != 512 && block != 640 && block != 832) ||
-                     nX_ % t2Width(block)))
-      return fail(err, "two steps per pass need k-omega off, one node per lane and a tile of "
-                       "512 (64 x 8), 832 (64 x 13) or 640 (128 x 5) nodes (block_x) whose "
-                       "width divides nX");
+    if (spp == 2 && (ko_ || npt != 1 || !t2Tile(block) || nX_ % t2Width(block)))
+      return fail(err, "two steps per pass need k-omega off, nodes_per_lane 1 and a tile "
+                       "(block_x) of 512, 640, 832, 1024, 1536 or 1664 nodes whose width "
+                       "divides nX");
     if (pad < 0 || pad % 64) return fail(err, "plane padding must be a non-negative multiple of 64");
     int rc = relayout(pad, err);
     if (rc) return rc;
@@ -1191,7 +1208,7 @@ class EngineT final : public Engine {
           }
         }
       if (!ko_)
-        for (int block : {512, 640, 832}) {
+        for (int block : {512, 640, 832, 1024, 1536, 1664}) {
           if (nX_ % t2Width(block)) continue;
           auto pA = p, pB = p;
           pA.applyIbb = 1;
@@ -1376,10 +1393,23 @@ class EnsembleT final : public Ensemble {
     return n;
   }
 
-  int setBlock(int block, std::string& err) override {
-    if (block != 128 && block != 256 && block != 512 && block != 1024)
+  int setBlock(int block, std::string& err) override { return setLaunch(block, 1, err); }
+
+  // One step per launch: block in {128, 256, 512, 1024}; two steps per
+  // launch: the tile's node count, 832 (64 x 13) or 1664 (64 x 26, two nodes
+  // per lane), every member's nX a multiple of 64.
+  int setLaunch(int block, int spp, std::string& err) override {
+    if (spp == 1 && block != 128 && block != 256 && block != 512 && block != 1024)
       return fail(err, "ensemble block width must be 128, 256, 512 or 1024");
+    if (spp == 2) {
+      if (block != 832 && block != 1664)
+        return fail(err, "two steps per launch need a tile of 832 or 1664 nodes");
+      for (auto* e : m_)
+        if (e->nX_ % 64) return fail(err, "two steps per launch need nX % 64 == 0 in every member");
+    }
+    if (spp != 1 && spp != 2) return fail(err, "steps per launch must be 1 or 2");
     block_ = block;
+    spp_ = spp;
     return regroup(err);
   }
 
@@ -1441,7 +1471,7 @@ class EnsembleT final : public Ensemble {
     std::vector<EngineT<S, R>*> m;
     Buffer ents[2], eager;
     hipGraphExec_t graph = nullptr;
-    int totalTiles = 0;
+    int totalTiles = 0, eagerTiles = 0;
     bool built = false;
     unsigned builtOptions = 0;
     std::vector<int> builtParity;
@@ -1459,32 +1489,34 @@ class EnsembleT final : public Ensemble {
 
     int step(EnsembleT& en, int n, unsigned options, std::string& err) {
       hipStream_t stream_ = en.stream_;
+      const int per = en.spp_;  // steps per launch
       int s = 0;
       // first step after an upload runs without the pending wall correction
       bool fresh = false;
       for (auto* e : m) fresh |= !e->preIbb_;
       if (fresh && s < n) {
-        int rc = fill(en, eager, options, 0, true, err);
+        int rc = fill(en, eager, options, 0, true, false, eagerTiles, err);
         if (rc) return rc;
-        en.launch(eager, totalTiles, int(m.size()));
+        en.launch(eager, eagerTiles, int(m.size()), false);
         advance(1);
         s++;
       }
       if (!built || builtOptions != options || builtParity != parity()) {
-        int rc = fill(en, ents[0], options, 0, false, err);
-        if (!rc) rc = fill(en, ents[1], options, 1, false, err);
+        int rc = fill(en, ents[0], options, 0, false, per == 2, totalTiles, err);
+        if (!rc) rc = fill(en, ents[1], options, 1, false, per == 2, totalTiles, err);
         if (rc) return rc;
         dropGraph();
         built = true;
         builtOptions = options;
         builtParity = parity();
       }
-      int k = 0;  // steps since the entries were built (selects ents[k & 1])
+      int k = 0;  // launches since the entries were built (selects ents[k & 1])
       while (n - s >= kGraphSteps) {
         if (!graph) {
           hipGraph_t g = nullptr;
           HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-          for (int j = 0; j < kGraphSteps; j++) en.launch(ents[j & 1], totalTiles, int(m.size()));
+          for (int j = 0; j < kGraphSteps / per; j++)
+            en.launch(ents[j & 1], totalTiles, int(m.size()), per == 2);
           HIP_TRY(hipStreamEndCapture(stream_, &g));
           hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
           (void)hipGraphDestroy(g);
@@ -1493,7 +1525,14 @@ class EnsembleT final : public Ensemble {
         HIP_TRY(hipGraphLaunch(graph, stream_));
         s += kGraphSteps;
       }
-      for (; s < n; s++, k++) en.launch(ents[k & 1], totalTiles, int(m.size()));
+      for (; n - s >= per; s += per, k++) en.launch(ents[k & 1], totalTiles, int(m.size()), per == 2);
+      if (s < n) {  // an odd last step after two-step launches
+        int rc = fill(en, eager, options, k, false, false, eagerTiles, err);
+        if (rc) return rc;
+        en.launch(eager, eagerTiles, int(m.size()), false);
+        s++;
+        k++;
+      }
       advance(k);
       // the built entries assume every member's current buffer; an odd tail
       // flips them, so rebuild next time
@@ -1501,8 +1540,10 @@ class EnsembleT final : public Ensemble {
       return 0;
     }
 
-    // entries for a step taken `shift` steps after now
-    int fill(EnsembleT& en, Buffer& buf, unsigned options, int shift, bool eager, std::string& err) {
+    // entries for a launch taken `shift` launches after now (two-step tiles
+    // with two)
+    int fill(EnsembleT& en, Buffer& buf, unsigned options, int shift, bool eager, bool two,
+             int& tilesOut, std::string& err) {
       std::vector<dev::BatchEntry<S, R>> h(m.size());
       int tiles = 0;
       for (size_t i = 0; i < m.size(); i++) {
@@ -1519,11 +1560,19 @@ class EnsembleT final : public Ensemble {
         b.uy = e->uy_.template as<R>();
         b.p = e->lbParams(options);
         b.p.applyIbb = eager ? (e->preIbb_ ? 1 : 0) : 1;
-        b.tilesX = (e->nX_ + en.block_ - 1) / en.block_;
-        b.tileStart = tiles;
-        tiles += b.tilesX * e->nY_;
+        if (two) {
+          const int tx = 64, ty = en.block_ / 64;
+          b.tilesX = e->nX_ / tx;
+          b.tileStart = tiles;
+          tiles += b.tilesX * ((e->nY_ + ty - 1) / ty);
+        } else {
+          const int bx = en.spp_ == 2 ? 256 : en.block_;
+          b.tilesX = (e->nX_ + bx - 1) / bx;
+          b.tileStart = tiles;
+          tiles += b.tilesX * e->nY_;
+        }
       }
-      totalTiles = tiles;
+      tilesOut = tiles;
       HIP_TRY(hipMemcpy(buf.as<void>(), h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice));
       return 0;
     }
@@ -1542,6 +1591,7 @@ class EnsembleT final : public Ensemble {
   hipStream_t stream_ = nullptr;
   hipEvent_t done_ = nullptr;
   int block_ = 512;
+  int spp_ = 1;
   // population bytes (both buffers) per group; 0 puts every member in one
   // launch.  The default fits the 256 MB last-level cache with room to spare.
   int64_t budget_ = int64_t(160) << 20;
@@ -1578,9 +1628,20 @@ class EnsembleT final : public Ensemble {
     return 0;
   }
 
-  void launch(const Buffer& buf, int totalTiles, int n) {
+  void launch(const Buffer& buf, int totalTiles, int n, bool two) {
     const auto* ents = buf.as<const dev::BatchEntry<S, R>>();
-    switch (block_) {
+    if (two) {
+      const dim3 grid(8 * ((totalTiles + 7) / 8));
+      if (block_ == 1664)
+        hipLaunchKernelGGL((dev::lbCollideStreamBatchT2<S, R, 64, 26, true>), grid,
+                           dim3(dev::t2pLanes<64, 26>()), 0, stream_, ents, n, totalTiles);
+      else
+        hipLaunchKernelGGL((dev::lbCollideStreamBatchT2<S, R, 64, 13, false>), grid,
+                           dim3(dev::t2Lanes<64, 13>()), 0, stream_, ents, n, totalTiles);
+      return;
+    }
+    const int bx = spp_ == 2 ? 256 : block_;
+    switch (bx) {
       case 128: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 128>), dim3(totalTiles), dim3(128), 0, stream_, ents, n); break;
       case 256: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 256>), dim3(totalTiles), dim3(256), 0, stream_, ents, n); break;
       case 1024: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 1024>), dim3(totalTiles), dim3(1024), 0, stream_, ents, n); break;

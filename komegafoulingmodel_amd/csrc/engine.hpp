@@ -69,6 +69,7 @@ class Ensemble {
   virtual int step(int n, unsigned options, std::string& err) = 0;
   virtual int timeSteps(int n, unsigned options, double& ms, std::string& err) = 0;
   virtual int setBlock(int block, std::string& err) = 0;
+  virtual int setLaunch(int block, int spp, std::string& err) = 0;
   virtual int setCacheBudget(int64_t bytes, std::string& err) = 0;
   virtual int groups() const = 0;
   virtual int synchronize(std::string& err) = 0;

@@ -4,8 +4,8 @@
 //
 // Storage.  Populations are kept as deviations from the lattice weights,
 // g_i = f_i - w_i, in the storage type S (float, _Float16, __bf16 or
-// double).  16-bit forms scale by 1024 / w_i before rounding so that every
-// direction uses the same part of the 16-bit range.  Arithmetic is done in
+// double).  16-bit forms scale by a power of two near 1024 / w_i before
+// rounding so that every direction uses the same part of the 16-bit range.  Arithmetic is done in
 // R (float, or double for the FP64 form).
 //
 // Time stepping follows the reference's push scheme (collide at the node,
@@ -67,16 +67,30 @@ struct Scaled<__bf16> {
   static constexpr bool value = true;
 };
 
+// 16-bit storage keeps s_i = g_i * scale16(i): a power of two near
+// 1024 / w_i (2048, 8192, 32768, so w_i * scale16(i) is 910 in every
+// direction).  A power of two makes the scaling (and 1 / scale16 on the way
+// back) exact, so the value stored is the single rounding of g_i *
+// scale16(i) whichever instruction sequence the compiler picks for the
+// multiply and the conversion, in every kernel.
+__host__ __device__ constexpr double scale16(int i) {
+  return i == 0 ? 2048. : (i < 5 ? 8192. : 32768.);
+}
+__host__ __device__ constexpr double unscale16(int i) { return 1. / scale16(i); }
+static_assert(scale16(0) * unscale16(0) == 1. && scale16(1) * unscale16(1) == 1. &&
+                  scale16(5) * unscale16(5) == 1.,
+              "16-bit scaling must round-trip exactly");
+
 // stored value <-> deviation g = f - w
 template <class S, class R>
 __device__ __forceinline__ R loadDev(const S* __restrict__ p, size_t idx, int dir) {
   R v = static_cast<R>(p[idx]);
-  if constexpr (Scaled<S>::value) v *= R(wgt(dir) / 1024.);
+  if constexpr (Scaled<S>::value) v *= R(unscale16(dir));
   return v;
 }
 template <class S, class R>
 __device__ __forceinline__ void storeDev(S* __restrict__ p, size_t idx, int dir, R g) {
-  if constexpr (Scaled<S>::value) g *= R(1024. / wgt(dir));
+  if constexpr (Scaled<S>::value) g *= R(scale16(dir));
   p[idx] = static_cast<S>(g);
 }
 
@@ -332,8 +346,8 @@ __global__ __launch_bounds__(BX) void lbCollideStream2(
     const Pair<S> v = *reinterpret_cast<const Pair<S>*>(src + g0 + size_t(p.P) * i);
     R a = static_cast<R>(v.a), b = static_cast<R>(v.b);
     if constexpr (Scaled<S>::value) {
-      a *= R(wgt(i) / 1024.);
-      b *= R(wgt(i) / 1024.);
+      a *= R(unscale16(i));
+      b *= R(unscale16(i));
     }
     g[0][i] = a;
     g[1][i] = b;
@@ -367,8 +381,8 @@ __global__ __launch_bounds__(BX) void lbCollideStream2(
     if (live[0] && live[1] && t[1] == t[0] + 1 && !(t[0] & 1)) {
       R va = g[0][i], vb = g[1][i];
       if constexpr (Scaled<S>::value) {
-        va *= R(1024. / wgt(slot[0]));
-        vb *= R(1024. / wgt(slot[1]));
+        va *= R(scale16(slot[0]));
+        vb *= R(scale16(slot[1]));
       }
       Pair<S> v;
       v.a = static_cast<S>(va);
@@ -426,16 +440,16 @@ constexpr int t2Lanes() {
 }
 
 template <class S, class R, int TX, int TY>
-__global__ __launch_bounds__((t2Lanes<TX, TY>())) void lbCollideStreamT2(
-    const S* __restrict__ src, S* __restrict__ dst, const uint16_t* __restrict__ mask,
-    const int32_t* __restrict__ ibbIdx, const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
-    R* __restrict__ uxOut, R* __restrict__ uyOut, LBParams<R> pA, LBParams<R> pB, TileMap tm) {
+__device__ __forceinline__ void t2Tile(const S* __restrict__ src, S* __restrict__ dst,
+                                       const uint16_t* __restrict__ mask,
+                                       const int32_t* __restrict__ ibbIdx,
+                                       const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
+                                       R* __restrict__ uxOut, R* __restrict__ uyOut,
+                                       const LBParams<R>& pA, const LBParams<R>& pB, int x0,
+                                       int y0) {
   constexpr int RX = TX + 2, RY = TY + 2, NR = RX * RY;
   static_assert(t2Lanes<TX, TY>() <= 1024, "tile and ring must fit one work-group");
   __shared__ S mid[9][NR];
-  int tbx = 0, tby = 0;
-  if (!tileOf(tm, tbx, tby)) return;
-  const int x0 = tbx * TX, y0 = tby * TY;
   const KOFields none{nullptr, nullptr, nullptr, nullptr, nullptr};
 
   // step 1 on the tile and its ring
@@ -464,7 +478,7 @@ __global__ __launch_bounds__((t2Lanes<TX, TY>())) void lbCollideStreamT2(
         slot = i;
       }
       R v = g[i];
-      if constexpr (Scaled<S>::value) v *= R(1024. / wgt(slot));
+      if constexpr (Scaled<S>::value) v *= R(scale16(slot));
       mid[slot][tr] = static_cast<S>(v);
     }
   }
@@ -481,7 +495,7 @@ __global__ __launch_bounds__((t2Lanes<TX, TY>())) void lbCollideStreamT2(
   const int r = (ix + 1) + RX * (iy + 1);
   auto lds = [&](int slot, int rr) {
     R v = static_cast<R>(mid[slot][rr]);
-    if constexpr (Scaled<S>::value) v *= R(wgt(slot) / 1024.);
+    if constexpr (Scaled<S>::value) v *= R(unscale16(slot));
     return v;
   };
   R g[9];
@@ -506,6 +520,165 @@ __global__ __launch_bounds__((t2Lanes<TX, TY>())) void lbCollideStreamT2(
     }
   }
 }
+
+template <class S, class R, int TX, int TY>
+__global__ __launch_bounds__((t2Lanes<TX, TY>())) void lbCollideStreamT2(
+    const S* __restrict__ src, S* __restrict__ dst, const uint16_t* __restrict__ mask,
+    const int32_t* __restrict__ ibbIdx, const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
+    R* __restrict__ uxOut, R* __restrict__ uyOut, LBParams<R> pA, LBParams<R> pB, TileMap tm) {
+  int tbx = 0, tby = 0;
+  if (!tileOf(tm, tbx, tby)) return;
+  t2Tile<S, R, TX, TY>(src, dst, mask, ibbIdx, ibbQ, rhoOut, uxOut, uyOut, pA, pB, tbx * TX,
+                       tby * TY);
+}
+
+// The two-step pass with two neighbouring nodes per lane, for 16-bit
+// storage: population planes are read and written as aligned two-element
+// vectors (4-byte accesses instead of 2-byte ones).  The ring is two columns
+// wide on each side so every region pair starts on an even x; the outer
+// column is collided but its pushes are not needed.  Requires TX even and
+// nX % TX == 0 (so nX is even).
+template <int TX, int TY>
+constexpr int t2pLanes() {
+  return ((TX + 4) / 2 * (TY + 2) + 63) / 64 * 64;
+}
+
+template <class S, class R, int TX, int TY>
+__device__ __forceinline__ void t2pTile(const S* __restrict__ src, S* __restrict__ dst,
+                                        const uint16_t* __restrict__ mask,
+                                        const int32_t* __restrict__ ibbIdx,
+                                        const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
+                                        R* __restrict__ uxOut, R* __restrict__ uyOut,
+                                        const LBParams<R>& pA, const LBParams<R>& pB, int x0,
+                                        int y0) {
+  constexpr int RX = TX + 4, RY = TY + 2, NR = RX * RY, NP = RX / 2 * RY;
+  static_assert(TX % 2 == 0 && t2pLanes<TX, TY>() <= 1024, "tile and ring must fit one work-group");
+  __shared__ S mid[9][NR];
+  const KOFields none{nullptr, nullptr, nullptr, nullptr, nullptr};
+
+  // step 1 on the tile and its ring, one aligned pair of nodes per lane
+  for (int q = threadIdx.x; q < NP; q += t2pLanes<TX, TY>()) {
+    const int rx0 = 2 * (q % (RX / 2)), ry = q / (RX / 2);
+    const int y = y0 - 1 + ry;
+    if (y < 0 || y >= pA.nY) continue;
+    int xa = x0 - 2 + rx0;
+    xa = xa < 0 ? xa + pA.nX : (xa >= pA.nX ? xa - pA.nX : xa);  // even, so xa + 1 < nX
+    const size_t g0 = size_t(xa) + size_t(pA.XF) * y;
+    const uint32_t mm = *reinterpret_cast<const uint32_t*>(mask + g0);
+    const uint32_t m[2] = {mm & 0xFFFFu, mm >> 16};
+    if ((m[0] & kSolid) && (m[1] & kSolid)) continue;
+    R g[2][9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const Pair<S> v = *reinterpret_cast<const Pair<S>*>(src + g0 + size_t(pA.P) * i);
+      R a = static_cast<R>(v.a), b = static_cast<R>(v.b);
+      if constexpr (Scaled<S>::value) {
+        a *= R(unscale16(i));
+        b *= R(unscale16(i));
+      }
+      g[0][i] = a;
+      g[1][i] = b;
+    }
+#pragma unroll
+    for (int n = 0; n < 2; n++) {
+      if (m[n] & kSolid) continue;
+      if (pA.applyIbb && (m[n] & kIbbNode))
+        ibbCorrect<S, R>(g[n], src, xa + n, y, ibbQ + 8 * size_t(ibbIdx[g0 + n]), pA);
+      collideNode<R, false>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pA);
+      const int r = rx0 + n + RX * ry;
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        int tr, slot;
+        if (i != 0 && (m[n] & boundBit(i))) {
+          tr = r;
+          slot = rev(i);
+        } else {
+          const int tx = rx0 + n + cx(i), ty = ry + cy(i);
+          if (tx < 0 || tx >= RX || ty < 0 || ty >= RY) continue;
+          tr = tx + RX * ty;
+          slot = i;
+        }
+        R v = g[n][i];
+        if constexpr (Scaled<S>::value) v *= R(scale16(slot));
+        mid[slot][tr] = static_cast<S>(v);
+      }
+    }
+  }
+  __syncthreads();
+
+  // step 2 on the tile, one aligned pair of nodes per lane
+  if (threadIdx.x >= TX / 2 * TY) return;
+  const int ix0 = 2 * (threadIdx.x % (TX / 2)), iy = threadIdx.x / (TX / 2);
+  const int x = x0 + ix0, y = y0 + iy;
+  if (y >= pB.nY) return;
+  const size_t g0 = size_t(x) + size_t(pB.XF) * y;
+  const uint32_t mm = *reinterpret_cast<const uint32_t*>(mask + g0);
+  const uint32_t m[2] = {mm & 0xFFFFu, mm >> 16};
+  const bool live[2] = {!(m[0] & kSolid), !(m[1] & kSolid)};
+  if (!live[0] && !live[1]) return;
+  auto lds = [&](int slot, int rr) {
+    R v = static_cast<R>(mid[slot][rr]);
+    if constexpr (Scaled<S>::value) v *= R(unscale16(slot));
+    return v;
+  };
+  R g[2][9];
+#pragma unroll
+  for (int n = 0; n < 2; n++) {
+    if (!live[n]) continue;
+    const int r = (ix0 + n + 2) + RX * (iy + 1);
+#pragma unroll
+    for (int i = 0; i < 9; i++) g[n][i] = lds(i, r);
+    if (m[n] & kIbbNode)
+      ibbCorrectWith<R>(g[n], ibbQ + 8 * size_t(ibbIdx[g0 + n]),
+                        [&](int d) { return lds(rev(d), r - cx(d) - RX * cy(d)); });
+    collideNode<R, false>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pB);
+  }
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const int a = cx(i), b = cy(i);
+    size_t t[2];
+    int slot[2];
+#pragma unroll
+    for (int n = 0; n < 2; n++) {
+      if (i != 0 && (m[n] & boundBit(i))) {
+        t[n] = g0 + n + size_t(pB.P) * rev(i);
+        slot[n] = rev(i);
+      } else {
+        int tx = x + n + a;
+        tx = tx >= pB.nX ? tx - pB.nX : (tx < 0 ? tx + pB.nX : tx);
+        t[n] = size_t(tx) + size_t(pB.XF) * (y + b) + size_t(pB.P) * i;
+        slot[n] = i;
+      }
+    }
+    if (live[0] && live[1] && t[1] == t[0] + 1 && !(t[0] & 1)) {
+      R va = g[0][i], vb = g[1][i];
+      if constexpr (Scaled<S>::value) {
+        va *= R(scale16(slot[0]));
+        vb *= R(scale16(slot[1]));
+      }
+      Pair<S> v;
+      v.a = static_cast<S>(va);
+      v.b = static_cast<S>(vb);
+      *reinterpret_cast<Pair<S>*>(dst + t[0]) = v;
+    } else {
+#pragma unroll
+      for (int n = 0; n < 2; n++)
+        if (live[n]) storeDev<S, R>(dst, t[n], slot[n], g[n][i]);
+    }
+  }
+}
+
+template <class S, class R, int TX, int TY>
+__global__ __launch_bounds__((t2pLanes<TX, TY>())) void lbCollideStreamT2P(
+    const S* __restrict__ src, S* __restrict__ dst, const uint16_t* __restrict__ mask,
+    const int32_t* __restrict__ ibbIdx, const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
+    R* __restrict__ uxOut, R* __restrict__ uyOut, LBParams<R> pA, LBParams<R> pB, TileMap tm) {
+  int tbx = 0, tby = 0;
+  if (!tileOf(tm, tbx, tby)) return;
+  t2pTile<S, R, TX, TY>(src, dst, mask, ibbIdx, ibbQ, rhoOut, uxOut, uyOut, pA, pB, tbx * TX,
+                        tby * TY);
+}
+
 
 // ---------------------------------------------------------------------------
 // ensemble of independent channels in one launch
@@ -565,6 +738,35 @@ __global__ __launch_bounds__(BX) void lbCollideStreamBatch(const BatchEntry<S, R
       storeDev<S, R>(e.dst, size_t(tx) + size_t(p.XF) * (y + b) + size_t(p.P) * i, i, g[i]);
     }
   }
+}
+
+// The ensemble launch with two steps per pass: tiles of TX x TY nodes (or
+// two nodes per lane with PAIRS), numbered channel after channel and dealt
+// to the XCDs in contiguous runs.
+template <class S, class R, int TX, int TY, bool PAIRS>
+__global__ __launch_bounds__((PAIRS ? t2pLanes<TX, TY>() : t2Lanes<TX, TY>())) void
+lbCollideStreamBatchT2(const BatchEntry<S, R>* __restrict__ ents, int n, int totalTiles) {
+  // XCD-aware order over all tiles (grid of 8 * per blocks)
+  const int per = (totalTiles + 7) / 8;
+  const int T = int(blockIdx.x & 7) * per + int(blockIdx.x >> 3);
+  if (T >= totalTiles) return;
+  int lo = 0, hi = n - 1;  // last channel whose first tile is <= T
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ents[mid].tileStart <= T) lo = mid; else hi = mid - 1;
+  }
+  const BatchEntry<S, R>& e = ents[lo];
+  const int t = T - e.tileStart;
+  LBParams<R> pA = e.p, pB = e.p;
+  pA.saveMacros = 0;
+  pB.applyIbb = 1;
+  const int x0 = (t % e.tilesX) * TX, y0 = (t / e.tilesX) * TY;
+  if constexpr (PAIRS)
+    t2pTile<S, R, TX, TY>(e.src, e.dst, e.mask, e.ibbIdx, e.ibbQ, e.rho, e.ux, e.uy, pA, pB, x0,
+                          y0);
+  else
+    t2Tile<S, R, TX, TY>(e.src, e.dst, e.mask, e.ibbIdx, e.ibbQ, e.rho, e.ux, e.uy, pA, pB, x0,
+                         y0);
 }
 
 // Reference-layout populations f = w + g (post interpolated bounce-back)

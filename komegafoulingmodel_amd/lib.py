@@ -46,7 +46,8 @@ EXPORTS = [
     "kofm_synchronize", "kofm_set_launch", "kofm_autotune", "kofm_set_graphs",
     "kofm_objective_run", "kofm_adjoint",
     "kofm_ensemble_create", "kofm_ensemble_step", "kofm_ensemble_time_steps",
-    "kofm_ensemble_set_block", "kofm_ensemble_set_cache_budget", "kofm_ensemble_last_error",
+    "kofm_ensemble_set_block", "kofm_ensemble_set_launch", "kofm_ensemble_set_cache_budget",
+    "kofm_ensemble_last_error",
     "kofm_ensemble_destroy",
 ]
 
@@ -143,6 +144,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
                                                C.POINTER(C.c_int64)]),
         "kofm_ensemble_set_block": (C.c_int, [P, C.c_int32]),
         "kofm_ensemble_set_cache_budget": (C.c_int, [P, C.c_int64, C.POINTER(C.c_int32)]),
+        "kofm_ensemble_set_launch": (C.c_int, [P, C.c_int32, C.c_int32]),
         "kofm_ensemble_last_error": (C.c_char_p, [P]),
         "kofm_ensemble_destroy": (None, [P]),
         "kofm_autotune": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -347,6 +349,9 @@ class Ensemble:
 
     def set_block(self, block_x: int):
         self._check(self.lib.kofm_ensemble_set_block(self.h, block_x))
+
+    def set_launch(self, block_x: int, steps_per_launch: int = 1):
+        self._check(self.lib.kofm_ensemble_set_launch(self.h, block_x, steps_per_launch))
 
     def set_cache_budget(self, nbytes: int) -> int:
         """Groups the members under a population-byte budget (0: one group);

@@ -198,7 +198,8 @@ def test_komega_graph_steps_match_eager_launches():
 
 @pytest.mark.parametrize("block,npt,pad,spp", [(128, 1, 0, 1), (1024, 1, 64, 1), (256, 2, 1088, 1),
                                                (512, 2, 0, 1), (512, 1, 1088, 2), (640, 1, 0, 2),
-                                               (832, 1, 64, 2)])
+                                               (832, 1, 64, 2), (1024, 1, 64, 2),
+                                               (1536, 1, 0, 2), (1664, 1, 1088, 2)])
 def test_launch_shapes_give_identical_results(block, npt, pad, spp):
     """Every collide-stream launch shape (work-group width, one or two nodes
     per lane, one or two steps per pass, plane padding) produces the same
@@ -215,11 +216,30 @@ def test_launch_shapes_give_identical_results(block, npt, pad, spp):
     assert np.array_equal(La.download("rho"), Lb.download("rho"))
 
 
-@pytest.mark.parametrize("budget,groups", [(0, 1), (2 << 20, 2)])
-def test_ensemble_matches_individual_runs(budget, groups):
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_16bit_two_step_kernels_match_one_step(storage):
+    """With 16-bit storage the two-step kernels (one and two nodes per lane)
+    round every intermediate population to the storage type, so they end
+    bit-identical to one-step launches."""
+    w = _mini_fouled()
+    ref, _ = make_pair(w, storage)
+    ref.set_launch(256, 2, 1088)
+    ref.step(41, 2)
+    for block, pad in ((512, 64), (1664, 0), (1024, 1088)):
+        L, _ = make_pair(w, storage)
+        L.set_launch(block, 1, pad, 2)
+        L.step(41, 2)
+        assert np.array_equal(L.download("f"), ref.download("f")), block
+        assert np.array_equal(L.download("ux"), ref.download("ux")), block
+
+
+@pytest.mark.parametrize("budget,groups,block,spp", [(0, 1, 512, 1), (2 << 20, 2, 512, 1),
+                                                     (0, 1, 832, 2), (2 << 20, 2, 1664, 2)])
+def test_ensemble_matches_individual_runs(budget, groups, block, spp):
     """Three channels of different sizes and fouling stepped by ensemble
-    launches (all in one group, or packed into two cache-sized groups) end
-    bit-identical to the same channels stepped alone."""
+    launches (all in one group, or packed into two cache-sized groups; one
+    step per launch, or two with one or two nodes per lane) end bit-identical
+    to the same channels stepped alone."""
     from komegafoulingmodel_amd import lib
     ws = [channel.fouled_channel(256, 48, Fval=1e-5, seed=31, amp=(0, 3), width=(10, 40)),
           channel.fouled_channel(384, 64, Fval=2e-5, seed=32, amp=(0, 3), width=(10, 40)),
@@ -228,6 +248,7 @@ def test_ensemble_matches_individual_runs(budget, groups):
     group = [make_pair(w, "f32")[0] for w in ws]
     ens = lib.Ensemble(group)
     assert ens.set_cache_budget(budget) == groups
+    ens.set_launch(block, spp)
     for n in (1, 70, 33):
         ens.step(n, 2)
         for L in solo:
