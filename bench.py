@@ -159,8 +159,8 @@ def measure_ensemble(storage, steps=300):
 
 def kernel_name(launch, storage="float"):
     if launch.get("steps_per_pass", 1) == 2:
-        tile = {512: (64, 8), 640: (128, 5), 832: (64, 13), 1024: (64, 16), 1536: (128, 12),
-                1664: (64, 26)}[launch["block_x"]]
+        tile = {512: (64, 8), 640: (128, 5), 832: (64, 13), 1856: (64, 29), 1024: (64, 16),
+                1536: (128, 12), 1664: (64, 26)}[launch["block_x"]]
         k = "lbCollideStreamT2P" if launch["block_x"] in (1024, 1536, 1664) else "lbCollideStreamT2"
         return f"{k}<{storage},float,{tile[0]},{tile[1]}>"
     k = "lbCollideStream2" if launch.get("nodes_per_lane", 1) == 2 else "lbCollideStream"

@@ -1038,13 +1038,14 @@ class EngineT final : public Engine {
   }
 
   // The temporally blocked kernels, selected by the tile's node count: one
-  // node per lane on 512 (64 x 8), 832 (64 x 13) or 640 (128 x 5) node
-  // tiles; two neighbouring nodes per lane (aligned two-element accesses)
-  // on 1024 (64 x 16), 1664 (64 x 26) or 1536 (128 x 12) node tiles.
+  // node per lane on 512 (64 x 8), 832 (64 x 13), 640 (128 x 5) or 1856
+  // (64 x 29, two nodes per lane in each step) node tiles; two neighbouring
+  // nodes per lane (aligned two-element accesses) on 1024 (64 x 16), 1664
+  // (64 x 26) or 1536 (128 x 12) node tiles.
   static int t2Width(int nodes) { return (nodes == 640 || nodes == 1536) ? 128 : 64; }
   static bool t2Tile(int nodes) {
-    return nodes == 512 || nodes == 640 || nodes == 832 || nodes == 1024 || nodes == 1536 ||
-           nodes == 1664;
+    return nodes == 512 || nodes == 640 || nodes == 832 || nodes == 1856 || nodes == 1024 ||
+           nodes == 1536 || nodes == 1664;
   }
   template <int TX, int TY>
   void launchT2PT(const dev::LBParams<R>& pA, const dev::LBParams<R>& pB, const S* src, S* dst) {
@@ -1073,6 +1074,7 @@ class EngineT final : public Engine {
     switch (nodes) {
       case 640: launchT2T<128, 5>(pA, pB, src, dst); break;
       case 832: launchT2T<64, 13>(pA, pB, src, dst); break;
+      case 1856: launchT2T<64, 29>(pA, pB, src, dst); break;
       case 1024: launchT2PT<64, 16>(pA, pB, src, dst); break;
       case 1536: launchT2PT<128, 12>(pA, pB, src, dst); break;
       case 1664: launchT2PT<64, 26>(pA, pB, src, dst); break;
@@ -1146,8 +1148,8 @@ class EngineT final : public Engine {
     if (spp != 1 && spp != 2) return fail(err, "steps per pass must be 1 or 2");
     if (spp == 2 && (ko_ || npt != 1 || !t2Tile(block) || nX_ % t2Width(block)))
       return fail(err, "two steps per pass need k-omega off, nodes_per_lane 1 and a tile "
-                       "(block_x) of 512, 640, 832, 1024, 1536 or 1664 nodes whose width "
-                       "divides nX");
+                       "(block_x) of 512, 640, 832, 1856, 1024, 1536 or 1664 nodes whose "
+                       "width divides nX");
     if (pad < 0 || pad % 64) return fail(err, "plane padding must be a non-negative multiple of 64");
     int rc = relayout(pad, err);
     if (rc) return rc;
@@ -1208,7 +1210,7 @@ class EngineT final : public Engine {
           }
         }
       if (!ko_)
-        for (int block : {512, 640, 832, 1024, 1536, 1664}) {
+        for (int block : {512, 640, 832, 1856, 1024, 1536, 1664}) {
           if (nX_ % t2Width(block)) continue;
           auto pA = p, pB = p;
           pA.applyIbb = 1;

@@ -194,12 +194,23 @@ template <class R>
 struct Moments {
   R drho, rho, inv, ux, uy;
 };
+// 1 / rho: the hardware reciprocal in FP32 (within 1 ulp, one instruction
+// instead of the IEEE division sequence), the IEEE quotient in FP64.
+template <class R>
+__device__ __forceinline__ R recipDensity(R x) {
+  return R(1) / x;
+}
+template <>
+__device__ __forceinline__ float recipDensity<float>(float x) {
+  return __builtin_amdgcn_rcpf(x);
+}
+
 template <class R>
 __device__ __forceinline__ Moments<R> moments(const R (&g)[9]) {
   Moments<R> m;
   m.drho = ((g[0] + g[1]) + (g[2] + g[3])) + ((g[4] + g[5]) + (g[6] + g[7])) + g[8];
   m.rho = R(1) + m.drho;
-  m.inv = R(1) / m.rho;
+  m.inv = recipDensity(m.rho);
   m.ux = m.inv * (((g[1] - g[2]) + (g[5] - g[6])) + (g[7] - g[8]));
   m.uy = m.inv * (((g[3] - g[4]) + (g[5] - g[6])) - (g[7] - g[8]));
   return m;
@@ -272,13 +283,27 @@ __device__ __forceinline__ void collideNode(R (&g)[9], size_t gi, R* __restrict_
     uyOut[gi] = uy;
   }
 
-  // exact-difference forcing: g += -w (g - geq(u)) + feq(u+F) - feq(u)
+  // exact-difference forcing, g += w (geq(u) - g) + feq(u+F) - feq(u), in
+  // the product form per axis (the collision is the instruction-heaviest
+  // part of the two-step pass, so the shared factors are formed once):
+  //   geq_i   = drho w_i + rho (w1(a) py_b + px_a phy_b),  phy_b = w1(b) + py_b
+  //   force_i = rho phy_b dS_a,  dS_a = dphi(a, ux + F) - dphi(a, ux)
+  // with px_a = dphi(a, ux), py_b = dphi(b, uy); multiply-adds are explicit.
+  const R u2 = ux * ux, v2 = uy * uy;
+  const R px[3] = {R(0.5) * (u2 - ux), -u2, R(0.5) * (u2 + ux)};
+  const R py[3] = {R(0.5) * (v2 - uy), -v2, R(0.5) * (v2 + uy)};
+  const R phy[3] = {w1<R>(-1) + py[0], w1<R>(0) + py[1], w1<R>(1) + py[2]};
+  const R rphy[3] = {rho * phy[0], rho * phy[1], rho * phy[2]};
+  const R F = p.Fval;
+  const R t = F * fma(R(2), ux, F);
+  const R dS[3] = {R(0.5) * (t - F), -t, R(0.5) * (t + F)};
+  const R dw[3] = {drho * R(wgt(0)), drho * R(wgt(1)), drho * R(wgt(5))};
 #pragma unroll
   for (int i = 0; i < 9; i++) {
-    const int a = cx(i), b = cy(i);
-    const R gq = geq<R>(i, drho, rho, ux, uy);
-    const R force = rho * (w1<R>(b) + dphi(b, uy)) * dphiShift(a, ux, p.Fval);
-    g[i] = g[i] - omega * (g[i] - gq) + force;
+    const int a = cx(i) + 1, b = cy(i) + 1;
+    const R tw = fma(px[a], phy[b], w1<R>(a - 1) * py[b]);
+    const R gq = fma(rho, tw, dw[i == 0 ? 0 : (i < 5 ? 1 : 2)]);
+    g[i] = fma(rphy[b], dS[a], fma(omega, gq - g[i], g[i]));
   }
 }
 
@@ -432,11 +457,12 @@ __device__ __forceinline__ bool tileOf(const TileMap& t, int& bx, int& by) {
 // (which share their ring columns) and the bands above and below run on the
 // same XCD and meet in its L2.
 // The block has one lane per node of the tile and its ring (rounded up to
-// whole waves), so the first step takes a single pass; the second step runs
-// on the first TX * TY lanes.
+// whole waves, at most 1024): a tile and ring of up to 1024 nodes takes a
+// single pass in each step, larger tiles loop.
 template <int TX, int TY>
 constexpr int t2Lanes() {
-  return ((TX + 2) * (TY + 2) + 63) / 64 * 64;
+  return ((TX + 2) * (TY + 2) + 63) / 64 * 64 < 1024 ? ((TX + 2) * (TY + 2) + 63) / 64 * 64
+                                                      : 1024;
 }
 
 template <class S, class R, int TX, int TY>
@@ -448,7 +474,6 @@ __device__ __forceinline__ void t2Tile(const S* __restrict__ src, S* __restrict_
                                        const LBParams<R>& pA, const LBParams<R>& pB, int x0,
                                        int y0) {
   constexpr int RX = TX + 2, RY = TY + 2, NR = RX * RY;
-  static_assert(t2Lanes<TX, TY>() <= 1024, "tile and ring must fit one work-group");
   __shared__ S mid[9][NR];
   const KOFields none{nullptr, nullptr, nullptr, nullptr, nullptr};
 
@@ -485,38 +510,39 @@ __device__ __forceinline__ void t2Tile(const S* __restrict__ src, S* __restrict_
   __syncthreads();
 
   // step 2 on the tile
-  if (threadIdx.x >= TX * TY) return;
-  const int ix = threadIdx.x % TX, iy = threadIdx.x / TX;
-  const int x = x0 + ix, y = y0 + iy;
-  if (y >= pB.nY) return;
-  const size_t gi = size_t(x) + size_t(pB.XF) * y;
-  const uint32_t m = mask[gi];
-  if (m & kSolid) return;
-  const int r = (ix + 1) + RX * (iy + 1);
-  auto lds = [&](int slot, int rr) {
-    R v = static_cast<R>(mid[slot][rr]);
-    if constexpr (Scaled<S>::value) v *= R(unscale16(slot));
-    return v;
-  };
-  R g[9];
-#pragma unroll
-  for (int i = 0; i < 9; i++) g[i] = lds(i, r);
-  if (m & kIbbNode)
-    ibbCorrectWith<R>(g, ibbQ + 8 * size_t(ibbIdx[gi]),
-                      [&](int d) { return lds(rev(d), r - cx(d) - RX * cy(d)); });
-  collideNode<R, false>(g, gi, rhoOut, uxOut, uyOut, none, pB);
-  const int xe = x < pB.nX - 1 ? x + 1 : 0;
-  const int xw = x > 0 ? x - 1 : pB.nX - 1;
-#pragma unroll
-  for (int i = 0; i < 9; i++) {
-    const int a = cx(i), b = cy(i);
-    if (i == 0) {
-      storeDev<S, R>(dst, gi, 0, g[0]);
-    } else if (m & boundBit(i)) {
-      storeDev<S, R>(dst, gi + size_t(pB.P) * rev(i), rev(i), g[i]);
-    } else {
-      const int tx = a > 0 ? xe : (a < 0 ? xw : x);
-      storeDev<S, R>(dst, size_t(tx) + size_t(pB.XF) * (y + b) + size_t(pB.P) * i, i, g[i]);
+  for (int q = threadIdx.x; q < TX * TY; q += t2Lanes<TX, TY>()) {
+    const int ix = q % TX, iy = q / TX;
+    const int x = x0 + ix, y = y0 + iy;
+    if (y >= pB.nY) continue;
+    const size_t gi = size_t(x) + size_t(pB.XF) * y;
+    const uint32_t m = mask[gi];
+    if (m & kSolid) continue;
+    const int r = (ix + 1) + RX * (iy + 1);
+    auto lds = [&](int slot, int rr) {
+      R v = static_cast<R>(mid[slot][rr]);
+      if constexpr (Scaled<S>::value) v *= R(unscale16(slot));
+      return v;
+    };
+    R g[9];
+  #pragma unroll
+    for (int i = 0; i < 9; i++) g[i] = lds(i, r);
+    if (m & kIbbNode)
+      ibbCorrectWith<R>(g, ibbQ + 8 * size_t(ibbIdx[gi]),
+                        [&](int d) { return lds(rev(d), r - cx(d) - RX * cy(d)); });
+    collideNode<R, false>(g, gi, rhoOut, uxOut, uyOut, none, pB);
+    const int xe = x < pB.nX - 1 ? x + 1 : 0;
+    const int xw = x > 0 ? x - 1 : pB.nX - 1;
+  #pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const int a = cx(i), b = cy(i);
+      if (i == 0) {
+        storeDev<S, R>(dst, gi, 0, g[0]);
+      } else if (m & boundBit(i)) {
+        storeDev<S, R>(dst, gi + size_t(pB.P) * rev(i), rev(i), g[i]);
+      } else {
+        const int tx = a > 0 ? xe : (a < 0 ? xw : x);
+        storeDev<S, R>(dst, size_t(tx) + size_t(pB.XF) * (y + b) + size_t(pB.P) * i, i, g[i]);
+      }
     }
   }
 }

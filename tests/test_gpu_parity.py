@@ -199,7 +199,8 @@ def test_komega_graph_steps_match_eager_launches():
 @pytest.mark.parametrize("block,npt,pad,spp", [(128, 1, 0, 1), (1024, 1, 64, 1), (256, 2, 1088, 1),
                                                (512, 2, 0, 1), (512, 1, 1088, 2), (640, 1, 0, 2),
                                                (832, 1, 64, 2), (1024, 1, 64, 2),
-                                               (1536, 1, 0, 2), (1664, 1, 1088, 2)])
+                                               (1536, 1, 0, 2), (1664, 1, 1088, 2),
+                                               (1856, 1, 64, 2)])
 def test_launch_shapes_give_identical_results(block, npt, pad, spp):
     """Every collide-stream launch shape (work-group width, one or two nodes
     per lane, one or two steps per pass, plane padding) produces the same
