@@ -82,6 +82,7 @@ def measure_adjoint(**kw):
     th = P.initial_theta()
     L = P.lattice()
     P.build(th)
+    launch = L.autotune(20)  # the forward it is compared with runs its best shape
     fluid = int(np.count_nonzero((L.geometry("nType").astype(np.int32) & 1) == 0))
     L.step(64)  # warm-up of the plain path
     L.synchronize()
@@ -102,6 +103,8 @@ def measure_adjoint(**kw):
     return {"workload": f"{P.nX}x{P.nY} serpentine channel, B-spline fouling, "
                         f"T={P.steps}, window {P.window}, checkpoints every {P.interval}",
             "fluid_nodes": fluid, "J": J, "grad_norm": float(np.linalg.norm(g)),
+            "launch": {"block_x": launch[0], "nodes_per_lane": launch[1],
+                       "steps_per_pass": launch[4], "plane_pad": launch[2]},
             "forward_ms": t_fwd * 1e3, "forward_mlups": fluid * P.steps / t_fwd / 1e6,
             "objective_run_ms": t_obj * 1e3, "adjoint_ms": t_adj * 1e3,
             "adjoint_cost_ratio": t_adj / t_fwd,

@@ -664,7 +664,10 @@ class EngineT final : public Engine {
   // from step w0 of the segment on (the state advances as with kofm_step).
   void forwardSegment(int len, int w0, const dev::ObjectiveNodes& on) {
     const dim3 og((on.n + 63) / 64);
-    for (int m = 0; m < len; m++) {
+    int m = 0;
+    if (lbSpp_ == 2)  // steps before the window need no objective term
+      for (; m + 2 <= w0; m += 2) launchLB2(0);
+    for (; m < len; m++) {
       const S* src = cur();
       auto p = lbParams(0);
       launchLB(0);
@@ -680,7 +683,22 @@ class EngineT final : public Engine {
   void adjointSegment(int len, int w0, int ap0, int a, const dev::AdjointBuffers& abuf,
                       const dev::ObjectiveNodes& on) {
     const dim3 og((on.n + 63) / 64), ig((nIbbNodes_ + 63) / 64);
-    for (int m = 0; m < len; m++)
+    // recompute every state of the segment; with a two-step launch shape the
+    // pairs run the temporally blocked kernel that also writes the middle state
+    int m = 0;
+    if (lbSpp_ == 2 && lbNpt_ == 1 && nX_ % 64 == 0) {
+      for (; m + 2 <= len; m += 2) {
+        auto pA = lbParams(0);
+        pA.applyIbb = m == 0 ? ap0 : 1;
+        pA.saveMacros = 0;
+        auto pB = lbParams(0);
+        pB.applyIbb = 1;
+        pB.saveMacros = 0;
+        launchT2T<64, 13>(pA, pB, seg_[m]->template as<S>(), seg_[m + 2]->template as<S>(),
+                          seg_[m + 1]->template as<S>());
+      }
+    }
+    for (; m < len; m++)
       launchLBFrom(seg_[m]->template as<S>(), seg_[m + 1]->template as<S>(), m == 0 ? ap0 : 1);
     for (int m = len - 1; m >= 0; m--) {
       const int apply = m == 0 ? ap0 : 1;
@@ -1059,7 +1077,8 @@ class EngineT final : public Engine {
                        uy_.as<R>(), pA, pB, tm);
   }
   template <int TX, int TY>
-  void launchT2T(const dev::LBParams<R>& pA, const dev::LBParams<R>& pB, const S* src, S* dst) {
+  void launchT2T(const dev::LBParams<R>& pA, const dev::LBParams<R>& pB, const S* src, S* dst,
+                 S* mid1 = nullptr) {
     dev::TileMap tm;
     tm.tilesX = nX_ / TX;
     tm.nTiles = tm.tilesX * ((nY_ + TY - 1) / TY);
@@ -1067,7 +1086,7 @@ class EngineT final : public Engine {
     hipLaunchKernelGGL((dev::lbCollideStreamT2<S, R, TX, TY>), dim3(8 * tm.per),
                        dim3(dev::t2Lanes<TX, TY>()), 0,
                        stream_, src, dst, mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(), ibbQ_.as<R>(),
-                       rho_.as<R>(), ux_.as<R>(), uy_.as<R>(), pA, pB, tm);
+                       rho_.as<R>(), ux_.as<R>(), uy_.as<R>(), pA, pB, tm, mid1);
   }
   void launchT2(int nodes, const dev::LBParams<R>& pA, const dev::LBParams<R>& pB, const S* src,
                 S* dst) {

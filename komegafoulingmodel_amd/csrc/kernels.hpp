@@ -465,6 +465,9 @@ constexpr int t2Lanes() {
                                                       : 1024;
 }
 
+// With `mid1` set, the tile's own nodes also push their first-step values
+// to HBM, so the intermediate populations are written exactly as a one-step
+// launch would write them (the adjoint's recompute needs every state).
 template <class S, class R, int TX, int TY>
 __device__ __forceinline__ void t2Tile(const S* __restrict__ src, S* __restrict__ dst,
                                        const uint16_t* __restrict__ mask,
@@ -472,7 +475,7 @@ __device__ __forceinline__ void t2Tile(const S* __restrict__ src, S* __restrict_
                                        const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
                                        R* __restrict__ uxOut, R* __restrict__ uyOut,
                                        const LBParams<R>& pA, const LBParams<R>& pB, int x0,
-                                       int y0) {
+                                       int y0, S* __restrict__ mid1 = nullptr) {
   constexpr int RX = TX + 2, RY = TY + 2, NR = RX * RY;
   __shared__ S mid[9][NR];
   const KOFields none{nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -490,6 +493,22 @@ __device__ __forceinline__ void t2Tile(const S* __restrict__ src, S* __restrict_
     R g[9];
     loadNode<S, R>(g, src, gi, x, y, m, ibbIdx, ibbQ, pA);
     collideNode<R, false>(g, gi, rhoOut, uxOut, uyOut, none, pA);
+    if (mid1 && rx >= 1 && rx <= TX && ry >= 1 && ry <= TY) {
+      const int xe = x < pA.nX - 1 ? x + 1 : 0;
+      const int xw = x > 0 ? x - 1 : pA.nX - 1;
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        if (i == 0) {
+          storeDev<S, R>(mid1, gi, 0, g[0]);
+        } else if (m & boundBit(i)) {
+          storeDev<S, R>(mid1, gi + size_t(pA.P) * rev(i), rev(i), g[i]);
+        } else {
+          const int tx = cx(i) > 0 ? xe : (cx(i) < 0 ? xw : x);
+          storeDev<S, R>(mid1, size_t(tx) + size_t(pA.XF) * (y + cy(i)) + size_t(pA.P) * i, i,
+                         g[i]);
+        }
+      }
+    }
 #pragma unroll
     for (int i = 0; i < 9; i++) {
       int tr, slot;
@@ -551,11 +570,12 @@ template <class S, class R, int TX, int TY>
 __global__ __launch_bounds__((t2Lanes<TX, TY>())) void lbCollideStreamT2(
     const S* __restrict__ src, S* __restrict__ dst, const uint16_t* __restrict__ mask,
     const int32_t* __restrict__ ibbIdx, const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
-    R* __restrict__ uxOut, R* __restrict__ uyOut, LBParams<R> pA, LBParams<R> pB, TileMap tm) {
+    R* __restrict__ uxOut, R* __restrict__ uyOut, LBParams<R> pA, LBParams<R> pB, TileMap tm,
+    S* __restrict__ mid1) {
   int tbx = 0, tby = 0;
   if (!tileOf(tm, tbx, tby)) return;
   t2Tile<S, R, TX, TY>(src, dst, mask, ibbIdx, ibbQ, rhoOut, uxOut, uyOut, pA, pB, tbx * TX,
-                       tby * TY);
+                       tby * TY, mid1);
 }
 
 // The two-step pass with two neighbouring nodes per lane, for 16-bit

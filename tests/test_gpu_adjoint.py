@@ -153,3 +153,21 @@ def test_torch_autograd_returns_the_adjoint():
     assert np.array_equal(t.grad.cpu().numpy(), g2)
     P.close()
     P2.close()
+
+
+def test_adjoint_is_independent_of_the_launch_shape():
+    """With the two-step launch shape the objective run steps in pairs before
+    the window and the recompute runs the two-step kernel that also writes
+    the middle state; J and the gradient are bit-identical to one-step
+    launches."""
+    res = []
+    for shape in (None, (832, 1, 64, 2)):
+        P = _problem("f32")
+        th = P.initial_theta()
+        if shape:
+            P.lattice().set_launch(*shape)
+        J = P.objective(th)
+        res.append((J, P.gradient()))
+        P.close()
+    assert res[0][0] == res[1][0]
+    assert np.array_equal(res[0][1], res[1][1])
