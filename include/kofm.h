@@ -296,20 +296,34 @@ typedef struct kofm_objective {
   double shear_weight; /* weight of the wall-shear integral */
 } kofm_objective;
 
-/* Runs nsteps plain steps (k-omega off) from the current state, returns J
- * and keeps the populations every checkpoint_interval steps (about
- * sqrt(nsteps) keeps the checkpoints and the recomputed interval the same
- * size).  The state advances exactly as with kofm_step. */
+/* Runs nsteps steps (with k-omega if the context has it) from the current
+ * state, returns J and keeps the state every checkpoint_interval steps
+ * (about sqrt(nsteps) keeps the checkpoints and the recomputed interval the
+ * same size).  The state advances exactly as with kofm_step.  With k-omega
+ * the traction uses each step's eddy viscosity, as kofm_wall_shear does. */
 int kofm_objective_run(kofm_ctx* ctx, int32_t nsteps, int32_t checkpoint_interval,
                        const kofm_objective* obj, double* J);
 
 /* Reverse sweep over the last kofm_objective_run: each checkpoint interval
- * is recomputed and the adjoint collide-stream runs backwards through it.
- * dJ_dy[nN] receives dJ/dy of every wall point of kofm_build_geometry (the
- * x coordinates held fixed; node types and link sets are held at the built
- * geometry); dJ_dq[nIbb] (may be NULL) receives dJ/dq of every boundary link
+ * is recomputed and the adjoint step runs backwards through it (with
+ * k-omega: the transposed implicit solves and the adjoints of the row
+ * assembly, the diffusivities and the k-omega collision; the transposed
+ * solves use the context's relative tolerances).  dJ_dy[nN] receives dJ/dy
+ * of every wall point of kofm_build_geometry (the x coordinates held fixed;
+ * node types, link sets, and the wall links that the wall distances and
+ * spacings were taken from, are held at the built geometry); dJ_dq[nIbb]
+ * (may be NULL) receives the part through the boundary-link q of every link
  * in KOFM_GEOM_IBB order (0 for a link repeated later in the list). */
 int kofm_adjoint(kofm_ctx* ctx, double* dJ_dy, double* dJ_dq);
+
+/* dJ/dy of the wall points from given sensitivities of the k-omega wall
+ * distance field (wall_d_bar[D], may be NULL) and of the axis wall spacings
+ * (dx_bar[4*D], planes E, W, N, S, may be NULL), each differentiated through
+ * the wall link (or link end point) the geometry build took it from; adds
+ * into dJ_dy[nN].  Host only (works on a kofm_create_host context).  No
+ * reference counterpart (the wall-field part of kofm_adjoint). */
+int kofm_geometry_field_gradient(const kofm_ctx* ctx, const double* wall_d_bar,
+                                 const double* dx_bar, double* dJ_dy);
 
 /* --- ensembles ------------------------------------------------------------- */
 

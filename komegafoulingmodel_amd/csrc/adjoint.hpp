@@ -293,8 +293,9 @@ __global__ __launch_bounds__(64) void objectiveForward(const S* __restrict__ sn,
                                                        const uint16_t* __restrict__ mask,
                                                        const int32_t* __restrict__ ibbIdx,
                                                        const R* __restrict__ ibbQ,
-                                                       const R* __restrict__ cfss, ObjectiveNodes o,
-                                                       LBParams<R> p) {
+                                                       const R* __restrict__ cfss,
+                                                       const double* __restrict__ nut,
+                                                       ObjectiveNodes o, LBParams<R> p) {
   const int k = blockIdx.x * 64 + threadIdx.x;
   if (k >= o.n) return;
   const int gi = o.node[k];
@@ -308,22 +309,25 @@ __global__ __launch_bounds__(64) void objectiveForward(const S* __restrict__ sn,
   const int w = o.wall[k];
   if (w >= 0) {
     loadCorrected<S, R>(sn1, gi, m, 1, ibbIdx, ibbQ, p, g, nv, t, links);
-    const R tau = wallTraction<R>(nv, mo.rho, mo.ux + p.Fval * R(0.5), mo.uy,
-                                  cfss + 16 * size_t(w), shearScale<R>(R(0), p.nu));
+    const R tau = wallTraction<R>(nv, mo.rho, mo.ux + p.Fval * R(0.5), mo.uy, cfss + 16 * size_t(w),
+                                  shearScale<R>(nut ? R(nut[gi]) : R(0), p.nu));
     o.accTau[k] += double(tau);
   }
 }
 
 // Adjoint of one window step's objective terms with respect to the corrected
 // values at level n+1 (traction) and level n (density, traction's macros):
-// bar[18k .. 18k+8] and bar[18k+9 .. 18k+17].
+// bar[18k .. 18k+8] and bar[18k+9 .. 18k+17]; with the step's eddy viscosity
+// nut, its adjoint is added to nutBar at the node.
 template <class S, class R>
 __global__ __launch_bounds__(64) void objectiveGrad(const S* __restrict__ sn,
                                                     const S* __restrict__ sn1,
                                                     const uint16_t* __restrict__ mask,
                                                     const int32_t* __restrict__ ibbIdx,
                                                     const R* __restrict__ ibbQ,
-                                                    const R* __restrict__ cfss, ObjectiveNodes o,
+                                                    const R* __restrict__ cfss,
+                                                    const double* __restrict__ nut,
+                                                    double* __restrict__ nutBar, ObjectiveNodes o,
                                                     LBParams<R> p) {
   const int k = blockIdx.x * 64 + threadIdx.x;
   if (k >= o.n) return;
@@ -344,7 +348,8 @@ __global__ __launch_bounds__(64) void objectiveGrad(const S* __restrict__ sn,
     loadCorrected<S, R>(sn1, gi, m, 1, ibbIdx, ibbQ, p, h, hv, t, links);
     const R r = mo.rho, ux = mo.ux + p.Fval * R(0.5), uy = mo.uy;
     const R* cf = cfss + 16 * size_t(w);
-    const R ve = shearScale<R>(R(0), p.nu);
+    const R nt = nut ? R(nut[gi]) : R(0);
+    const R ve = shearScale<R>(nt, p.nu);
     const R dr = r - R(1);
     R tx = R(0), ty = R(0);
 #pragma unroll
@@ -359,6 +364,11 @@ __global__ __launch_bounds__(64) void objectiveGrad(const S* __restrict__ sn,
       const R sg = tx < R(0) ? R(-1) : R(1);
       const R txb = T * sg * tx / (mag * r), tyb = T * sg * ty / (mag * r);
       rb -= T * (sg * mag / r) / r;
+      if (nutBar) {
+        // tau is proportional to ve = (nt + nu) / (3 (nt + nu) + 1/2)
+        const double e = 3. * (double(nt) + double(p.nu)) + 0.5;
+        nutBar[gi] += double(T) * double(sg * mag / r) / double(ve) * (0.5 / (e * e));
+      }
 #pragma unroll
       for (int a = 0; a < 8; a++) {
         const R fb = ve * (txb * cf[a] + tyb * cf[8 + a]);
@@ -444,6 +454,416 @@ __global__ __launch_bounds__(64) void objectiveUpstream(R* __restrict__ abv,
     sx = sx >= p.nX ? sx - p.nX : (sx < 0 ? sx + p.nX : sx);
     abv[size_t(sx) + size_t(p.XF) * (y - cy(d)) + size_t(p.P) * rev(d)] += up[d - 1];
   }
+}
+
+// ---------------------------------------------------------------------------
+// k-omega adjoint
+// ---------------------------------------------------------------------------
+//
+// A k-omega step maps (s^n, k^n, omega^n, S^n) to (s^{n+1}, k^{n+1},
+// omega^{n+1}, S^{n+1}): the collision clamps k and omega, forms the eddy
+// viscosity from them and S^n, relaxes with omega_t(nu_t) and writes the
+// strain S^{n+1} and the macros; the diffusivity and row kernels assemble two
+// five-point systems from those, and the implicit solve gives k^{n+1},
+// omega^{n+1}.  Its transpose runs in reverse order:
+//   1. the transposed systems A^T lambda = (adjoint of k^{n+1}, omega^{n+1});
+//   2. koRowsAdjoint: row adjoints (b = lambda, A_ij = -lambda_i x_j) back
+//      through the row assembly to the diffusivities (own node and the four
+//      neighbours, one plane each), m grad k, macros, k, omega, F1, nu_t,
+//      S^{n+1}, wall distance and spacings;
+//   3. koDiffAdjoint: gathers the diffusivity adjoints and goes back through
+//      the diffusivity map to k and omega (own node and neighbours), nu_t,
+//      density, wall distance and spacings;
+//   4. lbAdjointStepKO: gathers the k and omega adjoints, then the collide-
+//      stream transpose with the adjoint of omega_t(nu_t), of the macros and
+//      of the strain, and back through nu_t and the clamps to k^n, omega^n,
+//      S^n.
+// The per-node maps are differentiated by forward-mode evaluation of the
+// functions the forward kernels run (komega_local.hpp).  The forward step's
+// intermediates come from a tape recorded while a checkpoint segment is
+// recomputed.
+
+// One recorded step.
+struct KOTape {
+  const double* K0;    // k, omega before the step (before the clamps)
+  const double* O0;
+  const double* S0;    // 3 planes: strain of the previous step
+  const double* K1;    // k, omega, strain after the step
+  const double* O1;
+  const double* S1;
+  const double* nut;   // the step's eddy viscosity
+  const double* Dk;    // diffusivities, F1, m grad k (2 planes)
+  const double* Do;
+  const double* F1;
+  const double* dkdo;
+  const void* rho;     // stored macros (arithmetic type)
+  const void* ux;
+  const void* uy;
+};
+
+// Per-node planes of adjoint quantities (FP64).
+enum KOLocalPlane {
+  kLocMx = 0, kLocMy, kLocUx, kLocUy, kLocRho, kLocK, kLocO, kLocF1, kLocNut, kLocPlanes
+};
+struct KOAdjoint {
+  double* dk;    // 5 planes: adjoint of Dk at the node, and sent to E, W, N, S
+  double* dw;    // 5 planes: the same for the omega diffusivity
+  double* loc;   // kLocPlanes planes
+  double* kc;    // 5 planes: adjoint of clamped k at the node, and sent to E, W, N, S
+  double* oc;    // 5 planes: the same for omega
+  double* Sb;    // 3 planes: adjoint of the strain state
+  double* Kb;    // adjoint of the k and omega state
+  double* Ob;
+  double* lam[2];  // solutions of the transposed systems (k, omega)
+  double* wdb;   // accumulated dJ/d(wall distance)
+  double* dxb;   // 4 planes: accumulated dJ/d(wall spacing)
+};
+
+__device__ __forceinline__ double clampK(double k) { return k < 0. ? 1e-16 : k; }
+__device__ __forceinline__ double clampO(double o) { return o < 0. ? -o : o; }
+
+// Sum of the contributions the four neighbours sent to node g through the
+// planes a[1..4] (sent to E, W, N, S): the E plane of the west neighbour and
+// so on; solid neighbours sent nothing.
+__device__ __forceinline__ double gatherSent(const double* __restrict__ a, const uint16_t* __restrict__ mask,
+                                             const size_t (&nb)[4], int64_t D) {
+  // nb = E, W, N, S of g; the west neighbour sent east, etc.
+  double s = 0.;
+  if (!(mask[nb[1]] & kSolid)) s += a[nb[1] + 1 * D];
+  if (!(mask[nb[0]] & kSolid)) s += a[nb[0] + 2 * D];
+  if (!(mask[nb[3]] & kSolid)) s += a[nb[3] + 3 * D];
+  if (!(mask[nb[2]] & kSolid)) s += a[nb[2] + 4 * D];
+  return s;
+}
+
+// Transposed rows: AT[j][d] = A[nb_d(j)][opposite d] for a fluid neighbour.
+__global__ __launch_bounds__(kBlockX) void koTranspose(const uint16_t* __restrict__ mask,
+                                                       const double* __restrict__ Ak,
+                                                       const double* __restrict__ Ao,
+                                                       double* __restrict__ Tk,
+                                                       double* __restrict__ To, KOParams p) {
+  int bx, y;
+  if (!tileOf(p.tiles, bx, y)) return;
+  const int x = bx * kBlockX + threadIdx.x;
+  if (x >= p.nX) return;
+  const size_t g = size_t(x) + size_t(p.XF) * y;
+  if (mask[g] & kSolid) return;
+  size_t nb[4];
+  nbIndex(p, x, g, nb);
+  const int opp[4] = {1, 0, 3, 2};
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    const bool fl = !(mask[nb[d]] & kSolid);
+    Tk[g + size_t(d) * p.D] = fl ? Ak[nb[d] + size_t(opp[d]) * p.D] : 0.;
+    To[g + size_t(d) * p.D] = fl ? Ao[nb[d] + size_t(opp[d]) * p.D] : 0.;
+  }
+}
+
+// r = b - A x, rh = r for both systems; partials (r, r).
+__global__ __launch_bounds__(kBlockX) void bicgResidual(const uint16_t* __restrict__ mask, BicgVecs V,
+                                                        KOParams p) {
+  double a[2][2] = {{0., 0.}, {0., 0.}};
+  forOwnTiles(p, [&](int x, int y) {
+    const size_t g = size_t(x) + size_t(p.XF) * y;
+    if (mask[g] & kSolid) return;
+    size_t nb[4];
+    nbIndex(p, x, g, nb);
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      const double* A = V.A[v];
+      const double* X = V.x[v];
+      const double ax = X[g] + (A[g] * X[nb[0]] + A[g + p.D] * X[nb[1]] + A[g + 2 * p.D] * X[nb[2]] +
+                                A[g + 3 * p.D] * X[nb[3]]);
+      const double r = V.b[v][g] - ax;
+      V.r[v][g] = r;
+      V.rh[v][g] = r;
+      a[v][0] += r * r;
+    }
+  });
+  blockPartials(a, V.partial);
+}
+
+// Step 2: adjoint of the row assembly.
+template <class M>
+__global__ __launch_bounds__(kBlockX) void koRowsAdjoint(const uint16_t* __restrict__ mask,
+                                                         const double* __restrict__ Wd,
+                                                         const double* __restrict__ dX, KOTape t,
+                                                         KOAdjoint a, KOParams p) {
+  int bx, y;
+  if (!tileOf(p.tiles, bx, y)) return;
+  const int x = bx * kBlockX + threadIdx.x;
+  if (x >= p.nX) return;
+  const size_t g = size_t(x) + size_t(p.XF) * y;
+  const uint32_t m = mask[g];
+  if (m & kSolid) return;
+  size_t nb[4];
+  nbIndex(p, x, g, nb);
+  const M* rho = static_cast<const M*>(t.rho);
+  const M* ux = static_cast<const M*>(t.ux);
+  const M* uy = static_cast<const M*>(t.uy);
+  const double D4[4] = {dX[g], dX[g + p.D], dX[g + 2 * p.D], dX[g + 3 * p.D]};
+  const double in[ko::kRowInputs] = {
+      t.Dk[g],  t.Dk[nb[0]], t.Dk[nb[1]], t.Dk[nb[2]], t.Dk[nb[3]],
+      t.Do[g],  t.Do[nb[0]], t.Do[nb[1]], t.Do[nb[2]], t.Do[nb[3]],
+      t.dkdo[g], t.dkdo[g + p.D], double(ux[g]), double(uy[g]), double(rho[g]),
+      clampK(t.K0[g]), clampO(t.O0[g]), t.F1[g], t.nut[g],
+      t.S1[g], t.S1[g + p.D], t.S1[g + 2 * p.D], Wd[g], D4[0], D4[1], D4[2], D4[3]};
+  const double lk = a.lam[0][g], lo = a.lam[1][g];
+  double yb[ko::kRowOutputs];
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    yb[ko::kRowAk + d] = -lk * t.K1[nb[d]];
+    yb[ko::kRowAo + d] = -lo * t.O1[nb[d]];
+  }
+  yb[ko::kRowBk] = lk;
+  yb[ko::kRowBo] = lo;
+  double xb[ko::kRowInputs];
+#pragma unroll
+  for (int i = 0; i < ko::kRowInputs; i++) xb[i] = 0.;
+  // spacings of 1 are no wall crossing: their sensitivity is not used
+  const bool plain = D4[0] == 1. && D4[1] == 1. && D4[2] == 1. && D4[3] == 1.;
+  const bool nesw = (m & kNesw) != 0;
+  ko::transposeApply<ko::kRowInputs, ko::kRowOutputs>(
+      in, yb, xb,
+      [&](const ko::Dual (&v)[ko::kRowInputs], ko::Dual (&o)[ko::kRowOutputs]) {
+        ko::koRowT<ko::Dual>(v, nesw, p.nu, p.dt, o);
+      },
+      [&](int i) { return plain && i >= ko::kRowDx; });
+#pragma unroll
+  for (int j = 0; j < 5; j++) {
+    a.dk[g + size_t(j) * p.D] = xb[ko::kRowDk + j];
+    a.dw[g + size_t(j) * p.D] = xb[ko::kRowDo + j];
+  }
+  a.loc[g + kLocMx * p.D] = xb[ko::kRowMx];
+  a.loc[g + kLocMy * p.D] = xb[ko::kRowMy];
+  a.loc[g + kLocUx * p.D] = xb[ko::kRowUx];
+  a.loc[g + kLocUy * p.D] = xb[ko::kRowUy];
+  a.loc[g + kLocRho * p.D] = xb[ko::kRowRho];
+  a.loc[g + kLocK * p.D] = xb[ko::kRowK];
+  a.loc[g + kLocO * p.D] = xb[ko::kRowO];
+  a.loc[g + kLocF1 * p.D] = xb[ko::kRowF1];
+  a.loc[g + kLocNut * p.D] = xb[ko::kRowNut];
+#pragma unroll
+  for (int c = 0; c < 3; c++) a.Sb[g + size_t(c) * p.D] += xb[ko::kRowS + c];
+  a.wdb[g] += xb[ko::kRowYw];
+  if (!plain)
+#pragma unroll
+    for (int d = 0; d < 4; d++) a.dxb[g + size_t(d) * p.D] += xb[ko::kRowDx + d];
+}
+
+// Step 3: adjoint of the diffusivity map.
+template <class M>
+__global__ __launch_bounds__(kBlockX) void koDiffAdjoint(const uint16_t* __restrict__ mask,
+                                                         const double* __restrict__ Wd,
+                                                         const double* __restrict__ dX, KOTape t,
+                                                         KOAdjoint a, KOParams p) {
+  int bx, y;
+  if (!tileOf(p.tiles, bx, y)) return;
+  const int x = bx * kBlockX + threadIdx.x;
+  if (x >= p.nX) return;
+  const size_t g = size_t(x) + size_t(p.XF) * y;
+  if (mask[g] & kSolid) return;
+  size_t nb[4];
+  nbIndex(p, x, g, nb);
+  const M* rho = static_cast<const M*>(t.rho);
+  // the forward read k and omega after the collision, which clamped fluid nodes only
+  auto Kc = [&](size_t i) { return (mask[i] & kSolid) ? t.K0[i] : clampK(t.K0[i]); };
+  auto Oc = [&](size_t i) { return (mask[i] & kSolid) ? t.O0[i] : clampO(t.O0[i]); };
+  const double D4[4] = {dX[g], dX[g + p.D], dX[g + 2 * p.D], dX[g + 3 * p.D]};
+  const double in[ko::kDiffInputs] = {
+      Kc(g), Kc(nb[0]), Kc(nb[1]), Kc(nb[2]), Kc(nb[3]),
+      Oc(g), Oc(nb[0]), Oc(nb[1]), Oc(nb[2]), Oc(nb[3]),
+      t.nut[g], Wd[g], double(rho[g]), D4[0], D4[1], D4[2], D4[3]};
+  double yb[ko::kDiffOutputs];
+  yb[ko::kDiffMx] = a.loc[g + kLocMx * p.D];
+  yb[ko::kDiffMy] = a.loc[g + kLocMy * p.D];
+  yb[ko::kDiffF1] = a.loc[g + kLocF1 * p.D];
+  yb[ko::kDiffDk] = a.dk[g] + gatherSent(a.dk, mask, nb, p.D);
+  yb[ko::kDiffDo] = a.dw[g] + gatherSent(a.dw, mask, nb, p.D);
+  double xb[ko::kDiffInputs];
+#pragma unroll
+  for (int i = 0; i < ko::kDiffInputs; i++) xb[i] = 0.;
+  const bool plain = D4[0] == 1. && D4[1] == 1. && D4[2] == 1. && D4[3] == 1.;
+  ko::transposeApply<ko::kDiffInputs, ko::kDiffOutputs>(
+      in, yb, xb,
+      [&](const ko::Dual (&v)[ko::kDiffInputs], ko::Dual (&o)[ko::kDiffOutputs]) {
+        ko::koDiffT<ko::Dual>(v, p.nu, o);
+      },
+      [&](int i) { return plain && i >= ko::kDiffDx; });
+  a.kc[g] = xb[ko::kDiffK] + a.loc[g + kLocK * p.D];
+  a.oc[g] = xb[ko::kDiffO] + a.loc[g + kLocO * p.D];
+#pragma unroll
+  for (int d = 1; d < 5; d++) {
+    a.kc[g + size_t(d) * p.D] = xb[ko::kDiffK + d];
+    a.oc[g + size_t(d) * p.D] = xb[ko::kDiffO + d];
+  }
+  a.loc[g + kLocNut * p.D] += xb[ko::kDiffNut];
+  a.loc[g + kLocRho * p.D] += xb[ko::kDiffRho];
+  a.wdb[g] += xb[ko::kDiffYw];
+  if (!plain)
+#pragma unroll
+    for (int d = 0; d < 4; d++) a.dxb[g + size_t(d) * p.D] += xb[ko::kDiffDx + d];
+}
+
+// Transpose of the collision with a per-node relaxation frequency and with
+// adjoints of the stored macros (rho, u + F/2, uy) added: gb (adjoint of the
+// corrected pre-collision values) and omb (adjoint of omega).
+template <class R>
+__device__ __forceinline__ void collideAdjointKO(const R (&g)[9], R omega, R F, const R (&gsb)[9],
+                                                 R rb0, R uxb0, R uyb0, R (&gb)[9], R& omb) {
+  const Moments<R> m = moments(g);
+  R rb = rb0, uxb = uxb0, uyb = uyb0;
+  omb = R(0);
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const int a = cx(i), c = cy(i);
+    gb[i] = (R(1) - omega) * gsb[i];
+    geqAdjoint<R>(i, m.rho, m.ux, m.uy, omega * gsb[i], rb, uxb, uyb);
+    omb += gsb[i] * (geq<R>(i, m.drho, m.rho, m.ux, m.uy) - g[i]);
+    const R fy = w1<R>(c) + dphi(c, m.uy);
+    const R sh = dphiShift(a, m.ux, F);
+    const R dsh = a == 0 ? R(-2) * F : F;
+    rb += gsb[i] * (fy * sh);
+    uxb += gsb[i] * (m.rho * fy * dsh);
+    uyb += gsb[i] * (m.rho * dphiDu(c, m.uy) * sh);
+  }
+  momentsAdjoint(m, rb, uxb, uyb, gb);
+}
+
+// Step 4: adjoint of the k-omega collide-stream (see lbAdjointStep).
+template <class S, class R, int BX>
+__global__ __launch_bounds__(BX) void lbAdjointStepKO(const S* __restrict__ src,
+                                                      const R* __restrict__ abNext,
+                                                      R* __restrict__ abCur,
+                                                      const uint16_t* __restrict__ mask,
+                                                      const int32_t* __restrict__ ibbIdx,
+                                                      const R* __restrict__ ibbQ,
+                                                      const double* __restrict__ Wd,
+                                                      AdjointBuffers ab, KOTape t, KOAdjoint ka,
+                                                      LBParams<R> p, KOParams kp) {
+  const int x = blockIdx.x * BX + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= p.nX) return;
+  const size_t gi = size_t(x) + size_t(p.XF) * y;
+  const uint32_t m = mask[gi];
+  if (m & kSolid) return;
+
+  R g[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) g[i] = loadDev<S, R>(src, gi + size_t(p.P) * i, i);
+  const bool links = p.applyIbb && (m & kIbbNode);
+  NodeLinks<R> tl;
+  R nv[9];
+  if (links) {
+    loadLinks<S, R>(tl, src, x, y, ibbQ + 8 * size_t(ibbIdx[gi]), p);
+    linksForward<R>(g, tl, nv);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 9; i++) nv[i] = g[i];
+  }
+
+  const int xe = x < p.nX - 1 ? x + 1 : 0;
+  const int xw = x > 0 ? x - 1 : p.nX - 1;
+  R gsb[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const int a = cx(i), b = cy(i);
+    size_t t2;
+    if (i == 0) {
+      t2 = gi;
+    } else if (m & boundBit(i)) {
+      t2 = gi + size_t(p.P) * rev(i);
+    } else {
+      const int tx = a > 0 ? xe : (a < 0 ? xw : x);
+      t2 = size_t(tx) + size_t(p.XF) * (y + b) + size_t(p.P) * i;
+    }
+    gsb[i] = abNext[t2];
+  }
+
+  // the forward's eddy viscosity and relaxation frequency
+  const double nu = double(p.nu);
+  const double nt = t.nut[gi];
+  const double omd = 1. / (3. * (nt + nu) + 0.5);
+  const R omega = R(omd);
+  const double* L = ka.loc;
+  R gb[9], omb;
+  collideAdjointKO<R>(nv, omega, p.Fval, gsb, R(L[gi + kLocRho * p.D]), R(L[gi + kLocUx * p.D]),
+                      R(L[gi + kLocUy * p.D]), gb, omb);
+
+  // strain S^{n+1} from the corrected values (FP64)
+  {
+    double gd[9], xb[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      gd[i] = double(nv[i]);
+      xb[i] = 0.;
+    }
+    const double sb[3] = {ka.Sb[gi], ka.Sb[gi + p.D], ka.Sb[gi + 2 * p.D]};
+    ko::transposeApply<9, 3>(
+        gd, sb, xb,
+        [&](const ko::Dual (&v)[9], ko::Dual (&o)[3]) { ko::strainT<ko::Dual>(v, double(p.omega0), o); },
+        [](int) { return false; });
+#pragma unroll
+    for (int i = 0; i < 9; i++) gb[i] += R(xb[i]);
+  }
+
+  // eddy viscosity: relaxation frequency, the objective's and the k-omega
+  // kernels' uses; back to clamped k, omega, wall distance and S^n
+  size_t nb[4];
+  nbIndex(kp, x, gi, nb);
+  double Kcb = ka.kc[gi] + gatherSent(ka.kc, mask, nb, p.D);
+  double Ocb = ka.oc[gi] + gatherSent(ka.oc, mask, nb, p.D);
+  const double nutb = L[gi + kLocNut * p.D] + double(omb) * (-3. * omd * omd);
+  const double K0 = t.K0[gi], O0 = t.O0[gi];
+  const double nin[ko::kNutInputs] = {clampK(K0), clampO(O0), Wd[gi],
+                                      t.S0[gi], t.S0[gi + p.D], t.S0[gi + 2 * p.D]};
+  double nxb[ko::kNutInputs] = {0., 0., 0., 0., 0., 0.};
+  const double nyb[1] = {nutb};
+  ko::transposeApply<ko::kNutInputs, 1>(
+      nin, nyb, nxb,
+      [&](const ko::Dual (&v)[ko::kNutInputs], ko::Dual (&o)[1]) {
+        o[0] = ko::nutT<ko::Dual>(v, nu, double(p.maxNut));
+      },
+      [](int) { return false; });
+  Kcb += nxb[ko::kNutK];
+  Ocb += nxb[ko::kNutO];
+  ka.wdb[gi] += nxb[ko::kNutYw];
+#pragma unroll
+  for (int c = 0; c < 3; c++) ka.Sb[gi + size_t(c) * p.D] = nxb[ko::kNutS + c];
+  ka.Kb[gi] = K0 < 0. ? 0. : Kcb;
+  ka.Ob[gi] = O0 < 0. ? -Ocb : Ocb;
+
+  if (links) {
+    R sb[9], upb[8];
+    double qb[8];
+    linksAdjoint<R>(g, tl, gb, sb, upb, qb);
+    const size_t row = 8 * size_t(ibbIdx[gi]);
+    R* up = static_cast<R*>(ab.upBar);
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+      up[row + d] = upb[d];
+      ab.qBar[row + d] += qb[d];
+    }
+#pragma unroll
+    for (int i = 0; i < 9; i++) abCur[gi + size_t(p.P) * i] = sb[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 9; i++) abCur[gi + size_t(p.P) * i] = gb[i];
+  }
+}
+
+// Copies up to 24 planes (16-byte words) in one launch (the adjoint's tape).
+struct PlaneCopies {
+  const uint4* src[24];
+  uint4* dst[24];
+  int64_t words[24];
+  int n;
+};
+__global__ __launch_bounds__(256) void copyPlanes(PlaneCopies c) {
+  const int64_t stride = int64_t(gridDim.x) * 256;
+  for (int k = 0; k < c.n; k++)
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < c.words[k]; i += stride)
+      c.dst[k][i] = c.src[k][i];
 }
 
 // Zero fill of a range (used instead of memset so the sweep stays capturable).

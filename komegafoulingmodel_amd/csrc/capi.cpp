@@ -346,6 +346,18 @@ int kofm_adjoint(kofm_ctx* ctx, double* dJ_dy, double* dJ_dq) {
   return rc ? setErr(ctx, rc, err) : KOFM_OK;
 }
 
+int kofm_geometry_field_gradient(const kofm_ctx* ctx, const double* wall_d_bar,
+                                 const double* dx_bar, double* dJ_dy) {
+  if (!ctx || !dJ_dy) return KOFM_ERR_ARG;
+  if (!ctx->haveGeo) return KOFM_ERR_STATE;
+  std::vector<double> dy;
+  std::string err;
+  int rc = kofm::wallFieldGradient(ctx->geo.input, wall_d_bar, dx_bar, dy, err);
+  if (rc) return setErr(const_cast<kofm_ctx*>(ctx), rc, err);
+  for (size_t i = 0; i < dy.size(); i++) dJ_dy[i] += dy[i];
+  return KOFM_OK;
+}
+
 int kofm_set_graphs(kofm_ctx* ctx, int32_t on) {
   if (!ctx) return KOFM_ERR_ARG;
   NEED_ENGINE(ctx);

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
 #include <vector>
 
 #include "../../include/kofm.h"
@@ -484,7 +485,6 @@ class EngineT final : public Engine {
 
   int objectiveRun(int n, int interval, const Objective& o, double& J, std::string& err) override {
     run_.valid = false;
-    if (ko_) return fail(err, "the adjoint covers the step without k-omega");
     if (!haveState_) return fail(err, "state not initialised");
     if (n < 1 || interval < 1 || o.window < 1 || o.window > n || o.xIn < 0 || o.xIn >= nX_ ||
         o.xOut < 0 || o.xOut >= nX_) {
@@ -497,6 +497,7 @@ class EngineT final : public Engine {
     const int nCk = (n + interval - 1) / interval;
     const size_t state = 9 * size_t(P_) * sizeof(S);
     rc = reserve(ckpt_, nCk, state, err);
+    if (!rc && ko_) rc = reserve(ckKO_, nCk, 5 * size_t(D_) * sizeof(double), err);
     if (rc) return rc;
     ckIbb_.assign(size_t(nCk), 0);
     dev::ObjectiveNodes on = objNodes();
@@ -510,6 +511,18 @@ class EngineT final : public Engine {
                              stream_));
       ckIbb_[c] = preIbb_ ? 1 : 0;
       const int w0 = std::min(len, std::max(0, n - o.window - n0));
+      if (ko_) {
+        // k, omega and the strain state; the k-omega steps poll their solves
+        double* ck = ckKO_[c]->template as<double>();
+        HIP_TRY(hipMemcpyAsync(ck, K_.as<void>(), D_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        HIP_TRY(hipMemcpyAsync(ck + D_, O_.as<void>(), D_ * sizeof(double), hipMemcpyDeviceToDevice,
+                               stream_));
+        HIP_TRY(hipMemcpyAsync(ck + 2 * D_, S_.as<void>(), 3 * D_ * sizeof(double),
+                               hipMemcpyDeviceToDevice, stream_));
+        rc = forwardSegmentKO(len, w0, on, err);
+        if (rc) return rc;
+        continue;
+      }
       const std::vector<int64_t> key = {0, len, curBuf_, ckIbb_[c], w0};
       rc = replay(key, [&] { forwardSegment(len, w0, on); }, len, err);
       if (rc) return rc;
@@ -533,20 +546,12 @@ class EngineT final : public Engine {
 
   int adjoint(double* dJdy, double* dJdq, std::string& err) override {
     if (!run_.valid) return fail(err, "run kofm_objective_run on this geometry first");
+    if (ko_) return adjointKO(dJdy, dJdq, err);
     const int T = run_.n, K = run_.interval, W = run_.obj.window;
     const size_t state = 9 * size_t(P_) * sizeof(S);
     int rc = reserve(seg_, K + 1, state, err);
+    if (!rc) rc = adjointBuffers(err);
     if (rc) return rc;
-    const size_t nl = std::max<size_t>(8 * size_t(nIbbNodes_), 8);
-    if (ab_[0].bytes() < 9 * size_t(P_) * sizeof(R) || upBar_.bytes() < nl * sizeof(R) ||
-        qBar_.bytes() < nl * sizeof(double)) {
-      dropSweeps();
-      for (Buffer* b : {&ab_[0], &ab_[1]}) HIP_TRY(b->alloc(9 * size_t(P_) * sizeof(R)));
-      HIP_TRY(upBar_.alloc(nl * sizeof(R)));
-      HIP_TRY(qBar_.alloc(nl * sizeof(double)));
-    }
-    zero<double>(qBar_.as<double>(), int64_t(nl));
-    zero<R>(ab_[0].as<R>(), 9 * P_);
     dev::AdjointBuffers abuf{ibbNodes_.as<int32_t>(), upBar_.as<void>(), qBar_.as<double>(),
                              nIbbNodes_};
     dev::ObjectiveNodes on = objNodes();
@@ -598,6 +603,11 @@ class EngineT final : public Engine {
   } run_;
   std::vector<std::unique_ptr<Buffer>> ckpt_, seg_;
   std::vector<int> ckIbb_;
+  // k-omega adjoint: k, omega and strain at each checkpoint, the tape of a
+  // recomputed segment, the adjoint planes and the context's saved fields
+  std::vector<std::unique_ptr<Buffer>> ckKO_, tape_;
+  Buffer kaBuf_, koSave_;
+  int adjIters_ = 1;
   Buffer ab_[2], upBar_, qBar_, ibbNodes_;
   Buffer objNode_, objWall_, objCp_, objCt_, objAccR_, objAccT_, objBar_, objUp_;
   std::vector<int32_t> ibbNodesH_, slotLink_, objNodeH_, objWallH_;
@@ -632,6 +642,278 @@ class EngineT final : public Engine {
     for (auto& g : graphs_) (void)hipGraphExecDestroy(g.second);
     graphs_.clear();
     dropSweeps();
+  }
+
+  // The two adjoint population levels and the per-link buffers, zeroed.
+  int adjointBuffers(std::string& err) {
+    const size_t nl = std::max<size_t>(8 * size_t(nIbbNodes_), 8);
+    if (ab_[0].bytes() < 9 * size_t(P_) * sizeof(R) || upBar_.bytes() < nl * sizeof(R) ||
+        qBar_.bytes() < nl * sizeof(double)) {
+      dropSweeps();
+      for (Buffer* b : {&ab_[0], &ab_[1]}) HIP_TRY(b->alloc(9 * size_t(P_) * sizeof(R)));
+      HIP_TRY(upBar_.alloc(nl * sizeof(R)));
+      HIP_TRY(qBar_.alloc(nl * sizeof(double)));
+    }
+    zero<double>(qBar_.as<double>(), int64_t(nl));
+    zero<R>(ab_[0].as<R>(), 9 * P_);
+    return 0;
+  }
+
+  // ---- k-omega adjoint ----
+  // Tape slot m: k, omega, strain before step m (slot m + 1 holds them after
+  // it), the step's eddy viscosity, diffusivities, F1, m grad k, rows and
+  // macros.
+  enum TapePlane {
+    kTK0 = 0, kTO0 = 1, kTS0 = 2, kTNut = 5, kTDk = 6, kTDo = 7, kTF1 = 8, kTDkdo = 9,
+    kTAk = 11, kTAo = 15, kTPlanes = 19
+  };
+  // Adjoint planes (FP64): see dev::KOAdjoint.
+  enum AdjPlane {
+    kADk = 0, kADw = 5, kALoc = 10, kAKc = 19, kAOc = 24, kASb = 29, kAKb = 32, kAOb = 33,
+    kALam = 34, kAWdb = 36, kADxb = 37, kATk = 41, kATo = 45, kAPlanes = 49
+  };
+  size_t tapeBytes() const { return (kTPlanes * sizeof(double) + 3 * sizeof(R)) * size_t(D_); }
+  double* tapePlane(int m, int k) const { return tape_[size_t(m)]->template as<double>() + size_t(k) * D_; }
+  R* tapeMacro(int m, int k) const {
+    return reinterpret_cast<R*>(tape_[size_t(m)]->template as<double>() + kTPlanes * D_) + size_t(k) * D_;
+  }
+  double* adjPlane(int k) const { return kaBuf_.as<double>() + size_t(k) * D_; }
+
+  dev::KOTape koTape(int m) const {
+    dev::KOTape t;
+    t.K0 = tapePlane(m, kTK0);
+    t.O0 = tapePlane(m, kTO0);
+    t.S0 = tapePlane(m, kTS0);
+    t.K1 = tapePlane(m + 1, kTK0);
+    t.O1 = tapePlane(m + 1, kTO0);
+    t.S1 = tapePlane(m + 1, kTS0);
+    t.nut = tapePlane(m, kTNut);
+    t.Dk = tapePlane(m, kTDk);
+    t.Do = tapePlane(m, kTDo);
+    t.F1 = tapePlane(m, kTF1);
+    t.dkdo = tapePlane(m, kTDkdo);
+    t.rho = tapeMacro(m, 0);
+    t.ux = tapeMacro(m, 1);
+    t.uy = tapeMacro(m, 2);
+    return t;
+  }
+
+  dev::KOAdjoint koAdj() const {
+    dev::KOAdjoint a;
+    a.dk = adjPlane(kADk);
+    a.dw = adjPlane(kADw);
+    a.loc = adjPlane(kALoc);
+    a.kc = adjPlane(kAKc);
+    a.oc = adjPlane(kAOc);
+    a.Sb = adjPlane(kASb);
+    a.Kb = adjPlane(kAKb);
+    a.Ob = adjPlane(kAOb);
+    a.lam[0] = adjPlane(kALam);
+    a.lam[1] = adjPlane(kALam + 1);
+    a.wdb = adjPlane(kAWdb);
+    a.dxb = adjPlane(kADxb);
+    return a;
+  }
+
+  // The transposed systems share the forward solver's work vectors.
+  dev::BicgVecs adjVecs() const {
+    dev::BicgVecs V = bicgVecs();
+    for (int v = 0; v < 2; v++) {
+      V.x[v] = adjPlane(kALam + v);
+      V.A[v] = adjPlane(v == 0 ? kATk : kATo);
+      V.b[v] = adjPlane(v == 0 ? kAKb : kAOb);
+    }
+    return V;
+  }
+
+  struct Copy {
+    const void* src;
+    void* dst;
+    size_t bytes;
+  };
+  void copyPlanes(std::initializer_list<Copy> list) {
+    dev::PlaneCopies c{};
+    for (const Copy& e : list) {
+      c.src[c.n] = static_cast<const uint4*>(e.src);
+      c.dst[c.n] = static_cast<uint4*>(e.dst);
+      c.words[c.n] = int64_t(e.bytes / 16);
+      c.n++;
+    }
+    hipLaunchKernelGGL(dev::copyPlanes, dim3(1024), dim3(256), 0, stream_, c);
+  }
+
+  // The state after step m of a recomputed segment and the step's
+  // intermediates into the tape.
+  void recordStep(int m) {
+    const size_t d8 = size_t(D_) * sizeof(double), dr = size_t(D_) * sizeof(R);
+    copyPlanes({{nut_.as<void>(), tapePlane(m, kTNut), d8},
+                {Dk_.as<void>(), tapePlane(m, kTDk), d8},
+                {Do_.as<void>(), tapePlane(m, kTDo), d8},
+                {F1_.as<void>(), tapePlane(m, kTF1), d8},
+                {dkdo_.as<void>(), tapePlane(m, kTDkdo), 2 * d8},
+                {Ak_.as<void>(), tapePlane(m, kTAk), 4 * d8},
+                {Ao_.as<void>(), tapePlane(m, kTAo), 4 * d8},
+                {rho_.as<void>(), tapeMacro(m, 0), dr},
+                {ux_.as<void>(), tapeMacro(m, 1), dr},
+                {uy_.as<void>(), tapeMacro(m, 2), dr},
+                {K_.as<void>(), tapePlane(m + 1, kTK0), d8},
+                {O_.as<void>(), tapePlane(m + 1, kTO0), d8},
+                {S_.as<void>(), tapePlane(m + 1, kTS0), 3 * d8}});
+  }
+
+  // len k-omega steps of the objective run (the forward of the adjoint).
+  int forwardSegmentKO(int len, int w0, const dev::ObjectiveNodes& on, std::string& err) {
+    const dim3 og((on.n + 63) / 64);
+    for (int m = 0; m < len; m++) {
+      const S* src = cur();
+      auto p = lbParams(0);
+      int rc = koFullStep(0, err);
+      if (rc) return rc;
+      if (m >= w0)
+        hipLaunchKernelGGL((dev::objectiveForward<S, R>), og, dim3(64), 0, stream_, src,
+                           (const S*)cur(), mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(),
+                           ibbQ_.as<R>(), cfss_.as<R>(), (const double*)nut_.as<double>(), on, p);
+    }
+    return 0;
+  }
+
+  // kofm_adjoint with k-omega on.  The segment recompute runs through the
+  // context's k-omega buffers, which are saved first and restored after.
+  int adjointKO(double* dJdy, double* dJdq, std::string& err) {
+    const int T = run_.n, K = run_.interval, W = run_.obj.window;
+    const size_t state = 9 * size_t(P_) * sizeof(S);
+    const size_t d8 = size_t(D_) * sizeof(double), dr = size_t(D_) * sizeof(R);
+    int rc = reserve(seg_, K + 1, state, err);
+    if (!rc) rc = reserve(tape_, K + 1, tapeBytes(), err);
+    if (!rc) rc = adjointBuffers(err);
+    if (rc) return rc;
+    if (kaBuf_.bytes() < kAPlanes * d8) HIP_TRY(kaBuf_.alloc(kAPlanes * d8));
+    if (koSave_.bytes() < 6 * d8 + 3 * dr) HIP_TRY(koSave_.alloc(6 * d8 + 3 * dr));
+    char* sv = koSave_.as<char>();
+    const std::vector<Copy> saved = {{K_.as<void>(), sv, d8},
+                                     {O_.as<void>(), sv + d8, d8},
+                                     {S_.as<void>(), sv + 2 * d8, 3 * d8},
+                                     {nut_.as<void>(), sv + 5 * d8, d8},
+                                     {rho_.as<void>(), sv + 6 * d8, dr},
+                                     {ux_.as<void>(), sv + 6 * d8 + dr, dr},
+                                     {uy_.as<void>(), sv + 6 * d8 + 2 * dr, dr}};
+    for (const Copy& c : saved)
+      HIP_TRY(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToDevice, stream_));
+    const int itersSaved = lastIters_;
+    zero<double>(kaBuf_.as<double>(), int64_t(kAPlanes) * D_);
+    dev::AdjointBuffers abuf{ibbNodes_.as<int32_t>(), upBar_.as<void>(), qBar_.as<double>(),
+                             nIbbNodes_};
+    dev::ObjectiveNodes on = objNodes();
+    const int nCk = (T + K - 1) / K;
+    int a = 0;
+    for (int c = nCk - 1; c >= 0 && !rc; c--) {
+      const int n0 = c * K, len = std::min(K, T - n0);
+      HIP_TRY(hipMemcpyAsync(seg_[0]->template as<void>(), ckpt_[c]->template as<void>(), state,
+                             hipMemcpyDeviceToDevice, stream_));
+      const int w0 = std::min(len, std::max(0, T - W - n0));
+      rc = adjointSegmentKO(c, len, w0, ckIbb_[c], a, abuf, on, err);
+      a ^= len & 1;
+    }
+    for (const Copy& c : saved)
+      HIP_TRY(hipMemcpyAsync(const_cast<void*>(c.src), c.dst, c.bytes, hipMemcpyDeviceToDevice, stream_));
+    lastIters_ = itersSaved;
+    if (rc) {
+      HIP_TRY(hipStreamSynchronize(stream_));
+      return rc;
+    }
+    HIP_TRY(hipGetLastError());
+    std::vector<double> qb(8 * size_t(nIbbNodes_)), wdb(static_cast<size_t>(D_)),
+        dxb(4 * static_cast<size_t>(D_));
+    if (!qb.empty())
+      HIP_TRY(hipMemcpyAsync(qb.data(), qBar_.as<void>(), qb.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipMemcpyAsync(wdb.data(), adjPlane(kAWdb), d8, hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipMemcpyAsync(dxb.data(), adjPlane(kADxb), 4 * d8, hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    std::vector<double> dy(geo_.C0.size() / 2, 0.);
+    rc = wallFieldGradient(geo_.input, wdb.data(), dxb.data(), dy, err);
+    if (rc) return rc;
+    wallGradient(qb, dJdy, dJdq);
+    if (dJdy)
+      for (size_t i = 0; i < dy.size(); i++) dJdy[i] += dy[i];
+    return 0;
+  }
+
+  // Recomputes checkpoint segment c with the tape recorded, then runs the
+  // adjoint back through it; ab_[a] holds the population adjoint after the
+  // segment on entry.
+  int adjointSegmentKO(int c, int len, int w0, int ap0, int a, const dev::AdjointBuffers& abuf,
+                       const dev::ObjectiveNodes& on, std::string& err) {
+    const size_t d8 = size_t(D_) * sizeof(double);
+    const double* ck = ckKO_[size_t(c)]->template as<double>();
+    HIP_TRY(hipMemcpyAsync(K_.as<void>(), ck, d8, hipMemcpyDeviceToDevice, stream_));
+    HIP_TRY(hipMemcpyAsync(O_.as<void>(), ck + D_, d8, hipMemcpyDeviceToDevice, stream_));
+    HIP_TRY(hipMemcpyAsync(S_.as<void>(), ck + 2 * D_, 3 * d8, hipMemcpyDeviceToDevice, stream_));
+    HIP_TRY(hipMemcpyAsync(tapePlane(0, kTK0), ck, 5 * d8, hipMemcpyDeviceToDevice, stream_));
+    for (int m = 0; m < len; m++) {
+      auto p = lbParams(0);
+      p.applyIbb = m == 0 ? ap0 : 1;
+      launchLBBlock<true>(lbBlock_, lbNpt_, p, seg_[m]->template as<S>(), seg_[m + 1]->template as<S>());
+      const int batch = std::min(std::max(1, lastIters_), 8);
+      enqueueKoHead();
+      enqueueIters(batch);
+      int rc = koFinish(batch, err);
+      if (rc) return rc;
+      recordStep(m);
+    }
+    auto kp = koParams();
+    const dim3 grid(8 * kp.tiles.per), blk(dev::kBlockX);
+    const int nsb = solverBlocks();
+    const dim3 og((on.n + 63) / 64), ig((nIbbNodes_ + 63) / 64);
+    const dev::KOAdjoint ka = koAdj();
+    const dev::BicgVecs V = adjVecs();
+    auto* st = status_.as<dev::BicgState>();
+    const uint16_t* mask = mask_.as<uint16_t>();
+    for (int m = len - 1; m >= 0; m--) {
+      const dev::KOTape t = koTape(m);
+      // 1. the transposed implicit systems, warm-started from the last solution
+      hipLaunchKernelGGL(dev::koTranspose, grid, blk, 0, stream_, mask, (const double*)tapePlane(m, kTAk),
+                         (const double*)tapePlane(m, kTAo), adjPlane(kATk), adjPlane(kATo), kp);
+      hipLaunchKernelGGL(dev::bicgReset, dim3(1), dim3(64), 0, stream_, st, par_.relTol[0], 0.,
+                         par_.maxIters[0], par_.relTol[1], 0., par_.maxIters[1]);
+      hipLaunchKernelGGL(dev::bicgResidual, dim3(nsb), blk, 0, stream_, mask, V, kp);
+      hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_,
+                         (const double*)partial_.as<double>(), nsb, st, 0);
+      const int batch = std::min(std::max(1, adjIters_), 8);
+      enqueueIters(batch, V);
+      int rc = solveFinish(batch, V, adjIters_, err);
+      if (rc) return rc;
+      // 2, 3: rows and diffusivities
+      hipLaunchKernelGGL((dev::koRowsAdjoint<R>), grid, blk, 0, stream_, mask, wallD_.as<double>(),
+                         dX_.as<double>(), t, ka, kp);
+      hipLaunchKernelGGL((dev::koDiffAdjoint<R>), grid, blk, 0, stream_, mask, wallD_.as<double>(),
+                         dX_.as<double>(), t, ka, kp);
+      // objective terms and 4: the collide-stream
+      const int apply = m == 0 ? ap0 : 1;
+      const bool win = m >= w0;
+      const S* sn = seg_[m]->template as<S>();
+      const S* sn1 = seg_[m + 1]->template as<S>();
+      R* next = ab_[a].as<R>();
+      R* curA = ab_[a ^ 1].as<R>();
+      auto p = lbParams(0);
+      p.applyIbb = apply;
+      p.saveMacros = 0;
+      if (win) {
+        hipLaunchKernelGGL((dev::objectiveGrad<S, R>), og, dim3(64), 0, stream_, sn, sn1, mask,
+                           ibbIdx_.as<int32_t>(), ibbQ_.as<R>(), cfss_.as<R>(), t.nut,
+                           ka.loc + dev::kLocNut * D_, on, p);
+        objScatter(sn1, next, abuf, on, 0, 1, p);
+      }
+      hipLaunchKernelGGL((dev::lbAdjointStepKO<S, R, 256>), gridLB(256), dim3(256), 0, stream_, sn,
+                         (const R*)next, curA, mask, ibbIdx_.as<int32_t>(), ibbQ_.as<R>(),
+                         (const double*)wallD_.as<double>(), abuf, t, ka, p, kp);
+      if (apply && nIbbNodes_ > 0)
+        hipLaunchKernelGGL((dev::lbAdjointUpstream<R>), ig, dim3(64), 0, stream_, curA,
+                           ibbQ_.as<R>(), abuf, p);
+      if (win) objScatter(sn, curA, abuf, on, 1, apply, p);
+      a ^= 1;
+    }
+    return 0;
   }
 
   // Instantiates (once) the graph of kGraphSteps collide-stream launches for
@@ -674,7 +956,7 @@ class EngineT final : public Engine {
       if (m >= w0)
         hipLaunchKernelGGL((dev::objectiveForward<S, R>), og, dim3(64), 0, stream_, src,
                            (const S*)cur(), mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(),
-                           ibbQ_.as<R>(), cfss_.as<R>(), on, p);
+                           ibbQ_.as<R>(), cfss_.as<R>(), (const double*)nullptr, on, p);
     }
   }
 
@@ -713,7 +995,7 @@ class EngineT final : public Engine {
       if (win) {
         hipLaunchKernelGGL((dev::objectiveGrad<S, R>), og, dim3(64), 0, stream_, sn, sn1,
                            mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(), ibbQ_.as<R>(),
-                           cfss_.as<R>(), on, p);
+                           cfss_.as<R>(), (const double*)nullptr, (double*)nullptr, on, p);
         objScatter(sn1, next, abuf, on, 0, 1, p);
       }
       hipLaunchKernelGGL((dev::lbAdjointStep<S, R, 256>), gridLB(256), dim3(256), 0, stream_, sn,
@@ -1311,11 +1593,11 @@ class EngineT final : public Engine {
 
   // k BiCGStab iterations of both systems (launches past convergence return
   // at once).
-  void enqueueIters(int k) {
+  void enqueueIters(int k) { enqueueIters(k, bicgVecs()); }
+  void enqueueIters(int k, const dev::BicgVecs& V) {
     auto kp = koParams();
     const dim3 sgrid(solverBlocks()), blk(dev::kBlockX);
     const int nsb = solverBlocks();
-    const dev::BicgVecs V = bicgVecs();
     auto* st = status_.as<dev::BicgState>();
     const double* part = partial_.as<double>();
     for (int c = 0; c < k; c++) {
@@ -1360,6 +1642,30 @@ class EngineT final : public Engine {
     enqueueKoHead();
     enqueueIters(batch);
     return koFinish(batch, err);
+  }
+
+  // Polls a solve on the vectors V until both systems are done (adding
+  // chunks of four iterations); lastIters gets the iterations it took.
+  int solveFinish(int launched, const dev::BicgVecs& V, int& lastIters, std::string& err) {
+    const int maxIt = std::max(par_.maxIters[0], par_.maxIters[1]);
+    int it = launched;
+    auto* st = status_.as<dev::BicgState>();
+    dev::BicgState hs;
+    while (true) {
+      HIP_TRY(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, stream_));
+      HIP_TRY(hipStreamSynchronize(stream_));
+      if ((hs.done[0] && hs.done[1]) || it >= maxIt || hs.nan) break;
+      const int k = std::min(4, maxIt - it);
+      enqueueIters(k, V);
+      it += k;
+    }
+    lastIters = std::max(hs.iters[0], hs.iters[1]);
+    HIP_TRY(hipGetLastError());
+    if (hs.nan) {
+      err = "non-finite residual in a transposed k-omega solve";
+      return KOFM_ERR_SOLVER_NAN;
+    }
+    return 0;
   }
 
   int koFinish(int launched, std::string& err) {

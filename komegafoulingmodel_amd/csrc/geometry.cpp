@@ -6,6 +6,7 @@
 #include "geometry.hpp"
 
 #include <cmath>
+#include <unordered_map>
 
 #include "../../include/kofm.h"
 
@@ -29,6 +30,7 @@ class Builder {
     g_.nY = in_.nY;
     g_.C0 = in_.C0;
     g_.BL = in_.BL;
+    g_.input = in_;
     g_.cutoff = in_.cutoff;
     g_.XF = ((in_.nX + 127) / 128) * 128;
     g_.D = int64_t(g_.XF) * g_.nY;
@@ -63,6 +65,20 @@ class Builder {
   const GeometryInput& in_;
   Geometry& g_;
   int nN_ = 0, nBL_ = 0;
+
+ public:
+  // With tracking on, the build records which link each wall spacing and
+  // wall distance came from, and their derivatives in the y of the link's
+  // two points (for wallFieldGradient).
+  struct Source {
+    int32_t pt[2] = {-1, -1};
+    double dy[2] = {0., 0.};
+  };
+  bool track = false;
+  std::unordered_map<int64_t, Source> dxSrc;  // slot node + D * (dir - 1)
+  std::vector<Source> wdSrc;                  // per node
+
+ private:
 
   V2 pt(int idx) const { return {in_.C0[2 * idx], in_.C0[2 * idx + 1]}; }
   V2 linkP0(int bl) const { return pt(in_.BL[2 * bl]); }
@@ -164,7 +180,8 @@ class Builder {
 
   // bcFindIntersection (clVariablesLS.cpp:144-195) with
   // bcFindIntersectionLinePlane (:197-209) and testInside (:347-356).
-  bool crossing(V2 L0, V2 Ld, V2 P0, V2 P1, int cut0[2], int cut1[2], double& dist) {
+  bool crossing(V2 L0, V2 Ld, V2 P0, V2 P1, int cut0[2], int cut1[2], double& dist,
+                double* distFOut = nullptr) {
     V2 p10 = sub(P1, P0);
     V2 vn = {-p10.y, p10.x};
     double den = dot(vn, Ld);
@@ -174,6 +191,7 @@ class Builder {
     if (!(std::fabs(len(p10) - len(sub(cut, P0)) - len(sub(cut, P1))) < in_.eps)) return false;
     int ncut = int(std::ceil(distF));
     dist = 1. - (double(ncut) - distF);
+    if (distFOut) *distFOut = distF;
     cut1[0] = int(L0.x + Ld.x * ncut);
     cut1[1] = int(L0.y + Ld.y * ncut);
     cut0[0] = cut1[0] - int(Ld.x);
@@ -215,27 +233,42 @@ class Builder {
         // rays start on the bbox face the direction enters through
         if (Lattice::cx[dir] != 0) {
           int i = Lattice::cx[dir] == 1 ? lo[0] : hi[0];
-          for (int j = lo[1]; j <= hi[1]; j++) castRay({double(i), double(j)}, cd, dir, c0, c1);
+          for (int j = lo[1]; j <= hi[1]; j++) castRay({double(i), double(j)}, cd, dir, c0, c1, bl);
         }
         if (Lattice::cy[dir] != 0) {
           int j = Lattice::cy[dir] == 1 ? lo[1] : hi[1];
-          for (int i = lo[0]; i <= hi[0]; i++) castRay({double(i), double(j)}, cd, dir, c0, c1);
+          for (int i = lo[0]; i <= hi[0]; i++) castRay({double(i), double(j)}, cd, dir, c0, c1, bl);
         }
       }
     }
   }
 
-  void castRay(V2 L0, V2 cd, int dir, V2 c0, V2 c1) {
+  void castRay(V2 L0, V2 cd, int dir, V2 c0, V2 c1, int bl) {
     int a[2], b[2];
-    double dist;
-    if (!crossing(L0, cd, c0, c1, a, b, dist)) return;
+    double dist, distF;
+    if (!crossing(L0, cd, c0, c1, a, b, dist, &distF)) return;
     if (a[0] >= g_.nX || a[0] < 0 || a[1] >= g_.nY || a[1] < 0) return;
     b[0] = wrap(b[0], g_.nX);
     if (b[1] >= g_.nY || b[1] < 0) return;
     if (type(a[0], a[1]) & kM0Solid) return;
     if (!(type(b[0], b[1]) & kM0Solid)) return;
-    double& slot = g_.dX[a[0] + int64_t(g_.XF) * a[1] + g_.D * (dir - 1)];
+    const int64_t idx = a[0] + int64_t(g_.XF) * a[1] + g_.D * (dir - 1);
+    double& slot = g_.dX[idx];
     double m = std::min(dist, slot);
+    if (track && !(slot < dist)) {
+      // the new value is this crossing unless held at the minimum spacing;
+      // dist = distF + const, distF = n.(c0 - L0) / n.cd, n = (-(y1 - y0), x1 - x0)
+      Source src;
+      if (!(m < in_.minDx)) {
+        const double ex = c0.x - L0.x, dxs = c1.x - c0.x;
+        const double M = -(c1.y - c0.y) * cd.x + dxs * cd.y;
+        src.pt[0] = in_.BL[2 * bl];
+        src.pt[1] = in_.BL[2 * bl + 1];
+        src.dy[0] = (ex + dxs - distF * cd.x) / M;
+        src.dy[1] = (-ex + distF * cd.x) / M;
+      }
+      dxSrc[idx] = src;
+    }
     slot = std::max(m, in_.minDx);
   }
 
@@ -278,7 +311,7 @@ class Builder {
 
   // kOmega::findMinDist (kOmega.cpp:136-193): distance to the nearest link
   // of the bottom range [b0,b1) and the top range [t0,t1).
-  double nearestWall(int b0, int b1, int t0, int t1, V2 p) const {
+  double nearestWall(int b0, int b1, int t0, int t1, V2 p, Source* src = nullptr) const {
     double best = 1000.;
     for (int top = 0; top < 2; top++) {
       int lo = top ? t0 : b0, hi = top ? t1 : b1;
@@ -286,15 +319,33 @@ class Builder {
         V2 P0 = linkP0(i), P1 = linkP1(i);
         V2 t = sub(P1, P0);
         double tl = len(t);
-        double d = std::fabs(t.y * p.x - t.x * p.y + P1.x * P0.y - P1.y * P0.x) / tl;
-        t = {t.x / tl, t.y / tl};
-        V2 foot = {p.x - (-t.y) * d, p.y - t.x * d};
+        const double cr = t.y * p.x - t.x * p.y + P1.x * P0.y - P1.y * P0.x;
+        double d = std::fabs(cr) / tl;
+        V2 tn = {t.x / tl, t.y / tl};
+        V2 foot = {p.x - (-tn.y) * d, p.y - tn.x * d};
         bool before = top ? foot.x > P0.x : foot.x < P0.x;
         bool after = top ? foot.x < P1.x : foot.x > P1.x;
         if (before)
           d = len(sub(p, P0));
         else if (after)
           d = len(sub(p, P1));
+        if (src && !(best < d)) {
+          // d(distance)/d(y0, y1) of the chosen link: perpendicular distance
+          // |cr| / |P1 - P0|, or the distance to the end point
+          src->pt[0] = in_.BL[2 * i];
+          src->pt[1] = in_.BL[2 * i + 1];
+          if (before) {
+            src->dy[0] = d > 0. ? -(p.y - P0.y) / d : 0.;
+            src->dy[1] = 0.;
+          } else if (after) {
+            src->dy[0] = 0.;
+            src->dy[1] = d > 0. ? -(p.y - P1.y) / d : 0.;
+          } else {
+            const double sg = cr < 0. ? -1. : 1.;
+            src->dy[0] = sg * (P1.x - p.x) / tl + std::fabs(cr) * t.y / (tl * tl * tl);
+            src->dy[1] = sg * (p.x - P0.x) / tl - std::fabs(cr) * t.y / (tl * tl * tl);
+          }
+        }
         best = best < d ? best : d;
       }
     }
@@ -304,6 +355,7 @@ class Builder {
   // kOmega::iniWallD (kOmega.cpp:330-349).
   void wallDistance() {
     g_.wallD.assign(g_.D, 0.);
+    if (track) wdSrc.assign(size_t(g_.D), Source());
     const int R = in_.wallDSearchRad, half = nBL_ / 2;
     for (int i = 0; i < g_.nX; i++) {
       int lb = g_.lsMap[i], lt = g_.lsMap[i + g_.XF];
@@ -311,7 +363,9 @@ class Builder {
       int t0 = std::max(lt - R, half), t1 = std::min(lt + R, nBL_);
       for (int j = 0; j < g_.nY; j++) {
         if (type(i, j) & kMSolid) continue;
-        g_.wallD[i + int64_t(g_.XF) * j] = nearestWall(b0, b1, t0, t1, {double(i), double(j)});
+        const int64_t n = i + int64_t(g_.XF) * j;
+        g_.wallD[n] = nearestWall(b0, b1, t0, t1, {double(i), double(j)},
+                                  track ? &wdSrc[size_t(n)] : nullptr);
       }
     }
   }
@@ -487,6 +541,29 @@ int buildGeometry(const GeometryInput& in, Geometry& out, std::string& err) {
   }
   Builder b(in, out);
   return b.run(err);
+}
+
+int wallFieldGradient(const GeometryInput& in, const double* wdBar, const double* dxBar,
+                      std::vector<double>& dy, std::string& err) {
+  Geometry g;
+  Builder b(in, g);
+  b.track = true;
+  int rc = b.run(err);
+  if (rc) return rc;
+  dy.resize(in.C0.size() / 2, 0.);
+  auto add = [&](const Builder::Source& s, double w) {
+    for (int k = 0; k < 2; k++)
+      if (s.pt[k] >= 0) dy[size_t(s.pt[k])] += w * s.dy[k];
+  };
+  if (wdBar)
+    for (int64_t n = 0; n < g.D; n++)
+      if (wdBar[n] != 0. && !(g.nType[size_t(n)] & kMSolid)) add(b.wdSrc[size_t(n)], wdBar[n]);
+  if (dxBar)
+    for (const auto& e : b.dxSrc) {
+      const int64_t node = e.first % g.D;
+      if (!(g.nType[size_t(node)] & kMSolid)) add(e.second, dxBar[e.first]);
+    }
+  return 0;
 }
 
 }  // namespace kofm

@@ -83,6 +83,7 @@ struct Geometry {
   std::vector<int32_t> ibbSeg;    // nIbb: the link cuts the wall segment (s, s+1)
   std::vector<double> ibbDqDy;    // 2*nIbb: dq/dy of points s and s+1
   std::vector<int32_t> shearLink; // nBLshear: boundary link of each shear entry
+  GeometryInput input;            // what the geometry was built from
 
   int nBLshear() const { return bounds[1] + bounds[3]; }
 };
@@ -90,5 +91,15 @@ struct Geometry {
 // Builds everything from the wall polyline.  Returns 0 or a negative
 // KOFM_ERR_* code with a message in err.
 int buildGeometry(const GeometryInput& in, Geometry& out, std::string& err);
+
+// Adds to dy (one entry per wall point of `in`) the y-sensitivity of the
+// wall distance field and of the axis wall spacings, weighted by wdBar (D
+// values, one per node) and dxBar (4*D, planes E, W, N, S):
+//   dy[p] += sum_n wdBar[n] dWallD[n]/dy_p + sum_s dxBar[s] d dX[s]/dy_p.
+// Each value is differentiated through the wall link (or end point) that
+// produced it, as the geometry build chose it; spacings held at the minimum
+// spacing, and nodes farther than any link, contribute nothing.
+int wallFieldGradient(const GeometryInput& in, const double* wdBar, const double* dxBar,
+                      std::vector<double>& dy, std::string& err);
 
 }  // namespace kofm

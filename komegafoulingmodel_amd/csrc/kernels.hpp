@@ -23,6 +23,8 @@
 
 #include <cstdint>
 
+#include "komega_local.hpp"
+
 namespace kofm {
 namespace dev {
 
@@ -244,16 +246,8 @@ __device__ __forceinline__ void collideNode(R (&g)[9], size_t gi, R* __restrict_
       Kf[gi] = k;
     }
     const K nu = K(p.nu);
-    const K a = K(2) * sqrt(k) / (K(0.09) * om * yw);
-    const K b = K(500) * nu / (yw * yw * om);
-    const K arg2 = a > b ? a : b;
-    K F2 = tanh(arg2 * arg2);
-    const K sxx = Sx[gi], sxy = Sx[gi + p.D], syy = Sx[gi + 2 * p.D];
-    F2 *= sqrt(K(2) * (sxx * sxx + K(2) * sxy * sxy + syy * syy));
-    const K den = K(0.31) * om > F2 ? K(0.31) * om : F2;
-    K nt = K(0.31) * k / den;
-    nt = nt < K(p.maxNut) ? nt : K(p.maxNut);
-    nt = nt > K(1e-16) ? nt : K(1e-16);
+    const K nin[ko::kNutInputs] = {k, om, yw, Sx[gi], Sx[gi + p.D], Sx[gi + 2 * p.D]};
+    const K nt = ko::nutT<K>(nin, nu, K(p.maxNut));
     Nut[gi] = nt;
     omega = R(K(1) / (K(3) * (nt + nu) + K(0.5)));
     // strain rate from the pre-collision non-equilibrium with the laminar
@@ -261,20 +255,11 @@ __device__ __forceinline__ void collideNode(R (&g)[9], size_t gi, R* __restrict_
     K gd[9];
 #pragma unroll
     for (int i = 0; i < 9; i++) gd[i] = K(g[i]);
-    const K dr = ((gd[0] + gd[1]) + (gd[2] + gd[3])) + ((gd[4] + gd[5]) + (gd[6] + gd[7])) + gd[8];
-    const K rd = K(1) + dr;
-    const K uxd = (((gd[1] - gd[2]) + (gd[5] - gd[6])) + (gd[7] - gd[8])) / rd;
-    const K uyd = (((gd[3] - gd[4]) + (gd[5] - gd[6])) - (gd[7] - gd[8])) / rd;
-    K ne[9];
-#pragma unroll
-    for (int i = 1; i < 9; i++) ne[i] = gd[i] - geq<K>(i, dr, rd, uxd, uyd);
-    const K c = K(-1.5) * K(p.omega0) / rd;
-    const K exx = ((ne[1] + ne[2]) + (ne[5] + ne[6])) + (ne[7] + ne[8]);
-    const K eyy = ((ne[3] + ne[4]) + (ne[5] + ne[6])) + (ne[7] + ne[8]);
-    const K exy = (ne[5] + ne[6]) - (ne[7] + ne[8]);
-    Sx[gi] = c * exx;
-    Sx[gi + p.D] = c * exy;
-    Sx[gi + 2 * p.D] = c * eyy;
+    K sn[3];
+    ko::strainT<K>(gd, K(p.omega0), sn);
+    Sx[gi] = sn[0];
+    Sx[gi + p.D] = sn[1];
+    Sx[gi + 2 * p.D] = sn[2];
   }
 
   if (p.saveMacros) {
@@ -881,26 +866,6 @@ __global__ __launch_bounds__(kBlockX) void lbMacros(const S* __restrict__ src,
 // of kBlockX nodes: each XCD owns a contiguous band of rows and finds the
 // rows above and below a tile in its own L2.
 
-template <class R>
-__device__ __forceinline__ void fdCoeffs(R dp, R dm, R (&c)[6]) {
-  // kOmegaKernels.cl:37-55, 235-257
-  dp = dp > R(0.1) ? dp : R(0.1);
-  dm = dm > R(0.1) ? dm : R(0.1);
-  R dc = dp + dm;
-  dc = dc > R(0.1) ? dc : R(0.1);
-  c[0] = dm / (dp * dc);
-  c[1] = -dp / (dm * dc);
-  c[2] = (dp - dm) / (dp * dm);
-  if (fabs(c[2]) > R(10)) {
-    c[0] = R(1) / dc;
-    c[1] = R(-1) / dc;
-    c[2] = R(0);
-  }
-  c[3] = R(2) / (dp * dc);
-  c[4] = R(2) / (dm * dc);
-  c[5] = R(-2) / (dp * dm);
-}
-
 struct KOParams {
   int nX, nY, XF;
   int64_t D;
@@ -922,33 +887,20 @@ __global__ __launch_bounds__(kBlockX) void koDiffusivities(
   if (x >= p.nX) return;
   const size_t g = size_t(x) + size_t(p.XF) * y;
   if (mask[g] & kSolid) return;
-  R ax[6], ay[6];
-  fdCoeffs<R>(dX[g], dX[g + p.D], ax);
-  fdCoeffs<R>(dX[g + 2 * p.D], dX[g + 3 * p.D], ay);
   const size_t gw = x > 0 ? g - 1 : g + (p.nX - 1);
   const size_t ge = x < p.nX - 1 ? g + 1 : g - x;
   const size_t gn = g + p.XF, gs = g - p.XF;
-  const R om = O[g], k = K[g], yw = Wd[g];
-  const R dOdx = ax[0] * O[ge] + ax[1] * O[gw] + ax[2] * om;
-  const R dKdx = ax[0] * K[ge] + ax[1] * K[gw] + ax[2] * k;
-  const R dOdy = ay[0] * O[gn] + ay[1] * O[gs] + ay[2] * om;
-  const R dKdy = ay[0] * K[gn] + ay[1] * K[gs] + ay[2] * k;
-  const R mlt = R(2 * 0.856) / om;
-  const R mx = mlt * dKdx, my = mlt * dKdy;
-  dkdo[g] = mx;
-  dkdo[g + p.D] = my;
-  R cd = rho[g] * (mx * dOdx + my * dOdy);
-  cd = cd > R(1.0e-10) ? cd : R(1.0e-10);
-  const R a = sqrt(k) / (R(0.09) * om * yw), b = R(500) * p.nu / (yw * yw * om);
-  R arg = a > b ? a : b;
-  const R c = R(4 * 0.856) * k / (cd * yw * yw);
-  arg = arg < c ? arg : c;
-  const R a2 = arg * arg;
-  const R F1 = tanh(a2 * a2);
-  F1a[g] = F1;
-  const R nt = Nut[g];
-  Dk[g] = p.nu + (R(0.85) * F1 + R(1) * (R(1) - F1)) * nt;
-  Do[g] = p.nu + (R(0.5) * F1 + R(0.856) * (R(1) - F1)) * nt;
+  const R in[ko::kDiffInputs] = {K[g],   K[ge],  K[gw],          K[gn],          K[gs],
+                                 O[g],   O[ge],  O[gw],          O[gn],          O[gs],
+                                 Nut[g], Wd[g],  R(rho[g]),      dX[g],          dX[g + p.D],
+                                 dX[g + 2 * p.D], dX[g + 3 * p.D]};
+  R o[ko::kDiffOutputs];
+  ko::koDiffT<R>(in, p.nu, o);
+  dkdo[g] = o[ko::kDiffMx];
+  dkdo[g + p.D] = o[ko::kDiffMy];
+  F1a[g] = o[ko::kDiffF1];
+  Dk[g] = o[ko::kDiffDk];
+  Do[g] = o[ko::kDiffDo];
 }
 
 // Update_kOmega_Coeffs_Implicit (kOmegaKernels.cl:187-480): normalised
@@ -965,55 +917,24 @@ __device__ __forceinline__ void koCoeffRow(
     const M* __restrict__ rho, const M* __restrict__ uxa, const M* __restrict__ uya,
     const double* __restrict__ dX, const double* __restrict__ Wd, const KOParams& p, KORows& o) {
   using R = double;
-  R ax[6], ay[6];
-  fdCoeffs<R>(dX[g], dX[g + p.D], ax);
-  fdCoeffs<R>(dX[g + 2 * p.D], dX[g + 3 * p.D], ay);
   const size_t gw = x > 0 ? g - 1 : g + (p.nX - 1);
   const size_t ge = x < p.nX - 1 ? g + 1 : g - x;
   const size_t gn = g + p.XF, gs = g - p.XF;
-  const R dk = Dk[g], dw = Do[g];
-  const R dkdx = ax[0] * Dk[ge] + ax[1] * Dk[gw] + ax[2] * dk;
-  const R dwdx = ax[0] * Do[ge] + ax[1] * Do[gw] + ax[2] * dw;
-  const R dkdy = ay[0] * Dk[gn] + ay[1] * Dk[gs] + ay[2] * dk;
-  const R dwdy = ay[0] * Do[gn] + ay[1] * Do[gs] + ay[2] * dw;
-  const R Ux = uxa[g], Uy = uya[g], Rh = rho[g];
-  const R Jxw = Ux - dwdx - dkdo[g], Jyw = Uy - dwdy - dkdo[g + p.D];
-  const R Jxk = Ux - dkdx, Jyk = Uy - dkdy;
-  R om = O[g] > R(0) ? O[g] : R(0);
-  const R k = K[g] > R(0) ? K[g] : R(0);
-  const R F1 = F1a[g], nt = Nut[g];
-  const R sxx = Sx[g], sxy = Sx[g + p.D], syy = Sx[g + 2 * p.D];
-  R Pk = R(2) * nt * (sxx * sxx + syy * syy + R(2) * sxy * sxy) -
-         R(2. / 3.) * k * Rh * (sxx + syy);
-  const R lim = R(10 * 0.09) * k * om;
-  Pk = Pk < lim ? Pk : lim;
-  Pk = Pk > R(0) ? Pk : R(0);
-  const R gam = (F1 * R(5. / 9.) + (R(1) - F1) * R(0.44)) / nt;
-  R Sc = R(-0.09) * om;
-  R dt = p.dt;
-  const R Kc = R(1) + dt * (Jxk * ax[2] + Jyk * ay[2] - dk * (ax[5] + ay[5])) - Sc;
-  R iv = R(1) / Kc;
-  o.Ak[0] = dt * (Jxk * ax[0] - dk * ax[3]) * iv;
-  o.Ak[1] = dt * (Jxk * ax[1] - dk * ax[4]) * iv;
-  o.Ak[2] = dt * (Jyk * ay[0] - dk * ay[3]) * iv;
-  o.Ak[3] = dt * (Jyk * ay[1] - dk * ay[4]) * iv;
-  o.bk = (k + dt * Pk) * iv;
-  if (m & kNesw) {  // omega wall row (kOmegaKernels.cl:389-403)
-    dt = R(0);
-    Sc = R(0);
-    const R yp = Wd[g];
-    const R ov = (R(6) * p.nu / R(0.075)) / (yp * yp);
-    const R ol = sqrt(k) / (R(0.5477225575 * 0.41) * yp);
-    const R ow = sqrt(ov * ov + ol * ol);
-    om = ow < R(1000) ? ow : R(1000);
+  const R in[ko::kRowInputs] = {Dk[g],  Dk[ge],  Dk[gw],         Dk[gn],          Dk[gs],
+                                Do[g],  Do[ge],  Do[gw],         Do[gn],          Do[gs],
+                                dkdo[g], dkdo[g + p.D], R(uxa[g]), R(uya[g]),     R(rho[g]),
+                                K[g],   O[g],    F1a[g],         Nut[g],          Sx[g],
+                                Sx[g + p.D], Sx[g + 2 * p.D], Wd[g], dX[g],       dX[g + p.D],
+                                dX[g + 2 * p.D], dX[g + 3 * p.D]};
+  R out[ko::kRowOutputs];
+  ko::koRowT<R>(in, (m & kNesw) != 0, p.nu, p.dt, out);
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    o.Ak[d] = out[ko::kRowAk + d];
+    o.Ao[d] = out[ko::kRowAo + d];
   }
-  const R Wc = R(1) + dt * (Jxw * ax[2] + Jyw * ay[2] - dw * (ax[5] + ay[5])) - Sc;
-  iv = R(1) / Wc;
-  o.Ao[0] = dt * (Jxw * ax[0] - dw * ax[3]) * iv;
-  o.Ao[1] = dt * (Jxw * ax[1] - dw * ax[4]) * iv;
-  o.Ao[2] = dt * (Jyw * ay[0] - dw * ay[3]) * iv;
-  o.Ao[3] = dt * (Jyw * ay[1] - dw * ay[4]) * iv;
-  o.bo = (om + dt * (gam * Pk)) * iv;
+  o.bk = out[ko::kRowBk];
+  o.bo = out[ko::kRowBo];
 }
 
 // ---------------------------------------------------------------------------

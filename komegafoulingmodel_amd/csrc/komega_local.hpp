@@ -1,0 +1,301 @@
+// komega_local.hpp -- the per-node arithmetic of the k-omega part of the step,
+// written once for a scalar type T: double in the forward kernels, Dual (a
+// forward-mode derivative pair) in the adjoint kernels.  The adjoint applies
+// the transposed Jacobian of each of these local maps by evaluating it once
+// per input with that input's derivative seeded to 1, so the derivative is
+// always that of the exact forward arithmetic, branches (clamps, max/min,
+// the difference-formula fallback) included.
+//
+// The functions follow the reference kernels:
+//   fdCoeffsT  Update_kOmega_Diffusivities / _Coeffs_Implicit difference
+//              weights (kOmegaKernels.cl:37-55, 235-257)
+//   koDiffT    Update_kOmega_Diffusivities (kOmegaKernels.cl:7-95)
+//   koRowT     Update_kOmega_Coeffs_Implicit (kOmegaKernels.cl:187-480)
+//   nutT       eddy viscosity of LB_collision_SRT_Fluid_w_kOmega
+//              (lbKernels.cl:400-430)
+//   strainT    strain rate of the same kernel (lbKernels.cl:518-583)
+// The header has no HIP dependency so that host code can evaluate it too.
+#pragma once
+#include <cmath>
+
+#if defined(__HIPCC__)
+#define KOFM_HD __host__ __device__ inline
+#define KOFM_ROLLED _Pragma("unroll 1")
+#else
+#define KOFM_HD inline
+#define KOFM_ROLLED
+#endif
+
+namespace kofm {
+namespace ko {
+
+// value and derivative along one input direction
+struct Dual {
+  double v, d;
+  KOFM_HD Dual() : v(0.), d(0.) {}
+  KOFM_HD Dual(double x) : v(x), d(0.) {}
+  KOFM_HD Dual(double x, double dx) : v(x), d(dx) {}
+};
+
+KOFM_HD Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+KOFM_HD Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+KOFM_HD Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+KOFM_HD Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+KOFM_HD Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return Dual(q, (a.d - q * b.d) / b.v);
+}
+KOFM_HD Dual& operator*=(Dual& a, Dual b) { return a = a * b; }
+KOFM_HD bool operator<(Dual a, Dual b) { return a.v < b.v; }
+KOFM_HD bool operator>(Dual a, Dual b) { return a.v > b.v; }
+KOFM_HD bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
+KOFM_HD bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
+
+KOFM_HD double kSqrt(double x) { return sqrt(x); }
+KOFM_HD double kTanh(double x) { return tanh(x); }
+KOFM_HD double kAbs(double x) { return fabs(x); }
+// sqrt has no derivative at 0; every use here is either multiplied by zero
+// there or loses a max/min against another branch, so 0 is taken
+KOFM_HD Dual kSqrt(Dual a) {
+  const double s = sqrt(a.v);
+  return Dual(s, s > 0. ? a.d / (2. * s) : 0.);
+}
+// d tanh = 1 / cosh^2 (1 - tanh^2 cancels where tanh saturates)
+KOFM_HD Dual kTanh(Dual a) {
+  const double c = cosh(a.v);
+  return Dual(tanh(a.v), a.d / (c * c));
+}
+KOFM_HD Dual kAbs(Dual a) { return a.v < 0. ? -a : a; }
+#if !defined(__HIPCC__)
+// extended precision for host-side checks of the derivatives
+inline long double kSqrt(long double x) { return sqrtl(x); }
+inline long double kTanh(long double x) { return tanhl(x); }
+inline long double kAbs(long double x) { return fabsl(x); }
+#endif
+
+KOFM_HD constexpr int lcx(int i) {
+  return (i == 1 || i == 5 || i == 7) ? 1 : ((i == 2 || i == 6 || i == 8) ? -1 : 0);
+}
+KOFM_HD constexpr int lcy(int i) {
+  return (i == 3 || i == 5 || i == 8) ? 1 : ((i == 4 || i == 6 || i == 7) ? -1 : 0);
+}
+KOFM_HD constexpr double lw(int i) { return i == 0 ? 4. / 9. : (i < 5 ? 1. / 9. : 1. / 36.); }
+KOFM_HD constexpr double lw1(int c) { return c == 0 ? 2. / 3. : 1. / 6.; }
+
+template <class T>
+KOFM_HD T dphiT(int c, T u) {
+  return c == 0 ? -u * u : T(0.5) * (T(double(c)) * u + u * u);
+}
+
+// equilibrium deviation feq_i - w_i in product form (the same operation
+// order as the collision's geq)
+template <class T>
+KOFM_HD T geqT(int i, T drho, T rho, T ux, T uy) {
+  const int a = lcx(i), b = lcy(i);
+  const T px = dphiT(a, ux), py = dphiT(b, uy);
+  return drho * T(lw(i)) + rho * (T(lw1(a)) * py + T(lw1(b)) * px + px * py);
+}
+
+// Three-point weights for the spacings dp (forward) and dm (backward):
+// c[0..2] first derivative (next, previous, centre), c[3..5] second.
+template <class T>
+KOFM_HD void fdCoeffsT(T dp, T dm, T (&c)[6]) {
+  dp = dp > T(0.1) ? dp : T(0.1);
+  dm = dm > T(0.1) ? dm : T(0.1);
+  T dc = dp + dm;
+  dc = dc > T(0.1) ? dc : T(0.1);
+  c[0] = dm / (dp * dc);
+  c[1] = -dp / (dm * dc);
+  c[2] = (dp - dm) / (dp * dm);
+  if (kAbs(c[2]) > T(10)) {
+    c[0] = T(1) / dc;
+    c[1] = T(-1) / dc;
+    c[2] = T(0);
+  }
+  c[3] = T(2) / (dp * dc);
+  c[4] = T(2) / (dm * dc);
+  c[5] = T(-2) / (dp * dm);
+}
+
+// Stencil order of every five-point input: node, E, W, N, S.
+// Inputs of the diffusivity map at one node (k and omega after the
+// collision's clamps).
+enum DiffInput {
+  kDiffK = 0,     // 5: k at the stencil
+  kDiffO = 5,     // 5: omega at the stencil
+  kDiffNut = 10,  // eddy viscosity
+  kDiffYw = 11,   // wall distance
+  kDiffRho = 12,  // density
+  kDiffDx = 13,   // 4: wall spacings E, W, N, S
+  kDiffInputs = 17
+};
+enum DiffOutput { kDiffMx = 0, kDiffMy, kDiffF1, kDiffDk, kDiffDo, kDiffOutputs };
+
+template <class T>
+KOFM_HD void koDiffT(const T (&in)[kDiffInputs], double nu, T (&o)[kDiffOutputs]) {
+  T ax[6], ay[6];
+  fdCoeffsT<T>(in[kDiffDx], in[kDiffDx + 1], ax);
+  fdCoeffsT<T>(in[kDiffDx + 2], in[kDiffDx + 3], ay);
+  const T* K = in + kDiffK;
+  const T* O = in + kDiffO;
+  const T om = O[0], k = K[0], yw = in[kDiffYw];
+  const T dOdx = ax[0] * O[1] + ax[1] * O[2] + ax[2] * om;
+  const T dKdx = ax[0] * K[1] + ax[1] * K[2] + ax[2] * k;
+  const T dOdy = ay[0] * O[3] + ay[1] * O[4] + ay[2] * om;
+  const T dKdy = ay[0] * K[3] + ay[1] * K[4] + ay[2] * k;
+  const T mlt = T(2 * 0.856) / om;
+  const T mx = mlt * dKdx, my = mlt * dKdy;
+  o[kDiffMx] = mx;
+  o[kDiffMy] = my;
+  T cd = in[kDiffRho] * (mx * dOdx + my * dOdy);
+  cd = cd > T(1.0e-10) ? cd : T(1.0e-10);
+  const T a = kSqrt(k) / (T(0.09) * om * yw), b = T(500) * T(nu) / (yw * yw * om);
+  T arg = a > b ? a : b;
+  const T c = T(4 * 0.856) * k / (cd * yw * yw);
+  arg = arg < c ? arg : c;
+  const T a2 = arg * arg;
+  const T F1 = kTanh(a2 * a2);
+  o[kDiffF1] = F1;
+  const T nt = in[kDiffNut];
+  o[kDiffDk] = T(nu) + (T(0.85) * F1 + T(1) * (T(1) - F1)) * nt;
+  o[kDiffDo] = T(nu) + (T(0.5) * F1 + T(0.856) * (T(1) - F1)) * nt;
+}
+
+// Inputs of one node's pair of normalised rows.
+enum RowInput {
+  kRowDk = 0,    // 5: k diffusivity at the stencil
+  kRowDo = 5,    // 5: omega diffusivity at the stencil
+  kRowMx = 10,   // cross-diffusion m grad k, x and y
+  kRowMy = 11,
+  kRowUx = 12,   // stored velocity (u + F/2), density
+  kRowUy = 13,
+  kRowRho = 14,
+  kRowK = 15,    // k, omega after the collision's clamps
+  kRowO = 16,
+  kRowF1 = 17,
+  kRowNut = 18,
+  kRowS = 19,    // 3: strain Sxx, Sxy, Syy of this step
+  kRowYw = 22,
+  kRowDx = 23,   // 4: wall spacings
+  kRowInputs = 27
+};
+// Outputs: E, W, N, S coefficients of the k row, of the omega row, then the
+// two right-hand sides.
+enum RowOutput { kRowAk = 0, kRowAo = 4, kRowBk = 8, kRowBo = 9, kRowOutputs = 10 };
+
+template <class T>
+KOFM_HD void koRowT(const T (&in)[kRowInputs], bool nesw, double nu, double dtIn,
+                    T (&o)[kRowOutputs]) {
+  T ax[6], ay[6];
+  fdCoeffsT<T>(in[kRowDx], in[kRowDx + 1], ax);
+  fdCoeffsT<T>(in[kRowDx + 2], in[kRowDx + 3], ay);
+  const T* Dk = in + kRowDk;
+  const T* Do = in + kRowDo;
+  const T dk = Dk[0], dw = Do[0];
+  const T dkdx = ax[0] * Dk[1] + ax[1] * Dk[2] + ax[2] * dk;
+  const T dwdx = ax[0] * Do[1] + ax[1] * Do[2] + ax[2] * dw;
+  const T dkdy = ay[0] * Dk[3] + ay[1] * Dk[4] + ay[2] * dk;
+  const T dwdy = ay[0] * Do[3] + ay[1] * Do[4] + ay[2] * dw;
+  const T Ux = in[kRowUx], Uy = in[kRowUy], Rh = in[kRowRho];
+  const T Jxw = Ux - dwdx - in[kRowMx], Jyw = Uy - dwdy - in[kRowMy];
+  const T Jxk = Ux - dkdx, Jyk = Uy - dkdy;
+  T om = in[kRowO] > T(0) ? in[kRowO] : T(0);
+  const T k = in[kRowK] > T(0) ? in[kRowK] : T(0);
+  const T F1 = in[kRowF1], nt = in[kRowNut];
+  const T sxx = in[kRowS], sxy = in[kRowS + 1], syy = in[kRowS + 2];
+  T Pk = T(2) * nt * (sxx * sxx + syy * syy + T(2) * sxy * sxy) -
+         T(2. / 3.) * k * Rh * (sxx + syy);
+  const T lim = T(10 * 0.09) * k * om;
+  Pk = Pk < lim ? Pk : lim;
+  Pk = Pk > T(0) ? Pk : T(0);
+  const T gam = (F1 * T(5. / 9.) + (T(1) - F1) * T(0.44)) / nt;
+  T Sc = T(-0.09) * om;
+  T dt = T(dtIn);
+  const T Kc = T(1) + dt * (Jxk * ax[2] + Jyk * ay[2] - dk * (ax[5] + ay[5])) - Sc;
+  T iv = T(1) / Kc;
+  o[kRowAk + 0] = dt * (Jxk * ax[0] - dk * ax[3]) * iv;
+  o[kRowAk + 1] = dt * (Jxk * ax[1] - dk * ax[4]) * iv;
+  o[kRowAk + 2] = dt * (Jyk * ay[0] - dk * ay[3]) * iv;
+  o[kRowAk + 3] = dt * (Jyk * ay[1] - dk * ay[4]) * iv;
+  o[kRowBk] = (k + dt * Pk) * iv;
+  if (nesw) {  // omega wall row (kOmegaKernels.cl:389-403)
+    dt = T(0);
+    Sc = T(0);
+    const T yp = in[kRowYw];
+    const T ov = (T(6) * T(nu) / T(0.075)) / (yp * yp);
+    const T ol = kSqrt(k) / (T(0.5477225575 * 0.41) * yp);
+    const T ow = kSqrt(ov * ov + ol * ol);
+    om = ow < T(1000) ? ow : T(1000);
+  }
+  const T Wc = T(1) + dt * (Jxw * ax[2] + Jyw * ay[2] - dw * (ax[5] + ay[5])) - Sc;
+  iv = T(1) / Wc;
+  o[kRowAo + 0] = dt * (Jxw * ax[0] - dw * ax[3]) * iv;
+  o[kRowAo + 1] = dt * (Jxw * ax[1] - dw * ax[4]) * iv;
+  o[kRowAo + 2] = dt * (Jyw * ay[0] - dw * ay[3]) * iv;
+  o[kRowAo + 3] = dt * (Jyw * ay[1] - dw * ay[4]) * iv;
+  o[kRowBo] = (om + dt * (gam * Pk)) * iv;
+}
+
+// Eddy viscosity from k and omega after the clamps, the wall distance and
+// the strain of the previous step.
+enum NutInput { kNutK = 0, kNutO, kNutYw, kNutS, kNutInputs = 6 };
+
+template <class T>
+KOFM_HD T nutT(const T (&in)[kNutInputs], double nu, double maxNut) {
+  const T k = in[kNutK], om = in[kNutO], yw = in[kNutYw];
+  const T sxx = in[kNutS], sxy = in[kNutS + 1], syy = in[kNutS + 2];
+  const T a = T(2) * kSqrt(k) / (T(0.09) * om * yw);
+  const T b = T(500) * T(nu) / (yw * yw * om);
+  const T arg2 = a > b ? a : b;
+  T F2 = kTanh(arg2 * arg2);
+  F2 *= kSqrt(T(2) * (sxx * sxx + T(2) * sxy * sxy + syy * syy));
+  const T den = T(0.31) * om > F2 ? T(0.31) * om : F2;
+  T nt = T(0.31) * k / den;
+  nt = nt < T(maxNut) ? nt : T(maxNut);
+  nt = nt > T(1e-16) ? nt : T(1e-16);
+  return nt;
+}
+
+// Strain Sxx, Sxy, Syy from the nine pre-collision deviations (FP64), with
+// the laminar relaxation frequency omega0.
+template <class T>
+KOFM_HD void strainT(const T (&gd)[9], double omega0, T (&s)[3]) {
+  const T dr = ((gd[0] + gd[1]) + (gd[2] + gd[3])) + ((gd[4] + gd[5]) + (gd[6] + gd[7])) + gd[8];
+  const T rd = T(1) + dr;
+  const T uxd = (((gd[1] - gd[2]) + (gd[5] - gd[6])) + (gd[7] - gd[8])) / rd;
+  const T uyd = (((gd[3] - gd[4]) + (gd[5] - gd[6])) - (gd[7] - gd[8])) / rd;
+  T ne[9];
+  ne[0] = T(0);
+  for (int i = 1; i < 9; i++) ne[i] = gd[i] - geqT<T>(i, dr, rd, uxd, uyd);
+  const T c = T(-1.5) * T(omega0) / rd;
+  const T exx = ((ne[1] + ne[2]) + (ne[5] + ne[6])) + (ne[7] + ne[8]);
+  const T eyy = ((ne[3] + ne[4]) + (ne[5] + ne[6])) + (ne[7] + ne[8]);
+  const T exy = (ne[5] + ne[6]) - (ne[7] + ne[8]);
+  s[0] = c * exx;
+  s[1] = c * exy;
+  s[2] = c * eyy;
+}
+
+// xb += J^T yb for the local map f: NI inputs -> NO outputs at the point x,
+// one forward-mode evaluation per input (inputs with skip(i) true are left
+// alone).
+template <int NI, int NO, class F, class Skip>
+KOFM_HD void transposeApply(const double (&x)[NI], const double (&yb)[NO], double (&xb)[NI], F&& f,
+                            Skip&& skip) {
+  // one evaluation of f per input, kept as a loop (unrolled, the NI
+  // evaluations would not fit the register file)
+  KOFM_ROLLED
+  for (int i = 0; i < NI; i++) {
+    if (skip(i)) continue;
+    Dual in[NI];
+    for (int j = 0; j < NI; j++) in[j] = Dual(x[j], j == i ? 1. : 0.);
+    Dual out[NO];
+    f(in, out);
+    double s = 0.;
+    for (int o = 0; o < NO; o++) s += yb[o] * out[o].d;
+    xb[i] += s;
+  }
+}
+
+}  // namespace ko
+}  // namespace kofm

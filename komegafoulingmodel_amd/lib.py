@@ -44,7 +44,7 @@ EXPORTS = [
     "kofm_upload_state", "kofm_step", "kofm_time_steps", "kofm_wall_shear",
     "kofm_get_solver_stats", "kofm_reduce", "kofm_download", "kofm_save2file",
     "kofm_synchronize", "kofm_set_launch", "kofm_autotune", "kofm_set_graphs",
-    "kofm_objective_run", "kofm_adjoint",
+    "kofm_objective_run", "kofm_adjoint", "kofm_geometry_field_gradient",
     "kofm_ensemble_create", "kofm_ensemble_step", "kofm_ensemble_time_steps",
     "kofm_ensemble_set_block", "kofm_ensemble_set_launch", "kofm_ensemble_set_cache_budget",
     "kofm_ensemble_last_error",
@@ -138,6 +138,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "kofm_set_graphs": (C.c_int, [P, C.c_int32]),
         "kofm_objective_run": (C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(Objective), dp]),
         "kofm_adjoint": (C.c_int, [P, dp, dp]),
+        "kofm_geometry_field_gradient": (C.c_int, [P, dp, dp, dp]),
         "kofm_ensemble_create": (C.c_int, [C.POINTER(P), C.c_int32, C.POINTER(P)]),
         "kofm_ensemble_step": (C.c_int, [P, C.c_int32, C.c_uint32]),
         "kofm_ensemble_time_steps": (C.c_int, [P, C.c_int32, C.c_uint32, dp,
@@ -316,6 +317,16 @@ class Lattice:
         dq = np.zeros(max(self.counts().nIbb, 1), dtype=np.float64)
         self._check(self.lib.kofm_adjoint(self.h, _dptr(dy), _dptr(dq)))
         return dy, dq[:self.counts().nIbb]
+
+    def geometry_field_gradient(self, wall_d_bar=None, dx_bar=None):
+        """dJ/dy of the wall points for given sensitivities of the wall distance
+        field (D values) and the axis wall spacings (4*D values)."""
+        dy = np.zeros(self._nN, dtype=np.float64)
+        wd = None if wall_d_bar is None else np.ascontiguousarray(wall_d_bar, dtype=np.float64)
+        dx = None if dx_bar is None else np.ascontiguousarray(dx_bar, dtype=np.float64)
+        self._check(self.lib.kofm_geometry_field_gradient(
+            self.h, None if wd is None else _dptr(wd), None if dx is None else _dptr(dx), _dptr(dy)))
+        return dy
 
     def synchronize(self):
         self._check(self.lib.kofm_synchronize(self.h))

@@ -45,7 +45,8 @@ class WallShapeProblem:
 
     Defaults are the sensitivity workload: 2048 x 256, height H = 200 with
     walls displaced by 0.1 H sin(2 pi x / 1024) (H + 0.2 H fits the 252 rows
-    between the solid edge rows), Re_H = 5000 with nu = 0.005,
+    between the solid edge rows), Re_H = 5000 with nu = 0.005, k-omega on
+    (k = 1.5 (0.05 U_b)^2, omega = sqrt(k) / (0.09^(1/4) 0.07 H)),
     64 control points per wall drawn U[0, 4] from the seed, T = 4000 steps,
     J averaged over the last 1000, checkpoints every 64 steps."""
     nX: int = 2048
@@ -63,12 +64,19 @@ class WallShapeProblem:
     storage: str = "f32"
     seed: int = 11
     noise: float = 0.01
+    komega: bool = True
+    rel_tol: float = 1e-4     # k and omega solves (the reference's defaults)
+    abs_tol: float = 1e-7
+    max_iters: int = 1000
 
     def __post_init__(self):
         self.mid = (self.nY - 1) / 2.0
         self.Ub = self.Re * self.nu / self.H
         self.Fval = channel.turbulent_forcing(self.H, self.nu, self.Re)
         self.tau0 = 1.0 / (3.0 * self.nu + 0.5)
+        # k = 1.5 (0.05 U_b)^2, omega = sqrt(k) / (0.09^(1/4) 0.07 H)
+        self.k_ini = 1.5 * (0.05 * self.Ub) ** 2
+        self.omega_ini = math.sqrt(self.k_ini) / (0.09 ** 0.25 * 0.07 * self.H)
         self._lat = None
 
     def initial_theta(self) -> np.ndarray:
@@ -106,8 +114,9 @@ class WallShapeProblem:
 
     def lattice(self) -> lib.Lattice:
         if self._lat is None:
-            self._lat = lib.Lattice(self.nX, self.nY, storage=self.storage, komega=False)
-            self._lat.set_params(self.tau0, self.Fval, self.nu)
+            self._lat = lib.Lattice(self.nX, self.nY, storage=self.storage, komega=self.komega)
+            self._lat.set_params(self.tau0, self.Fval, self.nu, kappa_rel_tol=self.rel_tol,
+                                 kappa_abs_tol=self.abs_tol, kappa_max_iters=self.max_iters)
         return self._lat
 
     def build(self, theta):
@@ -123,6 +132,8 @@ class WallShapeProblem:
         ux = self.Ub * (1.0 + rng.uniform(-self.noise, self.noise, D))
         uy = self.Ub * rng.uniform(-self.noise, self.noise, D)
         L.init_state(np.where(fluid, 1.0, 0.0), np.where(fluid, ux, 0.0), np.where(fluid, uy, 0.0))
+        if self.komega:
+            L.init_komega(self.k_ini, self.omega_ini)
         self._C0 = C0
         return nType
 
