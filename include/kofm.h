@@ -292,9 +292,19 @@ typedef struct kofm_objective {
   int32_t window;      /* 1 <= window <= nsteps */
   int32_t x_in;        /* pressure column 1 (0 for the channel inlet) */
   int32_t x_out;       /* pressure column 2 (nX-1 for the outlet) */
-  int32_t reserved;    /* 0 */
+  int32_t flags;       /* KOFM_ADJOINT_* (0: the exact adjoint) */
   double shear_weight; /* weight of the wall-shear integral */
 } kofm_objective;
+
+/* With k-omega: kofm_adjoint holds the eddy viscosity of every step at its
+ * recorded value (the collide-stream is differentiated with that per-node
+ * relaxation frequency; the k-omega equations, the wall distances and the
+ * spacings are not).  The reference's k-omega rows are not monotone next to
+ * nodes very close to a curved wall: k and omega go negative, are clamped,
+ * and the eddy viscosity sits at its 1e-16 floor, where the production term
+ * (divided by the eddy viscosity) makes the exact tangent grow by orders of
+ * magnitude per step.  This gives a bounded gradient there. */
+#define KOFM_ADJOINT_FROZEN_TURBULENCE 1
 
 /* Runs nsteps steps (with k-omega if the context has it) from the current
  * state, returns J and keeps the state every checkpoint_interval steps

@@ -740,7 +740,7 @@ __global__ __launch_bounds__(BX) void lbAdjointStepKO(const S* __restrict__ src,
                                                       const R* __restrict__ ibbQ,
                                                       const double* __restrict__ Wd,
                                                       AdjointBuffers ab, KOTape t, KOAdjoint ka,
-                                                      LBParams<R> p, KOParams kp) {
+                                                      LBParams<R> p, KOParams kp, int frozen) {
   const int x = blockIdx.x * BX + threadIdx.x;
   const int y = blockIdx.y;
   if (x >= p.nX) return;
@@ -790,8 +790,9 @@ __global__ __launch_bounds__(BX) void lbAdjointStepKO(const S* __restrict__ src,
   collideAdjointKO<R>(nv, omega, p.Fval, gsb, R(L[gi + kLocRho * p.D]), R(L[gi + kLocUx * p.D]),
                       R(L[gi + kLocUy * p.D]), gb, omb);
 
-  // strain S^{n+1} from the corrected values (FP64)
-  {
+  // strain S^{n+1} from the corrected values (FP64); with the eddy
+  // viscosity frozen nothing downstream of the k-omega fields is followed
+  if (!frozen) {
     double gd[9], xb[9];
 #pragma unroll
     for (int i = 0; i < 9; i++) {
@@ -809,29 +810,31 @@ __global__ __launch_bounds__(BX) void lbAdjointStepKO(const S* __restrict__ src,
 
   // eddy viscosity: relaxation frequency, the objective's and the k-omega
   // kernels' uses; back to clamped k, omega, wall distance and S^n
-  size_t nb[4];
-  nbIndex(kp, x, gi, nb);
-  double Kcb = ka.kc[gi] + gatherSent(ka.kc, mask, nb, p.D);
-  double Ocb = ka.oc[gi] + gatherSent(ka.oc, mask, nb, p.D);
-  const double nutb = L[gi + kLocNut * p.D] + double(omb) * (-3. * omd * omd);
-  const double K0 = t.K0[gi], O0 = t.O0[gi];
-  const double nin[ko::kNutInputs] = {clampK(K0), clampO(O0), Wd[gi],
-                                      t.S0[gi], t.S0[gi + p.D], t.S0[gi + 2 * p.D]};
-  double nxb[ko::kNutInputs] = {0., 0., 0., 0., 0., 0.};
-  const double nyb[1] = {nutb};
-  ko::transposeApply<ko::kNutInputs, 1>(
-      nin, nyb, nxb,
-      [&](const ko::Dual (&v)[ko::kNutInputs], ko::Dual (&o)[1]) {
-        o[0] = ko::nutT<ko::Dual>(v, nu, double(p.maxNut));
-      },
-      [](int) { return false; });
-  Kcb += nxb[ko::kNutK];
-  Ocb += nxb[ko::kNutO];
-  ka.wdb[gi] += nxb[ko::kNutYw];
+  if (!frozen) {
+    size_t nb[4];
+    nbIndex(kp, x, gi, nb);
+    double Kcb = ka.kc[gi] + gatherSent(ka.kc, mask, nb, p.D);
+    double Ocb = ka.oc[gi] + gatherSent(ka.oc, mask, nb, p.D);
+    const double nutb = L[gi + kLocNut * p.D] + double(omb) * (-3. * omd * omd);
+    const double K0 = t.K0[gi], O0 = t.O0[gi];
+    const double nin[ko::kNutInputs] = {clampK(K0), clampO(O0), Wd[gi],
+                                        t.S0[gi], t.S0[gi + p.D], t.S0[gi + 2 * p.D]};
+    double nxb[ko::kNutInputs] = {0., 0., 0., 0., 0., 0.};
+    const double nyb[1] = {nutb};
+    ko::transposeApply<ko::kNutInputs, 1>(
+        nin, nyb, nxb,
+        [&](const ko::Dual (&v)[ko::kNutInputs], ko::Dual (&o)[1]) {
+          o[0] = ko::nutT<ko::Dual>(v, nu, double(p.maxNut));
+        },
+        [](int) { return false; });
+    Kcb += nxb[ko::kNutK];
+    Ocb += nxb[ko::kNutO];
+    ka.wdb[gi] += nxb[ko::kNutYw];
 #pragma unroll
-  for (int c = 0; c < 3; c++) ka.Sb[gi + size_t(c) * p.D] = nxb[ko::kNutS + c];
-  ka.Kb[gi] = K0 < 0. ? 0. : Kcb;
-  ka.Ob[gi] = O0 < 0. ? -Ocb : Ocb;
+    for (int c = 0; c < 3; c++) ka.Sb[gi + size_t(c) * p.D] = nxb[ko::kNutS + c];
+    ka.Kb[gi] = K0 < 0. ? 0. : Kcb;
+    ka.Ob[gi] = O0 < 0. ? -Ocb : Ocb;
+  }
 
   if (links) {
     R sb[9], upb[8];

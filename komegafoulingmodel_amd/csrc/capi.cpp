@@ -333,6 +333,9 @@ int kofm_objective_run(kofm_ctx* ctx, int32_t nsteps, int32_t checkpoint_interva
   o.xIn = obj->x_in;
   o.xOut = obj->x_out;
   o.shearWeight = obj->shear_weight;
+  o.flags = obj->flags;
+  if (o.flags & ~KOFM_ADJOINT_FROZEN_TURBULENCE)
+    return setErr(ctx, KOFM_ERR_ARG, "unknown kofm_objective flags");
   std::string err;
   int rc = ctx->engine->objectiveRun(nsteps, checkpoint_interval, o, *J, err);
   return rc ? setErr(ctx, rc, err) : KOFM_OK;

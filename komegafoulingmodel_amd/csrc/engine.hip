@@ -869,25 +869,28 @@ class EngineT final : public Engine {
     const dev::BicgVecs V = adjVecs();
     auto* st = status_.as<dev::BicgState>();
     const uint16_t* mask = mask_.as<uint16_t>();
+    const int frozen = (run_.obj.flags & KOFM_ADJOINT_FROZEN_TURBULENCE) ? 1 : 0;
     for (int m = len - 1; m >= 0; m--) {
       const dev::KOTape t = koTape(m);
-      // 1. the transposed implicit systems, warm-started from the last solution
-      hipLaunchKernelGGL(dev::koTranspose, grid, blk, 0, stream_, mask, (const double*)tapePlane(m, kTAk),
-                         (const double*)tapePlane(m, kTAo), adjPlane(kATk), adjPlane(kATo), kp);
-      hipLaunchKernelGGL(dev::bicgReset, dim3(1), dim3(64), 0, stream_, st, par_.relTol[0], 0.,
-                         par_.maxIters[0], par_.relTol[1], 0., par_.maxIters[1]);
-      hipLaunchKernelGGL(dev::bicgResidual, dim3(nsb), blk, 0, stream_, mask, V, kp);
-      hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_,
-                         (const double*)partial_.as<double>(), nsb, st, 0);
-      const int batch = std::min(std::max(1, adjIters_), 8);
-      enqueueIters(batch, V);
-      int rc = solveFinish(batch, V, adjIters_, err);
-      if (rc) return rc;
-      // 2, 3: rows and diffusivities
-      hipLaunchKernelGGL((dev::koRowsAdjoint<R>), grid, blk, 0, stream_, mask, wallD_.as<double>(),
-                         dX_.as<double>(), t, ka, kp);
-      hipLaunchKernelGGL((dev::koDiffAdjoint<R>), grid, blk, 0, stream_, mask, wallD_.as<double>(),
-                         dX_.as<double>(), t, ka, kp);
+      if (!frozen) {
+        // 1. the transposed implicit systems, warm-started from the last solution
+        hipLaunchKernelGGL(dev::koTranspose, grid, blk, 0, stream_, mask, (const double*)tapePlane(m, kTAk),
+                           (const double*)tapePlane(m, kTAo), adjPlane(kATk), adjPlane(kATo), kp);
+        hipLaunchKernelGGL(dev::bicgReset, dim3(1), dim3(64), 0, stream_, st, par_.relTol[0], 0.,
+                           par_.maxIters[0], par_.relTol[1], 0., par_.maxIters[1]);
+        hipLaunchKernelGGL(dev::bicgResidual, dim3(nsb), blk, 0, stream_, mask, V, kp);
+        hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_,
+                           (const double*)partial_.as<double>(), nsb, st, 0);
+        const int batch = std::min(std::max(1, adjIters_), 8);
+        enqueueIters(batch, V);
+        int rc = solveFinish(batch, V, adjIters_, err);
+        if (rc) return rc;
+        // 2, 3: rows and diffusivities
+        hipLaunchKernelGGL((dev::koRowsAdjoint<R>), grid, blk, 0, stream_, mask, wallD_.as<double>(),
+                           dX_.as<double>(), t, ka, kp);
+        hipLaunchKernelGGL((dev::koDiffAdjoint<R>), grid, blk, 0, stream_, mask, wallD_.as<double>(),
+                           dX_.as<double>(), t, ka, kp);
+      }
       // objective terms and 4: the collide-stream
       const int apply = m == 0 ? ap0 : 1;
       const bool win = m >= w0;
@@ -901,12 +904,12 @@ class EngineT final : public Engine {
       if (win) {
         hipLaunchKernelGGL((dev::objectiveGrad<S, R>), og, dim3(64), 0, stream_, sn, sn1, mask,
                            ibbIdx_.as<int32_t>(), ibbQ_.as<R>(), cfss_.as<R>(), t.nut,
-                           ka.loc + dev::kLocNut * D_, on, p);
+                           frozen ? (double*)nullptr : ka.loc + dev::kLocNut * D_, on, p);
         objScatter(sn1, next, abuf, on, 0, 1, p);
       }
       hipLaunchKernelGGL((dev::lbAdjointStepKO<S, R, 256>), gridLB(256), dim3(256), 0, stream_, sn,
                          (const R*)next, curA, mask, ibbIdx_.as<int32_t>(), ibbQ_.as<R>(),
-                         (const double*)wallD_.as<double>(), abuf, t, ka, p, kp);
+                         (const double*)wallD_.as<double>(), abuf, t, ka, p, kp, frozen);
       if (apply && nIbbNodes_ > 0)
         hipLaunchKernelGGL((dev::lbAdjointUpstream<R>), ig, dim3(64), 0, stream_, curA,
                            ibbQ_.as<R>(), abuf, p);
@@ -1662,7 +1665,8 @@ class EngineT final : public Engine {
     lastIters = std::max(hs.iters[0], hs.iters[1]);
     HIP_TRY(hipGetLastError());
     if (hs.nan) {
-      err = "non-finite residual in a transposed k-omega solve";
+      err = "non-finite residual in a transposed k-omega solve (the exact tangent of the k-omega "
+            "step overflowed; KOFM_ADJOINT_FROZEN_TURBULENCE gives a bounded gradient)";
       return KOFM_ERR_SOLVER_NAN;
     }
     return 0;

@@ -27,6 +27,7 @@ struct SolverStats {
 // with pbar the mean of rho / 3 over the fluid nodes of a column.
 struct Objective {
   int window = 1, xIn = 0, xOut = 0;
+  int flags = 0;  // KOFM_ADJOINT_*
   double shearWeight = 0.;
 };
 

@@ -90,7 +90,10 @@ class GeomCounts(C.Structure):
 
 class Objective(C.Structure):
     _fields_ = [("window", C.c_int32), ("x_in", C.c_int32), ("x_out", C.c_int32),
-                ("reserved", C.c_int32), ("shear_weight", C.c_double)]
+                ("flags", C.c_int32), ("shear_weight", C.c_double)]
+
+
+ADJOINT_FROZEN_TURBULENCE = 1
 
 
 class SolverStats(C.Structure):
@@ -302,10 +305,14 @@ class Lattice:
         self._check(self.lib.kofm_set_graphs(self.h, 1 if on else 0))
 
     def objective_run(self, nsteps: int, interval: int, window: int, x_in: int = 0,
-                      x_out: int | None = None, shear_weight: float = 0.1) -> float:
+                      x_out: int | None = None, shear_weight: float = 0.1,
+                      frozen_turbulence: bool = False) -> float:
         """Runs nsteps accumulating the wall objective J over the last `window`
-        steps and keeping checkpoints every `interval` steps; returns J."""
-        o = Objective(window, x_in, self.nX - 1 if x_out is None else x_out, 0, shear_weight)
+        steps and keeping checkpoints every `interval` steps; returns J.  With
+        frozen_turbulence the following adjoint holds the eddy viscosity at
+        its recorded values."""
+        o = Objective(window, x_in, self.nX - 1 if x_out is None else x_out,
+                      ADJOINT_FROZEN_TURBULENCE if frozen_turbulence else 0, shear_weight)
         J = C.c_double()
         self._check(self.lib.kofm_objective_run(self.h, nsteps, interval, C.byref(o), C.byref(J)))
         return J.value

@@ -65,6 +65,7 @@ class WallShapeProblem:
     seed: int = 11
     noise: float = 0.01
     komega: bool = True
+    frozen_turbulence: bool = False  # adjoint with the eddy viscosity held at its recorded values
     rel_tol: float = 1e-4     # k and omega solves (the reference's defaults)
     abs_tol: float = 1e-7
     max_iters: int = 1000
@@ -140,7 +141,8 @@ class WallShapeProblem:
     def objective(self, theta) -> float:
         self.build(theta)
         return self.lattice().objective_run(self.steps, self.interval, self.window,
-                                            shear_weight=self.shear_weight)
+                                            shear_weight=self.shear_weight,
+                                            frozen_turbulence=self.frozen_turbulence)
 
     def gradient(self) -> np.ndarray:
         """dJ/dtheta of the last objective() call."""

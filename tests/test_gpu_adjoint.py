@@ -262,3 +262,23 @@ def test_adjoint_is_independent_of_the_launch_shape():
         P.close()
     assert res[0][0] == res[1][0]
     assert np.array_equal(res[0][1], res[1][1])
+
+
+def test_frozen_turbulence_adjoint_reduces_to_the_laminar_adjoint():
+    """With k-omega on but k near zero the eddy viscosity stays at its 1e-16
+    floor, every node relaxes with the laminar frequency and the traction
+    sees nu alone: J and the frozen-eddy-viscosity gradient equal the
+    laminar ones."""
+    res = {}
+    for ko in (False, True):
+        P = _problem("f64", komega=ko, frozen_turbulence=True, steps=120, window=40)
+        P.k_ini = 1e-30
+        th = P.initial_theta()
+        J = P.objective(th)
+        res[ko] = (J, P.gradient())
+        P.close()
+    err = np.max(np.abs(res[True][1] - res[False][1])) / np.max(np.abs(res[False][1]))
+    print(f"frozen k-omega vs laminar: J {res[True][0]:.12e} {res[False][0]:.12e} grad {err:.2e}",
+          flush=True)
+    assert abs(res[True][0] - res[False][0]) <= 1e-12 * abs(res[False][0])
+    assert err < 1e-9
