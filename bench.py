@@ -165,6 +165,41 @@ def measure_ensemble(storage, steps=300):
     return out
 
 
+def measure_ensemble_ko(storage, steps=40):
+    """The 48 channels with k-omega on: ensemble steps (one batched
+    collide-stream launch and every member's implicit solves captured per
+    step, one poll per step) against the same channels stepped one after
+    another."""
+    from komegafoulingmodel_amd import channel, lib
+
+    ws = channel.ensemble_members(komega=True)
+    Ls, fluid = [], 0
+    for w in ws:
+        L, nf = setup(w, storage, komega=True, tune=False)
+        Ls.append(L)
+        fluid += nf
+    for L in Ls:  # warm (graphs captured, iteration predictions settled)
+        L.step(8)
+    for L in Ls:
+        L.synchronize()
+    t0 = time.perf_counter()
+    for L in Ls:
+        L.step(steps)
+    for L in Ls:
+        L.synchronize()
+    t_seq = time.perf_counter() - t0
+    ens = lib.Ensemble(Ls)
+    ens.step(8)
+    ms, _ = ens.time_steps(steps)
+    out = {"channels": len(ws), "fluid_nodes": fluid, "steps": steps, "komega": True,
+           "sequential_mlups": fluid * steps / t_seq / 1e6, "mlups": fluid / (ms * 1e-3) / 1e6,
+           "ms_per_step": ms, "speedup_vs_sequential": (t_seq / steps) / (ms * 1e-3)}
+    ens.close()
+    for L in Ls:
+        L.close()
+    return out
+
+
 def kernel_name(launch, storage="float"):
     if launch.get("steps_per_pass", 1) == 2:
         tile = {512: (64, 8), 640: (128, 5), 832: (64, 13), 1856: (64, 29), 1024: (64, 16),
@@ -237,7 +272,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--forms", default="f16,bf16,k32,adj32,adj32_exact,adj32_c3,ens32,ens16")
+    ap.add_argument("--forms", default="f16,bf16,k32,adj32,adj32_exact,adj32_c3,ens32,ens16,ens32ko,ens16ko")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true")
@@ -278,6 +313,8 @@ def main():
             Lk.close()
         elif form in ("ens32", "ens16"):
             forms[form] = measure_ensemble("f32" if form == "ens32" else "f16")
+        elif form in ("ens32ko", "ens16ko"):
+            forms[form] = measure_ensemble_ko("f32" if form == "ens32ko" else "f16")
         elif form == "adj32":
             # the 2048x256 k-omega sensitivity workload; its k-omega transient
             # takes k and omega through the clamps, where the exact tangent

@@ -161,14 +161,15 @@ def turbulent_forcing(H: float, nu: float, Re: float) -> float:
     return 0.5 * cf * Ub * Ub / (H / 2.0)
 
 
-def ensemble_members(seed: int = 5, n: int = 48) -> list:
+def ensemble_members(seed: int = 5, n: int = 48, komega: bool = False) -> list:
     """The batched ensemble: n plane channels, nX from {512, ..., 4096} in
     steps of 512, nY from {64, ..., 512} in steps of 64, Re_H ~ U[2000, 20000],
     each with its own seeded Gaussian fouling layer (channel i uses seed
     seed + i).  The fouling amplitude is scaled by nY / 1024 (U[0, 8 nY/1024])
     so the 16 Gaussians leave the narrowest channels open.  Each channel runs
     laminar BGK at bulk velocity 0.05: nu = 0.05 H / Re_H and the Poiseuille
-    force F = 12 nu U / H^2."""
+    force F = 12 nu U / H^2.  With komega, k-omega is on with the turbulent
+    forcing for Re_H and k = 1.5 (0.05 U)^2, omega = sqrt(k) / (0.09^(1/4) 0.07 H)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     out = []
     for i in range(n):
@@ -183,6 +184,13 @@ def ensemble_members(seed: int = 5, n: int = 48) -> list:
         hb = gaussian_fouling(nX, r, amp=a)
         ht = gaussian_fouling(nX, r, amp=a)
         C0, BL = wall_points(nX, lambda x: 0.5 + hb(x), lambda x: nY - 1.5 - ht(x))
+        if komega:
+            k0 = 1.5 * (0.05 * Ub) ** 2
+            out.append(Workload(f"ens{i}", nX, nY, C0, BL, pipe_radius=H / 2.0, tau0=_omega(nu),
+                                nu=nu, Fval=turbulent_forcing(H, nu, Re), komega=True, k_ini=k0,
+                                omega_ini=math.sqrt(k0) / (0.09 ** 0.25 * 0.07 * H), Re=Re,
+                                noise=0.01, u_init=Ub, seed=seed + i, steps=10000))
+            continue
         out.append(Workload(f"ens{i}", nX, nY, C0, BL, pipe_radius=H / 2.0, tau0=_omega(nu),
                             nu=nu, Fval=12.0 * nu * Ub / (H * H), noise=0.01, u_init=Ub,
                             seed=seed + i, steps=10000))

@@ -1702,9 +1702,12 @@ class EngineT final : public Engine {
 template <class S, class R>
 class EnsembleT final : public Ensemble {
  public:
-  explicit EnsembleT(std::vector<EngineT<S, R>*> m) : m_(std::move(m)) {}
+  explicit EnsembleT(std::vector<EngineT<S, R>*> m) : m_(std::move(m)) {
+    ko_ = !m_.empty() && m_[0]->komega();
+  }
   ~EnsembleT() override {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    dropKOGraphs();
     groups_.clear();
     if (done_) (void)hipEventDestroy(done_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -1739,6 +1742,7 @@ class EnsembleT final : public Ensemble {
         if (e->nX_ % 64) return fail(err, "two steps per launch need nX % 64 == 0 in every member");
     }
     if (spp != 1 && spp != 2) return fail(err, "steps per launch must be 1 or 2");
+    if (ko_ && spp != 1) return fail(err, "k-omega members take one step per launch");
     block_ = block;
     spp_ = spp;
     return regroup(err);
@@ -1760,9 +1764,14 @@ class EnsembleT final : public Ensemble {
       if (!e->haveState_) return fail(err, "every member needs an initialised state");
       HIP_TRY(hipStreamSynchronize(e->stream_));
     }
-    for (auto& g : groups_) {
-      int rc = g->step(*this, n, options, err);
+    if (ko_) {
+      int rc = stepKO(n, options, err);
       if (rc) return rc;
+    } else {
+      for (auto& g : groups_) {
+        int rc = g->step(*this, n, options, err);
+        if (rc) return rc;
+      }
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(done_, stream_));
@@ -1889,6 +1898,11 @@ class EnsembleT final : public Ensemble {
         b.rho = e->rho_.template as<R>();
         b.ux = e->ux_.template as<R>();
         b.uy = e->uy_.template as<R>();
+        b.ko = dev::KOFields{nullptr, nullptr, nullptr, nullptr, nullptr};
+        if (en.ko_)
+          b.ko = dev::KOFields{e->K_.template as<void>(), e->O_.template as<void>(),
+                               e->wallD_.template as<void>(), e->nut_.template as<void>(),
+                               e->S_.template as<void>()};
         b.p = e->lbParams(options);
         b.p.applyIbb = eager ? (e->preIbb_ ? 1 : 0) : 1;
         if (two) {
@@ -1923,6 +1937,12 @@ class EnsembleT final : public Ensemble {
   hipEvent_t done_ = nullptr;
   int block_ = 512;
   int spp_ = 1;
+  bool ko_ = false;  // every member runs k-omega
+  // k-omega path: per-member solver states, their gathered flags, and the
+  // captured steps keyed by the iteration predictions and the entry parity
+  Buffer koStates_, koFlags_;
+  std::vector<std::pair<std::vector<int>, hipGraphExec_t>> koGraphs_;
+  int koShift_ = 0;
   // population bytes (both buffers) per group; 0 puts every member in one
   // launch.  The default fits the 256 MB last-level cache with room to spare.
   int64_t budget_ = int64_t(160) << 20;
@@ -1932,9 +1952,11 @@ class EnsembleT final : public Ensemble {
     return KOFM_ERR_STATE;
   }
 
-  // First-fit decreasing packing of the members into groups under budget_.
+  // First-fit decreasing packing of the members into groups under budget_
+  // (k-omega members: one group, the solves dominate).
   int regroup(std::string& err) {
     if (stream_) HIP_TRY(hipStreamSynchronize(stream_));
+    dropKOGraphs();
     groups_.clear();
     std::vector<int> order(m_.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = int(i);
@@ -1943,7 +1965,7 @@ class EnsembleT final : public Ensemble {
     std::vector<int64_t> used;
     for (int i : order) {
       size_t g = 0;
-      if (budget_ > 0)
+      if (budget_ > 0 && !ko_)
         while (g < groups_.size() && used[g] + bytes(i) > budget_) g++;
       if (g == groups_.size()) {
         groups_.emplace_back(new Group);
@@ -1972,12 +1994,147 @@ class EnsembleT final : public Ensemble {
       return;
     }
     const int bx = spp_ == 2 ? 256 : block_;
-    switch (bx) {
-      case 128: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 128>), dim3(totalTiles), dim3(128), 0, stream_, ents, n); break;
-      case 256: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 256>), dim3(totalTiles), dim3(256), 0, stream_, ents, n); break;
-      case 1024: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 1024>), dim3(totalTiles), dim3(1024), 0, stream_, ents, n); break;
-      default: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 512>), dim3(totalTiles), dim3(512), 0, stream_, ents, n); break;
+    if (ko_) {
+      launchBX<true>(bx, ents, totalTiles, n);
+      return;
     }
+    launchBX<false>(bx, ents, totalTiles, n);
+  }
+
+  template <bool KO>
+  void launchBX(int bx, const dev::BatchEntry<S, R>* ents, int totalTiles, int n) {
+    const dim3 grid(totalTiles);
+    switch (bx) {
+      case 128: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 128, KO>), grid, dim3(128), 0, stream_, ents, n); break;
+      case 256: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 256, KO>), grid, dim3(256), 0, stream_, ents, n); break;
+      case 1024: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 1024, KO>), grid, dim3(1024), 0, stream_, ents, n); break;
+      default: hipLaunchKernelGGL((dev::lbCollideStreamBatch<S, R, 512, KO>), grid, dim3(512), 0, stream_, ents, n); break;
+    }
+  }
+
+  // ---- k-omega members ----
+  // Each step: one batched collide-stream launch over every member, then
+  // every member's k-omega solve (diffusivities, coefficients and as many
+  // iterations as its last solve needed), captured as one graph per entry
+  // parity and prediction; then one poll of every member's done flags, with
+  // chunks of four more iterations for the members not yet converged.
+  int stepKO(int n, unsigned options, std::string& err) {
+    const int nm = int(m_.size());
+    if (koStates_.bytes() == 0) {
+      std::vector<const dev::BicgState*> ptrs;
+      for (auto* e : m_) ptrs.push_back(e->status_.template as<dev::BicgState>());
+      HIP_TRY(koStates_.alloc(ptrs.size() * sizeof(ptrs[0])));
+      HIP_TRY(koFlags_.alloc(4 * ptrs.size() * sizeof(int)));
+      HIP_TRY(hipMemcpy(koStates_.as<void>(), ptrs.data(), ptrs.size() * sizeof(ptrs[0]),
+                        hipMemcpyHostToDevice));
+    }
+    Group& g = *groups_[0];
+    // member k-omega launches go to the ensemble's stream
+    struct Swap {
+      std::vector<EngineT<S, R>*>& m;
+      std::vector<hipStream_t> saved;
+      Swap(std::vector<EngineT<S, R>*>& mm, hipStream_t s) : m(mm) {
+        for (auto* e : m) {
+          saved.push_back(e->stream_);
+          e->stream_ = s;
+        }
+      }
+      ~Swap() {
+        for (size_t i = 0; i < m.size(); i++) m[i]->stream_ = saved[i];
+      }
+    } swap(m_, stream_);
+    auto solves = [&](const std::vector<int>& batch) {
+      for (int i = 0; i < nm; i++) {
+        m_[i]->enqueueKoHead();
+        m_[i]->enqueueIters(batch[i]);
+      }
+    };
+    for (int s = 0; s < n; s++) {
+      std::vector<int> batch(nm);
+      for (int i = 0; i < nm; i++) batch[i] = std::min(std::max(1, m_[i]->lastIters_), 8);
+      bool fresh = false;
+      for (auto* e : m_) fresh |= !e->preIbb_;
+      if (fresh) {
+        // the first step after an upload, without the pending correction
+        int rc = g.fill(*this, g.eager, options, 0, true, false, g.eagerTiles, err);
+        if (rc) return rc;
+        launch(g.eager, g.eagerTiles, nm, false);
+        solves(batch);
+        g.built = false;
+      } else {
+        // the entries were built koShift_ steps ago: every member's buffer
+        // parity has flipped koShift_ times since
+        std::vector<int> expect(g.builtParity);
+        for (int& v : expect) v ^= koShift_ & 1;
+        if (!g.built || g.builtOptions != options || expect != g.parity()) {
+          int rc = g.fill(*this, g.ents[0], options, 0, false, false, g.totalTiles, err);
+          if (!rc) rc = g.fill(*this, g.ents[1], options, 1, false, false, g.totalTiles, err);
+          if (rc) return rc;
+          dropKOGraphs();
+          g.built = true;
+          g.builtOptions = options;
+          g.builtParity = g.parity();
+          koShift_ = 0;
+        }
+        std::vector<int> key(batch);
+        key.push_back(koShift_ & 1);
+        hipGraphExec_t exec = nullptr;
+        for (auto& c : koGraphs_)
+          if (c.first == key) exec = c.second;
+        if (!exec) {
+          hipGraph_t gr = nullptr;
+          HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+          launch(g.ents[koShift_ & 1], g.totalTiles, nm, false);
+          solves(batch);
+          HIP_TRY(hipStreamEndCapture(stream_, &gr));
+          hipError_t e = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(gr);
+          HIP_TRY(e);
+          koGraphs_.push_back({key, exec});
+        }
+        HIP_TRY(hipGraphLaunch(exec, stream_));
+        koShift_++;
+      }
+      // poll every member; more iterations where needed
+      std::vector<int> it(batch), flags(4 * size_t(nm));
+      while (true) {
+        hipLaunchKernelGGL(dev::bicgGather, dim3((nm + 63) / 64), dim3(64), 0, stream_,
+                           (const dev::BicgState* const*)koStates_.as<const dev::BicgState*>(), nm,
+                           koFlags_.as<int>());
+        HIP_TRY(hipMemcpyAsync(flags.data(), koFlags_.as<void>(), flags.size() * sizeof(int),
+                               hipMemcpyDeviceToHost, stream_));
+        HIP_TRY(hipStreamSynchronize(stream_));
+        bool more = false;
+        for (int i = 0; i < nm; i++) {
+          if (flags[4 * i + 3]) {
+            err = "non-finite residual in the k-omega solve of an ensemble member";
+            return KOFM_ERR_SOLVER_NAN;
+          }
+          const EngineT<S, R>* e = m_[i];
+          const int maxIt = std::max(e->par_.maxIters[0], e->par_.maxIters[1]);
+          if (!(flags[4 * i] && flags[4 * i + 1]) && it[i] < maxIt) {
+            const int k = std::min(4, maxIt - it[i]);
+            m_[i]->enqueueIters(k);
+            it[i] += k;
+            more = true;
+          }
+        }
+        if (!more) break;
+      }
+      for (int i = 0; i < nm; i++) m_[i]->lastIters_ = flags[4 * i + 2];
+      for (auto* e : m_) {
+        e->curBuf_ ^= 1;
+        e->preIbb_ = true;
+      }
+    }
+    for (auto* e : m_) e->dropGraphs();
+    HIP_TRY(hipGetLastError());
+    return 0;
+  }
+
+  void dropKOGraphs() {
+    for (auto& c : koGraphs_) (void)hipGraphExecDestroy(c.second);
+    koGraphs_.clear();
   }
 };
 
@@ -1987,8 +2144,9 @@ std::unique_ptr<Ensemble> tryEnsemble(const std::vector<Engine*>& members, std::
   for (Engine* e : members) {
     auto* t = dynamic_cast<EngineT<S, R>*>(e);
     if (!t) return nullptr;
-    if (t->komega()) {
-      err = "ensemble members must have k-omega off";
+    if (t->komega() != (dynamic_cast<EngineT<S, R>*>(members[0]) &&
+                        dynamic_cast<EngineT<S, R>*>(members[0])->komega())) {
+      err = "ensemble members must all have k-omega on, or all off";
       return nullptr;
     }
     m.push_back(t);

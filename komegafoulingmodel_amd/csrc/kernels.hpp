@@ -728,12 +728,13 @@ struct BatchEntry {
   R* rho;
   R* ux;
   R* uy;
+  KOFields ko;  // the member's k-omega fields (unused without KO)
   LBParams<R> p;
   int tilesX;
   int tileStart;
 };
 
-template <class S, class R, int BX>
+template <class S, class R, int BX, bool KO>
 __global__ __launch_bounds__(BX) void lbCollideStreamBatch(const BatchEntry<S, R>* __restrict__ ents,
                                                            int n) {
   const int T = blockIdx.x;
@@ -753,8 +754,7 @@ __global__ __launch_bounds__(BX) void lbCollideStreamBatch(const BatchEntry<S, R
   if (m & kSolid) return;
   R g[9];
   loadNode<S, R>(g, e.src, gi, x, y, m, e.ibbIdx, e.ibbQ, p);
-  KOFields none{nullptr, nullptr, nullptr, nullptr, nullptr};
-  collideNode<R, false>(g, gi, e.rho, e.ux, e.uy, none, p);
+  collideNode<R, KO>(g, gi, e.rho, e.ux, e.uy, e.ko, p);
   const int xe = x < p.nX - 1 ? x + 1 : 0;
   const int xw = x > 0 ? x - 1 : p.nX - 1;
 #pragma unroll
@@ -1249,6 +1249,18 @@ __global__ __launch_bounds__(kBlockX) void bicgXR(const uint16_t* __restrict__ m
     }
   });
   blockPartials(a, V.partial);
+}
+
+// Done flags, iteration count and NaN flag of several solver states (an
+// ensemble's members) in one array: out[4 i .. 4 i + 3].
+__global__ void bicgGather(const BicgState* const* __restrict__ st, int n, int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const BicgState* s = st[i];
+  out[4 * i] = s->done[0];
+  out[4 * i + 1] = s->done[1];
+  out[4 * i + 2] = s->iters[0] > s->iters[1] ? s->iters[0] : s->iters[1];
+  out[4 * i + 3] = s->nan;
 }
 
 // Resets the solver state for a new solve (a kernel, not a memset, so the
