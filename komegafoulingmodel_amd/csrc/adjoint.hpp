@@ -617,38 +617,45 @@ __global__ __launch_bounds__(kBlockX) void koRowsAdjoint(const uint16_t* __restr
   }
   yb[ko::kRowBk] = lk;
   yb[ko::kRowBo] = lo;
-  double xb[ko::kRowInputs];
-#pragma unroll
-  for (int i = 0; i < ko::kRowInputs; i++) xb[i] = 0.;
-  // spacings of 1 are no wall crossing: their sensitivity is not used
-  const bool plain = D4[0] == 1. && D4[1] == 1. && D4[2] == 1. && D4[3] == 1.;
+  // each input's adjoint goes straight to its plane: the diffusivities of
+  // the stencil, the node's local quantities (the row inputs kRowMx ..
+  // kRowNut are the planes kLocMx .. kLocNut in order), the strain, the wall
+  // distance and spacings
+  auto sink = [&](int i, double v) {
+    if (i < ko::kRowDo)
+      a.dk[g + size_t(i) * p.D] = v;
+    else if (i < ko::kRowMx)
+      a.dw[g + size_t(i - ko::kRowDo) * p.D] = v;
+    else if (i < ko::kRowS)
+      a.loc[g + size_t(i - ko::kRowMx) * p.D] = v;
+    else if (i < ko::kRowYw)
+      a.Sb[g + size_t(i - ko::kRowS) * p.D] += v;
+    else if (i == ko::kRowYw)
+      a.wdb[g] += v;
+    else
+      a.dxb[g + size_t(i - ko::kRowDx) * p.D] += v;
+  };
   const bool nesw = (m & kNesw) != 0;
-  ko::transposeApply<ko::kRowInputs, ko::kRowOutputs>(
-      in, yb, xb,
+  // inputs other than the spacings, with the difference weights constant
+  double ax[6], ay[6];
+  ko::fdCoeffsT<double>(D4[0], D4[1], ax);
+  ko::fdCoeffsT<double>(D4[2], D4[3], ay);
+  ko::transposeRange<ko::kRowInputs, ko::kRowOutputs>(
+      in, yb, 0, ko::kRowDx,
       [&](const ko::Dual (&v)[ko::kRowInputs], ko::Dual (&o)[ko::kRowOutputs]) {
-        ko::koRowT<ko::Dual>(v, nesw, p.nu, p.dt, o);
+        ko::koRowCore<ko::Dual, double>(v, ax, ay, nesw, p.nu, p.dt, o);
       },
-      [&](int i) { return plain && i >= ko::kRowDx; });
-#pragma unroll
-  for (int j = 0; j < 5; j++) {
-    a.dk[g + size_t(j) * p.D] = xb[ko::kRowDk + j];
-    a.dw[g + size_t(j) * p.D] = xb[ko::kRowDo + j];
-  }
-  a.loc[g + kLocMx * p.D] = xb[ko::kRowMx];
-  a.loc[g + kLocMy * p.D] = xb[ko::kRowMy];
-  a.loc[g + kLocUx * p.D] = xb[ko::kRowUx];
-  a.loc[g + kLocUy * p.D] = xb[ko::kRowUy];
-  a.loc[g + kLocRho * p.D] = xb[ko::kRowRho];
-  a.loc[g + kLocK * p.D] = xb[ko::kRowK];
-  a.loc[g + kLocO * p.D] = xb[ko::kRowO];
-  a.loc[g + kLocF1 * p.D] = xb[ko::kRowF1];
-  a.loc[g + kLocNut * p.D] = xb[ko::kRowNut];
-#pragma unroll
-  for (int c = 0; c < 3; c++) a.Sb[g + size_t(c) * p.D] += xb[ko::kRowS + c];
-  a.wdb[g] += xb[ko::kRowYw];
+      sink);
+  // the spacings through the weights; spacings of 1 (no wall crossing) have
+  // no sensitivity that is used
+  const bool plain = D4[0] == 1. && D4[1] == 1. && D4[2] == 1. && D4[3] == 1.;
   if (!plain)
-#pragma unroll
-    for (int d = 0; d < 4; d++) a.dxb[g + size_t(d) * p.D] += xb[ko::kRowDx + d];
+    ko::transposeRange<ko::kRowInputs, ko::kRowOutputs>(
+        in, yb, ko::kRowDx, ko::kRowInputs,
+        [&](const ko::Dual (&v)[ko::kRowInputs], ko::Dual (&o)[ko::kRowOutputs]) {
+          ko::koRowT<ko::Dual>(v, nesw, p.nu, p.dt, o);
+        },
+        sink);
 }
 
 // Step 3: adjoint of the diffusivity map.
@@ -680,29 +687,42 @@ __global__ __launch_bounds__(kBlockX) void koDiffAdjoint(const uint16_t* __restr
   yb[ko::kDiffF1] = a.loc[g + kLocF1 * p.D];
   yb[ko::kDiffDk] = a.dk[g] + gatherSent(a.dk, mask, nb, p.D);
   yb[ko::kDiffDo] = a.dw[g] + gatherSent(a.dw, mask, nb, p.D);
-  double xb[ko::kDiffInputs];
-#pragma unroll
-  for (int i = 0; i < ko::kDiffInputs; i++) xb[i] = 0.;
-  const bool plain = D4[0] == 1. && D4[1] == 1. && D4[2] == 1. && D4[3] == 1.;
-  ko::transposeApply<ko::kDiffInputs, ko::kDiffOutputs>(
-      in, yb, xb,
+  const double locK = a.loc[g + kLocK * p.D], locO = a.loc[g + kLocO * p.D];
+  auto sink = [&](int i, double v) {
+    if (i == ko::kDiffK)
+      a.kc[g] = v + locK;
+    else if (i < ko::kDiffO)
+      a.kc[g + size_t(i - ko::kDiffK) * p.D] = v;
+    else if (i == ko::kDiffO)
+      a.oc[g] = v + locO;
+    else if (i < ko::kDiffNut)
+      a.oc[g + size_t(i - ko::kDiffO) * p.D] = v;
+    else if (i == ko::kDiffNut)
+      a.loc[g + kLocNut * p.D] += v;
+    else if (i == ko::kDiffYw)
+      a.wdb[g] += v;
+    else if (i == ko::kDiffRho)
+      a.loc[g + kLocRho * p.D] += v;
+    else
+      a.dxb[g + size_t(i - ko::kDiffDx) * p.D] += v;
+  };
+  double ax[6], ay[6];
+  ko::fdCoeffsT<double>(D4[0], D4[1], ax);
+  ko::fdCoeffsT<double>(D4[2], D4[3], ay);
+  ko::transposeRange<ko::kDiffInputs, ko::kDiffOutputs>(
+      in, yb, 0, ko::kDiffDx,
       [&](const ko::Dual (&v)[ko::kDiffInputs], ko::Dual (&o)[ko::kDiffOutputs]) {
-        ko::koDiffT<ko::Dual>(v, p.nu, o);
+        ko::koDiffCore<ko::Dual, double>(v, ax, ay, p.nu, o);
       },
-      [&](int i) { return plain && i >= ko::kDiffDx; });
-  a.kc[g] = xb[ko::kDiffK] + a.loc[g + kLocK * p.D];
-  a.oc[g] = xb[ko::kDiffO] + a.loc[g + kLocO * p.D];
-#pragma unroll
-  for (int d = 1; d < 5; d++) {
-    a.kc[g + size_t(d) * p.D] = xb[ko::kDiffK + d];
-    a.oc[g + size_t(d) * p.D] = xb[ko::kDiffO + d];
-  }
-  a.loc[g + kLocNut * p.D] += xb[ko::kDiffNut];
-  a.loc[g + kLocRho * p.D] += xb[ko::kDiffRho];
-  a.wdb[g] += xb[ko::kDiffYw];
+      sink);
+  const bool plain = D4[0] == 1. && D4[1] == 1. && D4[2] == 1. && D4[3] == 1.;
   if (!plain)
-#pragma unroll
-    for (int d = 0; d < 4; d++) a.dxb[g + size_t(d) * p.D] += xb[ko::kDiffDx + d];
+    ko::transposeRange<ko::kDiffInputs, ko::kDiffOutputs>(
+        in, yb, ko::kDiffDx, ko::kDiffInputs,
+        [&](const ko::Dual (&v)[ko::kDiffInputs], ko::Dual (&o)[ko::kDiffOutputs]) {
+          ko::koDiffT<ko::Dual>(v, p.nu, o);
+        },
+        sink);
 }
 
 // Transpose of the collision with a per-node relaxation frequency and with

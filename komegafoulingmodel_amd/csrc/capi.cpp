@@ -354,9 +354,7 @@ int kofm_geometry_field_gradient(const kofm_ctx* ctx, const double* wall_d_bar,
   if (!ctx || !dJ_dy) return KOFM_ERR_ARG;
   if (!ctx->haveGeo) return KOFM_ERR_STATE;
   std::vector<double> dy;
-  std::string err;
-  int rc = kofm::wallFieldGradient(ctx->geo.input, wall_d_bar, dx_bar, dy, err);
-  if (rc) return setErr(const_cast<kofm_ctx*>(ctx), rc, err);
+  kofm::wallFieldGradient(ctx->geo, wall_d_bar, dx_bar, dy);
   for (size_t i = 0; i < dy.size(); i++) dJ_dy[i] += dy[i];
   return KOFM_OK;
 }

@@ -88,6 +88,7 @@ class EngineT final : public Engine {
     if (stream_) (void)hipStreamSynchronize(stream_);
     dropGraphs();
     for (auto& e : evs_) (void)hipEventDestroy(e);
+    if (hostStatus_) (void)hipHostFree(hostStatus_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -111,6 +112,9 @@ class EngineT final : public Engine {
       HIP_TRY(Ao_.alloc(4 * D * sizeof(double)));
     }
     HIP_TRY(status_.alloc(sizeof(dev::BicgState)));
+    // pinned, so that the per-step poll of the solver state is one DMA
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&hostStatus_), sizeof(dev::BicgState),
+                          hipHostMallocDefault));
     if (ko_)
       for (int v = 0; v < 2; v++)
         for (Buffer* b : {&kr_[v], &krh_[v], &kp_[v][0], &kp_[v][1], &kv_[v][0], &kv_[v][1], &ks_[v], &kt_[v]})
@@ -594,6 +598,7 @@ class EngineT final : public Engine {
   Buffer ss_, cfss_, tau_, sInd_, wts_, tauBL_;
   Buffer status_, partial_, redPartial_, redOut_;
   int lastIters_ = 1;  // iterations the previous k-omega solve needed
+  dev::BicgState* hostStatus_ = nullptr;  // pinned copy of the solver state
   // adjoint: checkpoints of the objective run, the recomputed segment, the
   // two adjoint population levels, per-link buffers and the objective nodes
   struct ObjectiveRun {
@@ -827,12 +832,14 @@ class EngineT final : public Engine {
     if (!qb.empty())
       HIP_TRY(hipMemcpyAsync(qb.data(), qBar_.as<void>(), qb.size() * sizeof(double),
                              hipMemcpyDeviceToHost, stream_));
-    HIP_TRY(hipMemcpyAsync(wdb.data(), adjPlane(kAWdb), d8, hipMemcpyDeviceToHost, stream_));
-    HIP_TRY(hipMemcpyAsync(dxb.data(), adjPlane(kADxb), 4 * d8, hipMemcpyDeviceToHost, stream_));
+    if (!(run_.obj.flags & KOFM_ADJOINT_FROZEN_TURBULENCE)) {
+      HIP_TRY(hipMemcpyAsync(wdb.data(), adjPlane(kAWdb), d8, hipMemcpyDeviceToHost, stream_));
+      HIP_TRY(hipMemcpyAsync(dxb.data(), adjPlane(kADxb), 4 * d8, hipMemcpyDeviceToHost, stream_));
+    }
     HIP_TRY(hipStreamSynchronize(stream_));
     std::vector<double> dy(geo_.C0.size() / 2, 0.);
-    rc = wallFieldGradient(geo_.input, wdb.data(), dxb.data(), dy, err);
-    if (rc) return rc;
+    if (!(run_.obj.flags & KOFM_ADJOINT_FROZEN_TURBULENCE))
+      wallFieldGradient(geo_, wdb.data(), dxb.data(), dy);
     wallGradient(qb, dJdy, dJdq);
     if (dJdy)
       for (size_t i = 0; i < dy.size(); i++) dJdy[i] += dy[i];
@@ -1653,9 +1660,9 @@ class EngineT final : public Engine {
     const int maxIt = std::max(par_.maxIters[0], par_.maxIters[1]);
     int it = launched;
     auto* st = status_.as<dev::BicgState>();
-    dev::BicgState hs;
+    const dev::BicgState& hs = *hostStatus_;
     while (true) {
-      HIP_TRY(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, stream_));
+      HIP_TRY(hipMemcpyAsync(hostStatus_, st, sizeof(hs), hipMemcpyDeviceToHost, stream_));
       HIP_TRY(hipStreamSynchronize(stream_));
       if ((hs.done[0] && hs.done[1]) || it >= maxIt || hs.nan) break;
       const int k = std::min(4, maxIt - it);
@@ -1676,9 +1683,9 @@ class EngineT final : public Engine {
     const int maxIt = std::max(par_.maxIters[0], par_.maxIters[1]);
     int it = launched;
     auto* st = status_.as<dev::BicgState>();
-    dev::BicgState hs;
+    const dev::BicgState& hs = *hostStatus_;
     while (true) {
-      HIP_TRY(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, stream_));
+      HIP_TRY(hipMemcpyAsync(hostStatus_, st, sizeof(hs), hipMemcpyDeviceToHost, stream_));
       HIP_TRY(hipStreamSynchronize(stream_));
       if (hs.nan) nanSeen_ = true;
       if ((hs.done[0] && hs.done[1]) || it >= maxIt) break;
@@ -1709,6 +1716,7 @@ class EnsembleT final : public Ensemble {
     if (stream_) (void)hipStreamSynchronize(stream_);
     dropKOGraphs();
     groups_.clear();
+    if (koFlagsHost_) (void)hipHostFree(koFlagsHost_);
     if (done_) (void)hipEventDestroy(done_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
@@ -1941,6 +1949,7 @@ class EnsembleT final : public Ensemble {
   // k-omega path: per-member solver states, their gathered flags, and the
   // captured steps keyed by the iteration predictions and the entry parity
   Buffer koStates_, koFlags_;
+  int* koFlagsHost_ = nullptr;  // pinned
   std::vector<std::pair<std::vector<int>, hipGraphExec_t>> koGraphs_;
   int koShift_ = 0;
   // population bytes (both buffers) per group; 0 puts every member in one
@@ -2025,6 +2034,8 @@ class EnsembleT final : public Ensemble {
       for (auto* e : m_) ptrs.push_back(e->status_.template as<dev::BicgState>());
       HIP_TRY(koStates_.alloc(ptrs.size() * sizeof(ptrs[0])));
       HIP_TRY(koFlags_.alloc(4 * ptrs.size() * sizeof(int)));
+      HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&koFlagsHost_), 4 * ptrs.size() * sizeof(int),
+                            hipHostMallocDefault));
       HIP_TRY(hipMemcpy(koStates_.as<void>(), ptrs.data(), ptrs.size() * sizeof(ptrs[0]),
                         hipMemcpyHostToDevice));
     }
@@ -2096,12 +2107,13 @@ class EnsembleT final : public Ensemble {
         koShift_++;
       }
       // poll every member; more iterations where needed
-      std::vector<int> it(batch), flags(4 * size_t(nm));
+      std::vector<int> it(batch);
+      const int* flags = koFlagsHost_;
       while (true) {
         hipLaunchKernelGGL(dev::bicgGather, dim3((nm + 63) / 64), dim3(64), 0, stream_,
                            (const dev::BicgState* const*)koStates_.as<const dev::BicgState*>(), nm,
                            koFlags_.as<int>());
-        HIP_TRY(hipMemcpyAsync(flags.data(), koFlags_.as<void>(), flags.size() * sizeof(int),
+        HIP_TRY(hipMemcpyAsync(koFlagsHost_, koFlags_.as<void>(), 4 * size_t(nm) * sizeof(int),
                                hipMemcpyDeviceToHost, stream_));
         HIP_TRY(hipStreamSynchronize(stream_));
         bool more = false;

@@ -5,6 +5,7 @@
 // reference, lsKernels.cl:429-505) runs here once, deterministically.
 #include "geometry.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <unordered_map>
 
@@ -46,6 +47,16 @@ class Builder {
     promoteFluid();
     countSolid();
     wallSpacing();
+    g_.dxSlot.clear();
+    g_.dxLink.clear();
+    {
+      std::vector<std::pair<int64_t, int32_t>> v(dxSrc_.begin(), dxSrc_.end());
+      std::sort(v.begin(), v.end());
+      for (const auto& e : v) {
+        g_.dxSlot.push_back(e.first);
+        g_.dxLink.push_back(e.second);
+      }
+    }
     g_.ssMinX = std::max(int(std::floor(in_.xRelease)) - 10, 0);
     g_.ssMaxX = int(std::ceil(in_.xStop)) + 10;
     if (g_.ssMaxX > g_.nX - 1) {
@@ -66,19 +77,8 @@ class Builder {
   Geometry& g_;
   int nN_ = 0, nBL_ = 0;
 
- public:
-  // With tracking on, the build records which link each wall spacing and
-  // wall distance came from, and their derivatives in the y of the link's
-  // two points (for wallFieldGradient).
-  struct Source {
-    int32_t pt[2] = {-1, -1};
-    double dy[2] = {0., 0.};
-  };
-  bool track = false;
-  std::unordered_map<int64_t, Source> dxSrc;  // slot node + D * (dir - 1)
-  std::vector<Source> wdSrc;                  // per node
-
  private:
+  std::unordered_map<int64_t, int32_t> dxSrc_;  // slot -> link of the crossing that set it
 
   V2 pt(int idx) const { return {in_.C0[2 * idx], in_.C0[2 * idx + 1]}; }
   V2 linkP0(int bl) const { return pt(in_.BL[2 * bl]); }
@@ -180,8 +180,7 @@ class Builder {
 
   // bcFindIntersection (clVariablesLS.cpp:144-195) with
   // bcFindIntersectionLinePlane (:197-209) and testInside (:347-356).
-  bool crossing(V2 L0, V2 Ld, V2 P0, V2 P1, int cut0[2], int cut1[2], double& dist,
-                double* distFOut = nullptr) {
+  bool crossing(V2 L0, V2 Ld, V2 P0, V2 P1, int cut0[2], int cut1[2], double& dist) {
     V2 p10 = sub(P1, P0);
     V2 vn = {-p10.y, p10.x};
     double den = dot(vn, Ld);
@@ -191,7 +190,6 @@ class Builder {
     if (!(std::fabs(len(p10) - len(sub(cut, P0)) - len(sub(cut, P1))) < in_.eps)) return false;
     int ncut = int(std::ceil(distF));
     dist = 1. - (double(ncut) - distF);
-    if (distFOut) *distFOut = distF;
     cut1[0] = int(L0.x + Ld.x * ncut);
     cut1[1] = int(L0.y + Ld.y * ncut);
     cut0[0] = cut1[0] - int(Ld.x);
@@ -245,8 +243,8 @@ class Builder {
 
   void castRay(V2 L0, V2 cd, int dir, V2 c0, V2 c1, int bl) {
     int a[2], b[2];
-    double dist, distF;
-    if (!crossing(L0, cd, c0, c1, a, b, dist, &distF)) return;
+    double dist;
+    if (!crossing(L0, cd, c0, c1, a, b, dist)) return;
     if (a[0] >= g_.nX || a[0] < 0 || a[1] >= g_.nY || a[1] < 0) return;
     b[0] = wrap(b[0], g_.nX);
     if (b[1] >= g_.nY || b[1] < 0) return;
@@ -255,19 +253,13 @@ class Builder {
     const int64_t idx = a[0] + int64_t(g_.XF) * a[1] + g_.D * (dir - 1);
     double& slot = g_.dX[idx];
     double m = std::min(dist, slot);
-    if (track && !(slot < dist)) {
-      // the new value is this crossing unless held at the minimum spacing;
-      // dist = distF + const, distF = n.(c0 - L0) / n.cd, n = (-(y1 - y0), x1 - x0)
-      Source src;
-      if (!(m < in_.minDx)) {
-        const double ex = c0.x - L0.x, dxs = c1.x - c0.x;
-        const double M = -(c1.y - c0.y) * cd.x + dxs * cd.y;
-        src.pt[0] = in_.BL[2 * bl];
-        src.pt[1] = in_.BL[2 * bl + 1];
-        src.dy[0] = (ex + dxs - distF * cd.x) / M;
-        src.dy[1] = (-ex + distF * cd.x) / M;
-      }
-      dxSrc[idx] = src;
+    // remember the crossing that sets the value (none when held at the
+    // minimum spacing), for the wall-shape sensitivity
+    if (!(slot < dist)) {
+      if (!(m < in_.minDx))
+        dxSrc_[idx] = bl;
+      else
+        dxSrc_.erase(idx);
     }
     slot = std::max(m, in_.minDx);
   }
@@ -311,40 +303,28 @@ class Builder {
 
   // kOmega::findMinDist (kOmega.cpp:136-193): distance to the nearest link
   // of the bottom range [b0,b1) and the top range [t0,t1).
-  double nearestWall(int b0, int b1, int t0, int t1, V2 p, Source* src = nullptr) const {
+  double nearestWall(int b0, int b1, int t0, int t1, V2 p, int32_t& link, int8_t& which) const {
     double best = 1000.;
+    link = -1;
+    which = 0;
     for (int top = 0; top < 2; top++) {
       int lo = top ? t0 : b0, hi = top ? t1 : b1;
       for (int i = lo; i < hi; i++) {
         V2 P0 = linkP0(i), P1 = linkP1(i);
         V2 t = sub(P1, P0);
         double tl = len(t);
-        const double cr = t.y * p.x - t.x * p.y + P1.x * P0.y - P1.y * P0.x;
-        double d = std::fabs(cr) / tl;
-        V2 tn = {t.x / tl, t.y / tl};
-        V2 foot = {p.x - (-tn.y) * d, p.y - tn.x * d};
+        double d = std::fabs(t.y * p.x - t.x * p.y + P1.x * P0.y - P1.y * P0.x) / tl;
+        t = {t.x / tl, t.y / tl};
+        V2 foot = {p.x - (-t.y) * d, p.y - t.x * d};
         bool before = top ? foot.x > P0.x : foot.x < P0.x;
         bool after = top ? foot.x < P1.x : foot.x > P1.x;
         if (before)
           d = len(sub(p, P0));
         else if (after)
           d = len(sub(p, P1));
-        if (src && !(best < d)) {
-          // d(distance)/d(y0, y1) of the chosen link: perpendicular distance
-          // |cr| / |P1 - P0|, or the distance to the end point
-          src->pt[0] = in_.BL[2 * i];
-          src->pt[1] = in_.BL[2 * i + 1];
-          if (before) {
-            src->dy[0] = d > 0. ? -(p.y - P0.y) / d : 0.;
-            src->dy[1] = 0.;
-          } else if (after) {
-            src->dy[0] = 0.;
-            src->dy[1] = d > 0. ? -(p.y - P1.y) / d : 0.;
-          } else {
-            const double sg = cr < 0. ? -1. : 1.;
-            src->dy[0] = sg * (P1.x - p.x) / tl + std::fabs(cr) * t.y / (tl * tl * tl);
-            src->dy[1] = sg * (p.x - P0.x) / tl - std::fabs(cr) * t.y / (tl * tl * tl);
-          }
+        if (!(best < d)) {  // the link (and which part of it) the minimum comes from
+          link = i;
+          which = before ? 1 : (after ? 2 : 0);
         }
         best = best < d ? best : d;
       }
@@ -355,7 +335,8 @@ class Builder {
   // kOmega::iniWallD (kOmega.cpp:330-349).
   void wallDistance() {
     g_.wallD.assign(g_.D, 0.);
-    if (track) wdSrc.assign(size_t(g_.D), Source());
+    g_.wallDLink.assign(size_t(g_.D), -1);
+    g_.wallDCase.assign(size_t(g_.D), 0);
     const int R = in_.wallDSearchRad, half = nBL_ / 2;
     for (int i = 0; i < g_.nX; i++) {
       int lb = g_.lsMap[i], lt = g_.lsMap[i + g_.XF];
@@ -364,8 +345,8 @@ class Builder {
       for (int j = 0; j < g_.nY; j++) {
         if (type(i, j) & kMSolid) continue;
         const int64_t n = i + int64_t(g_.XF) * j;
-        g_.wallD[n] = nearestWall(b0, b1, t0, t1, {double(i), double(j)},
-                                  track ? &wdSrc[size_t(n)] : nullptr);
+        g_.wallD[n] = nearestWall(b0, b1, t0, t1, {double(i), double(j)}, g_.wallDLink[size_t(n)],
+                                  g_.wallDCase[size_t(n)]);
       }
     }
   }
@@ -543,27 +524,54 @@ int buildGeometry(const GeometryInput& in, Geometry& out, std::string& err) {
   return b.run(err);
 }
 
-int wallFieldGradient(const GeometryInput& in, const double* wdBar, const double* dxBar,
-                      std::vector<double>& dy, std::string& err) {
-  Geometry g;
-  Builder b(in, g);
-  b.track = true;
-  int rc = b.run(err);
-  if (rc) return rc;
+void wallFieldGradient(const Geometry& g, const double* wdBar, const double* dxBar,
+                       std::vector<double>& dy) {
+  const auto& in = g.input;
   dy.resize(in.C0.size() / 2, 0.);
-  auto add = [&](const Builder::Source& s, double w) {
-    for (int k = 0; k < 2; k++)
-      if (s.pt[k] >= 0) dy[size_t(s.pt[k])] += w * s.dy[k];
+  auto pt = [&](int idx) { return V2{in.C0[2 * size_t(idx)], in.C0[2 * size_t(idx) + 1]}; };
+  auto add = [&](int link, double w, double d0, double d1) {
+    dy[size_t(in.BL[2 * size_t(link)])] += w * d0;
+    dy[size_t(in.BL[2 * size_t(link) + 1])] += w * d1;
   };
   if (wdBar)
-    for (int64_t n = 0; n < g.D; n++)
-      if (wdBar[n] != 0. && !(g.nType[size_t(n)] & kMSolid)) add(b.wdSrc[size_t(n)], wdBar[n]);
-  if (dxBar)
-    for (const auto& e : b.dxSrc) {
-      const int64_t node = e.first % g.D;
-      if (!(g.nType[size_t(node)] & kMSolid)) add(e.second, dxBar[e.first]);
+    for (int64_t n = 0; n < g.D; n++) {
+      const int link = g.wallDLink[size_t(n)];
+      if (wdBar[n] == 0. || link < 0 || (g.nType[size_t(n)] & kMSolid)) continue;
+      const V2 p = {double(n % g.XF), double(n / g.XF)};
+      const V2 P0 = pt(in.BL[2 * size_t(link)]), P1 = pt(in.BL[2 * size_t(link) + 1]);
+      const int which = g.wallDCase[size_t(n)];
+      if (which == 1) {  // distance to the first point
+        const double d = len(sub(p, P0));
+        if (d > 0.) add(link, wdBar[n], -(p.y - P0.y) / d, 0.);
+      } else if (which == 2) {
+        const double d = len(sub(p, P1));
+        if (d > 0.) add(link, wdBar[n], 0., -(p.y - P1.y) / d);
+      } else {
+        // |cr| / |P1 - P0| with cr = t.y p.x - t.x p.y + P1.x P0.y - P1.y P0.x
+        const V2 t = sub(P1, P0);
+        const double tl = len(t);
+        const double cr = t.y * p.x - t.x * p.y + P1.x * P0.y - P1.y * P0.x;
+        const double sg = cr < 0. ? -1. : 1.;
+        const double c3 = std::fabs(cr) * t.y / (tl * tl * tl);
+        add(link, wdBar[n], sg * (P1.x - p.x) / tl + c3, sg * (p.x - P0.x) / tl - c3);
+      }
     }
-  return 0;
+  if (dxBar)
+    for (size_t k = 0; k < g.dxSlot.size(); k++) {
+      const int64_t slot = g.dxSlot[k];
+      const int64_t node = slot % g.D;
+      const int dir = int(slot / g.D) + 1;
+      if (dxBar[slot] == 0. || (g.nType[size_t(node)] & kMSolid)) continue;
+      // the crossing cut = a + c dist of the link (c0, c1): with
+      // M = -(y1 - y0) c.x + (x1 - x0) c.y, d dist/dy0 = (x1 - cut.x) / M and
+      // d dist/dy1 = (cut.x - x0) / M
+      const int link = g.dxLink[k];
+      const V2 c0 = pt(in.BL[2 * size_t(link)]), c1 = pt(in.BL[2 * size_t(link) + 1]);
+      const double cdx = Lattice::cx[dir], cdy = Lattice::cy[dir];
+      const double cutx = double(node % g.XF) + cdx * g.dX[size_t(slot)];
+      const double M = -(c1.y - c0.y) * cdx + (c1.x - c0.x) * cdy;
+      add(link, dxBar[slot], (c1.x - cutx) / M, (cutx - c0.x) / M);
+    }
 }
 
 }  // namespace kofm

@@ -83,6 +83,10 @@ struct Geometry {
   std::vector<int32_t> ibbSeg;    // nIbb: the link cuts the wall segment (s, s+1)
   std::vector<double> ibbDqDy;    // 2*nIbb: dq/dy of points s and s+1
   std::vector<int32_t> shearLink; // nBLshear: boundary link of each shear entry
+  std::vector<int32_t> wallDLink; // D: link the wall distance was taken from (-1: none)
+  std::vector<int8_t> wallDCase;  // D: 0 perpendicular foot, 1 first point, 2 second point
+  std::vector<int64_t> dxSlot;    // wall spacings set by a crossing: slot node + D * k
+  std::vector<int32_t> dxLink;    // and the link crossed
   GeometryInput input;            // what the geometry was built from
 
   int nBLshear() const { return bounds[1] + bounds[3]; }
@@ -92,14 +96,14 @@ struct Geometry {
 // KOFM_ERR_* code with a message in err.
 int buildGeometry(const GeometryInput& in, Geometry& out, std::string& err);
 
-// Adds to dy (one entry per wall point of `in`) the y-sensitivity of the
+// Adds to dy (one entry per wall point) the y-sensitivity of the
 // wall distance field and of the axis wall spacings, weighted by wdBar (D
 // values, one per node) and dxBar (4*D, planes E, W, N, S):
 //   dy[p] += sum_n wdBar[n] dWallD[n]/dy_p + sum_s dxBar[s] d dX[s]/dy_p.
 // Each value is differentiated through the wall link (or end point) that
 // produced it, as the geometry build chose it; spacings held at the minimum
 // spacing, and nodes farther than any link, contribute nothing.
-int wallFieldGradient(const GeometryInput& in, const double* wdBar, const double* dxBar,
-                      std::vector<double>& dy, std::string& err);
+void wallFieldGradient(const Geometry& g, const double* wdBar, const double* dxBar,
+                       std::vector<double>& dy);
 
 }  // namespace kofm

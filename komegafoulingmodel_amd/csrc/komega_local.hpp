@@ -46,6 +46,14 @@ KOFM_HD Dual operator/(Dual a, Dual b) {
   return Dual(q, (a.d - q * b.d) / b.v);
 }
 KOFM_HD Dual& operator*=(Dual& a, Dual b) { return a = a * b; }
+// with a constant operand
+KOFM_HD Dual operator+(Dual a, double b) { return Dual(a.v + b, a.d); }
+KOFM_HD Dual operator+(double a, Dual b) { return Dual(a + b.v, b.d); }
+KOFM_HD Dual operator-(Dual a, double b) { return Dual(a.v - b, a.d); }
+KOFM_HD Dual operator-(double a, Dual b) { return Dual(a - b.v, -b.d); }
+KOFM_HD Dual operator*(Dual a, double b) { return Dual(a.v * b, a.d * b); }
+KOFM_HD Dual operator*(double a, Dual b) { return Dual(a * b.v, a * b.d); }
+KOFM_HD Dual operator/(Dual a, double b) { return Dual(a.v / b, a.d / b); }
 KOFM_HD bool operator<(Dual a, Dual b) { return a.v < b.v; }
 KOFM_HD bool operator>(Dual a, Dual b) { return a.v > b.v; }
 KOFM_HD bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
@@ -131,11 +139,11 @@ enum DiffInput {
 };
 enum DiffOutput { kDiffMx = 0, kDiffMy, kDiffF1, kDiffDk, kDiffDo, kDiffOutputs };
 
-template <class T>
-KOFM_HD void koDiffT(const T (&in)[kDiffInputs], double nu, T (&o)[kDiffOutputs]) {
-  T ax[6], ay[6];
-  fdCoeffsT<T>(in[kDiffDx], in[kDiffDx + 1], ax);
-  fdCoeffsT<T>(in[kDiffDx + 2], in[kDiffDx + 3], ay);
+// The map with its difference weights given (C = double holds them
+// constant; the spacing inputs are then not read).
+template <class T, class C>
+KOFM_HD void koDiffCore(const T (&in)[kDiffInputs], const C (&ax)[6], const C (&ay)[6], double nu,
+                        T (&o)[kDiffOutputs]) {
   const T* K = in + kDiffK;
   const T* O = in + kDiffO;
   const T om = O[0], k = K[0], yw = in[kDiffYw];
@@ -161,6 +169,14 @@ KOFM_HD void koDiffT(const T (&in)[kDiffInputs], double nu, T (&o)[kDiffOutputs]
   o[kDiffDo] = T(nu) + (T(0.5) * F1 + T(0.856) * (T(1) - F1)) * nt;
 }
 
+template <class T>
+KOFM_HD void koDiffT(const T (&in)[kDiffInputs], double nu, T (&o)[kDiffOutputs]) {
+  T ax[6], ay[6];
+  fdCoeffsT<T>(in[kDiffDx], in[kDiffDx + 1], ax);
+  fdCoeffsT<T>(in[kDiffDx + 2], in[kDiffDx + 3], ay);
+  koDiffCore<T, T>(in, ax, ay, nu, o);
+}
+
 // Inputs of one node's pair of normalised rows.
 enum RowInput {
   kRowDk = 0,    // 5: k diffusivity at the stencil
@@ -183,12 +199,9 @@ enum RowInput {
 // two right-hand sides.
 enum RowOutput { kRowAk = 0, kRowAo = 4, kRowBk = 8, kRowBo = 9, kRowOutputs = 10 };
 
-template <class T>
-KOFM_HD void koRowT(const T (&in)[kRowInputs], bool nesw, double nu, double dtIn,
-                    T (&o)[kRowOutputs]) {
-  T ax[6], ay[6];
-  fdCoeffsT<T>(in[kRowDx], in[kRowDx + 1], ax);
-  fdCoeffsT<T>(in[kRowDx + 2], in[kRowDx + 3], ay);
+template <class T, class C>
+KOFM_HD void koRowCore(const T (&in)[kRowInputs], const C (&ax)[6], const C (&ay)[6], bool nesw,
+                       double nu, double dtIn, T (&o)[kRowOutputs]) {
   const T* Dk = in + kRowDk;
   const T* Do = in + kRowDo;
   const T dk = Dk[0], dw = Do[0];
@@ -236,6 +249,15 @@ KOFM_HD void koRowT(const T (&in)[kRowInputs], bool nesw, double nu, double dtIn
   o[kRowBo] = (om + dt * (gam * Pk)) * iv;
 }
 
+template <class T>
+KOFM_HD void koRowT(const T (&in)[kRowInputs], bool nesw, double nu, double dtIn,
+                    T (&o)[kRowOutputs]) {
+  T ax[6], ay[6];
+  fdCoeffsT<T>(in[kRowDx], in[kRowDx + 1], ax);
+  fdCoeffsT<T>(in[kRowDx + 2], in[kRowDx + 3], ay);
+  koRowCore<T, T>(in, ax, ay, nesw, nu, dtIn, o);
+}
+
 // Eddy viscosity from k and omega after the clamps, the wall distance and
 // the strain of the previous step.
 enum NutInput { kNutK = 0, kNutO, kNutYw, kNutS, kNutInputs = 6 };
@@ -274,6 +296,23 @@ KOFM_HD void strainT(const T (&gd)[9], double omega0, T (&s)[3]) {
   s[0] = c * exx;
   s[1] = c * exy;
   s[2] = c * eyy;
+}
+
+// sink(i, (J^T yb)_i) for the inputs lo <= i < hi of the local map f: NI
+// inputs -> NO outputs at the point x, one forward-mode evaluation per input.
+template <int NI, int NO, class F, class Sink>
+KOFM_HD void transposeRange(const double (&x)[NI], const double (&yb)[NO], int lo, int hi, F&& f,
+                            Sink&& sink) {
+  KOFM_ROLLED
+  for (int i = lo; i < hi; i++) {
+    Dual in[NI];
+    for (int j = 0; j < NI; j++) in[j] = Dual(x[j], j == i ? 1. : 0.);
+    Dual out[NO];
+    f(in, out);
+    double s = 0.;
+    for (int o = 0; o < NO; o++) s += yb[o] * out[o].d;
+    sink(i, s);
+  }
 }
 
 // xb += J^T yb for the local map f: NI inputs -> NO outputs at the point x,

@@ -112,6 +112,42 @@ int main() {
     for (double v : xb)
       if (!std::isfinite(v)) bad++;
   }
+  // the adjoint kernels evaluate the non-spacing inputs with the difference
+  // weights held as constants, the spacings through the full map: the same
+  // transpose
+  for (int trial = 0; trial < 50; trial++) {
+    double rx[kRowInputs], yb[kRowOutputs], full[kRowInputs] = {}, split[kRowInputs] = {};
+    for (int j = 0; j < kRowInputs; j++) rx[j] = U(0.01, 0.02);
+    rx[kRowUx] = U(0., 0.1);
+    rx[kRowK] = U(1e-5, 1e-3);
+    rx[kRowO] = U(1e-3, 0.5);
+    rx[kRowYw] = U(0.5, 40.);
+    for (int j = 0; j < 4; j++) rx[kRowDx + j] = U(0.15, 1.);
+    for (double& v : yb) v = U(-1., 1.);
+    const bool nesw = trial % 3 == 0;
+    transposeApply<kRowInputs, kRowOutputs>(
+        rx, yb, full,
+        [&](const Dual (&v)[kRowInputs], Dual (&o)[kRowOutputs]) { koRowT(v, nesw, 0.01, 1., o); },
+        [](int) { return false; });
+    double ax[6], ay[6];
+    fdCoeffsT<double>(rx[kRowDx], rx[kRowDx + 1], ax);
+    fdCoeffsT<double>(rx[kRowDx + 2], rx[kRowDx + 3], ay);
+    transposeRange<kRowInputs, kRowOutputs>(
+        rx, yb, 0, kRowDx,
+        [&](const Dual (&v)[kRowInputs], Dual (&o)[kRowOutputs]) {
+          koRowCore<Dual, double>(v, ax, ay, nesw, 0.01, 1., o);
+        },
+        [&](int i, double g) { split[i] = g; });
+    transposeRange<kRowInputs, kRowOutputs>(
+        rx, yb, kRowDx, kRowInputs,
+        [&](const Dual (&v)[kRowInputs], Dual (&o)[kRowOutputs]) { koRowT(v, nesw, 0.01, 1., o); },
+        [&](int i, double g) { split[i] = g; });
+    for (int i = 0; i < kRowInputs; i++)
+      if (std::fabs(full[i] - split[i]) > 1e-13 * (std::fabs(full[i]) + 1e-300)) {
+        std::printf("split row transpose input %d: %.17e vs %.17e\n", i, split[i], full[i]);
+        bad++;
+      }
+  }
   std::printf("failures %d\n", bad);
   return bad ? 1 : 0;
 }
