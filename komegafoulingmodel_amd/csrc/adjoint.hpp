@@ -478,8 +478,10 @@ __global__ __launch_bounds__(64) void objectiveUpstream(R* __restrict__ abv,
 //      stream transpose with the adjoint of omega_t(nu_t), of the macros and
 //      of the strain, and back through nu_t and the clamps to k^n, omega^n,
 //      S^n.
-// The per-node maps are differentiated by forward-mode evaluation of the
-// functions the forward kernels run (komega_local.hpp).  The forward step's
+// The per-node maps are differentiated in one reverse pass each
+// (komega_local.hpp, checked against forward-mode evaluation of the functions
+// the forward kernels run); the spacing inputs of near-wall nodes use the
+// forward mode.  The forward step's
 // intermediates come from a tape recorded while a checkpoint segment is
 // recomputed.
 
@@ -636,16 +638,16 @@ __global__ __launch_bounds__(kBlockX) void koRowsAdjoint(const uint16_t* __restr
       a.dxb[g + size_t(i - ko::kRowDx) * p.D] += v;
   };
   const bool nesw = (m & kNesw) != 0;
-  // inputs other than the spacings, with the difference weights constant
-  double ax[6], ay[6];
-  ko::fdCoeffsT<double>(D4[0], D4[1], ax);
-  ko::fdCoeffsT<double>(D4[2], D4[3], ay);
-  ko::transposeRange<ko::kRowInputs, ko::kRowOutputs>(
-      in, yb, 0, ko::kRowDx,
-      [&](const ko::Dual (&v)[ko::kRowInputs], ko::Dual (&o)[ko::kRowOutputs]) {
-        ko::koRowCore<ko::Dual, double>(v, ax, ay, nesw, p.nu, p.dt, o);
-      },
-      sink);
+  // inputs other than the spacings (difference weights constant): one
+  // reverse pass
+  {
+    double ax[6], ay[6], xb[ko::kRowInputs];
+    ko::fdCoeffsT<double>(D4[0], D4[1], ax);
+    ko::fdCoeffsT<double>(D4[2], D4[3], ay);
+    ko::koRowCoreAdj(in, ax, ay, nesw, p.nu, p.dt, yb, xb);
+#pragma unroll
+    for (int i = 0; i < ko::kRowDx; i++) sink(i, xb[i]);
+  }
   // the spacings through the weights; spacings of 1 (no wall crossing) have
   // no sensitivity that is used
   const bool plain = D4[0] == 1. && D4[1] == 1. && D4[2] == 1. && D4[3] == 1.;
@@ -706,15 +708,14 @@ __global__ __launch_bounds__(kBlockX) void koDiffAdjoint(const uint16_t* __restr
     else
       a.dxb[g + size_t(i - ko::kDiffDx) * p.D] += v;
   };
-  double ax[6], ay[6];
-  ko::fdCoeffsT<double>(D4[0], D4[1], ax);
-  ko::fdCoeffsT<double>(D4[2], D4[3], ay);
-  ko::transposeRange<ko::kDiffInputs, ko::kDiffOutputs>(
-      in, yb, 0, ko::kDiffDx,
-      [&](const ko::Dual (&v)[ko::kDiffInputs], ko::Dual (&o)[ko::kDiffOutputs]) {
-        ko::koDiffCore<ko::Dual, double>(v, ax, ay, p.nu, o);
-      },
-      sink);
+  {
+    double ax[6], ay[6], xb[ko::kDiffInputs];
+    ko::fdCoeffsT<double>(D4[0], D4[1], ax);
+    ko::fdCoeffsT<double>(D4[2], D4[3], ay);
+    ko::koDiffCoreAdj(in, ax, ay, p.nu, yb, xb);
+#pragma unroll
+    for (int i = 0; i < ko::kDiffDx; i++) sink(i, xb[i]);
+  }
   const bool plain = D4[0] == 1. && D4[1] == 1. && D4[2] == 1. && D4[3] == 1.;
   if (!plain)
     ko::transposeRange<ko::kDiffInputs, ko::kDiffOutputs>(
@@ -815,15 +816,9 @@ __global__ __launch_bounds__(BX) void lbAdjointStepKO(const S* __restrict__ src,
   if (!frozen) {
     double gd[9], xb[9];
 #pragma unroll
-    for (int i = 0; i < 9; i++) {
-      gd[i] = double(nv[i]);
-      xb[i] = 0.;
-    }
+    for (int i = 0; i < 9; i++) gd[i] = double(nv[i]);
     const double sb[3] = {ka.Sb[gi], ka.Sb[gi + p.D], ka.Sb[gi + 2 * p.D]};
-    ko::transposeApply<9, 3>(
-        gd, sb, xb,
-        [&](const ko::Dual (&v)[9], ko::Dual (&o)[3]) { ko::strainT<ko::Dual>(v, double(p.omega0), o); },
-        [](int) { return false; });
+    ko::strainAdj(gd, double(p.omega0), sb, xb);
 #pragma unroll
     for (int i = 0; i < 9; i++) gb[i] += R(xb[i]);
   }
@@ -839,14 +834,8 @@ __global__ __launch_bounds__(BX) void lbAdjointStepKO(const S* __restrict__ src,
     const double K0 = t.K0[gi], O0 = t.O0[gi];
     const double nin[ko::kNutInputs] = {clampK(K0), clampO(O0), Wd[gi],
                                         t.S0[gi], t.S0[gi + p.D], t.S0[gi + 2 * p.D]};
-    double nxb[ko::kNutInputs] = {0., 0., 0., 0., 0., 0.};
-    const double nyb[1] = {nutb};
-    ko::transposeApply<ko::kNutInputs, 1>(
-        nin, nyb, nxb,
-        [&](const ko::Dual (&v)[ko::kNutInputs], ko::Dual (&o)[1]) {
-          o[0] = ko::nutT<ko::Dual>(v, nu, double(p.maxNut));
-        },
-        [](int) { return false; });
+    double nxb[ko::kNutInputs];
+    ko::nutAdj(nin, nu, double(p.maxNut), nutb, nxb);
     Kcb += nxb[ko::kNutK];
     Ocb += nxb[ko::kNutO];
     ka.wdb[gi] += nxb[ko::kNutYw];

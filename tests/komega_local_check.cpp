@@ -148,6 +148,106 @@ int main() {
         bad++;
       }
   }
+  // the hand-written reverse of the row and diffusivity maps (constant
+  // difference weights) against their dual-number transposes
+  for (int trial = 0; trial < 400; trial++) {
+    const double nu = U(0.002, 0.02);
+    double ax[6], ay[6];
+    fdCoeffsT<double>(trial % 3 ? 1. : U(0.15, 1.), trial % 5 ? 1. : U(0.15, 1.), ax);
+    fdCoeffsT<double>(trial % 2 ? 1. : U(0.15, 1.), U(0.15, 1.), ay);
+    double rx[kRowInputs], yb[kRowOutputs], dual[kRowInputs] = {}, rev[kRowInputs] = {};
+    for (int j = 0; j < 10; j++) rx[j] = nu + U(0., 5e-3);
+    rx[kRowMx] = U(-1e-2, 1e-2);
+    rx[kRowMy] = U(-1e-2, 1e-2);
+    rx[kRowUx] = U(0., 0.1);
+    rx[kRowUy] = U(-0.01, 0.01);
+    rx[kRowRho] = U(0.99, 1.01);
+    rx[kRowK] = trial % 7 == 0 ? -1e-6 : U(1e-6, 1e-3);
+    rx[kRowO] = trial % 11 == 0 ? -1e-3 : U(1e-3, 0.5);
+    rx[kRowF1] = U(0.05, 0.95);
+    rx[kRowNut] = U(1e-4, 1e-2);
+    // strains from small (production limited) to large (unlimited), signs mixed
+    const double sm = trial % 4 == 0 ? 1e-1 : 1e-3;
+    for (int j = 0; j < 3; j++) rx[kRowS + j] = U(-sm, sm);
+    rx[kRowYw] = trial % 6 == 0 ? U(0.01, 0.1) : U(0.5, 40.);
+    for (int j = 0; j < 4; j++) rx[kRowDx + j] = 1.;
+    for (double& v : yb) v = U(-1., 1.);
+    const bool nesw = trial % 4 == 1 || trial % 6 == 0;
+    transposeRange<kRowInputs, kRowOutputs>(
+        rx, yb, 0, kRowDx,
+        [&](const Dual (&v)[kRowInputs], Dual (&o)[kRowOutputs]) {
+          koRowCore<Dual, double>(v, ax, ay, nesw, nu, 1., o);
+        },
+        [&](int i, double g) { dual[i] = g; });
+    koRowCoreAdj(rx, ax, ay, nesw, nu, 1., yb, rev);
+    double scale = 0.;
+    for (int i = 0; i < kRowDx; i++) scale = std::fmax(scale, std::fabs(dual[i] * rx[i]));
+    for (int i = 0; i < kRowDx; i++)
+      if (std::fabs(dual[i] - rev[i]) * std::fabs(rx[i]) > 1e-12 * scale + 1e-300) {
+        std::printf("row reverse input %d (nesw %d): %.17e vs %.17e\n", i, int(nesw), rev[i], dual[i]);
+        bad++;
+      }
+    double dxi[kDiffInputs], dyb[kDiffOutputs], ddual[kDiffInputs] = {}, drev[kDiffInputs] = {};
+    for (int j = 0; j < 5; j++) dxi[kDiffK + j] = U(1e-7, 1e-3);
+    for (int j = 0; j < 5; j++) dxi[kDiffO + j] = U(1e-3, 0.5);
+    dxi[kDiffNut] = U(1e-4, 1e-2);
+    dxi[kDiffYw] = trial % 3 == 0 ? U(0.02, 0.5) : U(0.5, 40.);
+    dxi[kDiffRho] = U(0.99, 1.01);
+    for (int j = 0; j < 4; j++) dxi[kDiffDx + j] = 1.;
+    for (double& v : dyb) v = U(-1., 1.);
+    transposeRange<kDiffInputs, kDiffOutputs>(
+        dxi, dyb, 0, kDiffDx,
+        [&](const Dual (&v)[kDiffInputs], Dual (&o)[kDiffOutputs]) {
+          koDiffCore<Dual, double>(v, ax, ay, nu, o);
+        },
+        [&](int i, double g) { ddual[i] = g; });
+    koDiffCoreAdj(dxi, ax, ay, nu, dyb, drev);
+    scale = 0.;
+    for (int i = 0; i < kDiffDx; i++) scale = std::fmax(scale, std::fabs(ddual[i] * dxi[i]));
+    for (int i = 0; i < kDiffDx; i++)
+      if (std::fabs(ddual[i] - drev[i]) * std::fabs(dxi[i]) > 1e-12 * scale + 1e-300) {
+        std::printf("diffusivity reverse input %d: %.17e vs %.17e\n", i, drev[i], ddual[i]);
+        bad++;
+      }
+  }
+  // the hand-written reverse of the eddy viscosity and the strain
+  for (int trial = 0; trial < 400; trial++) {
+    const double nu = U(0.002, 0.02);
+    const double sm = trial % 3 == 0 ? 1e-5 : 1e-2;
+    double nx[kNutInputs] = {trial % 9 == 0 ? 0. : U(1e-7, 1e-3), U(1e-3, 0.5),
+                             trial % 4 == 0 ? U(0.01, 0.3) : U(0.5, 40.), U(-sm, sm), U(-sm, sm),
+                             U(-sm, sm)};
+    const double maxNut = trial % 5 == 0 ? 1e-3 : 1e3;
+    const double nb = U(-1., 1.);
+    double dual[kNutInputs] = {}, rev[kNutInputs] = {};
+    const double ybn[1] = {nb};
+    transposeRange<kNutInputs, 1>(
+        nx, ybn, 0, kNutInputs,
+        [&](const Dual (&v)[kNutInputs], Dual (&o)[1]) { o[0] = nutT(v, nu, maxNut); },
+        [&](int i, double g) { dual[i] = g; });
+    nutAdj(nx, nu, maxNut, nb, rev);
+    double scale = 1e-300;
+    for (int i = 0; i < kNutInputs; i++) scale = std::fmax(scale, std::fabs(dual[i] * nx[i]));
+    for (int i = 0; i < kNutInputs; i++)
+      if (std::fabs(dual[i] - rev[i]) * std::fmax(std::fabs(nx[i]), 1e-300) > 1e-12 * scale) {
+        std::printf("nut reverse input %d: %.17e vs %.17e\n", i, rev[i], dual[i]);
+        bad++;
+      }
+    double gx[9], sb[3], sd[9] = {}, sr[9];
+    for (int j = 0; j < 9; j++) gx[j] = U(-3e-2, 3e-2);
+    for (double& v : sb) v = U(-1., 1.);
+    transposeRange<9, 3>(
+        gx, sb, 0, 9, [&](const Dual (&v)[9], Dual (&o)[3]) { strainT(v, 1.6, o); },
+        [&](int i, double g) { sd[i] = g; });
+    strainAdj(gx, 1.6, sb, sr);
+    double ss = 0.;
+    for (int i = 0; i < 9; i++) ss = std::fmax(ss, std::fabs(sd[i]));
+    for (int i = 0; i < 9; i++)
+      if (std::fabs(sd[i] - sr[i]) > 1e-12 * ss) {
+        std::printf("strain reverse input %d: %.17e vs %.17e\n", i, sr[i], sd[i]);
+        bad++;
+      }
+  }
   std::printf("failures %d\n", bad);
   return bad ? 1 : 0;
 }
