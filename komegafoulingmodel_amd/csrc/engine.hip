@@ -1714,9 +1714,7 @@ class EnsembleT final : public Ensemble {
   }
   ~EnsembleT() override {
     if (stream_) (void)hipStreamSynchronize(stream_);
-    dropKOGraphs();
     groups_.clear();
-    if (koFlagsHost_) (void)hipHostFree(koFlagsHost_);
     if (done_) (void)hipEventDestroy(done_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
@@ -1773,8 +1771,12 @@ class EnsembleT final : public Ensemble {
       HIP_TRY(hipStreamSynchronize(e->stream_));
     }
     if (ko_) {
-      int rc = stepKO(n, options, err);
-      if (rc) return rc;
+      // group after group, as for laminar members: a group whose k-omega
+      // working set fits the last-level cache steps from there
+      for (auto& g : groups_) {
+        int rc = stepKO(*g, n, options, err);
+        if (rc) return rc;
+      }
     } else {
       for (auto& g : groups_) {
         int rc = g->step(*this, n, options, err);
@@ -1824,10 +1826,26 @@ class EnsembleT final : public Ensemble {
     unsigned builtOptions = 0;
     std::vector<int> builtParity;
 
-    ~Group() { dropGraph(); }
+    // k-omega members: the solver states, their gathered flags (pinned on
+    // the host), and the captured steps keyed by the iteration predictions
+    // and the entry parity
+    Buffer koStates, koFlags;
+    int* koFlagsHost = nullptr;
+    std::vector<std::pair<std::vector<int>, hipGraphExec_t>> koGraphs;
+    int koShift = 0;
+
+    ~Group() {
+      dropGraph();
+      dropKOGraphs();
+      if (koFlagsHost) (void)hipHostFree(koFlagsHost);
+    }
     void dropGraph() {
       if (graph) (void)hipGraphExecDestroy(graph);
       graph = nullptr;
+    }
+    void dropKOGraphs() {
+      for (auto& c : koGraphs) (void)hipGraphExecDestroy(c.second);
+      koGraphs.clear();
     }
     std::vector<int> parity() const {
       std::vector<int> v;
@@ -1946,12 +1964,6 @@ class EnsembleT final : public Ensemble {
   int block_ = 512;
   int spp_ = 1;
   bool ko_ = false;  // every member runs k-omega
-  // k-omega path: per-member solver states, their gathered flags, and the
-  // captured steps keyed by the iteration predictions and the entry parity
-  Buffer koStates_, koFlags_;
-  int* koFlagsHost_ = nullptr;  // pinned
-  std::vector<std::pair<std::vector<int>, hipGraphExec_t>> koGraphs_;
-  int koShift_ = 0;
   // population bytes (both buffers) per group; 0 puts every member in one
   // launch.  The default fits the 256 MB last-level cache with room to spare.
   int64_t budget_ = int64_t(160) << 20;
@@ -1961,20 +1973,23 @@ class EnsembleT final : public Ensemble {
     return KOFM_ERR_STATE;
   }
 
-  // First-fit decreasing packing of the members into groups under budget_
-  // (k-omega members: one group, the solves dominate).
+  // First-fit decreasing packing of the members into groups under budget_:
+  // the population buffers, and with k-omega the FP64 k-omega planes (the
+  // fields, rows, diffusivities and solver vectors: 42 per node).
   int regroup(std::string& err) {
     if (stream_) HIP_TRY(hipStreamSynchronize(stream_));
-    dropKOGraphs();
     groups_.clear();
     std::vector<int> order(m_.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = int(i);
-    auto bytes = [&](int i) { return 2 * 9 * m_[i]->P_ * int64_t(sizeof(S)); };
+    auto bytes = [&](int i) {
+      return 2 * 9 * m_[i]->P_ * int64_t(sizeof(S)) +
+             (ko_ ? 42 * m_[i]->D_ * int64_t(sizeof(double)) : 0);
+    };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return bytes(a) > bytes(b); });
     std::vector<int64_t> used;
     for (int i : order) {
       size_t g = 0;
-      if (budget_ > 0 && !ko_)
+      if (budget_ > 0)
         while (g < groups_.size() && used[g] + bytes(i) > budget_) g++;
       if (g == groups_.size()) {
         groups_.emplace_back(new Group);
@@ -2027,19 +2042,18 @@ class EnsembleT final : public Ensemble {
   // iterations as its last solve needed), captured as one graph per entry
   // parity and prediction; then one poll of every member's done flags, with
   // chunks of four more iterations for the members not yet converged.
-  int stepKO(int n, unsigned options, std::string& err) {
-    const int nm = int(m_.size());
-    if (koStates_.bytes() == 0) {
+  int stepKO(Group& g, int n, unsigned options, std::string& err) {
+    const int nm = int(g.m.size());
+    if (g.koStates.bytes() == 0) {
       std::vector<const dev::BicgState*> ptrs;
-      for (auto* e : m_) ptrs.push_back(e->status_.template as<dev::BicgState>());
-      HIP_TRY(koStates_.alloc(ptrs.size() * sizeof(ptrs[0])));
-      HIP_TRY(koFlags_.alloc(4 * ptrs.size() * sizeof(int)));
-      HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&koFlagsHost_), 4 * ptrs.size() * sizeof(int),
+      for (auto* e : g.m) ptrs.push_back(e->status_.template as<dev::BicgState>());
+      HIP_TRY(g.koStates.alloc(ptrs.size() * sizeof(ptrs[0])));
+      HIP_TRY(g.koFlags.alloc(4 * ptrs.size() * sizeof(int)));
+      HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.koFlagsHost), 4 * ptrs.size() * sizeof(int),
                             hipHostMallocDefault));
-      HIP_TRY(hipMemcpy(koStates_.as<void>(), ptrs.data(), ptrs.size() * sizeof(ptrs[0]),
+      HIP_TRY(hipMemcpy(g.koStates.template as<void>(), ptrs.data(), ptrs.size() * sizeof(ptrs[0]),
                         hipMemcpyHostToDevice));
     }
-    Group& g = *groups_[0];
     // member k-omega launches go to the ensemble's stream
     struct Swap {
       std::vector<EngineT<S, R>*>& m;
@@ -2053,18 +2067,18 @@ class EnsembleT final : public Ensemble {
       ~Swap() {
         for (size_t i = 0; i < m.size(); i++) m[i]->stream_ = saved[i];
       }
-    } swap(m_, stream_);
+    } swap(g.m, stream_);
     auto solves = [&](const std::vector<int>& batch) {
       for (int i = 0; i < nm; i++) {
-        m_[i]->enqueueKoHead();
-        m_[i]->enqueueIters(batch[i]);
+        g.m[i]->enqueueKoHead();
+        g.m[i]->enqueueIters(batch[i]);
       }
     };
     for (int s = 0; s < n; s++) {
       std::vector<int> batch(nm);
-      for (int i = 0; i < nm; i++) batch[i] = std::min(std::max(1, m_[i]->lastIters_), 8);
+      for (int i = 0; i < nm; i++) batch[i] = std::min(std::max(1, g.m[i]->lastIters_), 8);
       bool fresh = false;
-      for (auto* e : m_) fresh |= !e->preIbb_;
+      for (auto* e : g.m) fresh |= !e->preIbb_;
       if (fresh) {
         // the first step after an upload, without the pending correction
         int rc = g.fill(*this, g.eager, options, 0, true, false, g.eagerTiles, err);
@@ -2073,47 +2087,47 @@ class EnsembleT final : public Ensemble {
         solves(batch);
         g.built = false;
       } else {
-        // the entries were built koShift_ steps ago: every member's buffer
-        // parity has flipped koShift_ times since
+        // the entries were built g.koShift steps ago: every member's buffer
+        // parity has flipped g.koShift times since
         std::vector<int> expect(g.builtParity);
-        for (int& v : expect) v ^= koShift_ & 1;
+        for (int& v : expect) v ^= g.koShift & 1;
         if (!g.built || g.builtOptions != options || expect != g.parity()) {
           int rc = g.fill(*this, g.ents[0], options, 0, false, false, g.totalTiles, err);
           if (!rc) rc = g.fill(*this, g.ents[1], options, 1, false, false, g.totalTiles, err);
           if (rc) return rc;
-          dropKOGraphs();
+          g.dropKOGraphs();
           g.built = true;
           g.builtOptions = options;
           g.builtParity = g.parity();
-          koShift_ = 0;
+          g.koShift = 0;
         }
         std::vector<int> key(batch);
-        key.push_back(koShift_ & 1);
+        key.push_back(g.koShift & 1);
         hipGraphExec_t exec = nullptr;
-        for (auto& c : koGraphs_)
+        for (auto& c : g.koGraphs)
           if (c.first == key) exec = c.second;
         if (!exec) {
           hipGraph_t gr = nullptr;
           HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-          launch(g.ents[koShift_ & 1], g.totalTiles, nm, false);
+          launch(g.ents[g.koShift & 1], g.totalTiles, nm, false);
           solves(batch);
           HIP_TRY(hipStreamEndCapture(stream_, &gr));
           hipError_t e = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
           (void)hipGraphDestroy(gr);
           HIP_TRY(e);
-          koGraphs_.push_back({key, exec});
+          g.koGraphs.push_back({key, exec});
         }
         HIP_TRY(hipGraphLaunch(exec, stream_));
-        koShift_++;
+        g.koShift++;
       }
       // poll every member; more iterations where needed
       std::vector<int> it(batch);
-      const int* flags = koFlagsHost_;
+      const int* flags = g.koFlagsHost;
       while (true) {
         hipLaunchKernelGGL(dev::bicgGather, dim3((nm + 63) / 64), dim3(64), 0, stream_,
-                           (const dev::BicgState* const*)koStates_.as<const dev::BicgState*>(), nm,
-                           koFlags_.as<int>());
-        HIP_TRY(hipMemcpyAsync(koFlagsHost_, koFlags_.as<void>(), 4 * size_t(nm) * sizeof(int),
+                           (const dev::BicgState* const*)g.koStates.template as<const dev::BicgState*>(), nm,
+                           g.koFlags.template as<int>());
+        HIP_TRY(hipMemcpyAsync(g.koFlagsHost, g.koFlags.template as<void>(), 4 * size_t(nm) * sizeof(int),
                                hipMemcpyDeviceToHost, stream_));
         HIP_TRY(hipStreamSynchronize(stream_));
         bool more = false;
@@ -2122,32 +2136,29 @@ class EnsembleT final : public Ensemble {
             err = "non-finite residual in the k-omega solve of an ensemble member";
             return KOFM_ERR_SOLVER_NAN;
           }
-          const EngineT<S, R>* e = m_[i];
+          const EngineT<S, R>* e = g.m[i];
           const int maxIt = std::max(e->par_.maxIters[0], e->par_.maxIters[1]);
           if (!(flags[4 * i] && flags[4 * i + 1]) && it[i] < maxIt) {
             const int k = std::min(4, maxIt - it[i]);
-            m_[i]->enqueueIters(k);
+            g.m[i]->enqueueIters(k);
             it[i] += k;
             more = true;
           }
         }
         if (!more) break;
       }
-      for (int i = 0; i < nm; i++) m_[i]->lastIters_ = flags[4 * i + 2];
-      for (auto* e : m_) {
+      for (int i = 0; i < nm; i++) g.m[i]->lastIters_ = flags[4 * i + 2];
+      for (auto* e : g.m) {
         e->curBuf_ ^= 1;
         e->preIbb_ = true;
       }
     }
-    for (auto* e : m_) e->dropGraphs();
+    for (auto* e : g.m) e->dropGraphs();
     HIP_TRY(hipGetLastError());
     return 0;
   }
 
-  void dropKOGraphs() {
-    for (auto& c : koGraphs_) (void)hipGraphExecDestroy(c.second);
-    koGraphs_.clear();
-  }
+
 };
 
 template <class S, class R>

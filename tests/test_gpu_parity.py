@@ -260,11 +260,12 @@ def test_ensemble_matches_individual_runs(budget, groups, block, spp):
     ens.close()
 
 
-def test_komega_ensemble_matches_individual_runs():
-    """Three k-omega channels of different sizes stepped by the ensemble (one
-    batched collide-stream launch per step, then every member's implicit
-    solves, captured together) end bit-identical to the same channels
-    stepped alone."""
+@pytest.mark.parametrize("budget,groups", [(0, 1), (1, 3)])
+def test_komega_ensemble_matches_individual_runs(budget, groups):
+    """Three k-omega channels of different sizes stepped by the ensemble (per
+    group and step, one batched collide-stream launch, then every member's
+    implicit solves, captured together; all in one group, or one group per
+    member) end bit-identical to the same channels stepped alone."""
     from komegafoulingmodel_amd import lib
     ws = [channel.wavy_channel(256, 72, H=56.0, amp=4.0, period=128.0, nu=0.01, Re=500.0,
                                Fval=channel.turbulent_forcing(56.0, 0.01, 500.0), seed=7),
@@ -275,6 +276,7 @@ def test_komega_ensemble_matches_individual_runs():
     solo = [make_pair(w, "f32")[0] for w in ws]
     group = [make_pair(w, "f32")[0] for w in ws]
     ens = lib.Ensemble(group)
+    assert ens.set_cache_budget(budget) == groups
     for n in (1, 6, 3):
         ens.step(n, 2)
         for L in solo:
