@@ -140,25 +140,18 @@ def measure_ensemble(storage, steps=300):
         L.synchronize()
     t_seq = time.perf_counter() - t0
     ens = lib.Ensemble(Ls)
+    # the ensemble's own autotuner picks the launch shape (one step per launch,
+    # or two with 64 x 13 / 64 x 26 tiles) and the cache budget of the groups
+    block, spp, budget, _ = ens.autotune(64)
+    groups = ens.set_cache_budget(budget)
+    ens.step(64)
+    ms, _ = ens.time_steps(steps)
     out = {"channels": len(ws), "fluid_nodes": fluid, "steps": steps,
-           "sequential_mlups": fluid * steps / t_seq / 1e6}
-    # one launch per step over all members or members grouped to fit the
-    # last-level cache; one step per launch or two (64 x 13 tiles, or 64 x 26
-    # with two nodes per lane); the best is reported
-    out["shapes"] = {}
-    for budget in (0, 160 << 20):
-        for block, spp in ((512, 1), (832, 2), (1664, 2)):
-            groups = ens.set_cache_budget(budget)
-            ens.set_launch(block, spp)
-            ens.step(64)
-            ms, nodes = ens.time_steps(steps)
-            out["shapes"][f"budget{budget >> 20}MiB_block{block}_spp{spp}"] = {
-                "groups": groups, "mlups": fluid / (ms * 1e-3) / 1e6, "ms_per_step": ms,
-                "speedup_vs_sequential": (t_seq / steps) / (ms * 1e-3)}
-    best = max(out["shapes"].items(), key=lambda kv: kv[1]["mlups"])
-    out["best"] = best[0]
-    out["mlups"] = best[1]["mlups"]
-    out["speedup_vs_sequential"] = best[1]["speedup_vs_sequential"]
+           "sequential_mlups": fluid * steps / t_seq / 1e6,
+           "autotuned": {"block_x": block, "steps_per_launch": spp, "cache_budget": budget,
+                         "groups": groups},
+           "mlups": fluid / (ms * 1e-3) / 1e6, "ms_per_step": ms,
+           "speedup_vs_sequential": (t_seq / steps) / (ms * 1e-3)}
     ens.close()
     for L in Ls:
         L.close()
@@ -189,22 +182,19 @@ def measure_ensemble_ko(storage, steps=40):
         L.synchronize()
     t_seq = time.perf_counter() - t0
     ens = lib.Ensemble(Ls)
+    # the autotuner picks the work-group width and the cache budget of the
+    # groups (all members every step, or groups whose k-omega working set
+    # fits the last-level cache, stepped one group at a time)
+    block, spp, budget, _ = ens.autotune(24)
+    groups = ens.set_cache_budget(budget)
+    ens.step(8)
+    ms, _ = ens.time_steps(steps)
     out = {"channels": len(ws), "fluid_nodes": fluid, "steps": steps, "komega": True,
-           "sequential_mlups": fluid * steps / t_seq / 1e6, "budgets": {}}
-    # all members in one group, or groups whose k-omega working set fits
-    # the last-level cache (stepped one group at a time)
-    for budget in (0, 160 << 20):
-        groups = ens.set_cache_budget(budget)
-        ens.step(8)
-        ms, _ = ens.time_steps(steps)
-        out["budgets"][f"{budget >> 20}MiB"] = {
-            "groups": groups, "mlups": fluid / (ms * 1e-3) / 1e6, "ms_per_step": ms,
-            "speedup_vs_sequential": (t_seq / steps) / (ms * 1e-3)}
-    best = max(out["budgets"].items(), key=lambda kv: kv[1]["mlups"])
-    out["best"] = best[0]
-    out["mlups"] = best[1]["mlups"]
-    out["ms_per_step"] = best[1]["ms_per_step"]
-    out["speedup_vs_sequential"] = best[1]["speedup_vs_sequential"]
+           "sequential_mlups": fluid * steps / t_seq / 1e6,
+           "autotuned": {"block_x": block, "steps_per_launch": spp, "cache_budget": budget,
+                         "groups": groups},
+           "mlups": fluid / (ms * 1e-3) / 1e6, "ms_per_step": ms,
+           "speedup_vs_sequential": (t_seq / steps) / (ms * 1e-3)}
     ens.close()
     for L in Ls:
         L.close()

@@ -338,9 +338,10 @@ int kofm_geometry_field_gradient(const kofm_ctx* ctx, const double* wall_d_bar,
 /* --- ensembles ------------------------------------------------------------- */
 
 /* Several independent channels (contexts with geometry and state, one
- * population storage type, k-omega off) stepped together: one collide-stream
- * launch per step covers every tile of a group of members, and whole blocks
- * of 32 steps replay a captured HIP graph (see
+ * population storage type, k-omega all off or all on) stepped together: one
+ * collide-stream launch per step covers every tile of a group of members;
+ * without k-omega whole blocks of 32 steps replay a captured HIP graph, with
+ * k-omega each step (the launch and every member's implicit solves) does (see
  * kofm_ensemble_set_cache_budget for the grouping).  Members keep their own arrays and are read
  * with kofm_download as usual; calls on a member are ordered after the
  * ensemble's steps.  The reference runs one channel per process; this has no
@@ -367,6 +368,15 @@ int kofm_ensemble_set_launch(kofm_ensemble* e, int32_t block_x, int32_t steps_pe
  * one launch per step.  The default is 160 MiB.  *groups (may be NULL)
  * receives the number of groups.  Results do not depend on the grouping. */
 int kofm_ensemble_set_cache_budget(kofm_ensemble* e, int64_t bytes, int32_t* groups);
+/* Times `steps` ensemble steps for each launch shape (one step per launch
+ * with 256 or 512 wide groups; with k-omega off also the two-step 832 and
+ * 1664 tiles when every nX is a multiple of 64) and each cache budget (0, 96
+ * and 160 MiB), keeps the fastest and reports it.  Every member's state
+ * (populations, and with k-omega k, omega, strain, eddy viscosity and
+ * macros) is restored after each trial, so the run continues as if no
+ * tuning had happened. */
+int kofm_ensemble_autotune(kofm_ensemble* e, int32_t steps, int32_t* block_x,
+                           int32_t* steps_per_launch, int64_t* cache_budget, double* ms_per_step);
 const char* kofm_ensemble_last_error(const kofm_ensemble* e);
 void kofm_ensemble_destroy(kofm_ensemble* e);
 

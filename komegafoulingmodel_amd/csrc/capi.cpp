@@ -435,6 +435,21 @@ int kofm_ensemble_set_cache_budget(kofm_ensemble* e, int64_t bytes, int32_t* gro
   return rc ? rc : KOFM_OK;
 }
 
+int kofm_ensemble_autotune(kofm_ensemble* e, int32_t steps, int32_t* block_x,
+                           int32_t* steps_per_launch, int64_t* cache_budget, double* ms_per_step) {
+  if (!e || steps < 1) return KOFM_ERR_ARG;
+  int block = 0, spp = 0;
+  int64_t budget = 0;
+  double ms = 0.;
+  int rc = e->impl->autotune(steps, block, spp, budget, ms, e->err);
+  if (rc) return rc;
+  if (block_x) *block_x = block;
+  if (steps_per_launch) *steps_per_launch = spp;
+  if (cache_budget) *cache_budget = budget;
+  if (ms_per_step) *ms_per_step = ms;
+  return KOFM_OK;
+}
+
 const char* kofm_ensemble_last_error(const kofm_ensemble* e) {
   return e ? e->err.c_str() : g_createError.c_str();
 }

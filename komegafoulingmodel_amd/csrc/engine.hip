@@ -1762,6 +1762,102 @@ class EnsembleT final : public Ensemble {
 
   int groups() const override { return int(groups_.size()); }
 
+  int autotune(int steps, int& bestBlock, int& bestSpp, int64_t& bestBudget, double& bestMs,
+               std::string& err) override {
+    for (auto* e : m_)
+      if (!e->haveState_) return fail(err, "every member needs an initialised state");
+    // save every member's state
+    struct Saved {
+      Buffer pops, ko;
+      int curBuf;
+      bool preIbb;
+      int lastIters;
+    };
+    std::vector<Saved> saved(m_.size());
+    auto planes = [&](EngineT<S, R>* e) {
+      const size_t d8 = size_t(e->D_) * sizeof(double), dr = size_t(e->D_) * sizeof(R);
+      return std::vector<std::pair<Buffer*, size_t>>{
+          {&e->K_, d8}, {&e->O_, d8}, {&e->S_, 3 * d8}, {&e->nut_, d8},
+          {&e->rho_, dr}, {&e->ux_, dr}, {&e->uy_, dr}};
+    };
+    for (size_t i = 0; i < m_.size(); i++) {
+      EngineT<S, R>* e = m_[i];
+      HIP_TRY(hipStreamSynchronize(e->stream_));
+      const size_t pb = 9 * size_t(e->P_) * sizeof(S);
+      HIP_TRY(saved[i].pops.alloc(pb));
+      HIP_TRY(hipMemcpy(saved[i].pops.template as<void>(), e->cur(), pb, hipMemcpyDeviceToDevice));
+      saved[i].curBuf = e->curBuf_;
+      saved[i].preIbb = e->preIbb_;
+      saved[i].lastIters = e->lastIters_;
+      if (ko_) {
+        size_t tot = 0;
+        for (auto& pl : planes(e)) tot += pl.second;
+        HIP_TRY(saved[i].ko.alloc(tot));
+        char* dst = saved[i].ko.template as<char>();
+        for (auto& pl : planes(e)) {
+          HIP_TRY(hipMemcpy(dst, pl.first->template as<void>(), pl.second, hipMemcpyDeviceToDevice));
+          dst += pl.second;
+        }
+      }
+    }
+    auto restore = [&]() -> int {
+      HIP_TRY(hipStreamSynchronize(stream_));
+      for (size_t i = 0; i < m_.size(); i++) {
+        EngineT<S, R>* e = m_[i];
+        e->curBuf_ = saved[i].curBuf;
+        e->preIbb_ = saved[i].preIbb;
+        e->lastIters_ = saved[i].lastIters;
+        HIP_TRY(hipMemcpy(e->cur(), saved[i].pops.template as<void>(), 9 * size_t(e->P_) * sizeof(S),
+                          hipMemcpyDeviceToDevice));
+        if (ko_) {
+          const char* src = saved[i].ko.template as<char>();
+          for (auto& pl : planes(e)) {
+            HIP_TRY(hipMemcpy(pl.first->template as<void>(), src, pl.second, hipMemcpyDeviceToDevice));
+            src += pl.second;
+          }
+        }
+        e->dropGraphs();
+      }
+      return 0;
+    };
+    bool all64 = true;
+    for (auto* e : m_) all64 &= e->nX_ % 64 == 0;
+    std::vector<std::pair<int, int>> shapes = {{512, 1}, {256, 1}};
+    if (!ko_ && all64) {
+      shapes.push_back({832, 2});
+      shapes.push_back({1664, 2});
+    }
+    const int64_t budgets[] = {0, int64_t(96) << 20, int64_t(160) << 20};
+    bestMs = 1e30;
+    bestBlock = block_;
+    bestSpp = spp_;
+    bestBudget = budget_;
+    int rc = 0;
+    for (auto& sh : shapes) {
+      for (int64_t b : budgets) {
+        rc = setLaunch(sh.first, sh.second, err);
+        if (!rc) rc = setCacheBudget(b, err);
+        double ms = 0.;
+        // two settling steps capture the graphs of both buffer parities
+        if (!rc) rc = step(2, 0, err);
+        if (!rc) rc = timeSteps(steps, 0, ms, err);
+        const int rr = restore();
+        if (!rc) rc = rr;
+        if (rc) break;
+        if (ms < bestMs) {
+          bestMs = ms;
+          bestBlock = sh.first;
+          bestSpp = sh.second;
+          bestBudget = b;
+        }
+      }
+      if (rc) break;
+    }
+    if (!rc) rc = setLaunch(bestBlock, bestSpp, err);
+    if (!rc) rc = setCacheBudget(bestBudget, err);
+    return rc;
+  }
+
   // Every group advances n steps before the next group starts: the members
   // are independent, and a group whose populations fit the last-level cache
   // then streams its steps from there.

@@ -72,6 +72,8 @@ class Ensemble {
   virtual int setBlock(int block, std::string& err) = 0;
   virtual int setLaunch(int block, int spp, std::string& err) = 0;
   virtual int setCacheBudget(int64_t bytes, std::string& err) = 0;
+  virtual int autotune(int steps, int& block, int& spp, int64_t& budget, double& ms,
+                       std::string& err) = 0;
   virtual int groups() const = 0;
   virtual int synchronize(std::string& err) = 0;
   virtual int64_t fluidNodes() const = 0;

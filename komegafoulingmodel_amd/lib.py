@@ -47,6 +47,7 @@ EXPORTS = [
     "kofm_objective_run", "kofm_adjoint", "kofm_geometry_field_gradient",
     "kofm_ensemble_create", "kofm_ensemble_step", "kofm_ensemble_time_steps",
     "kofm_ensemble_set_block", "kofm_ensemble_set_launch", "kofm_ensemble_set_cache_budget",
+    "kofm_ensemble_autotune",
     "kofm_ensemble_last_error",
     "kofm_ensemble_destroy",
 ]
@@ -149,6 +150,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "kofm_ensemble_set_block": (C.c_int, [P, C.c_int32]),
         "kofm_ensemble_set_cache_budget": (C.c_int, [P, C.c_int64, C.POINTER(C.c_int32)]),
         "kofm_ensemble_set_launch": (C.c_int, [P, C.c_int32, C.c_int32]),
+        "kofm_ensemble_autotune": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int32),
+                                             C.POINTER(C.c_int32), C.POINTER(C.c_int64), dp]),
         "kofm_ensemble_last_error": (C.c_char_p, [P]),
         "kofm_ensemble_destroy": (None, [P]),
         "kofm_autotune": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -370,6 +373,15 @@ class Ensemble:
 
     def set_launch(self, block_x: int, steps_per_launch: int = 1):
         self._check(self.lib.kofm_ensemble_set_launch(self.h, block_x, steps_per_launch))
+
+    def autotune(self, steps: int = 20):
+        """Times each launch shape and cache budget on the members (their
+        state is restored), keeps the fastest; returns (block_x,
+        steps_per_launch, cache_budget, ms_per_step)."""
+        b, s, g, ms = C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()
+        self._check(self.lib.kofm_ensemble_autotune(self.h, steps, C.byref(b), C.byref(s),
+                                                     C.byref(g), C.byref(ms)))
+        return b.value, s.value, g.value, ms.value
 
     def set_cache_budget(self, nbytes: int) -> int:
         """Groups the members under a population-byte budget (0: one group);

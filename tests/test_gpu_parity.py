@@ -285,3 +285,57 @@ def test_komega_ensemble_matches_individual_runs(budget, groups):
             for f in ("f", "k", "omega", "nut", "ux"):
                 assert np.array_equal(a.download(f), b.download(f)), (n, f)
     ens.close()
+
+
+@pytest.mark.parametrize("storage,tol", [("f16", 1e-3), ("bf16", 3e-2)])
+def test_16bit_velocity_and_wall_shear_against_fp64(storage, tol):
+    """16-bit storage of the scaled deviations with FP32 collisions against
+    the FP64 form from the same state: velocity and wall shear (node and
+    boundary-link) after 300 steps of the fouled channel, relative to FP64
+    (max norm).  FP16 keeps both within 1e-3 (measured 7e-4 and 6e-4); BF16,
+    with an 8-bit mantissa, stays near 2e-2 and is held to 3e-2."""
+    w = _mini_fouled()
+    L16, _ = make_pair(w, storage)
+    L64, _ = make_pair(w, "f64")
+    for L in (L16, L64):
+        L.step(300, 2)
+        L.wall_shear()
+    m = fluid_mask(L64.geometry("nType"))
+    eu = vec_rel_err(L16.download("ux")[m], L16.download("uy")[m],
+                     L64.download("ux")[m], L64.download("uy")[m])
+    en = rel_err(L16.download("tau_node"), L64.download("tau_node"))
+    eb = rel_err(L16.download("tau_bl"), L64.download("tau_bl"))
+    print(f"{storage} vs f64: u {eu:.2e} shear node {en:.2e} link {eb:.2e}", flush=True)
+    assert eu < tol and en < tol and eb < tol
+
+
+@pytest.mark.parametrize("komega", [False, True])
+def test_ensemble_autotune_preserves_the_state(komega):
+    """kofm_ensemble_autotune times every shape and budget on the members and
+    restores their state after each trial: stepping on afterwards gives the
+    populations (and k, omega) of members that were never tuned."""
+    from komegafoulingmodel_amd import lib
+    if komega:
+        ws = [channel.wavy_channel(192, 48, H=36.0, amp=3.0, period=96.0, nu=0.02, Re=300.0,
+                                   Fval=channel.turbulent_forcing(36.0, 0.02, 300.0), seed=8),
+              channel.wavy_channel(128, 40, H=30.0, amp=2.0, period=64.0, nu=0.02, Re=200.0,
+                                   Fval=channel.turbulent_forcing(30.0, 0.02, 200.0), seed=9)]
+    else:
+        ws = [channel.fouled_channel(256, 48, Fval=1e-5, seed=31, amp=(0, 3), width=(10, 40)),
+              channel.fouled_channel(384, 64, Fval=2e-5, seed=32, amp=(0, 3), width=(10, 40))]
+    solo = [make_pair(w, "f32")[0] for w in ws]
+    group = [make_pair(w, "f32")[0] for w in ws]
+    for L in solo + group:
+        L.step(5, 2)
+    ens = lib.Ensemble(group)
+    block, spp, budget, ms = ens.autotune(4)
+    print(f"autotuned: block {block} steps/launch {spp} budget {budget} {ms:.3f} ms", flush=True)
+    assert ms > 0
+    ens.step(7, 2)
+    for L in solo:
+        L.step(7, 2)
+    fields = ("f", "ux", "k", "omega") if komega else ("f", "ux")
+    for a, b in zip(group, solo):
+        for f in fields:
+            assert np.array_equal(a.download(f), b.download(f)), f
+    ens.close()
