@@ -28,7 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-B_LU = {"f32": 74.0, "f16": 38.0, "bf16": 38.0, "f64": 146.0, "k32": 94.0}
+B_LU = {"f32": 74.0, "f16": 38.0, "bf16": 38.0, "f64": 146.0, "k32": 94.0, "k16": 58.0}
 
 
 def log(msg):
@@ -273,7 +273,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--forms", default="f16,bf16,k32,adj32,adj32_exact,adj32_c3,ens32,ens16,ens32ko,ens16ko")
+    ap.add_argument("--forms", default="f16,bf16,k32,k16,adj32,adj32_exact,adj32_c3,ens32,ens16,ens32ko,ens16ko")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true")
@@ -302,15 +302,15 @@ def main():
             forms["p" + form[1:] if form == "f16" else "pbf16"] = measure(
                 Lf, nf, args.steps, args.warmup, B_LU[form])
             Lf.close()
-        elif form == "k32":
+        elif form in ("k32", "k16"):
             wk = channel.config("c2")
-            log(f"k32 on c2 {wk.nX}x{wk.nY}")
-            Lk, nk = setup(wk, "f32", komega=True)
+            log(f"{form} on c2 {wk.nX}x{wk.nY}")
+            Lk, nk = setup(wk, "f32" if form == "k32" else "f16", komega=True)
             # past the start-up transient (the first steps need tens of
             # Krylov iterations), then a timed window at the reference tolerances
-            forms["k32"] = measure(Lk, nk, 50, 200, B_LU["k32"])
+            forms[form] = measure(Lk, nk, 50, 200, B_LU[form])
             st = Lk.solver_stats()
-            forms["k32"]["solver_sweeps_last"] = [st.kappa_iters, st.omega_iters]
+            forms[form]["solver_sweeps_last"] = [st.kappa_iters, st.omega_iters]
             Lk.close()
         elif form in ("ens32", "ens16"):
             forms[form] = measure_ensemble("f32" if form == "ens32" else "f16")
