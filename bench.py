@@ -71,7 +71,7 @@ def measure(L, fluid, steps, warmup, blu, options=0):
     return out
 
 
-def measure_adjoint(**kw):
+def measure_adjoint(theta_scale=1.0, **kw):
     """Wall-shape gradient on the 2048x256 serpentine channel (64 control
     points per wall, T = 4000, J over the last 1000 steps, checkpoints every
     64, k-omega on unless komega=False): plain forward, objective run with
@@ -80,7 +80,7 @@ def measure_adjoint(**kw):
     from komegafoulingmodel_amd import sensitivity as S
 
     P = S.WallShapeProblem(**kw)
-    th = P.initial_theta()
+    th = theta_scale * P.initial_theta()
     L = P.lattice()
     P.build(th)
     launch = L.autotune(20)  # the forward it is compared with runs its best shape
@@ -104,7 +104,8 @@ def measure_adjoint(**kw):
     P.close()
     mode = ("k-omega, eddy viscosity frozen in the adjoint" if P.komega and P.frozen_turbulence
             else ("k-omega, exact adjoint" if P.komega else "laminar"))
-    return {"workload": f"{P.nX}x{P.nY} serpentine channel, B-spline fouling, {mode}, "
+    shape = "serpentine channel" if P.amp else "flat-wall channel (walls at half-integer rows)"
+    return {"workload": f"{P.nX}x{P.nY} {shape}, B-spline fouling x {theta_scale:g}, {mode}, "
                         f"T={P.steps}, window {P.window}, checkpoints every {P.interval}",
             "fluid_nodes": fluid, "J": J, "grad_norm": float(np.linalg.norm(g)),
             "grad_finite": bool(np.all(np.isfinite(g))),
@@ -273,7 +274,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--forms", default="f16,bf16,k32,k16,adj32,adj32_exact,adj32_c3,ens32,ens16,ens32ko,ens16ko")
+    ap.add_argument("--forms", default="f16,bf16,k32,k16,adj32,adj32_exact,adj32_exact_smooth,adj32_c3,ens32,ens16,ens32ko,ens16ko")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true")
@@ -325,6 +326,11 @@ def main():
             # the exact k-omega adjoint (transposed solves, row, diffusivity and
             # collision adjoints) on the same channel over a horizon it stays finite
             forms["adj32_exact"] = measure_adjoint(steps=64, window=16, interval=16)
+        elif form == "adj32_exact_smooth":
+            # the exact k-omega adjoint over the full T = 4000 on the same grid
+            # with flat walls and a fouling layer of at most 0.2 nodes, where k
+            # and omega stay positive and the clamps never act
+            forms["adj32_exact_smooth"] = measure_adjoint(theta_scale=0.05, amp=0.0)
         elif form == "adj32_c3":
             # the laminar gradient at the 8192x1024 roofline shape (bandwidth bound)
             forms["adj32_c3"] = measure_adjoint(nX=8192, nY=1024, H=840.0, amp=84.0,
