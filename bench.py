@@ -212,13 +212,16 @@ def kernel_name(launch, storage="float"):
     return f"{k}<float,float,false,{launch['block_x']}>"
 
 
-def read_traffic(name):
-    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary), or None."""
+def read_traffic(name, kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (committed summary, one `traffic_bytes_per_launch[<kernel>]:` line per
+    measured kernel), or None when that kernel was not measured."""
     path = os.path.join(ROOT, "profiles", f"traffic_{name}.txt")
     if not os.path.exists(path):
         return None
+    key = f"traffic_bytes_per_launch[{kernel}]:"
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key):
             return float(line.split(":", 1)[1])
     return None
 
@@ -278,6 +281,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--launch", default="",
+                    help="block_x,nodes_per_lane,steps_per_pass,plane_pad for the headline "
+                         "form instead of the autotuner's choice (for profiling one kernel)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -291,7 +297,11 @@ def main():
 
     w = channel.config("c3")
     log(f"workload c3 {w.nX}x{w.nY} fouled, FP32, {args.steps} steps")
-    L, fluid = setup(w, "f32")
+    L, fluid = setup(w, "f32", tune=not args.launch)
+    if args.launch:
+        b, n, sp, pad = (int(v) for v in args.launch.split(","))
+        L.set_launch(b, n, pad, sp)
+        L.launch = (b, n, pad, 0.0, sp)
     r = measure(L, fluid, args.steps, args.warmup, B_LU["f32"])
     L.close()
     log(f"p32: {r}")
@@ -348,7 +358,7 @@ def main():
         log("cpu baseline")
         cpu = cpu_baseline(w, seconds=args.cpu_seconds)
 
-    traffic = read_traffic("c3_p32")
+    traffic = read_traffic("c3_p32", kernel_name(r["launch"]))
     out = {
         "metric": "MLUPS + % HBM roofline: FP32 vs FP16-store fwd, adjoint cost ratio",
         "value": round(r["mlups"], 3),

@@ -83,17 +83,31 @@ static_assert(scale16(0) * unscale16(0) == 1. && scale16(1) * unscale16(1) == 1.
                   scale16(5) * unscale16(5) == 1.,
               "16-bit scaling must round-trip exactly");
 
+// Element idx of a population array as a 32-bit byte offset from the
+// array's (uniform) base: every population array is below 4 GiB (the engine
+// checks), and a 32-bit offset lets the load or store use the scalar base
+// register instead of 64-bit address arithmetic per lane.
+template <class S>
+__device__ __forceinline__ const S* popAt(const S* __restrict__ p, size_t idx) {
+  return reinterpret_cast<const S*>(reinterpret_cast<const char*>(p) +
+                                    uint32_t(idx) * uint32_t(sizeof(S)));
+}
+template <class S>
+__device__ __forceinline__ S* popAt(S* __restrict__ p, size_t idx) {
+  return reinterpret_cast<S*>(reinterpret_cast<char*>(p) + uint32_t(idx) * uint32_t(sizeof(S)));
+}
+
 // stored value <-> deviation g = f - w
 template <class S, class R>
 __device__ __forceinline__ R loadDev(const S* __restrict__ p, size_t idx, int dir) {
-  R v = static_cast<R>(p[idx]);
+  R v = static_cast<R>(*popAt(p, idx));
   if constexpr (Scaled<S>::value) v *= R(unscale16(dir));
   return v;
 }
 template <class S, class R>
 __device__ __forceinline__ void storeDev(S* __restrict__ p, size_t idx, int dir, R g) {
   if constexpr (Scaled<S>::value) g *= R(scale16(dir));
-  p[idx] = static_cast<S>(g);
+  *popAt(p, idx) = static_cast<S>(g);
 }
 
 template <class R>
@@ -353,7 +367,7 @@ __global__ __launch_bounds__(BX) void lbCollideStream2(
   R g[2][9];
 #pragma unroll
   for (int i = 0; i < 9; i++) {
-    const Pair<S> v = *reinterpret_cast<const Pair<S>*>(src + g0 + size_t(p.P) * i);
+    const Pair<S> v = *reinterpret_cast<const Pair<S>*>(popAt(src, g0 + size_t(p.P) * i));
     R a = static_cast<R>(v.a), b = static_cast<R>(v.b);
     if constexpr (Scaled<S>::value) {
       a *= R(unscale16(i));
@@ -397,7 +411,7 @@ __global__ __launch_bounds__(BX) void lbCollideStream2(
       Pair<S> v;
       v.a = static_cast<S>(va);
       v.b = static_cast<S>(vb);
-      *reinterpret_cast<Pair<S>*>(dst + t[0]) = v;
+      *reinterpret_cast<Pair<S>*>(popAt(dst, t[0])) = v;
     } else {
 #pragma unroll
       for (int n = 0; n < 2; n++)
@@ -601,7 +615,7 @@ __device__ __forceinline__ void t2pTile(const S* __restrict__ src, S* __restrict
     R g[2][9];
 #pragma unroll
     for (int i = 0; i < 9; i++) {
-      const Pair<S> v = *reinterpret_cast<const Pair<S>*>(src + g0 + size_t(pA.P) * i);
+      const Pair<S> v = *reinterpret_cast<const Pair<S>*>(popAt(src, g0 + size_t(pA.P) * i));
       R a = static_cast<R>(v.a), b = static_cast<R>(v.b);
       if constexpr (Scaled<S>::value) {
         a *= R(unscale16(i));
@@ -690,7 +704,7 @@ __device__ __forceinline__ void t2pTile(const S* __restrict__ src, S* __restrict
       Pair<S> v;
       v.a = static_cast<S>(va);
       v.b = static_cast<S>(vb);
-      *reinterpret_cast<Pair<S>*>(dst + t[0]) = v;
+      *reinterpret_cast<Pair<S>*>(popAt(dst, t[0])) = v;
     } else {
 #pragma unroll
       for (int n = 0; n < 2; n++)
