@@ -144,7 +144,10 @@ typedef struct kofm_solver_stats {
 
 /* Allocates device storage for an nX x nY lattice on the current HIP device.
  * Replaces clVariablesLB::allocateArrays/allocateBuffers
- * (clVariablesLB.cpp:47-73) and kOmega::allocateArrays (kOmega.cpp:76-108). */
+ * (clVariablesLB.cpp:47-73) and kOmega::allocateArrays (kOmega.cpp:76-108).
+ * One population array (9 planes of XsizeFull * nY plus padding) must stay
+ * below 4 GiB (about 119M nodes in FP32, 59M in FP64); larger lattices are
+ * split over GPUs.  KOFM_ERR_ARG otherwise. */
 int kofm_create(const kofm_grid* grid, kofm_ctx** out);
 
 /* A context without device state, for geometry preparation on machines

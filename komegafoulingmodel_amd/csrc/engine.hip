@@ -92,7 +92,15 @@ class EngineT final : public Engine {
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
+  // The kernels address populations by 32-bit byte offsets from the array
+  // base: one population array (9 planes) must stay below 4 GiB.
+  static bool popsFit(int64_t P) { return 9 * P * int64_t(sizeof(S)) < (int64_t(1) << 32); }
+
   int create(std::string& err) {
+    if (!popsFit(P_)) {
+      err = "lattice too large for one GPU: a population array (9 planes) must stay below 4 GiB";
+      return KOFM_ERR_ARG;
+    }
     HIP_TRY(hipStreamCreate(&stream_));
     const size_t D = size_t(D_);
     HIP_TRY(fA_.alloc(9 * size_t(P_) * sizeof(S)));
@@ -1421,6 +1429,7 @@ class EngineT final : public Engine {
   int relayout(int64_t pad, std::string& err) {
     const int64_t P = D_ + pad;
     if (P == P_) return 0;
+    if (!popsFit(P)) return fail(err, "plane padding makes a population array 4 GiB or larger");
     Buffer tmp;
     if (haveState_) {
       HIP_TRY(tmp.alloc(9 * size_t(D_) * sizeof(double)));
@@ -1490,6 +1499,7 @@ class EngineT final : public Engine {
     bestPad = P_ - D_;
     int rc = 0;
     for (int64_t pad : pads) {
+      if (!popsFit(D_ + pad)) continue;
       Buffer a, b;
       if (hipSuccess != a.alloc(9 * size_t(D_ + pad) * sizeof(S)) ||
           hipSuccess != b.alloc(9 * size_t(D_ + pad) * sizeof(S))) {
