@@ -62,11 +62,12 @@ int kofm_create(const kofm_grid* grid, kofm_ctx** out) {
   c->storage = grid->storage;
   c->komega = grid->use_komega != 0;
   std::string err;
-  c->engine = kofm::makeEngine(c->nX, c->nY, c->storage, c->komega, err);
+  int rc = 0;
+  c->engine = kofm::makeEngine(c->nX, c->nY, c->storage, c->komega, rc, err);
   if (!c->engine) {
     g_createError = err.empty() ? "engine creation failed" : err;
     delete c;
-    return KOFM_ERR_DEVICE;
+    return rc < 0 ? rc : KOFM_ERR_DEVICE;
   }
   c->engine->setParams(c->params);
   *out = c;

@@ -2302,9 +2302,10 @@ std::unique_ptr<Ensemble> makeEnsemble(const std::vector<Engine*>& members, std:
   return e;
 }
 
-std::unique_ptr<Engine> makeEngine(int nX, int nY, int storage, bool komega, std::string& err) {
+std::unique_ptr<Engine> makeEngine(int nX, int nY, int storage, bool komega, int& rc,
+                                   std::string& err) {
   std::unique_ptr<Engine> e;
-  int rc = 0;
+  rc = 0;
   switch (storage) {
     case KOFM_STORE_F32: {
       auto* p = new EngineT<float, float>(nX, nY, komega);
@@ -2332,6 +2333,7 @@ std::unique_ptr<Engine> makeEngine(int nX, int nY, int storage, bool komega, std
     }
     default:
       err = "unknown storage type";
+      rc = KOFM_ERR_ARG;
       return nullptr;
   }
   if (rc) return nullptr;

@@ -80,7 +80,9 @@ class Ensemble {
 };
 
 // storage: KOFM_STORE_*
-std::unique_ptr<Engine> makeEngine(int nX, int nY, int storage, bool komega, std::string& err);
+// rc receives the negative KOFM_ERR_* code of a failure
+std::unique_ptr<Engine> makeEngine(int nX, int nY, int storage, bool komega, int& rc,
+                                   std::string& err);
 std::unique_ptr<Ensemble> makeEnsemble(const std::vector<Engine*>& members, std::string& err);
 
 }  // namespace kofm

@@ -339,3 +339,14 @@ def test_ensemble_autotune_preserves_the_state(komega):
         for f in fields:
             assert np.array_equal(a.download(f), b.download(f)), f
     ens.close()
+
+
+def test_create_rejects_population_arrays_of_4gib_or_more():
+    """Populations are addressed by 32-bit byte offsets: a lattice whose
+    population array would reach 4 GiB (16384 x 8192 in FP64: 9.7 GB) is
+    refused with KOFM_ERR_ARG before anything is allocated."""
+    from komegafoulingmodel_amd import lib
+    with pytest.raises(lib.KofmError) as e:
+        lib.Lattice(16384, 8192, storage="f64")
+    assert e.value.code == -1
+    assert "4 GiB" in str(e.value)
