@@ -216,6 +216,30 @@ def test_komega_wall_omega_row_and_blending_limits():
     assert np.allclose(Dk[wl.nY // 2, :64], wl.nu + 1.0 * nut0, rtol=1e-4)
 
 
+def test_komega_strain_limiter_bradshaw_and_free_stream_limits():
+    """The SST eddy viscosity nu_t = a1 k / max(a1 omega, F2 |S|) (a1 = 0.31)
+    in its two limits, from a prescribed previous-step strain (Sxy = 0.01,
+    |S| = sqrt(2 S:S) = 0.02) with uniform k, omega: next to a wall F2 -> 1
+    and the limiter gives Bradshaw's relation nu_t |S| = a1 k; at mid-height
+    of a tall channel F2 is small, a1 omega wins and nu_t = k / omega
+    (lbKernels.cl:411-417)."""
+    k0, w0, sxy = 1e-4, 1e-2, 0.01
+    wl = _komega_channel(nY=200, k0=k0, w0=w0)
+    g, orc = _lattice(wl)
+    D = g["D"]
+    orc.init_state(np.ones(D), np.zeros(D), np.zeros(D))
+    orc.init_komega(k0, w0)
+    fluid = (g["nType"].astype(np.int32) & 1) == 0
+    orc.S[D:2 * D] = np.where(fluid, sxy, 0.0)
+    orc.step(1)
+    N = orc.nut.reshape(wl.nY, wl.XF)
+    smag = 2.0 * sxy
+    assert 0.31 * w0 < smag  # the limiter can act where F2 ~ 1
+    assert np.allclose(N[1, :64] * smag, 0.31 * k0, rtol=1e-12)
+    assert np.allclose(N[wl.nY - 2, :64] * smag, 0.31 * k0, rtol=1e-12)
+    assert np.allclose(N[wl.nY // 2, :64], k0 / w0, rtol=1e-13)
+
+
 @pytest.mark.parametrize("n", [2 ** 20, 2 ** 20 - 54, 2 ** 20 - 256 + 10, 2 ** 20 - 108, 2 ** 20 + 4343])
 def test_reduction_known_answers(n):
     """ReduceTests.h:20-143 cases as assertions: sums of ones count the
