@@ -588,70 +588,147 @@ constexpr int t2pLanes() {
   return ((TX + 4) / 2 * (TY + 2) + 63) / 64 * 64;
 }
 
-template <class S, class R, int TX, int TY>
-__device__ __forceinline__ void t2pTile(const S* __restrict__ src, S* __restrict__ dst,
-                                        const uint16_t* __restrict__ mask,
-                                        const int32_t* __restrict__ ibbIdx,
-                                        const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
-                                        R* __restrict__ uxOut, R* __restrict__ uyOut,
-                                        const LBParams<R>& pA, const LBParams<R>& pB, int x0,
-                                        int y0) {
-  constexpr int RX = TX + 4, RY = TY + 2, NR = RX * RY, NP = RX / 2 * RY;
-  static_assert(TX % 2 == 0 && t2pLanes<TX, TY>() <= 1024, "tile and ring must fit one work-group");
-  __shared__ S mid[9][NR];
-  const KOFields none{nullptr, nullptr, nullptr, nullptr, nullptr};
+// Node types the fast path of the two-step pair kernel leaves to the general
+// path: solid, owner of interpolated links, or next to a solid node in any
+// direction (a halfway bounce-back).
+constexpr uint32_t kNotPlain = kSolid | kIbbNode | kBoundMask;
 
-  // step 1 on the tile and its ring, one aligned pair of nodes per lane
-  for (int q = threadIdx.x; q < NP; q += t2pLanes<TX, TY>()) {
-    const int rx0 = 2 * (q % (RX / 2)), ry = q / (RX / 2);
-    const int y = y0 - 1 + ry;
-    if (y < 0 || y >= pA.nY) continue;
-    int xa = x0 - 2 + rx0;
-    xa = xa < 0 ? xa + pA.nX : (xa >= pA.nX ? xa - pA.nX : xa);  // even, so xa + 1 < nX
-    const size_t g0 = size_t(xa) + size_t(pA.XF) * y;
-    const uint32_t mm = *reinterpret_cast<const uint32_t*>(mask + g0);
-    const uint32_t m[2] = {mm & 0xFFFFu, mm >> 16};
-    if ((m[0] & kSolid) && (m[1] & kSolid)) continue;
-    R g[2][9];
+// stored value of deviation g in slot `slot`, and back
+template <class S, class R>
+__device__ __forceinline__ S toStored(R g, int slot) {
+  if constexpr (Scaled<S>::value) g *= R(scale16(slot));
+  return static_cast<S>(g);
+}
+template <class S, class R>
+__device__ __forceinline__ R fromStored(S v, int slot) {
+  R g = static_cast<R>(v);
+  if constexpr (Scaled<S>::value) g *= R(unscale16(slot));
+  return g;
+}
+
+// The LDS planes hold the tile and its ring (RX x RY nodes) with one more
+// cell on every side (two columns left and right, so node pairs stay
+// aligned): a push out of the ring lands in a cell that is never read, so
+// no push needs a bounds check.  Ring node (rx, ry) is cell
+// (rx + 2) + LX (ry + 1).
+//
+// Both steps take a fast path in a wave whose active lanes hold only plain
+// fluid nodes (no solid, bounce-back or interpolated link: the bulk of a
+// channel).  There every push goes to the neighbour's cell at a fixed
+// offset from the node's own, the rest, N and S pushes of a pair are
+// aligned pairs, and in the second step (tiles off the periodic seam) the
+// global stores use one row offset per row and a uniform base per plane.
+// The arithmetic is the same collideNode either way, so the paths agree
+// bit for bit.
+template <int TX, int TY>
+struct T2PShape {
+  static constexpr int RX = TX + 4, RY = TY + 2, NP = RX / 2 * RY;
+  static constexpr int LX = RX + 4, LY = RY + 2, NL = LX * LY;
+  static_assert(TX % 2 == 0 && t2pLanes<TX, TY>() <= 1024, "tile and ring must fit one work-group");
+  static_assert(t2pLanes<TX, TY>() >= NP, "one pair of the tile and ring per lane");
+  static_assert(NL % 2 == 0, "node pairs must stay aligned in every plane");
+};
+
+// A lane's first-step input: its pair of nodes of the tile and ring (the
+// nine stored population pairs and the two mask words).
+template <class S>
+struct T2PIn {
+  Pair<S> v[9];
+  uint32_t mm;
+  int xa, y;  // first node of the pair; y < 0 marks a lane with no pair in the lattice
+};
+
+template <class S, class R, int TX, int TY>
+__device__ __forceinline__ void t2pLoad(T2PIn<S>& in, const S* __restrict__ src,
+                                        const uint16_t* __restrict__ mask,
+                                        const LBParams<R>& pA, int x0, int y0) {
+  using Sh = T2PShape<TX, TY>;
+  const int q = threadIdx.x;
+  const int rx0 = 2 * (q % (Sh::RX / 2)), ry = q / (Sh::RX / 2);
+  const int y = y0 - 1 + ry;
+  in.y = -1;
+  if (q >= Sh::NP || y < 0 || y >= pA.nY) return;
+  int xa = x0 - 2 + rx0;
+  xa = xa < 0 ? xa + pA.nX : (xa >= pA.nX ? xa - pA.nX : xa);  // even, so xa + 1 < nX
+  in.xa = xa;
+  in.y = y;
+  const size_t g0 = size_t(xa) + size_t(pA.XF) * y;
+  in.mm = *reinterpret_cast<const uint32_t*>(mask + g0);
+#pragma unroll
+  for (int i = 0; i < 9; i++)
+    in.v[i] = *reinterpret_cast<const Pair<S>*>(popAt(src, g0 + size_t(pA.P) * i));
+}
+
+// Step 1 on the tile and its ring from the loaded pairs, pushing into LDS.
+template <class S, class R, int TX, int TY>
+__device__ __forceinline__ void t2pStep1(const T2PIn<S>& in, S (*__restrict__ mid)[T2PShape<TX, TY>::NL],
+                                         const S* __restrict__ src,
+                                         const int32_t* __restrict__ ibbIdx,
+                                         const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
+                                         R* __restrict__ uxOut, R* __restrict__ uyOut,
+                                         const LBParams<R>& pA) {
+  using Sh = T2PShape<TX, TY>;
+  constexpr int LX = Sh::LX;
+  const KOFields none{nullptr, nullptr, nullptr, nullptr, nullptr};
+  if (in.y < 0) return;
+  const uint32_t m[2] = {in.mm & 0xFFFFu, in.mm >> 16};
+  if ((m[0] & kSolid) && (m[1] & kSolid)) return;
+  const int q = threadIdx.x;
+  const int rx0 = 2 * (q % (Sh::RX / 2)), ry = q / (Sh::RX / 2);
+  const size_t g0 = size_t(in.xa) + size_t(pA.XF) * in.y;
+  R g[2][9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    g[0][i] = fromStored<S, R>(in.v[i].a, i);
+    g[1][i] = fromStored<S, R>(in.v[i].b, i);
+  }
+  const int c0 = (rx0 + 2) + LX * (ry + 1);  // even
+  if (__all(((m[0] | m[1]) & kNotPlain) == 0)) {
+#pragma unroll
+    for (int n = 0; n < 2; n++) collideNode<R, false>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pA);
 #pragma unroll
     for (int i = 0; i < 9; i++) {
-      const Pair<S> v = *reinterpret_cast<const Pair<S>*>(popAt(src, g0 + size_t(pA.P) * i));
-      R a = static_cast<R>(v.a), b = static_cast<R>(v.b);
-      if constexpr (Scaled<S>::value) {
-        a *= R(unscale16(i));
-        b *= R(unscale16(i));
+      const int t = c0 + cx(i) + LX * cy(i);
+      if (cx(i) == 0) {
+        Pair<S> v;
+        v.a = toStored<S, R>(g[0][i], i);
+        v.b = toStored<S, R>(g[1][i], i);
+        *reinterpret_cast<Pair<S>*>(&mid[i][t]) = v;
+      } else {
+        mid[i][t] = toStored<S, R>(g[0][i], i);
+        mid[i][t + 1] = toStored<S, R>(g[1][i], i);
       }
-      g[0][i] = a;
-      g[1][i] = b;
     }
+    return;
+  }
 #pragma unroll
-    for (int n = 0; n < 2; n++) {
-      if (m[n] & kSolid) continue;
-      if (pA.applyIbb && (m[n] & kIbbNode))
-        ibbCorrect<S, R>(g[n], src, xa + n, y, ibbQ + 8 * size_t(ibbIdx[g0 + n]), pA);
-      collideNode<R, false>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pA);
-      const int r = rx0 + n + RX * ry;
+  for (int n = 0; n < 2; n++) {
+    if (m[n] & kSolid) continue;
+    if (pA.applyIbb && (m[n] & kIbbNode))
+      ibbCorrect<S, R>(g[n], src, in.xa + n, in.y, ibbQ + 8 * size_t(ibbIdx[g0 + n]), pA);
+    collideNode<R, false>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pA);
+    const int r = c0 + n;
 #pragma unroll
-      for (int i = 0; i < 9; i++) {
-        int tr, slot;
-        if (i != 0 && (m[n] & boundBit(i))) {
-          tr = r;
-          slot = rev(i);
-        } else {
-          const int tx = rx0 + n + cx(i), ty = ry + cy(i);
-          if (tx < 0 || tx >= RX || ty < 0 || ty >= RY) continue;
-          tr = tx + RX * ty;
-          slot = i;
-        }
-        R v = g[n][i];
-        if constexpr (Scaled<S>::value) v *= R(scale16(slot));
-        mid[slot][tr] = static_cast<S>(v);
-      }
+    for (int i = 0; i < 9; i++) {
+      if (i != 0 && (m[n] & boundBit(i)))
+        mid[rev(i)][r] = toStored<S, R>(g[n][i], rev(i));
+      else
+        mid[i][r + cx(i) + LX * cy(i)] = toStored<S, R>(g[n][i], i);
     }
   }
-  __syncthreads();
+}
 
-  // step 2 on the tile, one aligned pair of nodes per lane
+// Step 2 on the tile, one aligned pair of nodes per lane, from LDS to HBM.
+template <class S, class R, int TX, int TY>
+__device__ __forceinline__ void t2pStep2(const S (*__restrict__ mid)[T2PShape<TX, TY>::NL],
+                                         S* __restrict__ dst, const uint16_t* __restrict__ mask,
+                                         const int32_t* __restrict__ ibbIdx,
+                                         const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
+                                         R* __restrict__ uxOut, R* __restrict__ uyOut,
+                                         const LBParams<R>& pB, int x0, int y0) {
+  using Sh = T2PShape<TX, TY>;
+  constexpr int LX = Sh::LX;
+  const KOFields none{nullptr, nullptr, nullptr, nullptr, nullptr};
   if (threadIdx.x >= TX / 2 * TY) return;
   const int ix0 = 2 * (threadIdx.x % (TX / 2)), iy = threadIdx.x / (TX / 2);
   const int x = x0 + ix0, y = y0 + iy;
@@ -661,21 +738,48 @@ __device__ __forceinline__ void t2pTile(const S* __restrict__ src, S* __restrict
   const uint32_t m[2] = {mm & 0xFFFFu, mm >> 16};
   const bool live[2] = {!(m[0] & kSolid), !(m[1] & kSolid)};
   if (!live[0] && !live[1]) return;
-  auto lds = [&](int slot, int rr) {
-    R v = static_cast<R>(mid[slot][rr]);
-    if constexpr (Scaled<S>::value) v *= R(unscale16(slot));
-    return v;
-  };
+  const int c0 = (ix0 + 4) + LX * (iy + 2);  // ring node (ix0 + 2, iy + 1); even
   R g[2][9];
+  const bool seam = x0 == 0 || x0 + TX >= pB.nX;  // uniform: pushes wrap around x
+  if (!seam && __all(((m[0] | m[1]) & kNotPlain) == 0)) {
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const Pair<S> v = *reinterpret_cast<const Pair<S>*>(&mid[i][c0]);
+      g[0][i] = fromStored<S, R>(v.a, i);
+      g[1][i] = fromStored<S, R>(v.b, i);
+    }
+#pragma unroll
+    for (int n = 0; n < 2; n++) collideNode<R, false>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pB);
+    // element offsets of the pair's column in rows y - 1, y, y + 1 (32-bit:
+    // a population array is below 4 GiB)
+    const uint32_t row[3] = {uint32_t(g0) - uint32_t(pB.XF), uint32_t(g0),
+                             uint32_t(g0) + uint32_t(pB.XF)};
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      S* plane = dst + size_t(pB.P) * i;
+      const uint32_t o = row[cy(i) + 1] + uint32_t(cx(i));
+      if (cx(i) == 0) {
+        Pair<S> v;
+        v.a = toStored<S, R>(g[0][i], i);
+        v.b = toStored<S, R>(g[1][i], i);
+        *reinterpret_cast<Pair<S>*>(popAt(plane, o)) = v;
+      } else {
+        *popAt(plane, o) = toStored<S, R>(g[0][i], i);
+        *popAt(plane, o + 1u) = toStored<S, R>(g[1][i], i);
+      }
+    }
+    return;
+  }
+  auto lds = [&](int slot, int rr) { return fromStored<S, R>(mid[slot][rr], slot); };
 #pragma unroll
   for (int n = 0; n < 2; n++) {
     if (!live[n]) continue;
-    const int r = (ix0 + n + 2) + RX * (iy + 1);
+    const int r = c0 + n;
 #pragma unroll
     for (int i = 0; i < 9; i++) g[n][i] = lds(i, r);
     if (m[n] & kIbbNode)
       ibbCorrectWith<R>(g[n], ibbQ + 8 * size_t(ibbIdx[g0 + n]),
-                        [&](int d) { return lds(rev(d), r - cx(d) - RX * cy(d)); });
+                        [&](int d) { return lds(rev(d), r - cx(d) - LX * cy(d)); });
     collideNode<R, false>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pB);
   }
 #pragma unroll
@@ -696,14 +800,9 @@ __device__ __forceinline__ void t2pTile(const S* __restrict__ src, S* __restrict
       }
     }
     if (live[0] && live[1] && t[1] == t[0] + 1 && !(t[0] & 1)) {
-      R va = g[0][i], vb = g[1][i];
-      if constexpr (Scaled<S>::value) {
-        va *= R(scale16(slot[0]));
-        vb *= R(scale16(slot[1]));
-      }
       Pair<S> v;
-      v.a = static_cast<S>(va);
-      v.b = static_cast<S>(vb);
+      v.a = toStored<S, R>(g[0][i], slot[0]);
+      v.b = toStored<S, R>(g[1][i], slot[1]);
       *reinterpret_cast<Pair<S>*>(popAt(dst, t[0])) = v;
     } else {
 #pragma unroll
@@ -713,8 +812,28 @@ __device__ __forceinline__ void t2pTile(const S* __restrict__ src, S* __restrict
   }
 }
 
+// With FP32 arithmetic the kernel is held to 64 VGPRs (8 waves per SIMD), so
+// two work-groups of up to 16 waves share a CU and one loads while the other
+// computes; at 65-72 VGPRs only one fits.
+// Both steps on the tile at (x0, y0).
 template <class S, class R, int TX, int TY>
-__global__ __launch_bounds__((t2pLanes<TX, TY>())) void lbCollideStreamT2P(
+__device__ __forceinline__ void t2pTile(const S* __restrict__ src, S* __restrict__ dst,
+                                        const uint16_t* __restrict__ mask,
+                                        const int32_t* __restrict__ ibbIdx,
+                                        const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
+                                        R* __restrict__ uxOut, R* __restrict__ uyOut,
+                                        const LBParams<R>& pA, const LBParams<R>& pB, int x0,
+                                        int y0) {
+  __shared__ alignas(2 * sizeof(S)) S mid[9][T2PShape<TX, TY>::NL];
+  T2PIn<S> in;
+  t2pLoad<S, R, TX, TY>(in, src, mask, pA, x0, y0);
+  t2pStep1<S, R, TX, TY>(in, mid, src, ibbIdx, ibbQ, rhoOut, uxOut, uyOut, pA);
+  __syncthreads();
+  t2pStep2<S, R, TX, TY>(mid, dst, mask, ibbIdx, ibbQ, rhoOut, uxOut, uyOut, pB, x0, y0);
+}
+
+template <class S, class R, int TX, int TY>
+__global__ __launch_bounds__((t2pLanes<TX, TY>()), (sizeof(R) == 4 ? 8 : 1)) void lbCollideStreamT2P(
     const S* __restrict__ src, S* __restrict__ dst, const uint16_t* __restrict__ mask,
     const int32_t* __restrict__ ibbIdx, const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
     R* __restrict__ uxOut, R* __restrict__ uyOut, LBParams<R> pA, LBParams<R> pB, TileMap tm) {
@@ -789,7 +908,8 @@ __global__ __launch_bounds__(BX) void lbCollideStreamBatch(const BatchEntry<S, R
 // two nodes per lane with PAIRS), numbered channel after channel and dealt
 // to the XCDs in contiguous runs.
 template <class S, class R, int TX, int TY, bool PAIRS>
-__global__ __launch_bounds__((PAIRS ? t2pLanes<TX, TY>() : t2Lanes<TX, TY>())) void
+__global__ __launch_bounds__((PAIRS ? t2pLanes<TX, TY>() : t2Lanes<TX, TY>()),
+                             (PAIRS && sizeof(R) == 4 ? 8 : 1)) void
 lbCollideStreamBatchT2(const BatchEntry<S, R>* __restrict__ ents, int n, int totalTiles) {
   // XCD-aware order over all tiles (grid of 8 * per blocks)
   const int per = (totalTiles + 7) / 8;

@@ -234,6 +234,26 @@ def test_16bit_two_step_kernels_match_one_step(storage):
         assert np.array_equal(L.download("ux"), ref.download("ux")), block
 
 
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16", "f64"])
+def test_two_step_pair_kernels_match_one_step_across_tiles(storage):
+    """The two-step pair kernels take a fast path in waves of plain fluid
+    nodes and, in the second step, only in tiles off the periodic seam: on a
+    640-wide fouled channel every tile shape has seam tiles, interior tiles,
+    plain waves and waves next to the walls, and all of them end
+    bit-identical to one-step launches."""
+    w = channel.fouled_channel(640, 72, Fval=1e-5, seed=17, amp=(0.0, 5.0), width=(10.0, 80.0))
+    ref, _ = make_pair(w, storage)
+    ref.set_launch(256, 2, 1088)
+    ref.step(41, 2)
+    for block, pad in ((1536, 64), (1024, 0), (1664, 1088)):
+        L, _ = make_pair(w, storage)
+        L.set_launch(block, 1, pad, 2)
+        L.step(41, 2)
+        assert np.array_equal(L.download("f"), ref.download("f")), block
+        assert np.array_equal(L.download("ux"), ref.download("ux")), block
+        L.close()
+
+
 @pytest.mark.parametrize("budget,groups,block,spp", [(0, 1, 512, 1), (2 << 20, 2, 512, 1),
                                                      (0, 1, 832, 2), (2 << 20, 2, 1664, 2)])
 def test_ensemble_matches_individual_runs(budget, groups, block, spp):
