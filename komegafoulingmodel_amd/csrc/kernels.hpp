@@ -346,6 +346,51 @@ struct alignas(2 * sizeof(S)) Pair {
   S a, b;
 };
 
+// Node types the fast path of the two-step pair kernel leaves to the general
+// path: solid, owner of interpolated links, or next to a solid node in any
+// direction (a halfway bounce-back).
+constexpr uint32_t kNotPlain = kSolid | kIbbNode | kBoundMask;
+
+// stored value of deviation g in slot `slot`, and back
+template <class S, class R>
+__device__ __forceinline__ S toStored(R g, int slot) {
+  if constexpr (Scaled<S>::value) g *= R(scale16(slot));
+  return static_cast<S>(g);
+}
+template <class S, class R>
+__device__ __forceinline__ R fromStored(S v, int slot) {
+  R g = static_cast<R>(v);
+  if constexpr (Scaled<S>::value) g *= R(unscale16(slot));
+  return g;
+}
+
+// One half of a stored FP16 pair widened and unscaled in one instruction:
+// v_fma_mix_f32 forms h * s + (-0) with h the selected half.  The product by
+// a power of two is exact, so the result (sign of zero included) is the
+// same as widening and then multiplying, which the compiler emits as two.
+template <bool HI>
+__device__ __forceinline__ float fromHalfPair(uint32_t packed, float s) {
+  float r;
+  if constexpr (HI)
+    asm("v_fma_mix_f32 %0, %1, %2, neg(0) op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+        : "=v"(r)
+        : "v"(packed), "s"(s));
+  else
+    asm("v_fma_mix_f32 %0, %1, %2, neg(0) op_sel_hi:[1,0,0]" : "=v"(r) : "v"(packed), "s"(s));
+  return r;
+}
+template <class S, class R>
+__device__ __forceinline__ void fromStoredPair(const Pair<S>& v, int slot, R& a, R& b) {
+  if constexpr (__is_same(S, _Float16) && __is_same(R, float)) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, v);
+    a = fromHalfPair<false>(u, float(unscale16(slot)));
+    b = fromHalfPair<true>(u, float(unscale16(slot)));
+  } else {
+    a = fromStored<S, R>(v.a, slot);
+    b = fromStored<S, R>(v.b, slot);
+  }
+}
+
 // Two neighbouring nodes (x, x+1) per lane, x even, nX even: the nine
 // population planes are read as two-element vectors, and a store is paired
 // whenever both nodes' targets are adjacent and aligned (rest, N and S
@@ -588,24 +633,6 @@ constexpr int t2pLanes() {
   return ((TX + 4) / 2 * (TY + 2) + 63) / 64 * 64;
 }
 
-// Node types the fast path of the two-step pair kernel leaves to the general
-// path: solid, owner of interpolated links, or next to a solid node in any
-// direction (a halfway bounce-back).
-constexpr uint32_t kNotPlain = kSolid | kIbbNode | kBoundMask;
-
-// stored value of deviation g in slot `slot`, and back
-template <class S, class R>
-__device__ __forceinline__ S toStored(R g, int slot) {
-  if constexpr (Scaled<S>::value) g *= R(scale16(slot));
-  return static_cast<S>(g);
-}
-template <class S, class R>
-__device__ __forceinline__ R fromStored(S v, int slot) {
-  R g = static_cast<R>(v);
-  if constexpr (Scaled<S>::value) g *= R(unscale16(slot));
-  return g;
-}
-
 // The LDS planes hold the tile and its ring (RX x RY nodes) with one more
 // cell on every side (two columns left and right, so node pairs stay
 // aligned): a push out of the ring lands in a cell that is never read, so
@@ -678,10 +705,7 @@ __device__ __forceinline__ void t2pStep1(const T2PIn<S>& in, S (*__restrict__ mi
   const size_t g0 = size_t(in.xa) + size_t(pA.XF) * in.y;
   R g[2][9];
 #pragma unroll
-  for (int i = 0; i < 9; i++) {
-    g[0][i] = fromStored<S, R>(in.v[i].a, i);
-    g[1][i] = fromStored<S, R>(in.v[i].b, i);
-  }
+  for (int i = 0; i < 9; i++) fromStoredPair<S, R>(in.v[i], i, g[0][i], g[1][i]);
   const int c0 = (rx0 + 2) + LX * (ry + 1);  // even
   if (__all(((m[0] | m[1]) & kNotPlain) == 0)) {
 #pragma unroll
@@ -743,11 +767,8 @@ __device__ __forceinline__ void t2pStep2(const S (*__restrict__ mid)[T2PShape<TX
   const bool seam = x0 == 0 || x0 + TX >= pB.nX;  // uniform: pushes wrap around x
   if (!seam && __all(((m[0] | m[1]) & kNotPlain) == 0)) {
 #pragma unroll
-    for (int i = 0; i < 9; i++) {
-      const Pair<S> v = *reinterpret_cast<const Pair<S>*>(&mid[i][c0]);
-      g[0][i] = fromStored<S, R>(v.a, i);
-      g[1][i] = fromStored<S, R>(v.b, i);
-    }
+    for (int i = 0; i < 9; i++)
+      fromStoredPair<S, R>(*reinterpret_cast<const Pair<S>*>(&mid[i][c0]), i, g[0][i], g[1][i]);
 #pragma unroll
     for (int n = 0; n < 2; n++) collideNode<R, false>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pB);
     // element offsets of the pair's column in rows y - 1, y, y + 1 (32-bit:
