@@ -205,8 +205,9 @@ def measure_ensemble_ko(storage, steps=40):
 def kernel_name(launch, storage="float"):
     if launch.get("steps_per_pass", 1) == 2:
         tile = {512: (64, 8), 640: (128, 5), 832: (64, 13), 1856: (64, 29), 1024: (64, 16),
-                1536: (128, 12), 1664: (64, 26)}[launch["block_x"]]
-        k = "lbCollideStreamT2P" if launch["block_x"] in (1024, 1536, 1664) else "lbCollideStreamT2"
+                1536: (128, 12), 1664: (64, 26), 3584: (128, 28)}[launch["block_x"]]
+        k = ("lbCollideStreamT2P" if launch["block_x"] in (1024, 1536, 1664, 3584)
+             else "lbCollideStreamT2")
         return f"{k}<{storage},float,{tile[0]},{tile[1]}>"
     k = "lbCollideStream2" if launch.get("nodes_per_lane", 1) == 2 else "lbCollideStream"
     return f"{k}<float,float,false,{launch['block_x']}>"

@@ -258,9 +258,10 @@ int kofm_save2file(kofm_ctx* ctx, const char* dir);
  * nX is even), steps per pass (1, or 2 for the temporally blocked kernels:
  * k-omega off, nodes_per_lane 1, and block_x is then the tile's node count,
  * 512 (64 x 8), 832 (64 x 13), 640 (128 x 5) or 1856 (64 x 29) with one
- * node per lane and step, or 1024 (64 x 16), 1664 (64 x 26) or 1536
- * (128 x 12) with two neighbouring nodes per lane; the tile width must
- * divide nX) and the padding (elements,
+ * node per lane and step, or 1024 (64 x 16), 1664 (64 x 26), 1536
+ * (128 x 12) or, with 16- or 32-bit storage, 3584 (128 x 28) with two
+ * neighbouring nodes per lane; the tile width must divide nX) and the
+ * padding (elements,
  * multiple of 64) between the nine population planes in HBM.  The state is preserved and
  * every shape gives bit-identical results.  No counterpart in the reference,
  * whose work-group is fixed at 128 x 1 (ConstDef.h:70-79). */

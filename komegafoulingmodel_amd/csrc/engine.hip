@@ -1359,11 +1359,16 @@ class EngineT final : public Engine {
   // node per lane on 512 (64 x 8), 832 (64 x 13), 640 (128 x 5) or 1856
   // (64 x 29, two nodes per lane in each step) node tiles; two neighbouring
   // nodes per lane (aligned two-element accesses) on 1024 (64 x 16), 1664
-  // (64 x 26) or 1536 (128 x 12) node tiles.
-  static int t2Width(int nodes) { return (nodes == 640 || nodes == 1536) ? 128 : 64; }
+  // (64 x 26), 1536 (128 x 12) or 3584 (128 x 28, two rounds of each step)
+  // node tiles.
+  static int t2Width(int nodes) {
+    return (nodes == 640 || nodes == 1536 || nodes == 3584) ? 128 : 64;
+  }
+  // (the 128 x 28 tile's LDS planes exceed a work-group's 160 KB with FP64
+  // storage)
   static bool t2Tile(int nodes) {
     return nodes == 512 || nodes == 640 || nodes == 832 || nodes == 1856 || nodes == 1024 ||
-           nodes == 1536 || nodes == 1664;
+           nodes == 1536 || nodes == 1664 || (nodes == 3584 && sizeof(S) <= 4);
   }
   template <int TX, int TY>
   void launchT2PT(const dev::LBParams<R>& pA, const dev::LBParams<R>& pB, const S* src, S* dst) {
@@ -1397,6 +1402,9 @@ class EngineT final : public Engine {
       case 1024: launchT2PT<64, 16>(pA, pB, src, dst); break;
       case 1536: launchT2PT<128, 12>(pA, pB, src, dst); break;
       case 1664: launchT2PT<64, 26>(pA, pB, src, dst); break;
+      case 3584:
+        if constexpr (sizeof(S) <= 4) launchT2PT<128, 28>(pA, pB, src, dst);
+        break;
       default: launchT2T<64, 8>(pA, pB, src, dst); break;
     }
   }
@@ -1468,7 +1476,8 @@ class EngineT final : public Engine {
     if (spp != 1 && spp != 2) return fail(err, "steps per pass must be 1 or 2");
     if (spp == 2 && (ko_ || npt != 1 || !t2Tile(block) || nX_ % t2Width(block)))
       return fail(err, "two steps per pass need k-omega off, nodes_per_lane 1 and a tile "
-                       "(block_x) of 512, 640, 832, 1856, 1024, 1536 or 1664 nodes whose "
+                       "(block_x) of 512, 640, 832, 1856, 1024, 1536, 1664 or (16- and 32-bit "
+                       "storage) 3584 nodes whose "
                        "width divides nX");
     if (pad < 0 || pad % 64) return fail(err, "plane padding must be a non-negative multiple of 64");
     int rc = relayout(pad, err);
@@ -1531,8 +1540,8 @@ class EngineT final : public Engine {
           }
         }
       if (!ko_)
-        for (int block : {512, 640, 832, 1856, 1024, 1536, 1664}) {
-          if (nX_ % t2Width(block)) continue;
+        for (int block : {512, 640, 832, 1856, 1024, 1536, 1664, 3584}) {
+          if (!t2Tile(block) || nX_ % t2Width(block)) continue;
           auto pA = p, pB = p;
           pA.applyIbb = 1;
           launchT2(block, pA, pB, a.as<S>(), b.as<S>());  // warm-up

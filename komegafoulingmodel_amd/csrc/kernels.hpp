@@ -630,7 +630,8 @@ __global__ __launch_bounds__((t2Lanes<TX, TY>())) void lbCollideStreamT2(
 // nX % TX == 0 (so nX is even).
 template <int TX, int TY>
 constexpr int t2pLanes() {
-  return ((TX + 4) / 2 * (TY + 2) + 63) / 64 * 64;
+  return ((TX + 4) / 2 * (TY + 2) + 63) / 64 * 64 < 1024 ? ((TX + 4) / 2 * (TY + 2) + 63) / 64 * 64
+                                                          : 1024;
 }
 
 // The LDS planes hold the tile and its ring (RX x RY nodes) with one more
@@ -651,13 +652,12 @@ template <int TX, int TY>
 struct T2PShape {
   static constexpr int RX = TX + 4, RY = TY + 2, NP = RX / 2 * RY;
   static constexpr int LX = RX + 4, LY = RY + 2, NL = LX * LY;
-  static_assert(TX % 2 == 0 && t2pLanes<TX, TY>() <= 1024, "tile and ring must fit one work-group");
-  static_assert(t2pLanes<TX, TY>() >= NP, "one pair of the tile and ring per lane");
+  static_assert(TX % 2 == 0, "the tile holds whole node pairs");
   static_assert(NL % 2 == 0, "node pairs must stay aligned in every plane");
 };
 
-// A lane's first-step input: its pair of nodes of the tile and ring (the
-// nine stored population pairs and the two mask words).
+// First-step input of pair q of the tile and ring (the nine stored
+// population pairs and the two mask words).
 template <class S>
 struct T2PIn {
   Pair<S> v[9];
@@ -666,11 +666,10 @@ struct T2PIn {
 };
 
 template <class S, class R, int TX, int TY>
-__device__ __forceinline__ void t2pLoad(T2PIn<S>& in, const S* __restrict__ src,
+__device__ __forceinline__ void t2pLoad(T2PIn<S>& in, int q, const S* __restrict__ src,
                                         const uint16_t* __restrict__ mask,
                                         const LBParams<R>& pA, int x0, int y0) {
   using Sh = T2PShape<TX, TY>;
-  const int q = threadIdx.x;
   const int rx0 = 2 * (q % (Sh::RX / 2)), ry = q / (Sh::RX / 2);
   const int y = y0 - 1 + ry;
   in.y = -1;
@@ -688,7 +687,8 @@ __device__ __forceinline__ void t2pLoad(T2PIn<S>& in, const S* __restrict__ src,
 
 // Step 1 on the tile and its ring from the loaded pairs, pushing into LDS.
 template <class S, class R, int TX, int TY>
-__device__ __forceinline__ void t2pStep1(const T2PIn<S>& in, S (*__restrict__ mid)[T2PShape<TX, TY>::NL],
+__device__ __forceinline__ void t2pStep1(const T2PIn<S>& in, int q,
+                                         S (*__restrict__ mid)[T2PShape<TX, TY>::NL],
                                          const S* __restrict__ src,
                                          const int32_t* __restrict__ ibbIdx,
                                          const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
@@ -700,7 +700,6 @@ __device__ __forceinline__ void t2pStep1(const T2PIn<S>& in, S (*__restrict__ mi
   if (in.y < 0) return;
   const uint32_t m[2] = {in.mm & 0xFFFFu, in.mm >> 16};
   if ((m[0] & kSolid) && (m[1] & kSolid)) return;
-  const int q = threadIdx.x;
   const int rx0 = 2 * (q % (Sh::RX / 2)), ry = q / (Sh::RX / 2);
   const size_t g0 = size_t(in.xa) + size_t(pA.XF) * in.y;
   R g[2][9];
@@ -744,7 +743,7 @@ __device__ __forceinline__ void t2pStep1(const T2PIn<S>& in, S (*__restrict__ mi
 
 // Step 2 on the tile, one aligned pair of nodes per lane, from LDS to HBM.
 template <class S, class R, int TX, int TY>
-__device__ __forceinline__ void t2pStep2(const S (*__restrict__ mid)[T2PShape<TX, TY>::NL],
+__device__ __forceinline__ void t2pStep2(int q, const S (*__restrict__ mid)[T2PShape<TX, TY>::NL],
                                          S* __restrict__ dst, const uint16_t* __restrict__ mask,
                                          const int32_t* __restrict__ ibbIdx,
                                          const R* __restrict__ ibbQ, R* __restrict__ rhoOut,
@@ -753,8 +752,7 @@ __device__ __forceinline__ void t2pStep2(const S (*__restrict__ mid)[T2PShape<TX
   using Sh = T2PShape<TX, TY>;
   constexpr int LX = Sh::LX;
   const KOFields none{nullptr, nullptr, nullptr, nullptr, nullptr};
-  if (threadIdx.x >= TX / 2 * TY) return;
-  const int ix0 = 2 * (threadIdx.x % (TX / 2)), iy = threadIdx.x / (TX / 2);
+  const int ix0 = 2 * (q % (TX / 2)), iy = q / (TX / 2);
   const int x = x0 + ix0, y = y0 + iy;
   if (y >= pB.nY) return;
   const size_t g0 = size_t(x) + size_t(pB.XF) * y;
@@ -836,7 +834,8 @@ __device__ __forceinline__ void t2pStep2(const S (*__restrict__ mid)[T2PShape<TX
 // With FP32 arithmetic the kernel is held to 64 VGPRs (8 waves per SIMD), so
 // two work-groups of up to 16 waves share a CU and one loads while the other
 // computes; at 65-72 VGPRs only one fits.
-// Both steps on the tile at (x0, y0).
+// Both steps on the tile at (x0, y0); a tile and ring of more than 1024 lanes'
+// pairs takes two rounds of each step.
 template <class S, class R, int TX, int TY>
 __device__ __forceinline__ void t2pTile(const S* __restrict__ src, S* __restrict__ dst,
                                         const uint16_t* __restrict__ mask,
@@ -845,12 +844,16 @@ __device__ __forceinline__ void t2pTile(const S* __restrict__ src, S* __restrict
                                         R* __restrict__ uxOut, R* __restrict__ uyOut,
                                         const LBParams<R>& pA, const LBParams<R>& pB, int x0,
                                         int y0) {
+  constexpr int L = t2pLanes<TX, TY>();
   __shared__ alignas(2 * sizeof(S)) S mid[9][T2PShape<TX, TY>::NL];
-  T2PIn<S> in;
-  t2pLoad<S, R, TX, TY>(in, src, mask, pA, x0, y0);
-  t2pStep1<S, R, TX, TY>(in, mid, src, ibbIdx, ibbQ, rhoOut, uxOut, uyOut, pA);
+  for (int q = threadIdx.x; q < T2PShape<TX, TY>::NP; q += L) {
+    T2PIn<S> in;
+    t2pLoad<S, R, TX, TY>(in, q, src, mask, pA, x0, y0);
+    t2pStep1<S, R, TX, TY>(in, q, mid, src, ibbIdx, ibbQ, rhoOut, uxOut, uyOut, pA);
+  }
   __syncthreads();
-  t2pStep2<S, R, TX, TY>(mid, dst, mask, ibbIdx, ibbQ, rhoOut, uxOut, uyOut, pB, x0, y0);
+  for (int q = threadIdx.x; q < TX / 2 * TY; q += L)
+    t2pStep2<S, R, TX, TY>(q, mid, dst, mask, ibbIdx, ibbQ, rhoOut, uxOut, uyOut, pB, x0, y0);
 }
 
 template <class S, class R, int TX, int TY>

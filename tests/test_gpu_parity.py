@@ -200,6 +200,7 @@ def test_komega_graph_steps_match_eager_launches():
                                                (512, 2, 0, 1), (512, 1, 1088, 2), (640, 1, 0, 2),
                                                (832, 1, 64, 2), (1024, 1, 64, 2),
                                                (1536, 1, 0, 2), (1664, 1, 1088, 2),
+                                               (3584, 1, 64, 2),
                                                (1856, 1, 64, 2)])
 def test_launch_shapes_give_identical_results(block, npt, pad, spp):
     """Every collide-stream launch shape (work-group width, one or two nodes
@@ -245,7 +246,7 @@ def test_two_step_pair_kernels_match_one_step_across_tiles(storage):
     ref, _ = make_pair(w, storage)
     ref.set_launch(256, 2, 1088)
     ref.step(41, 2)
-    for block, pad in ((1536, 64), (1024, 0), (1664, 1088)):
+    for block, pad in ((1536, 64), (1024, 0), (1664, 1088)) + (((3584, 64),) if storage != "f64" else ()):
         L, _ = make_pair(w, storage)
         L.set_launch(block, 1, pad, 2)
         L.step(41, 2)
