@@ -285,6 +285,8 @@ def main():
     ap.add_argument("--launch", default="",
                     help="block_x,nodes_per_lane,steps_per_pass,plane_pad for the headline "
                          "form instead of the autotuner's choice (for profiling one kernel)")
+    ap.add_argument("--launch-16bit", default="",
+                    help="the same for the f16 and bf16 forms")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -310,7 +312,11 @@ def main():
     forms = {"p32": r}
     for form in [f for f in args.forms.split(",") if f]:
         if form in ("f16", "bf16"):
-            Lf, nf = setup(w, form)
+            Lf, nf = setup(w, form, tune=not args.launch_16bit)
+            if args.launch_16bit:
+                b, n, sp, pad = (int(v) for v in args.launch_16bit.split(","))
+                Lf.set_launch(b, n, pad, sp)
+                Lf.launch = (b, n, pad, 0.0, sp)
             forms["p" + form[1:] if form == "f16" else "pbf16"] = measure(
                 Lf, nf, args.steps, args.warmup, B_LU[form])
             Lf.close()

@@ -362,8 +362,9 @@ int kofm_ensemble_time_steps(kofm_ensemble* e, int32_t nsteps, uint32_t options,
 int kofm_ensemble_set_block(kofm_ensemble* e, int32_t block_x);
 /* Launch shape: steps_per_launch 1 with block_x 128, 256, 512 or 1024, or
  * 2 (the temporally blocked kernels) with block_x the tile's node count,
- * 832 (64 x 13) or 1664 (64 x 26, two nodes per lane), every member's nX a
- * multiple of 64.  Results are bit-identical for every shape. */
+ * 832 (64 x 13), or with two nodes per lane 1664 (64 x 26), 1536 (128 x 12)
+ * or, with 16- or 32-bit storage, 3584 (128 x 28); every member's nX a
+ * multiple of the tile width.  Results are bit-identical for every shape. */
 int kofm_ensemble_set_launch(kofm_ensemble* e, int32_t block_x, int32_t steps_per_launch);
 /* Members are packed (largest first) into groups whose population arrays
  * (both buffers) total at most `bytes`; kofm_ensemble_step advances each
@@ -374,7 +375,8 @@ int kofm_ensemble_set_launch(kofm_ensemble* e, int32_t block_x, int32_t steps_pe
 int kofm_ensemble_set_cache_budget(kofm_ensemble* e, int64_t bytes, int32_t* groups);
 /* Times `steps` ensemble steps for each launch shape (one step per launch
  * with 256 or 512 wide groups; with k-omega off also the two-step 832 and
- * 1664 tiles when every nX is a multiple of 64) and each cache budget (0, 96
+ * 1664 tiles when every nX is a multiple of 64, and 1536 and 3584 when it
+ * is a multiple of 128) and each cache budget (0, 96
  * and 160 MiB), keeps the fastest and reports it.  Every member's state
  * (populations, and with k-omega k, omega, strain, eddy viscosity and
  * macros) is restored after each trial, so the run continues as if no

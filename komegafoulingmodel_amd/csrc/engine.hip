@@ -1755,16 +1755,21 @@ class EnsembleT final : public Ensemble {
   int setBlock(int block, std::string& err) override { return setLaunch(block, 1, err); }
 
   // One step per launch: block in {128, 256, 512, 1024}; two steps per
-  // launch: the tile's node count, 832 (64 x 13) or 1664 (64 x 26, two nodes
-  // per lane), every member's nX a multiple of 64.
+  // launch: the tile's node count, 832 (64 x 13), or with two nodes per lane
+  // 1664 (64 x 26), 1536 (128 x 12) or, with 16- and 32-bit storage, 3584
+  // (128 x 28); every member's nX a multiple of the tile width.
+  static int tileWidth(int block) { return (block == 1536 || block == 3584) ? 128 : 64; }
   int setLaunch(int block, int spp, std::string& err) override {
     if (spp == 1 && block != 128 && block != 256 && block != 512 && block != 1024)
       return fail(err, "ensemble block width must be 128, 256, 512 or 1024");
     if (spp == 2) {
-      if (block != 832 && block != 1664)
-        return fail(err, "two steps per launch need a tile of 832 or 1664 nodes");
+      if (block != 832 && block != 1664 && block != 1536 && !(block == 3584 && sizeof(S) <= 4))
+        return fail(err, "two steps per launch need a tile of 832, 1664, 1536 or (16- and "
+                         "32-bit storage) 3584 nodes");
       for (auto* e : m_)
-        if (e->nX_ % 64) return fail(err, "two steps per launch need nX % 64 == 0 in every member");
+        if (e->nX_ % tileWidth(block))
+          return fail(err, "two steps per launch need every member's nX a multiple of the tile "
+                           "width (64, or 128 for 1536 and 3584)");
     }
     if (spp != 1 && spp != 2) return fail(err, "steps per launch must be 1 or 2");
     if (ko_ && spp != 1) return fail(err, "k-omega members take one step per launch");
@@ -1839,12 +1844,19 @@ class EnsembleT final : public Ensemble {
       }
       return 0;
     };
-    bool all64 = true;
-    for (auto* e : m_) all64 &= e->nX_ % 64 == 0;
+    bool all64 = true, all128 = true;
+    for (auto* e : m_) {
+      all64 &= e->nX_ % 64 == 0;
+      all128 &= e->nX_ % 128 == 0;
+    }
     std::vector<std::pair<int, int>> shapes = {{512, 1}, {256, 1}};
     if (!ko_ && all64) {
       shapes.push_back({832, 2});
       shapes.push_back({1664, 2});
+    }
+    if (!ko_ && all128) {
+      shapes.push_back({1536, 2});
+      if (sizeof(S) <= 4) shapes.push_back({3584, 2});
     }
     const int64_t budgets[] = {0, int64_t(96) << 20, int64_t(160) << 20};
     bestMs = 1e30;
@@ -2047,7 +2059,7 @@ class EnsembleT final : public Ensemble {
         b.p = e->lbParams(options);
         b.p.applyIbb = eager ? (e->preIbb_ ? 1 : 0) : 1;
         if (two) {
-          const int tx = 64, ty = en.block_ / 64;
+          const int tx = tileWidth(en.block_), ty = en.block_ / tx;
           b.tilesX = e->nX_ / tx;
           b.tileStart = tiles;
           tiles += b.tilesX * ((e->nY_ + ty - 1) / ty);
@@ -2127,7 +2139,14 @@ class EnsembleT final : public Ensemble {
       if (block_ == 1664)
         hipLaunchKernelGGL((dev::lbCollideStreamBatchT2<S, R, 64, 26, true>), grid,
                            dim3(dev::t2pLanes<64, 26>()), 0, stream_, ents, n, totalTiles);
-      else
+      else if (block_ == 1536)
+        hipLaunchKernelGGL((dev::lbCollideStreamBatchT2<S, R, 128, 12, true>), grid,
+                           dim3(dev::t2pLanes<128, 12>()), 0, stream_, ents, n, totalTiles);
+      else if (block_ == 3584) {
+        if constexpr (sizeof(S) <= 4)
+          hipLaunchKernelGGL((dev::lbCollideStreamBatchT2<S, R, 128, 28, true>), grid,
+                             dim3(dev::t2pLanes<128, 28>()), 0, stream_, ents, n, totalTiles);
+      } else
         hipLaunchKernelGGL((dev::lbCollideStreamBatchT2<S, R, 64, 13, false>), grid,
                            dim3(dev::t2Lanes<64, 13>()), 0, stream_, ents, n, totalTiles);
       return;

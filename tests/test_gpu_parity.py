@@ -256,7 +256,8 @@ def test_two_step_pair_kernels_match_one_step_across_tiles(storage):
 
 
 @pytest.mark.parametrize("budget,groups,block,spp", [(0, 1, 512, 1), (2 << 20, 2, 512, 1),
-                                                     (0, 1, 832, 2), (2 << 20, 2, 1664, 2)])
+                                                     (0, 1, 832, 2), (2 << 20, 2, 1664, 2),
+                                                     (0, 1, 1536, 2), (2 << 20, 2, 3584, 2)])
 def test_ensemble_matches_individual_runs(budget, groups, block, spp):
     """Three channels of different sizes and fouling stepped by ensemble
     launches (all in one group, or packed into two cache-sized groups; one
