@@ -622,12 +622,14 @@ __global__ __launch_bounds__((t2Lanes<TX, TY>())) void lbCollideStreamT2(
                        tby * TY, mid1);
 }
 
-// The two-step pass with two neighbouring nodes per lane, for 16-bit
-// storage: population planes are read and written as aligned two-element
-// vectors (4-byte accesses instead of 2-byte ones).  The ring is two columns
-// wide on each side so every region pair starts on an even x; the outer
-// column is collided but its pushes are not needed.  Requires TX even and
-// nX % TX == 0 (so nX is even).
+// The two-step pass with two neighbouring nodes per lane: population planes
+// are read and written as aligned two-element vectors (8-byte accesses for
+// FP32, 4-byte instead of 2-byte ones for 16-bit storage).  The ring is two
+// columns wide on each side so every region pair starts on an even x; the
+// outer column is collided but its pushes are not needed.  Requires TX even
+// and nX % TX == 0 (so nX is even).  The work-group has one lane per pair of
+// the tile and ring, at most 1024; a larger tile takes two rounds of each
+// step.
 template <int TX, int TY>
 constexpr int t2pLanes() {
   return ((TX + 4) / 2 * (TY + 2) + 63) / 64 * 64 < 1024 ? ((TX + 4) / 2 * (TY + 2) + 63) / 64 * 64
