@@ -159,6 +159,46 @@ def measure_ensemble(storage, steps=300):
     return out
 
 
+def measure_precision(name="c3", steps=100_000):
+    """A channel run `steps` steps from the same seeded state with FP64, FP32,
+    FP16 and BF16 populations, each on its autotuned launch shape. For each
+    field, the error of the last step against the FP64 run: the largest
+    difference over fluid nodes (wall nodes and shear links for the wall
+    shear), relative to the largest FP64 magnitude. u is compared as a
+    vector; ux and uy are each relative to max |u|."""
+    from komegafoulingmodel_amd import channel, lib
+
+    w = channel.config(name)
+    runs = {}
+    for st in ("f64", "f32", "f16", "bf16"):
+        t0 = time.perf_counter()
+        L, _ = setup(w, st)
+        L.step(steps - 1)
+        L.step(1, lib.OPTION_SAVE_MACRO_FIELDS)
+        L.wall_shear()
+        fluid = (L.geometry("nType").astype(np.int32) & 1) == 0
+        runs[st] = {"ux": L.download("ux")[fluid], "uy": L.download("uy")[fluid],
+                    "tau_node": L.download("tau_node"), "tau_bl": L.download("tau_bl")}
+        L.close()
+        log(f"prec {name} {st}: {steps} steps in {time.perf_counter() - t0:.1f} s")
+    ref = runs["f64"]
+    umax = float(np.max(np.hypot(ref["ux"], ref["uy"])))
+    out = {"workload": f"{name}: {w.nX}x{w.nY}, k-omega off, seeded state", "steps": steps,
+           "reference": "FP64 populations and arithmetic on the GPU", "max_u_fp64": umax}
+    for st in ("f32", "f16", "bf16"):
+        r = runs[st]
+        out[st] = {
+            "u": float(np.max(np.hypot(r["ux"] - ref["ux"], r["uy"] - ref["uy"]))) / umax,
+            "ux": float(np.max(np.abs(r["ux"] - ref["ux"]))) / umax,
+            "uy": float(np.max(np.abs(r["uy"] - ref["uy"]))) / umax,
+            "tau_node": float(np.max(np.abs(r["tau_node"] - ref["tau_node"])
+                                     / np.max(np.abs(ref["tau_node"])))),
+            "tau_bl": float(np.max(np.abs(r["tau_bl"] - ref["tau_bl"])
+                                   / np.max(np.abs(ref["tau_bl"])))),
+        }
+    return out
+
+
 def measure_ensemble_ko(storage, steps=40):
     """The 48 channels with k-omega on: ensemble steps (one batched
     collide-stream launch and every member's implicit solves captured per
@@ -278,13 +318,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--forms", default="f16,bf16,k32,k16,adj32,adj32_exact,adj32_exact_smooth,adj32_c3,ens32,ens16,ens32ko,ens16ko")
+    ap.add_argument("--forms", default="f16,bf16,prec,k32,k16,adj32,adj32_exact,adj32_exact_smooth,adj32_c3,ens32,ens16,ens32ko,ens16ko")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--launch", default="",
                     help="block_x,nodes_per_lane,steps_per_pass,plane_pad for the headline "
                          "form instead of the autotuner's choice (for profiling one kernel)")
+    ap.add_argument("--prec-steps", type=int, default=100_000,
+                    help="steps of the precision form (errors against FP64 at 8192x1024)")
     ap.add_argument("--launch-16bit", default="",
                     help="the same for the f16 and bf16 forms")
     args = ap.parse_args()
@@ -334,6 +376,10 @@ def main():
             forms[form] = measure_ensemble("f32" if form == "ens32" else "f16")
         elif form in ("ens32ko", "ens16ko"):
             forms[form] = measure_ensemble_ko("f32" if form == "ens32ko" else "f16")
+        elif form == "prec":
+            # the configured lengths: 100,000 steps at 8192x1024 and 20,000 at 1024x128
+            forms["prec"] = measure_precision("c3", args.prec_steps)
+            forms["prec_c1"] = measure_precision("c1", max(1, args.prec_steps // 5))
         elif form == "adj32":
             # the 2048x256 k-omega sensitivity workload; its k-omega transient
             # takes k and omega through the clamps, where the exact tangent
