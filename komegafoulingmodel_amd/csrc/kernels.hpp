@@ -239,11 +239,38 @@ __device__ __forceinline__ Moments<R> moments(const R (&g)[9]) {
   return m;
 }
 
-template <class R, bool KO>
+// Moments summed in FP64 and rounded to R once.  The FP32 form uses them:
+// with FP32 sums the rounding of the moments fed the equilibrium a small
+// systematic momentum excess (the bulk velocity of a forced channel ran
+// ahead of FP64 by about 1e-8 of itself per step).
+template <class R>
+__device__ __forceinline__ Moments<R> momentsWide(const R (&g)[9]) {
+  double d[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) d[i] = double(g[i]);
+  const double drho = ((d[0] + d[1]) + (d[2] + d[3])) + ((d[4] + d[5]) + (d[6] + d[7])) + d[8];
+  const double rho = 1.0 + drho;
+  const double r0 = __builtin_amdgcn_rcp(rho);
+  const double inv = fma(fma(-rho, r0, 1.0), r0, r0);  // one Newton step
+  Moments<R> m;
+  m.drho = R(drho);
+  m.rho = R(rho);
+  m.inv = R(inv);
+  m.ux = R(inv * (((d[1] - d[2]) + (d[5] - d[6])) + (d[7] - d[8])));
+  m.uy = R(inv * (((d[3] - d[4]) + (d[5] - d[6])) - (d[7] - d[8])));
+  return m;
+}
+// FP64 moment sums for FP32 storage and arithmetic (the 16-bit forms keep
+// FP32 sums: their error is set by the storage, and their kernels are
+// bound by instruction issue)
+template <class S, class R>
+constexpr bool kWideMoments = __is_same(S, float) && __is_same(R, float);
+
+template <class R, bool KO, bool WIDE = false>
 __device__ __forceinline__ void collideNode(R (&g)[9], size_t gi, R* __restrict__ rhoOut,
                                             R* __restrict__ uxOut, R* __restrict__ uyOut,
                                             const KOFields& ko, const LBParams<R>& p) {
-  const Moments<R> mo = moments(g);
+  const Moments<R> mo = WIDE ? momentsWide(g) : moments(g);
   const R drho = mo.drho, rho = mo.rho, ux = mo.ux, uy = mo.uy;
 
   R omega = p.omega0;
@@ -330,7 +357,7 @@ __global__ __launch_bounds__(BX) void lbCollideStream(
 
   R g[9];
   loadNode<S, R>(g, src, gi, x, y, m, ibbIdx, ibbQ, p);
-  collideNode<R, KO>(g, gi, rhoOut, uxOut, uyOut, ko, p);
+  collideNode<R, KO, kWideMoments<S, R>>(g, gi, rhoOut, uxOut, uyOut, ko, p);
 
   const int xe = x < p.nX - 1 ? x + 1 : 0;
   const int xw = x > 0 ? x - 1 : p.nX - 1;
@@ -433,7 +460,7 @@ __global__ __launch_bounds__(BX) void lbCollideStream2(
     if (!live[n]) continue;
     if (p.applyIbb && (m[n] & kIbbNode))
       ibbCorrect<S, R>(g[n], src, x0 + n, y, ibbQ + 8 * size_t(ibbIdx[g0 + n]), p);
-    collideNode<R, KO>(g[n], g0 + n, rhoOut, uxOut, uyOut, ko, p);
+    collideNode<R, KO, kWideMoments<S, R>>(g[n], g0 + n, rhoOut, uxOut, uyOut, ko, p);
   }
 
 #pragma unroll
@@ -543,7 +570,7 @@ __device__ __forceinline__ void t2Tile(const S* __restrict__ src, S* __restrict_
     if (m & kSolid) continue;
     R g[9];
     loadNode<S, R>(g, src, gi, x, y, m, ibbIdx, ibbQ, pA);
-    collideNode<R, false>(g, gi, rhoOut, uxOut, uyOut, none, pA);
+    collideNode<R, false, kWideMoments<S, R>>(g, gi, rhoOut, uxOut, uyOut, none, pA);
     if (mid1 && rx >= 1 && rx <= TX && ry >= 1 && ry <= TY) {
       const int xe = x < pA.nX - 1 ? x + 1 : 0;
       const int xw = x > 0 ? x - 1 : pA.nX - 1;
@@ -599,7 +626,7 @@ __device__ __forceinline__ void t2Tile(const S* __restrict__ src, S* __restrict_
     if (m & kIbbNode)
       ibbCorrectWith<R>(g, ibbQ + 8 * size_t(ibbIdx[gi]),
                         [&](int d) { return lds(rev(d), r - cx(d) - RX * cy(d)); });
-    collideNode<R, false>(g, gi, rhoOut, uxOut, uyOut, none, pB);
+    collideNode<R, false, kWideMoments<S, R>>(g, gi, rhoOut, uxOut, uyOut, none, pB);
     const int xe = x < pB.nX - 1 ? x + 1 : 0;
     const int xw = x > 0 ? x - 1 : pB.nX - 1;
   #pragma unroll
@@ -717,7 +744,7 @@ __device__ __forceinline__ void t2pStep1(const T2PIn<S>& in, int q,
   const int c0 = (rx0 + 2) + LX * (ry + 1);  // even
   if (__all(((m[0] | m[1]) & kNotPlain) == 0)) {
 #pragma unroll
-    for (int n = 0; n < 2; n++) collideNode<R, false>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pA);
+    for (int n = 0; n < 2; n++) collideNode<R, false, kWideMoments<S, R>>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pA);
 #pragma unroll
     for (int i = 0; i < 9; i++) {
       const int t = c0 + cx(i) + LX * cy(i);
@@ -738,7 +765,7 @@ __device__ __forceinline__ void t2pStep1(const T2PIn<S>& in, int q,
     if (m[n] & kSolid) continue;
     if (pA.applyIbb && (m[n] & kIbbNode))
       ibbCorrect<S, R>(g[n], src, in.xa + n, in.y, ibbQ + 8 * size_t(ibbIdx[g0 + n]), pA);
-    collideNode<R, false>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pA);
+    collideNode<R, false, kWideMoments<S, R>>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pA);
     const int r = c0 + n;
 #pragma unroll
     for (int i = 0; i < 9; i++) {
@@ -777,7 +804,7 @@ __device__ __forceinline__ void t2pStep2(int q, const S (*__restrict__ mid)[T2PS
     for (int i = 0; i < 9; i++)
       fromStoredPair<S, R>(*reinterpret_cast<const Pair<S>*>(&mid[i][c0]), i, g[0][i], g[1][i]);
 #pragma unroll
-    for (int n = 0; n < 2; n++) collideNode<R, false>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pB);
+    for (int n = 0; n < 2; n++) collideNode<R, false, kWideMoments<S, R>>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pB);
     // element offsets of the pair's column in rows y - 1, y, y + 1 (32-bit:
     // a population array is below 4 GiB)
     const uint32_t row[3] = {uint32_t(g0) - uint32_t(pB.XF), uint32_t(g0),
@@ -808,7 +835,7 @@ __device__ __forceinline__ void t2pStep2(int q, const S (*__restrict__ mid)[T2PS
     if (m[n] & kIbbNode)
       ibbCorrectWith<R>(g[n], ibbQ + 8 * size_t(ibbIdx[g0 + n]),
                         [&](int d) { return lds(rev(d), r - cx(d) - LX * cy(d)); });
-    collideNode<R, false>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pB);
+    collideNode<R, false, kWideMoments<S, R>>(g[n], g0 + n, rhoOut, uxOut, uyOut, none, pB);
   }
 #pragma unroll
   for (int i = 0; i < 9; i++) {
@@ -920,7 +947,7 @@ __global__ __launch_bounds__(BX) void lbCollideStreamBatch(const BatchEntry<S, R
   if (m & kSolid) return;
   R g[9];
   loadNode<S, R>(g, e.src, gi, x, y, m, e.ibbIdx, e.ibbQ, p);
-  collideNode<R, KO>(g, gi, e.rho, e.ux, e.uy, e.ko, p);
+  collideNode<R, KO, kWideMoments<S, R>>(g, gi, e.rho, e.ux, e.uy, e.ko, p);
   const int xe = x < p.nX - 1 ? x + 1 : 0;
   const int xw = x > 0 ? x - 1 : p.nX - 1;
 #pragma unroll
