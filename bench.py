@@ -334,7 +334,7 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the lattice step runs on a single MI355X)", file=sys.stderr)
         sys.exit(2)
 
