@@ -136,6 +136,28 @@ def test_interpolated_bounce_back_wall_position(delta):
     assert np.max(np.abs(u_bb - u_an)) / np.max(u_an) > 2.5 * err
 
 
+def test_wall_shear_balances_the_body_force_in_steady_poiseuille():
+    """Steady plane Poiseuille flow, flat walls on the halfway planes: the
+    shear stress is linear across the channel, rho F (H/2 - d) at distance d
+    from a wall, so every wall node (d = 1/2) reports F (H/2 - 1/2)
+    (TR_shear_1, trKernelsShear.cl:174-222, divides by rho), with the drag
+    downstream (positive) on both walls.  The boundary-link values
+    (TR_shear_2, :300-316) are normalised weighted means of equal node
+    values, so they equal it too.  A wrong scale (the nu_e / (3 nu_e + 1/2)
+    factor or the 3 of the CFSS coefficients), normal or sign fails this."""
+    nX, nY, F = 64, 20, 2e-5
+    w = channel.flat_channel(nX, nY, Fval=F, noise=0.0)
+    g, orc = _lattice(w)
+    D = g["D"]
+    orc.init_state(np.ones(D), np.zeros(D), np.zeros(D))
+    orc.step(20000)
+    tn, tb = orc.shear()
+    expect = F * ((nY - 2.0) / 2 - 0.5)
+    assert len(tn) > 100 and len(tb) > 50
+    assert np.allclose(tn, expect, rtol=1e-9)
+    assert np.allclose(tb, expect, rtol=1e-9)
+
+
 def test_fd_coefficients_exact_for_quadratics():
     """The nonuniform three-point first and second derivatives
     (kOmegaKernels.cl:37-55, 235-257) are exact for quadratics; beyond the
