@@ -255,20 +255,21 @@ def test_two_step_pair_kernels_match_one_step_across_tiles(storage):
         L.close()
 
 
-@pytest.mark.parametrize("budget,groups,block,spp", [(0, 1, 512, 1), (2 << 20, 2, 512, 1),
-                                                     (0, 1, 832, 2), (2 << 20, 2, 1664, 2),
-                                                     (0, 1, 1536, 2), (2 << 20, 2, 3584, 2)])
-def test_ensemble_matches_individual_runs(budget, groups, block, spp):
+@pytest.mark.parametrize("budget,groups,block,spp,storage", [
+    (0, 1, 512, 1, "f32"), (2 << 20, 2, 512, 1, "f32"), (0, 1, 832, 2, "f32"),
+    (2 << 20, 2, 1664, 2, "f32"), (0, 1, 1536, 2, "f32"), (2 << 20, 2, 3584, 2, "f32"),
+    (0, 1, 1536, 2, "f16"), (1 << 20, 2, 3584, 2, "f16"), (0, 1, 1664, 2, "bf16")])
+def test_ensemble_matches_individual_runs(budget, groups, block, spp, storage):
     """Three channels of different sizes and fouling stepped by ensemble
     launches (all in one group, or packed into two cache-sized groups; one
     step per launch, or two with one or two nodes per lane) end bit-identical
-    to the same channels stepped alone."""
+    to the same channels stepped alone, for FP32 and 16-bit storage."""
     from komegafoulingmodel_amd import lib
     ws = [channel.fouled_channel(256, 48, Fval=1e-5, seed=31, amp=(0, 3), width=(10, 40)),
           channel.fouled_channel(384, 64, Fval=2e-5, seed=32, amp=(0, 3), width=(10, 40)),
           channel.flat_channel(128, 24, Fval=1e-5, seed=33)]
-    solo = [make_pair(w, "f32")[0] for w in ws]
-    group = [make_pair(w, "f32")[0] for w in ws]
+    solo = [make_pair(w, storage)[0] for w in ws]
+    group = [make_pair(w, storage)[0] for w in ws]
     ens = lib.Ensemble(group)
     assert ens.set_cache_budget(budget) == groups
     ens.set_launch(block, spp)
