@@ -446,6 +446,9 @@ int orc_geometry(int nX, int XF, int nY, const double *C0, int nN, const int32_t
                  int16_t *nType, double *dX, int32_t *ssArr, int32_t *ssIndMap,
                  int32_t *ssInds, int *nSS, uint16_t *lsMap, double *WallD, int32_t *ibb,
                  double *ibbq, int ibbCap, int *nIbb) {
+  /* ssArrInds is indexed up to ssMaxX (TR_Find_Shear_Coeffs2), so the shear
+   * window has to end inside the lattice */
+  if (ssMinX < 0 || ssMaxX > nX - 1) return -5;
   int rc = fill_map0(nX, XF, nY, C0, BL, nBL, nType);
   if (rc) return rc;
   rc = boundary_info(nX, XF, nY, nType, 0);
@@ -1074,16 +1077,38 @@ double orc_reduce(const double *a, long n, int op) {
 /* Whole steps                                                               */
 /* ------------------------------------------------------------------------ */
 
+/* Solver settings and the k-omega time step control state (kOmega.h:22,
+ * 204-205): sub-steps per LB step, their time step and the steps since the
+ * last change.  inject_nan makes the next solve report a non-finite residual
+ * (a test of the restart path without a non-finite value in any array);
+ * halvings counts the restarts. */
 typedef struct {
   double rtol, atol;
   int maxit;
   int last_iters_k, last_iters_o;
+  int steps_per_lb;
+  double dt_cur;
+  int since_change;
+  int inject_nan;
+  int halvings;
 } orc_solver;
 
-/* One reference time step (clVariablesLB::Solve clVariablesLB.h:375-441,
- * kOmega::Solve kOmega.cpp:31-73 with one sub-step): collide+stream FA->FB,
- * IBB on FB, then (k-omega on) diffusivities, coefficients, omega solve,
- * k solve.  The caller swaps FA/FB.  work: 21*D doubles. */
+#define ORC_DT_QUIET 20          /* kOmega.h:22 timeBtwIncreaseTimeStep */
+#define ORC_MAX_STEPS_PER_LB (1 << 16)
+
+/* One reference time step (clVariablesLB::Solve clVariablesLB.h:375-441):
+ * collide+stream FA->FB, IBB on FB, then (k-omega on) kOmega::Solve
+ * (kOmega.cpp:31-73): the time step doubles back after ORC_DT_QUIET steps
+ * without a change (doubleTimeStep :401-414), k and omega are kept
+ * (copyToPrevSolution), and each of the steps_per_lb sub-steps runs the
+ * diffusivities, the coefficients with the current time step, the omega
+ * solve and the k solve.  A non-finite residual restores k and omega, halves
+ * the time step and doubles the sub-steps (halveTimeStep :387-399 via
+ * GMRESSolver::checkConvergence, GMRESSolver.cpp:1535-1546); the sub-steps
+ * then start again from the first.  (The reference resumes its loop at the
+ * second sub-step after running the k solve on rows assembled with the old
+ * time step; here the restart is clean.)  The caller swaps FA/FB.  work:
+ * 23*D doubles. */
 int orc_step(const orc_par *p, const int16_t *nType, const int32_t *ibb, const double *q,
              int nIbb, const double *FA, double *FB, double *rho, double *ux, double *uy,
              double *K, double *O, const double *WallD, double *Nut, double *Sxy,
@@ -1094,15 +1119,49 @@ int orc_step(const orc_par *p, const int16_t *nType, const int32_t *ibb, const d
   if (!komega) return 0;
   double *Dk = work, *Do = work + D, *F1 = work + 2 * D, *dkdo = work + 3 * D;
   double *Ak = work + 5 * D, *Ao = Ak + 4 * D, *bk = Ao + 4 * D, *bo = bk + D;
-  double *sw = bo + D; /* solver work, 6*D: work must hold 21*D doubles */
-  orc_ko_diffusivities(p, nType, K, O, Nut, WallD, Dk, Do, F1, dkdo, rho, dX);
-  orc_ko_coeffs(p, nType, K, O, Nut, Dk, Do, F1, dkdo, Sxy, rho, ux, uy, dX, WallD, Ak, Ao,
-                bk, bo);
-  int io = orc_ko_solve(p, nType, Ao, bo, O, sv->rtol, sv->atol, sv->maxit, 0, 0, sw);
-  int ik = orc_ko_solve(p, nType, Ak, bk, K, sv->rtol, sv->atol, sv->maxit, 0, 0, sw);
-  sv->last_iters_o = io;
-  sv->last_iters_k = ik;
-  return (io < 0 || ik < 0) ? -1 : 0;
+  double *sw = bo + D; /* solver work, 6*D */
+  double *Kp = sw + 6 * D, *Op = Kp + D; /* restart point: work must hold 23*D doubles */
+  if (sv->steps_per_lb < 1) {
+    sv->steps_per_lb = 1;
+    sv->dt_cur = p->dt;
+  }
+  if (++sv->since_change >= ORC_DT_QUIET) {
+    sv->since_change = 0;
+    if (sv->steps_per_lb > 1) {
+      sv->steps_per_lb /= 2;
+      sv->dt_cur *= 2.;
+    }
+  }
+  memcpy(Kp, K, sizeof(double) * D);
+  memcpy(Op, O, sizeof(double) * D);
+  orc_par pp = *p;
+  for (int sub = 0; sub < sv->steps_per_lb;) {
+    pp.dt = sv->dt_cur;
+    orc_ko_diffusivities(&pp, nType, K, O, Nut, WallD, Dk, Do, F1, dkdo, rho, dX);
+    orc_ko_coeffs(&pp, nType, K, O, Nut, Dk, Do, F1, dkdo, Sxy, rho, ux, uy, dX, WallD, Ak, Ao,
+                  bk, bo);
+    int io = orc_ko_solve(&pp, nType, Ao, bo, O, sv->rtol, sv->atol, sv->maxit, 0, 0, sw);
+    int ik = io < 0 ? -1 : orc_ko_solve(&pp, nType, Ak, bk, K, sv->rtol, sv->atol, sv->maxit, 0, 0, sw);
+    if (sv->inject_nan) {
+      sv->inject_nan = 0;
+      io = -1;
+    }
+    sv->last_iters_o = io;
+    sv->last_iters_k = ik;
+    if (io < 0 || ik < 0) {
+      if (sv->steps_per_lb >= ORC_MAX_STEPS_PER_LB) return -1;
+      memcpy(K, Kp, sizeof(double) * D);
+      memcpy(O, Op, sizeof(double) * D);
+      sv->steps_per_lb *= 2;
+      sv->dt_cur /= 2.;
+      sv->since_change = 0;
+      sv->halvings++;
+      sub = 0;
+      continue;
+    }
+    sub++;
+  }
+  return 0;
 }
 
 /* n steps with FA/FB ping-pong; on return the newest populations are in FA. */

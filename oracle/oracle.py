@@ -25,7 +25,9 @@ class OrcPar(C.Structure):
 
 class OrcSolver(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("maxit", C.c_int),
-                ("last_iters_k", C.c_int), ("last_iters_o", C.c_int)]
+                ("last_iters_k", C.c_int), ("last_iters_o", C.c_int),
+                ("steps_per_lb", C.c_int), ("dt_cur", C.c_double), ("since_change", C.c_int),
+                ("inject_nan", C.c_int), ("halvings", C.c_int)]
 
 
 def build():
@@ -123,7 +125,7 @@ class OracleLattice:
         self.g = geo
         nX, nY, XF, D = geo["nX"], geo["nY"], geo["XF"], geo["D"]
         self.par = OrcPar(nX, nY, XF, tau0, Fval, nu, max_turb_visc, dt, 0x2, threads)
-        self.sv = OrcSolver(rtol, atol, maxit, 0, 0)
+        self.sv = OrcSolver(rtol, atol, maxit, 0, 0, max(1, int(1.0 / dt)), dt, 0, 0, 0)
         self.komega = komega
         self.D = D
         self.FA = np.zeros(9 * D)
@@ -135,7 +137,7 @@ class OracleLattice:
         self.O = np.zeros(D)
         self.nut = np.zeros(D)
         self.S = np.zeros(3 * D)
-        self.work = np.zeros(21 * D)
+        self.work = np.zeros(23 * D)
         self.dX = np.ascontiguousarray(geo["dX"])
         self.wallD = np.ascontiguousarray(geo["wallD"])
         self.nType = geo["nType"]

@@ -107,11 +107,15 @@ def test_startup_poiseuille_series():
     assert np.max(np.abs(u_lb - u_an)) / np.max(u_an) < 6e-3
 
 
-@pytest.mark.parametrize("delta", [0.15, 0.35])
+@pytest.mark.parametrize("delta", [0.15, 0.35, -0.2, -0.35])
 def test_interpolated_bounce_back_wall_position(delta):
     """Steady Poiseuille flow with walls off the halfway plane: Bouzidi
     interpolation places the wall where it is (error ~0.5%), plain halfway
-    bounce-back would be off by ~2 delta / H (>2%)."""
+    bounce-back would be off by ~2 delta / H (>2%).  delta > 0 gives
+    q = 0.15, 0.35 (the 2q <= 1 branch of lbIBB, lbKernels.cl:640-668,
+    interpolating with the node's own incoming value); delta < 0 gives
+    q = 0.7, 0.85 (the 2q > 1 branch, interpolating with the value streamed
+    back to the upstream neighbour)."""
     nX, nY, F = 64, 26, 2e-5
     yb, yt = 0.5 + delta, nY - 1.5 - delta
     C0, BL = channel.wall_points(nX, lambda x: np.full_like(x, yb), lambda x: np.full_like(x, yt))
@@ -323,3 +327,350 @@ def test_ibb_links_brute_force():
         assert best is not None and abs(best - q) < 1e-9, (x, y, d, q, best)
         nb = ((x + CX[d]) % w.nX) + XF * (y + CY[d])
         assert g["nType"][nb] & 1, "the link must point into a solid node"
+
+
+def _clamp_free_ko():
+    nu, Re, H = 0.01, 500.0, 56.0
+    return channel.wavy_channel(128, 72, H=H, amp=0.0, period=128.0, nu=nu, Re=Re,
+                                Fval=channel.turbulent_forcing(H, nu, Re), seed=7)
+
+
+def _ko_pair():
+    w = _clamp_free_ko()
+    out = []
+    for _ in range(2):
+        _, orc = _lattice(w, rtol=1e-15, maxit=2000)
+        rho, ux, uy = w.initial_velocity(orc.nType)
+        orc.init_state(rho, ux, uy)
+        orc.init_komega(w.k_ini, w.omega_ini)
+        out.append(orc)
+    return out
+
+
+def test_komega_time_step_halving_restarts_the_step():
+    """kOmega::halveTimeStep (kOmega.cpp:387-399) on a non-finite residual:
+    the step that saw it ends exactly as a step that started with twice the
+    sub-steps at half the time step (k and omega restored from the start of
+    the step, nothing else kept from the failed attempt).  20 steps later the
+    time step doubles back (doubleTimeStep, kOmega.cpp:31-36, 401-414)."""
+    a, b = _ko_pair()
+    a.step(2)
+    b.step(2)
+    a.sv.inject_nan = 1
+    b.sv.steps_per_lb, b.sv.dt_cur, b.sv.since_change = 2, 0.5, -1
+    a.step(1)
+    b.step(1)
+    assert (a.sv.halvings, a.sv.steps_per_lb, a.sv.dt_cur) == (1, 2, 0.5)
+    for f in ("K", "O", "nut", "FA", "ux"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    # a step with two half steps differs from one full step
+    c, _ = _ko_pair()
+    c.step(3)
+    assert not np.array_equal(c.K, a.K)
+    a.step(19)
+    assert (a.sv.steps_per_lb, a.sv.dt_cur) == (2, 0.5)
+    a.step(1)
+    assert (a.sv.steps_per_lb, a.sv.dt_cur, a.sv.halvings) == (1, 1.0, 1)
+
+
+def test_strain_rate_output_in_steady_poiseuille():
+    """The strain rate the collision writes (lbKernels.cl:518-583:
+    S_ab = -1.5 omega0 / rho sum_i c_ia c_ib (f_i - feq_i(u)) from the
+    pre-collision populations) is the symmetric velocity gradient: in steady
+    plane Poiseuille flow u = F/(2 nu) (y - yb)(yt - y), S_xy = du/dy / 2 =
+    F/(4 nu) (yt + yb - 2y), and S_yy = 0.  S_xx carries only the O(F u)
+    part that the forcing leaves in the pre-collision non-equilibrium (three
+    orders below S_xy here).  A wrong factor (1.5, omega0, 1/rho), a sign or
+    the wrong moment fails this."""
+    nX, nY, F = 64, 24, 2e-5
+    w = channel.flat_channel(nX, nY, Fval=F, noise=0.0)
+    w.komega, w.k_ini, w.omega_ini = True, 1e-12, 1.0  # S is written by the k-omega collision
+    g, orc = _lattice(w, rtol=1e-14, maxit=400)
+    D = g["D"]
+    orc.init_state(np.ones(D), np.zeros(D), np.zeros(D))
+    orc.init_komega(1e-12, 1.0)
+    orc.step(15000)
+    nu = w.nu
+    yb, yt = 0.5, nY - 1.5
+    y = np.arange(1, nY - 1)
+    S = orc.S.reshape(3, nY, w.XF)[:, 1:nY - 1, :nX]
+    expect = F / (4 * nu) * (yt + yb - 2 * y)
+    smax = np.max(np.abs(expect))
+    err = np.max(np.abs(S[1] - expect[:, None])) / smax
+    print(f"strain: S_xy rel err {err:.2e}, |S_xx| {np.abs(S[0]).max():.2e}, |S_yy| {np.abs(S[2]).max():.2e}")
+    assert err < 1e-10, err
+    umax = F / (2 * nu) * ((yt - yb) / 2) ** 2
+    assert np.max(np.abs(S[0])) < 10 * w.tau0 * F * umax and np.max(np.abs(S[2])) < 1e-12 * smax
+    # the eddy viscosity stays at its floor-free value k/omega = 1e-12, so the flow is laminar
+    assert np.max(orc.nut.reshape(nY, w.XF)[1:nY - 1, :nX]) < 1e-11
+
+
+def _ko_rows(orc, S):
+    """Diffusivities and the assembled rows (orc_ko_diffusivities,
+    orc_ko_coeffs) for the lattice's current fields and a given strain S."""
+    import ctypes as C
+    lib = oracle.load()
+    p = oracle.p
+    D = orc.D
+    Dk, Do, F1, dkdo = np.zeros(D), np.zeros(D), np.zeros(D), np.zeros(2 * D)
+    lib.orc_ko_diffusivities(C.byref(orc.par), p(orc.nType), p(orc.K), p(orc.O), p(orc.nut),
+                             p(orc.wallD), p(Dk), p(Do), p(F1), p(dkdo), p(orc.rho), p(orc.dX))
+    Ak, Ao, bk, bo = np.zeros(4 * D), np.zeros(4 * D), np.zeros(D), np.zeros(D)
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    lib.orc_ko_coeffs(C.byref(orc.par), p(orc.nType), p(orc.K), p(orc.O), p(orc.nut), p(Dk),
+                      p(Do), p(F1), p(dkdo), p(S), p(orc.rho), p(orc.ux), p(orc.uy), p(orc.dX),
+                      p(orc.wallD), p(Ak), p(Ao), p(bk), p(bo))
+    return dict(Dk=Dk, Do=Do, F1=F1, Ak=Ak.reshape(4, D), Ao=Ao.reshape(4, D), bk=bk, bo=bo)
+
+
+@pytest.mark.parametrize("sxx,sxy,syy", [(0.0, 1e-3, 0.0), (1e-3, 0.0, -1e-3), (1e-3, 0.0, 1e-3),
+                                         (-1e-3, 0.0, -1e-3), (0.0, 1e-2, 0.0)])
+def test_komega_production_rows_under_a_prescribed_strain(sxx, sxy, syy):
+    """Fluid at rest with uniform k, omega and a uniform prescribed strain:
+    the advection and diffusion parts of an interior row cancel for a uniform
+    field, so the uniform solution of the row is
+      k' = (k + dt P_k) / (1 + beta* omega),
+      omega' = (omega + dt gamma P_k) / (1 + beta* omega),
+    with P_k = 2 nu_t (Sxx^2 + Syy^2 + 2 Sxy^2) - 2/3 k rho (Sxx + Syy) clipped
+    to [0, 10 beta* k omega] and gamma = (F1 gamma1 + (1 - F1) gamma2) / nu_t,
+    gamma1 = 5/9, gamma2 = 0.44 (kOmegaKernels.cl:187-480, Heading.cl:398-410).
+    The cases cover pure shear, pure normal strain, a dilatation with either
+    sign (the -2/3 k div u term decides, one case clips at zero) and a strain
+    large enough for the 10 beta* k omega limit."""
+    k0, w0 = 1e-4, 1e-2
+    wl = _komega_channel(nY=64, k0=k0, w0=w0)
+    g, orc = _lattice(wl)
+    D = g["D"]
+    orc.init_state(np.ones(D), np.zeros(D), np.zeros(D))
+    orc.init_komega(k0, w0)
+    fluid = (g["nType"].astype(np.int32) & 1) == 0
+    S = np.zeros(3 * D)
+    for m, v in enumerate((sxx, sxy, syy)):
+        S[m * D:(m + 1) * D] = np.where(fluid, v, 0.0)
+    r = _ko_rows(orc, S)
+    nut = k0 / w0
+    P = 2 * nut * (sxx ** 2 + syy ** 2 + 2 * sxy ** 2) - 2.0 / 3.0 * k0 * (sxx + syy)
+    P = min(max(P, 0.0), 10 * 0.09 * k0 * w0)
+    rows = [j * wl.XF + i for j in range(wl.nY // 2 - 5, wl.nY // 2 + 5) for i in range(64)]
+    for n in rows:
+        kk = r["bk"][n] / (1.0 + r["Ak"][:, n].sum())
+        ww = r["bo"][n] / (1.0 + r["Ao"][:, n].sum())
+        F1 = r["F1"][n]
+        gam = (F1 * 5.0 / 9.0 + (1.0 - F1) * 0.44) / nut
+        assert abs(kk - (k0 + P) / (1 + 0.09 * w0)) < 1e-13 * k0, (n, kk)
+        assert abs(ww - (w0 + gam * P) / (1 + 0.09 * w0)) < 1e-13 * w0, (n, ww)
+    if (sxx, sxy, syy) == (0.0, 1e-2, 0.0):
+        assert P == 10 * 0.09 * k0 * w0  # the limit acts
+    if (sxx, sxy, syy) == (1e-3, 0.0, 1e-3):
+        assert P == 0.0  # the dilatation term outweighs the shear term
+
+
+def test_komega_transport_of_a_sinusoidal_k():
+    """A sinusoidal k(x) = k0 (1 + eps sin(theta x)) carried by a uniform
+    flow u = U (omega uniform, no strain): to first order in eps one implicit
+    step multiplies the Fourier mode by the discrete symbol of the k row,
+      1 / (1 + beta* omega + dt (i U sin(theta) + 2 D0 (1 - cos(theta)))),
+    central differences on the unit lattice, D0 = nu + (0.85 F1 + (1 - F1)) nu_t
+    (kOmegaKernels.cl:7-95, 187-480).  The amplitude and phase of the mode at
+    mid-height are checked; a dropped or mis-signed advection term, a one-sided
+    difference, a wrong diffusivity or a missing sink fails this."""
+    k0, w0, U, eps = 1e-4, 1e-2, 0.05, 1e-5
+    nX, nY, per = 64, 64, 16
+    C0, BL = channel.wall_points(nX, lambda x: np.full_like(x, 0.5),
+                                 lambda x: np.full_like(x, nY - 1.5))
+    nu = 0.01
+    wl = channel.Workload("ko", nX, nY, C0, BL, pipe_radius=(nY - 2) / 2, tau0=1 / (3 * nu + 0.5),
+                          nu=nu, Fval=0.0, komega=True, k_ini=k0, omega_ini=w0)
+    g, orc = _lattice(wl, rtol=1e-15, maxit=2000)
+    D, XF = g["D"], g["XF"]
+    fluid = (g["nType"].astype(np.int32) & 1) == 0
+    orc.init_state(np.ones(D), np.where(fluid, U, 0.0), np.zeros(D))
+    orc.init_komega(k0, w0)
+    th = 2 * math.pi / per
+    x = np.arange(D) % XF
+    orc.K[:] = np.where(fluid, k0 * (1.0 + eps * np.sin(th * x)), 0.0)
+    orc.nut[:] = np.where(fluid, orc.K / w0, 0.0)
+    orc.step(1)
+    F1 = orc.work[2 * D:3 * D]
+    mid = nY // 2
+    row = slice(mid * XF, mid * XF + nX)
+    nut0 = k0 / w0
+    D0 = nu + (0.85 * F1[row].mean() + (1 - F1[row].mean())) * nut0
+    sym = 1.0 / (1.0 + 0.09 * w0 + (1j * U * math.sin(th) + 2 * D0 * (1 - math.cos(th))))
+    kr = orc.K[row]
+    # k - mean = Im(A e^{i theta x}) = (A e^{i theta x} - conj) / 2i: the DFT
+    # coefficient at theta is A / 2i, so A = 2i * (sum / nX)
+    A = 2j * np.sum((kr - kr.mean()) * np.exp(-1j * th * np.arange(nX))) / nX
+    expect = eps * k0 * sym
+    assert abs(kr.mean() - k0 / (1 + 0.09 * w0)) < 1e-6 * k0
+    err = abs(A - expect) / abs(expect)
+    print(f"transport mode {A:.6e} expected {expect:.6e} rel err {err:.2e}")
+    assert err < 1e-5, err
+    got = A
+    # the row's imaginary part is the advection: a mis-signed U would flip the phase
+    assert np.sign(np.angle(got)) == np.sign(np.angle(expect)) and abs(np.angle(expect)) > 1e-2
+
+
+def _seg_dist(p, a, b):
+    """Euclidean distance from point p to segment ab."""
+    t = b - a
+    s = np.clip(np.dot(p - a, t) / np.dot(t, t), 0.0, 1.0)
+    return float(np.linalg.norm(p - (a + s * t)))
+
+
+def _sawtooth_channel(nX=96, nY=64):
+    """Bottom wall of +-45 degree segments (kinks on wall points every 24
+    nodes), flat top wall."""
+    def bottom(x):
+        u = np.mod(np.asarray(x) - 0.25, 48.0)
+        return 8.37 + 24.0 - np.abs(u - 24.0)
+    C0, BL = channel.wall_points(nX, bottom, lambda x: np.full_like(x, nY - 1.5))
+    return C0, BL
+
+
+def test_shear_normal_on_45_degree_walls():
+    """TR_Find_Shear_Coeffs1 (trKernelsShear.cl:332-378) on a +-45 degree
+    wall: every wall node of a straight diagonal stretch has solid
+    neighbours to the south and to the east (rising wall) or west (falling
+    wall), so by symmetry the flag-sum normal is (1, -1)/sqrt2 or
+    (-1, -1)/sqrt2 (pointing into the wall), and
+    CFSS_a = 3 (c.n)(c_a - n_a (c.n)) for each moving direction."""
+    nX, nY = 96, 64
+    C0, BL = _sawtooth_channel(nX, nY)
+    g = oracle.geometry(nX, nY, C0, BL, x_release_pos=2.0, x_stop_pos=nX - 12.0)
+    XF = g["XF"]
+    cf = g["cfss"].reshape(-1, 16)
+    checked = 0
+    for ii, node in enumerate(g["ss"]):
+        x, y = node % XF, node // XF
+        if y > nY // 2:
+            continue
+        u = (x - 0.25) % 48.0
+        if 6.0 <= u <= 18.0:
+            n = np.array([1.0, -1.0]) / math.sqrt(2)
+        elif 30.0 <= u <= 42.0:
+            n = np.array([-1.0, -1.0]) / math.sqrt(2)
+        else:
+            continue
+        s = CX[1:] * n[0] + CY[1:] * n[1]
+        assert np.allclose(cf[ii, :8], 3 * s * (CX[1:] - n[0] * s), rtol=0, atol=1e-14), (x, y)
+        assert np.allclose(cf[ii, 8:], 3 * s * (CY[1:] - n[1] * s), rtol=0, atol=1e-14), (x, y)
+        checked += 1
+    assert checked > 40
+
+
+@pytest.mark.parametrize("shape", ["fouled", "sawtooth"])
+def test_shear_link_neighbours_and_weights_brute_force(shape):
+    """TR_Find_Shear_Coeffs2 (trKernelsShear.cl:414-496): for each shear
+    boundary link, the (at most) four wall nodes nearest to the link's
+    midpoint within the cutoff radius, found here by a search over every wall
+    node, with Gaussian weights exp(-(d/cutoff)^2 / 2) normalised over the
+    four, both listed in wall-node order."""
+    if shape == "fouled":
+        w = channel.fouled_channel(128, 40, seed=21, amp=(0, 4), width=(8, 30))
+        nX, nY, C0, BL = w.nX, w.nY, w.C0, w.BL
+    else:
+        nX, nY = 96, 64
+        C0, BL = _sawtooth_channel(nX, nY)
+    cutoff = 5.0
+    g = oracle.geometry(nX, nY, C0, BL, x_release_pos=10.0, x_stop_pos=nX - 12.0,
+                        cutoff_radius=cutoff)
+    XF = g["XF"]
+    nBL = BL.shape[0]
+    b = g["bounds"]
+    nodes = np.stack([g["ss"] % XF, g["ss"] // XF], axis=1).astype(float)
+    nb = int(b[1] + b[3])
+    assert nb > 50
+    for ii in range(nb):
+        bl = ii + b[0] if ii < b[1] else ii + b[2] - b[1]
+        link = bl if bl < nBL // 2 else (nBL - 1 - bl) + nBL // 2
+        mid = (C0[BL[link, 0]] + C0[BL[link, 1]]) / 2
+        d = np.linalg.norm(nodes - mid, axis=1)
+        cand = [j for j in np.argsort(d, kind="stable") if d[j] < cutoff][:4]
+        wts = {j: math.exp(-0.5 * (d[j] / cutoff) ** 2) for j in cand}
+        tot = sum(wts.values())
+        idx = sorted(cand)
+        got_i = g["sInd"][4 * ii:4 * ii + 4]
+        got_w = g["weights"][4 * ii:4 * ii + 4]
+        real = got_i >= 0
+        assert list(got_i[real]) == idx, (ii, list(got_i), idx)
+        assert np.allclose(got_w[real], [wts[j] / tot for j in idx], rtol=1e-12, atol=0), ii
+        assert np.all(got_w[~real] < 1e-80)
+
+
+@pytest.mark.parametrize("shape", ["fouled", "sawtooth"])
+def test_wall_distance_and_spacings_brute_force(shape):
+    """kOmega::iniWallD (kOmega.cpp:136-193, 330-349): the wall distance of
+    every fluid node is its distance to the nearest wall segment, here from a
+    search over every segment of both walls.  inidXArrays
+    (clVariablesLS.cpp:650-682): the E/W/N/S spacing of a fluid node whose
+    axis neighbour is solid is the fraction of the way to the nearest wall
+    crossing on that axis (at least the minimum spacing); 1 where no wall is
+    crossed."""
+    if shape == "fouled":
+        w = channel.fouled_channel(128, 40, seed=21, amp=(0, 4), width=(8, 30))
+        nX, nY, C0, BL = w.nX, w.nY, w.C0, w.BL
+    else:
+        nX, nY = 96, 64
+        C0, BL = _sawtooth_channel(nX, nY)
+    mindx = 0.001
+    g = oracle.geometry(nX, nY, C0, BL, min_dx=mindx)
+    XF, D = g["XF"], g["D"]
+    nt = g["nType"].astype(np.int32)
+    segs = [(C0[a], C0[b]) for a, b in BL]
+    dX = g["dX"].reshape(4, D)
+    axis = [(1, 0), (-1, 0), (0, 1), (0, -1)]
+    nchk = 0
+    for y in range(nY):
+        for x in range(nX):
+            n = x + XF * y
+            if nt[n] & 1:
+                continue
+            p = np.array([x, y], float)
+            wd = min(_seg_dist(p, a, b) for a, b in segs)
+            assert abs(g["wallD"][n] - wd) < 1e-12, (x, y, g["wallD"][n], wd)
+            for k, (cx, cy) in enumerate(axis):
+                m = ((x + cx) % nX) + XF * (y + cy)
+                if not (nt[m] & 0x4):  # M0_SOLID_NODE: the neighbour is initially solid
+                    continue
+                c = np.array([cx, cy], float)
+                best = None
+                for a, b in segs:
+                    mat = np.array([[c[0], a[0] - b[0]], [c[1], a[1] - b[1]]])
+                    if abs(np.linalg.det(mat)) < 1e-14:
+                        continue
+                    t, s = np.linalg.solve(mat, a - p)
+                    if -1e-12 <= s <= 1 + 1e-12 and 0.0 <= t <= 1.0 + 1e-12:
+                        best = t if best is None else min(best, t)
+                if best is None:
+                    continue
+                assert abs(dX[k, n] - max(best, mindx)) < 1e-9, (x, y, k, dX[k, n], best)
+                nchk += 1
+    assert nchk > 2 * nX
+
+
+@pytest.mark.parametrize("shape", ["fouled", "sawtooth"])
+def test_solid_fluid_map_brute_force(shape):
+    """iniFillMap0 (clVariablesLS.cpp:810-891) with its column fill: a node
+    of a channel whose walls are single-valued in x is fluid exactly when it
+    lies between the bottom and the top wall polylines at its x (checked
+    for every node farther than 1e-6 from a wall)."""
+    if shape == "fouled":
+        w = channel.fouled_channel(128, 40, seed=21, amp=(0, 4), width=(8, 30))
+        nX, nY, C0, BL = w.nX, w.nY, w.C0, w.BL
+    else:
+        nX, nY = 96, 64
+        C0, BL = _sawtooth_channel(nX, nY)
+    g = oracle.geometry(nX, nY, C0, BL)
+    XF = g["XF"]
+    half = C0.shape[0] // 2
+    xb, yb = C0[:half, 0], C0[:half, 1]
+    xt, yt = C0[half:, 0][::-1], C0[half:, 1][::-1]
+    nt = g["nType"].astype(np.int32).reshape(nY, XF)[:, :nX]
+    for x in range(nX):
+        lo, hi = np.interp(x, xb, yb), np.interp(x, xt, yt)
+        for y in range(nY):
+            if min(abs(y - lo), abs(y - hi)) < 1e-6:
+                continue
+            fluid = lo < y < hi
+            assert bool(nt[y, x] & 1) != fluid, (x, y, lo, hi)
