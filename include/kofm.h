@@ -160,8 +160,8 @@ int kofm_create_host(const kofm_grid* grid, kofm_ctx** out);
 /* Releases the context and all its device memory. */
 void kofm_destroy(kofm_ctx* ctx);
 
-/* Last error text of this context (or of the last failed kofm_create when
- * ctx is NULL). */
+/* Last error text of this context (or, when ctx is NULL, of the last failed
+ * kofm_create or kofm_reduce_device on this thread). */
 const char* kofm_last_error(const kofm_ctx* ctx);
 
 /* Relaxation, forcing and solver constants.  Replaces
@@ -240,6 +240,25 @@ int kofm_get_solver_stats(kofm_ctx* ctx, kofm_solver_stats* out);
 /* Sum/min/max of one field over the zero-padded power-of-two domain,
  * accumulated in FP64 (Reducer::reduceSingle, Reducer.h:240-273). */
 int kofm_reduce(kofm_ctx* ctx, int32_t field, int32_t op, double* out);
+
+/* The reduction of kofm_reduce on n values at a device address, enqueued on
+ * `stream` (a hipStream_t; NULL for the null stream) with nothing allocated
+ * and nothing synchronised: the result goes to the device double *out.
+ * elem_bytes is 4 (FP32 values) or 8 (FP64); n lies in
+ * [1, KOFM_REDUCE_DEVICE_MAX_N].  The per-work-group partials live in a
+ * fixed device array of KOFM_REDUCE_DEVICE_PARTIALS FP64 values (8 KiB per
+ * GPU, held until the process ends); the work-group count is capped at that
+ * number, so any n in range fits.  All calls on one GPU within a process
+ * share that array and it keeps the last call's partials: calls that overlap
+ * on one GPU (on different streams or from different threads) race and give
+ * wrong results.  For a given n the result has the same bits as kofm_reduce
+ * over the same values.  Errors are reported by kofm_last_error(NULL).
+ * No counterpart in the reference: this is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops). */
+#define KOFM_REDUCE_DEVICE_PARTIALS 1024
+#define KOFM_REDUCE_DEVICE_MAX_N (((int64_t)1) << 40)
+int kofm_reduce_device(const void* field, int32_t elem_bytes, int64_t n, int32_t op, double* out,
+                       void* stream);
 
 /* Copies one field (KOFM_FIELD_*) to host FP64 in the reference layout. */
 int kofm_download(kofm_ctx* ctx, int32_t field, double* host);

@@ -39,6 +39,10 @@ int setErr(kofm_ctx* c, int rc, const std::string& msg) {
 }
 }  // namespace
 
+namespace kofm {
+void setGlobalError(const std::string& msg) { g_createError = msg; }
+}  // namespace kofm
+
 extern "C" {
 
 int kofm_create(const kofm_grid* grid, kofm_ctx** out) {

@@ -130,7 +130,7 @@ class EngineT final : public Engine {
     const size_t nTiles = size_t((nX_ + dev::kBlockX - 1) / dev::kBlockX) * nY_;
     const size_t nblk = 8 * ((nTiles + 7) / 8);
     HIP_TRY(partial_.alloc(4 * nblk * sizeof(double)));
-    HIP_TRY(redPartial_.alloc(1024 * sizeof(double)));
+    HIP_TRY(redPartial_.alloc(dev::kReducePartials * sizeof(double)));
     HIP_TRY(redOut_.alloc(sizeof(double)));
     return 0;
   }
@@ -429,9 +429,7 @@ class EngineT final : public Engine {
     if (!f.p) return fail(err, "field cannot be reduced");
     if (op != KOFM_RED_SUM && op != KOFM_RED_MAX && op != KOFM_RED_MIN) return fail(err, "bad op");
     const int64_t n = fieldSize(field);
-    int64_t full = 2;
-    while (full < n) full *= 2;
-    const int blocks = int(std::min<int64_t>(1024, (n + 255) / 256));
+    const int blocks = dev::reduceBlocks(n);
     if (f.dbl)
       hipLaunchKernelGGL((dev::reducePartial<double>), dim3(blocks), dim3(256), 0, stream_,
                          (const double*)f.p, n, op, redPartial_.as<double>());
@@ -439,7 +437,7 @@ class EngineT final : public Engine {
       hipLaunchKernelGGL((dev::reducePartial<float>), dim3(blocks), dim3(256), 0, stream_,
                          (const float*)f.p, n, op, redPartial_.as<double>());
     hipLaunchKernelGGL(dev::reduceFinal, dim3(1), dim3(256), 0, stream_,
-                       (const double*)redPartial_.as<double>(), blocks, op, full > n ? 1 : 0,
+                       (const double*)redPartial_.as<double>(), blocks, op, dev::reducePadded(n),
                        redOut_.as<double>());
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(&out, redOut_.as<void>(), sizeof(double), hipMemcpyDeviceToHost, stream_));
@@ -2369,3 +2367,84 @@ std::unique_ptr<Engine> makeEngine(int nX, int nY, int storage, bool komega, int
 }
 
 }  // namespace kofm
+
+// ---------------------------------------------------------------------------
+// kofm_reduce_device: the reduction of kofm_reduce on a caller's device array,
+// with no allocation.  Its per-work-group partials live in `scratch`, a
+// file-scope device array (one copy per GPU, held until the process ends).
+// Host code never passes `scratch` itself to a kernel (on the host that name
+// is a stand-in, not a device address): the device address of each GPU's
+// copy is looked up once, with that GPU current, by hipGetSymbolAddress and
+// kept in g_scratch.
+// ---------------------------------------------------------------------------
+#include <mutex>
+
+namespace {
+
+static __device__ __align__(256) double scratch[kofm::dev::kReducePartials];
+static_assert(kofm::dev::kReducePartials == KOFM_REDUCE_DEVICE_PARTIALS,
+              "the header's partial count is the kernels' work-group cap");
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+double* g_scratch[kMaxDevices] = {};
+
+// Device address of the current GPU's scratch array (nullptr on failure).
+double* scratchOnCurrentDevice(int& dev, hipError_t& e) {
+  e = hipGetDevice(&dev);
+  if (e != hipSuccess) return nullptr;
+  if (dev < 0 || dev >= kMaxDevices) {
+    e = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return nullptr;
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  return g_scratch[dev];
+}
+
+int reduceDeviceFail(int rc, const std::string& msg) {
+  kofm::setGlobalError(msg);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kofm_reduce_device(const void* field, int32_t elem_bytes, int64_t n, int32_t op, double* out,
+                       void* stream) {
+  if (!field || !out) return reduceDeviceFail(KOFM_ERR_ARG, "null device pointer");
+  if (elem_bytes != 4 && elem_bytes != 8)
+    return reduceDeviceFail(KOFM_ERR_ARG, "elem_bytes must be 4 (FP32) or 8 (FP64)");
+  if (op != KOFM_RED_SUM && op != KOFM_RED_MAX && op != KOFM_RED_MIN)
+    return reduceDeviceFail(KOFM_ERR_ARG, "op must be KOFM_RED_SUM, KOFM_RED_MAX or KOFM_RED_MIN");
+  if (n < 1 || n > KOFM_REDUCE_DEVICE_MAX_N)
+    return reduceDeviceFail(KOFM_ERR_ARG, "n must lie in [1, KOFM_REDUCE_DEVICE_MAX_N]: " +
+                                              std::to_string(n) + " elements requested");
+  int dev = 0;
+  hipError_t e = hipSuccess;
+  double* part = scratchOnCurrentDevice(dev, e);
+  if (!part)
+    return reduceDeviceFail(KOFM_ERR_DEVICE, std::string("scratch address: ") + hipGetErrorString(e));
+  auto st = static_cast<hipStream_t>(stream);
+  const int blocks = kofm::dev::reduceBlocks(n);
+  if (elem_bytes == 8)
+    hipLaunchKernelGGL((kofm::dev::reducePartial<double>), dim3(blocks), dim3(256), 0, st,
+                       static_cast<const double*>(field), n, op, part);
+  else
+    hipLaunchKernelGGL((kofm::dev::reducePartial<float>), dim3(blocks), dim3(256), 0, st,
+                       static_cast<const float*>(field), n, op, part);
+  hipLaunchKernelGGL(kofm::dev::reduceFinal, dim3(1), dim3(256), 0, st, (const double*)part,
+                     blocks, op, kofm::dev::reducePadded(n), out);
+  e = hipGetLastError();
+  if (e != hipSuccess)
+    return reduceDeviceFail(KOFM_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
+  return KOFM_OK;
+}
+
+}  // extern "C"

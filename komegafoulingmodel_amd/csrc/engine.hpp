@@ -79,6 +79,9 @@ class Ensemble {
   virtual int64_t fluidNodes() const = 0;
 };
 
+// Error text returned by kofm_last_error(NULL) (calls without a context).
+void setGlobalError(const std::string& msg);
+
 // storage: KOFM_STORE_*
 // rc receives the negative KOFM_ERR_* code of a failure
 std::unique_ptr<Engine> makeEngine(int nX, int nY, int storage, bool komega, int& rc,

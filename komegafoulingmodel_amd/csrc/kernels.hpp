@@ -1541,6 +1541,21 @@ __global__ __launch_bounds__(64) void shearLinks(const R* __restrict__ w,
 // reductions (FP64 accumulation, deterministic two-pass)
 // ---------------------------------------------------------------------------
 
+// At most kReducePartials work-groups of 256 lanes sweep the field (grid
+// stride), each writing one FP64 partial; reduceFinal combines them in a
+// fixed order, so a given n always gives the same bits.
+constexpr int kReducePartials = 1024;
+inline int reduceBlocks(int64_t n) {
+  return int(n < int64_t(kReducePartials) * 256 ? (n + 255) / 256 : kReducePartials);
+}
+// Reducer sums over the zero-padded power-of-two buffer (ReduceGenerator.cpp:35-43):
+// when n is not a power of two, the padding's zeros take part in min and max.
+inline int reducePadded(int64_t n) {
+  int64_t full = 2;
+  while (full < n) full *= 2;
+  return full > n ? 1 : 0;
+}
+
 template <class R>
 __global__ __launch_bounds__(256) void reducePartial(const R* __restrict__ a, int64_t n, int op,
                                                      double* __restrict__ partial) {

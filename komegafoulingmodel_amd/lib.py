@@ -50,6 +50,7 @@ EXPORTS = [
     "kofm_ensemble_autotune",
     "kofm_ensemble_last_error",
     "kofm_ensemble_destroy",
+    "kofm_reduce_device",
 ]
 
 
@@ -135,6 +136,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "kofm_wall_shear": (C.c_int, [P]),
         "kofm_get_solver_stats": (C.c_int, [P, C.POINTER(SolverStats)]),
         "kofm_reduce": (C.c_int, [P, C.c_int32, C.c_int32, dp]),
+        "kofm_reduce_device": (C.c_int, [P, C.c_int32, C.c_int64, C.c_int32, P, P]),
         "kofm_download": (C.c_int, [P, C.c_int32, dp]),
         "kofm_save2file": (C.c_int, [P, C.c_char_p]),
         "kofm_synchronize": (C.c_int, [P]),
@@ -169,6 +171,18 @@ def _dptr(a):
     if a is None:
         return None
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def reduce_device(field_ptr: int, elem_bytes: int, n: int, op: int, out_ptr: int,
+                  stream: int = 0) -> None:
+    """kofm_reduce_device: reduction of n values at a device address into the
+    device double at out_ptr, enqueued on `stream` with nothing allocated and
+    nothing synchronised (the caller synchronises before reading out_ptr)."""
+    lib = load()
+    rc = lib.kofm_reduce_device(C.c_void_p(field_ptr), elem_bytes, n, op, C.c_void_p(out_ptr),
+                                C.c_void_p(stream) if stream else None)
+    if rc != 0:
+        raise KofmError(rc, lib.kofm_last_error(None).decode())
 
 
 class Lattice:
