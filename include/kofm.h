@@ -48,6 +48,9 @@ typedef struct kofm_ctx kofm_ctx;
 #define KOFM_STORE_F16 1  /* FP16 scaled deviations (f_i - w_i) * 1024 / w_i */
 #define KOFM_STORE_BF16 2 /* BF16 scaled deviations (f_i - w_i) * 1024 / w_i */
 #define KOFM_STORE_F64 3  /* FP64 deviations, FP64 arithmetic (GPU FP64 oracle) */
+#define KOFM_STORE_F32X 4 /* FP32 deviations, FP64 arithmetic, stochastically rounded
+                             stores: FP32 bytes with long-run drift near FP64 (one step
+                             per pass, no captured graphs, no ensemble or adjoint) */
 
 /* options for kofm_step (Heading.cl:309) */
 #define KOFM_OPTION_SAVE_MACRO_FIELDS 0x2
@@ -64,6 +67,9 @@ typedef struct kofm_ctx kofm_ctx;
 #define KOFM_FIELD_WALLD 8    /* D   WallD                                      */
 #define KOFM_FIELD_TAU_NODE 9 /* nSS shearStress::Tau (per wall node)           */
 #define KOFM_FIELD_TAU_BL 10  /* nBLshear  BL.Tau over the shear link range     */
+#define KOFM_FIELD_DIFF_K 11  /* D   kOmega::Diff_K (k diffusivity; a node keeps its
+                                 last value when a wall update makes it solid)   */
+#define KOFM_FIELD_DIFF_OMEGA 12 /* D kOmega::Diff_Omega                           */
 
 /* reduction operators (ReduceGenerator.cpp:219-305) */
 #define KOFM_RED_SUM 1
@@ -137,7 +143,10 @@ typedef struct kofm_solver_stats {
   int32_t omega_iters;  /* sweeps of the last omega solve */
   double kappa_res0, kappa_res;
   double omega_res0, omega_res;
-  int32_t nan_flag;     /* a non-finite residual was seen since the last query */
+  int32_t nan_flag;     /* non-finite residuals seen since the last query (each one
+                           halved the k-omega time step) */
+  int32_t steps_per_lb; /* k-omega sub-steps per LB step (kOmega::stepsPerLB) */
+  double time_step;     /* their time step (kOmega::timeStep) */
 } kofm_solver_stats;
 
 /* --- lifecycle ------------------------------------------------------------ */
@@ -186,6 +195,21 @@ int kofm_build_geometry(kofm_ctx* ctx, const double* C0, int32_t nN, const int32
 
 int kofm_geometry_counts(const kofm_ctx* ctx, kofm_geom_counts* out);
 
+/* Moves the walls to the points C (nN points, x y interleaved, the same
+ * points and links as kofm_build_geometry; a fouling layer grows between
+ * the built walls and C).  The fouling-layer update of clVariablesFL::update
+ * (clVariablesFL.cpp:745-780): clVariablesLS::update (clVariablesLS.cpp:
+ * 1020-1083) with updateNType (fluid nodes between the built and the current
+ * wall become solid, their rho 1, u, k, omega and nu_t 0), updateNodeBoundaryInfo
+ * and updateBoundaryNodes (links and wall spacings, lsKernels.cl:47-634),
+ * kOmega::updateWallD (kOmegaKernels.cl:813-840), and the wall-node list and
+ * shear coefficients of shearStress (shearStress.cpp:135-139).  Node types,
+ * links, spacings and wall distances are computed on the device (each node
+ * finds its own links: the result does not depend on scheduling); the
+ * populations keep their values.  kofm_geometry_array returns the updated
+ * arrays; kofm_objective_run needs a rebuilt geometry afterwards. */
+int kofm_update_walls(kofm_ctx* ctx, const double* C, int32_t nN);
+
 /* Copies one host-side geometry array (KOFM_GEOM_*) into out. */
 int kofm_geometry_array(const kofm_ctx* ctx, int32_t which, void* out);
 
@@ -199,12 +223,42 @@ int kofm_geometry_array(const kofm_ctx* ctx, int32_t which, void* out);
  * tensor to zero. */
 int kofm_init_state(kofm_ctx* ctx, const double* rho, const double* ux, const double* uy);
 
+/* The initial velocity field of clVariablesLB::iniDists(Umax)
+ * (clVariablesLB.cpp:427-496), "Ini Poiseuille": walking up each column from
+ * its first fluid node, whose height above the wall below is its S-plane
+ * wall spacing, one node at a time, u_x is the parabola
+ * 1.5 Re nu / R (1 - (y - R)^2 / R^2) or, with turbulent_profile ("Ini Turb
+ * Velocity"), the law of the wall u_tau u+(y+) with u_tau = sqrt(3 Re) nu / R;
+ * with perturb ("Perturb Vel Field") streaks and wall-normal waves whose
+ * amplitudes are Box-Muller draws (HelperFuncs.cpp:107-126) from the C
+ * library generator seeded by srand(seed) (the reference calls srand(1),
+ * clProblem.cpp:80).  rho is set to rho.  Host only (works on a
+ * kofm_create_host context with geometry); the arrays [D] then go to
+ * kofm_init_state.  Nodes outside the walked columns are left unchanged. */
+typedef struct kofm_ini_flow {
+  double Re;                 /* "Re" */
+  double nu;                 /* MuVal */
+  double pipe_radius;        /* "Pipe Radius" */
+  double rho;                /* "Rho" */
+  int32_t turbulent_profile; /* "Ini Turb Velocity" */
+  int32_t perturb;           /* "Perturb Vel Field" */
+  double duplus_mult;        /* "DUplus Perturb Multiplier" */
+  double epsilon_mult;       /* "Epsilon Perturb Multiplier" */
+  double beta_mult;          /* "Beta Perturb Multiplier" */
+  double alpha_mult;         /* "Alpha Perturb Multiplier" */
+  double sigma_mult;         /* "Sigma Perturb Multiplier" */
+  uint32_t seed;             /* srand seed */
+} kofm_ini_flow;
+int kofm_ini_poiseuille(const kofm_ctx* ctx, const kofm_ini_flow* p, double* rho, double* ux,
+                        double* uy);
+
 /* k, omega and nu_t = k/omega on fluid nodes (kOmega::setInitialValues,
  * kOmega.cpp:530-570, fillByNodeType Array.h:903-918).  Values <= 0 select
- * the reference's turbulence-intensity estimate with Re and "Geometric
- * Penalty Factor" = 1: umean = Re nu / R / 4 when the current mean velocity
- * is zero. */
-int kofm_init_komega(kofm_ctx* ctx, double k_ini, double omega_ini, double Re);
+ * the reference's turbulence-intensity estimate from the mean velocity
+ * sum(u_x)/sum(rho), or, when that is zero, umean = Re nu / R / 4 *
+ * geom_penalty_factor ("Geometric Penalty Factor", kOmega.cpp:541). */
+int kofm_init_komega(kofm_ctx* ctx, double k_ini, double omega_ini, double Re,
+                     double geom_penalty_factor);
 
 /* Uploads populations (9*D, the reference FA layout as loaded from
  * load/lbf.bin, clVariablesLB.cpp:883-902) and, when non-NULL, k, omega
@@ -223,8 +277,12 @@ int kofm_upload_state(kofm_ctx* ctx, const double* f, const double* k, const dou
 int kofm_step(kofm_ctx* ctx, int32_t nsteps, uint32_t options);
 
 /* Runs nsteps as kofm_step and returns the mean device time per step in
- * *ms_per_step and of the dominant (collide-stream) kernel in
- * *ms_per_kernel, both measured with HIP events on the context stream. */
+ * *ms_per_step, measured with HIP events on the context stream around the
+ * whole run.  With ms_per_kernel non-NULL, events also bracket every
+ * collide-stream launch and *ms_per_kernel receives that kernel's mean time
+ * per step (-1 for captured k-omega steps, which are timed whole); those
+ * inner events then add their own cost to *ms_per_step, so time the step
+ * and the kernel in separate calls. */
 int kofm_time_steps(kofm_ctx* ctx, int32_t nsteps, uint32_t options, double* ms_per_step,
                     double* ms_per_kernel);
 
@@ -234,6 +292,31 @@ int kofm_time_steps(kofm_ctx* ctx, int32_t nsteps, uint32_t options, double* ms_
 int kofm_wall_shear(kofm_ctx* ctx);
 
 int kofm_get_solver_stats(kofm_ctx* ctx, kofm_solver_stats* out);
+
+/* The k-omega time step (k-omega contexts).  Each kofm_step runs
+ * kOmega::Solve's sub-steps (kOmega.cpp:31-73): after 20 steps without a
+ * change the time step doubles back towards its initial value, and a
+ * non-finite residual in either solve restores k and omega to the start of
+ * the step, halves the time step, doubles the sub-steps per LB step and
+ * starts the sub-steps again (GMRESSolver::checkConvergence,
+ * GMRESSolver.cpp:1535-1546).  The actions:
+ *   KOFM_DT_QUERY       report only;
+ *   KOFM_DT_HALVE       kOmega::halveTimeStep (kOmega.cpp:387-399): k and
+ *                       omega back to the start of the last k-omega step
+ *                       (when one has run), half the time step, twice the
+ *                       sub-steps;
+ *   KOFM_DT_DOUBLE      kOmega::doubleTimeStep (kOmega.cpp:401-414);
+ *   KOFM_DT_INJECT_NAN  the next solve is taken to have a non-finite
+ *                       residual (tests the restart path without one; no
+ *                       reference counterpart).
+ * *steps_per_lb and *time_step (either may be NULL) receive the state after
+ * the action. */
+#define KOFM_DT_QUERY 0
+#define KOFM_DT_HALVE 1
+#define KOFM_DT_DOUBLE 2
+#define KOFM_DT_INJECT_NAN 3
+int kofm_komega_time_step(kofm_ctx* ctx, int32_t action, int32_t* steps_per_lb,
+                          double* time_step);
 
 /* --- output --------------------------------------------------------------- */
 
@@ -269,6 +352,20 @@ int kofm_download(kofm_ctx* ctx, int32_t field, double* host);
  * the reference format (Array.h:2528-2546: one row per x, tab-separated
  * values per y, 6 significant digits) into dir. */
 int kofm_save2file(kofm_ctx* ctx, const char* dir);
+
+/* saveRestartFiles of clVariablesLB, kOmega and clVariablesLS
+ * (clVariablesLB.cpp:752-765, kOmega.cpp:492-496, clVariablesLS.cpp:534-537):
+ * lbf.bin, lbkappa.bin and lbomega.bin (k-omega contexts) and lsc.bin (the
+ * wall points of the current geometry, x y interleaved, 2*nN doubles), raw
+ * FP64 with no header (Array.h:1178-1194), into dir. */
+int kofm_save_restart(kofm_ctx* ctx, const char* dir);
+
+/* The debug dumps of clVariablesLS::saveDebug (clVariablesLS.cpp:545-556)
+ * and kOmega::saveDebug (kOmega.cpp:422-480) in the save2file text layout:
+ * nType.txt (integers), dXarr_0.txt .. dXarr_3.txt (E, W, N, S wall
+ * spacings, one file per plane, Array.h:3193-3220) and, with k-omega,
+ * lbnut.txt and wallD.txt, into dir. */
+int kofm_save_debug(kofm_ctx* ctx, const char* dir);
 
 /* --- launch shape ---------------------------------------------------------- */
 

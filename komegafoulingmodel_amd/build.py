@@ -24,7 +24,7 @@ CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("KOFM_ARCH", "gfx950")
 
-SOURCES = ["engine.hip", "capi.cpp", "geometry.cpp", "io.cpp"]
+SOURCES = ["engine.hip", "capi.cpp", "geometry.cpp", "io.cpp", "init.cpp"]
 
 COMMON = [
     "-O3",

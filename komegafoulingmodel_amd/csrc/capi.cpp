@@ -9,6 +9,7 @@
 #include "../../include/kofm.h"
 #include "engine.hpp"
 #include "geometry.hpp"
+#include "init.hpp"
 #include "io.hpp"
 
 struct kofm_ctx {
@@ -155,6 +156,19 @@ int kofm_build_geometry(kofm_ctx* ctx, const double* C0, int32_t nN, const int32
   return KOFM_OK;
 }
 
+int kofm_update_walls(kofm_ctx* ctx, const double* C, int32_t nN) {
+  if (!ctx || !C || nN < 2) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  if (!ctx->haveGeo) return setErr(ctx, KOFM_ERR_STATE, "build the geometry first");
+  if (size_t(2 * int64_t(nN)) != ctx->geo.C0.size())
+    return setErr(ctx, KOFM_ERR_ARG, "the wall update needs the built geometry's number of wall points");
+  std::string err;
+  int rc = ctx->engine->updateWalls(std::vector<double>(C, C + 2 * size_t(nN)), err);
+  if (rc) return setErr(ctx, rc, err);
+  ctx->geo = ctx->engine->geometry();
+  return KOFM_OK;
+}
+
 int kofm_geometry_counts(const kofm_ctx* ctx, kofm_geom_counts* out) {
   if (!ctx || !out) return KOFM_ERR_ARG;
   const kofm::Geometry& g = ctx->geo;
@@ -209,6 +223,15 @@ int kofm_geometry_array(const kofm_ctx* ctx, int32_t which, void* out) {
   }
 }
 
+int kofm_ini_poiseuille(const kofm_ctx* ctx, const kofm_ini_flow* p, double* rho, double* ux,
+                        double* uy) {
+  if (!ctx || !p || !rho || !ux || !uy) return KOFM_ERR_ARG;
+  if (!ctx->haveGeo) return KOFM_ERR_STATE;
+  if (!(p->pipe_radius > 0.) || !(p->nu > 0.) || !(p->Re > 0.)) return KOFM_ERR_ARG;
+  kofm::iniPoiseuille(ctx->geo, *p, rho, ux, uy);
+  return KOFM_OK;
+}
+
 int kofm_init_state(kofm_ctx* ctx, const double* rho, const double* ux, const double* uy) {
   if (!ctx) return KOFM_ERR_ARG;
   NEED_ENGINE(ctx);
@@ -218,11 +241,12 @@ int kofm_init_state(kofm_ctx* ctx, const double* rho, const double* ux, const do
   return rc ? setErr(ctx, rc, err) : KOFM_OK;
 }
 
-int kofm_init_komega(kofm_ctx* ctx, double k_ini, double omega_ini, double Re) {
+int kofm_init_komega(kofm_ctx* ctx, double k_ini, double omega_ini, double Re,
+                     double geom_penalty_factor) {
   if (!ctx) return KOFM_ERR_ARG;
   NEED_ENGINE(ctx);
   std::string err;
-  int rc = ctx->engine->initKOmega(k_ini, omega_ini, Re, ctx->pipeRadius, err);
+  int rc = ctx->engine->initKOmega(k_ini, omega_ini, Re, geom_penalty_factor, ctx->pipeRadius, err);
   return rc ? setErr(ctx, rc, err) : KOFM_OK;
 }
 
@@ -245,10 +269,10 @@ int kofm_step(kofm_ctx* ctx, int32_t nsteps, uint32_t options) {
 
 int kofm_time_steps(kofm_ctx* ctx, int32_t nsteps, uint32_t options, double* ms_per_step,
                     double* ms_per_kernel) {
-  if (!ctx || !ms_per_step || !ms_per_kernel) return KOFM_ERR_ARG;
+  if (!ctx || !ms_per_step) return KOFM_ERR_ARG;
   NEED_ENGINE(ctx);
   std::string err;
-  int rc = ctx->engine->timeSteps(nsteps, options, *ms_per_step, *ms_per_kernel, err);
+  int rc = ctx->engine->timeSteps(nsteps, options, *ms_per_step, ms_per_kernel, err);
   return rc ? setErr(ctx, rc, err) : KOFM_OK;
 }
 
@@ -258,6 +282,20 @@ int kofm_wall_shear(kofm_ctx* ctx) {
   std::string err;
   int rc = ctx->engine->wallShear(err);
   return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_komega_time_step(kofm_ctx* ctx, int32_t action, int32_t* steps_per_lb,
+                          double* time_step) {
+  if (!ctx) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int spl = 0;
+  double dt = 0.;
+  int rc = ctx->engine->komegaTimeStep(action, spl, dt, err);
+  if (rc) return setErr(ctx, rc, err);
+  if (steps_per_lb) *steps_per_lb = spl;
+  if (time_step) *time_step = dt;
+  return KOFM_OK;
 }
 
 int kofm_get_solver_stats(kofm_ctx* ctx, kofm_solver_stats* out) {
@@ -274,6 +312,8 @@ int kofm_get_solver_stats(kofm_ctx* ctx, kofm_solver_stats* out) {
   out->omega_res0 = s.res0[1];
   out->omega_res = s.res[1];
   out->nan_flag = s.nan;
+  out->steps_per_lb = s.stepsPerLB;
+  out->time_step = s.timeStep;
   return KOFM_OK;
 }
 
@@ -298,6 +338,24 @@ int kofm_save2file(kofm_ctx* ctx, const char* dir) {
   NEED_ENGINE(ctx);
   std::string err;
   int rc = kofm::saveFields(*ctx->engine, ctx->nX, ctx->nY, ctx->komega, dir, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_save_restart(kofm_ctx* ctx, const char* dir) {
+  if (!ctx || !dir) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  if (!ctx->haveGeo) return setErr(ctx, KOFM_ERR_STATE, "build the geometry first");
+  std::string err;
+  int rc = kofm::saveRestart(*ctx->engine, ctx->nX, ctx->nY, ctx->komega, ctx->geo.C0, dir, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_save_debug(kofm_ctx* ctx, const char* dir) {
+  if (!ctx || !dir) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  if (!ctx->haveGeo) return setErr(ctx, KOFM_ERR_STATE, "build the geometry first");
+  std::string err;
+  int rc = kofm::saveDebug(*ctx->engine, ctx->geo, ctx->komega, dir, err);
   return rc ? setErr(ctx, rc, err) : KOFM_OK;
 }
 

@@ -29,6 +29,15 @@ static int printParams(const std::string& loadDir) {
               vlb.Re, vlb.Fval, vlb.RhoVal);
   std::printf("UseTurbModel %d\nStopTime %u\ntrStepsWall %u\n",
               vlb.kOmegaClass.kOmegaSolverFlag ? 1 : 0, p.StopTime, p.trSteps_wall);
+  std::printf("IniPoiseuille %d\nIniTurbVelocity %d\nPerturbVelField %d\nSaveAvgInfo %d\n"
+              "LoadVelText %d\nUseTurbVelBC %d\nGeomPenaltyFactor %.17g\nlbOutMaxLines %d\n",
+              vlb.iniPoiseuille, vlb.kOmegaClass.iniTurbVelocity, vlb.kOmegaClass.perturbVelField,
+              vlb.saveAvgInfo, vlb.loadVelTxtFlag, vlb.turbVelBCFlag, vlb.geomPenaltyFactor,
+              vlb.lbOutSize);
+  std::printf("DUplus %.17g\nEpsilonMult %.17g\nBetaMult %.17g\nAlphaMult %.17g\nSigmaMult %.17g\n",
+              vlb.kOmegaClass.perturbDUPlus, vlb.kOmegaClass.perturbEpsilon,
+              vlb.kOmegaClass.perturbBeta, vlb.kOmegaClass.perturbAlpha,
+              vlb.kOmegaClass.perturbSigma);
   return 0;
 }
 
@@ -51,35 +60,51 @@ int main(int argc, char** argv) {
     vls.loadParams(p, loadDir);
     clVariablesLB vlb;
     vlb.loadParams(p);
-    vlb.ini(p, vls, loadDir);
+    vlb.ini(p, vls, loadDir, outDir);
+    // "Save Macros On Start" (clVariablesLB.cpp:362-363)
+    if (vlb.saveMacroStart && !vlb.restarted) vlb.save2file(outDir);
     std::printf("nX %d nY %d tau0 %.6f nu %.6g Fval %.6g k-omega %s\n", p.nX, p.nY, vlb.tau,
                 vlb.MuVal, vlb.Fval, vlb.kOmegaClass.kOmegaSolverFlag ? "on" : "off");
     if (p.RestartTime == 0 && vlb.tlbmNumIniSteps_LB > 0) vlb.getInitialFlow(p, outDir);
     shearStress ss;
-    unsigned nextAvg = p.saveAvgStartTime + p.saveTimeStepAvg;
-    unsigned nextDump = p.dumpStartTime + p.dumpTimeStep;
+    // a run restarted from a RunParam.yaml without the cadence keys resumes
+    // at the next cadence point after its current time
+    while (p.nextSaveAvgTime <= p.Time) p.nextSaveAvgTime += p.saveTimeStepAvg;
+    while (p.nextDumpStepTime <= p.Time) p.nextDumpStepTime += p.dumpTimeStep;
     while (p.Time < p.StopTime) {
       unsigned n = std::min(p.StopTime - p.Time, p.trSteps_wall);
-      n = std::min(n, nextAvg - p.Time);
-      n = std::min(n, nextDump - p.Time);
+      n = std::min(n, p.nextSaveAvgTime - p.Time);
+      n = std::min(n, p.nextDumpStepTime - p.Time);
       vlb.Solve(int(n));
       p.Time += n;
       if (p.Time % p.trSteps_wall == 0) ss.calculateShear(vlb.ctx);
-      if (p.Time >= nextAvg) {
+      if (p.Time >= p.nextSaveAvgTime) {
+        // rows held in memory, written out every "lbOut Max Lines" rows
         vlb.updateTimeData(p.Time);
-        vlb.saveTimeData(outDir);
         ss.updateTimeData(vlb.ctx, p.Time, outDir);
-        nextAvg += p.saveTimeStepAvg;
+        p.nextSaveAvgTime += p.saveTimeStepAvg;
       }
-      if (p.Time >= nextDump) {
+      if (p.Time >= p.nextDumpStepTime) {
+        // clProblem::step's dump (clProblem.cpp:130-168): the save2file text
+        // dumps and the held time data, RunParam.yaml with the current time
+        // and the cadence state; every "Save Bin Step" dumps the restart
+        // files and RunParam.yaml go to the load directory, so the next run
+        // restarts from them
+        p.nextDumpStepTime += p.dumpTimeStep;
+        const bool bin = ++p.currentDumpStep >= p.nextDumpBin;
+        if (bin) p.nextDumpBin += int(p.saveBinStep);
         vlb.save2file(outDir);
-        nextDump += p.dumpTimeStep;
+        vlb.saveTimeData();
+        p.saveParameters(outDir);
+        if (bin) {
+          vlb.saveRestartFiles(loadDir);
+          p.saveParameters(loadDir);
+        }
       }
     }
     vlb.save2file(outDir);
-    p.params.setInt(kofm::RunParams::kSystem, "Current Time", int(p.Time));
-    p.params.setBool(kofm::RunParams::kSystem, "Restart Run", true);
-    p.params.save(outDir + "/RunParam.yaml");
+    vlb.saveTimeData();
+    p.saveParameters(outDir);
     std::printf("done at time %u, Umean %g\n", p.Time, vlb.calcUmean());
   } catch (const std::exception& e) {
     std::fprintf(stderr, "kofm_run: %s\n", e.what());

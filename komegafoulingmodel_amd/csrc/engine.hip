@@ -21,6 +21,7 @@
 #include "engine.hpp"
 #include "adjoint.hpp"
 #include "kernels.hpp"
+#include "walls.hpp"
 
 namespace kofm {
 namespace {
@@ -79,12 +80,19 @@ class EngineT final : public Engine {
   friend class EnsembleT<S, R>;
 
  public:
+  // FP32 storage with FP64 collisions and stochastically rounded stores:
+  // every launch takes its step's index, so steps are launched one by one
+  // (no captured graphs, whose launches would repeat their indices)
+  static constexpr bool kSR = dev::kStochastic<S, R>;
+
   EngineT(int nX, int nY, bool komega) : nX_(nX), nY_(nY), ko_(komega) {
+    useGraphs_ = !kSR;
     XF_ = ((nX + 127) / 128) * 128;
     D_ = int64_t(XF_) * nY;
     P_ = D_ + kDefaultPad;
   }
   ~EngineT() override {
+    for (Ensemble* u : users_) u->memberGone(this);
     if (stream_) (void)hipStreamSynchronize(stream_);
     dropGraphs();
     for (auto& e : evs_) (void)hipEventDestroy(e);
@@ -118,6 +126,8 @@ class EngineT final : public Engine {
       HIP_TRY(dX_.alloc(4 * D * sizeof(double)));
       HIP_TRY(Ak_.alloc(4 * D * sizeof(double)));
       HIP_TRY(Ao_.alloc(4 * D * sizeof(double)));
+      HIP_TRY(Kprev_.alloc(D * sizeof(double)));
+      HIP_TRY(Oprev_.alloc(D * sizeof(double)));
     }
     HIP_TRY(status_.alloc(sizeof(dev::BicgState)));
     // pinned, so that the per-step poll of the solver state is one DMA
@@ -136,6 +146,7 @@ class EngineT final : public Engine {
   }
 
   bool komega() const { return ko_; }
+  const Geometry& geometry() const override { return geo_; }
 
   int64_t fieldSize(int field) const override {
     switch (field) {
@@ -149,6 +160,8 @@ class EngineT final : public Engine {
 
   int uploadGeometry(const Geometry& g, std::string& err) override {
     geo_ = g;
+    wallsMoved_ = false;
+    gen_++;
     dropGraphs();
     const size_t D = size_t(D_);
     // device mask and wall-node q table
@@ -188,11 +201,33 @@ class EngineT final : public Engine {
                         hipMemcpyHostToDevice));
     run_.valid = false;
     HIP_TRY(hipMemcpy(mask_.as<void>(), m.data(), D * sizeof(uint16_t), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(dev::solverPairs, dim3(unsigned((D_ + dev::kBlockX - 1) / dev::kBlockX)),
+                       dim3(dev::kBlockX), 0, stream_, mask_.as<uint16_t>(), nX_, XF_, nY_);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpy(ibbIdx_.as<void>(), idx.data(), D * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(ibbQ_.alloc(std::max<size_t>(q.size(), 8) * sizeof(R)));
     if (!q.empty())
       HIP_TRY(hipMemcpy(ibbQ_.as<void>(), q.data(), q.size() * sizeof(R), hipMemcpyHostToDevice));
-    // wall-shear tables
+    int rc = uploadShear(err);
+    if (rc) return rc;
+    // the node types, and with k-omega the wall spacings of the clean
+    // walls, for the moving-wall update
+    HIP_TRY(ntype_.alloc(D * sizeof(int16_t)));
+    HIP_TRY(hipMemcpy(ntype_.as<void>(), g.nType.data(), D * sizeof(int16_t), hipMemcpyHostToDevice));
+    wallC0_ = g.C0;
+    if (ko_) {
+      HIP_TRY(upload<double>(wallD_, g.wallD));
+      HIP_TRY(upload<double>(dX_, g.dX));
+      HIP_TRY(dX0_.alloc(4 * D * sizeof(double)));
+      HIP_TRY(hipMemcpy(dX0_.as<void>(), dX_.as<void>(), 4 * D * sizeof(double), hipMemcpyDeviceToDevice));
+    }
+    haveGeo_ = true;
+    return 0;
+  }
+
+  // Wall-shear tables of geo_ to the device.
+  int uploadShear(std::string& err) {
+    const Geometry& g = geo_;
     const size_t nSS = g.ss.size(), nb = size_t(g.nBLshear());
     HIP_TRY(ss_.alloc(std::max<size_t>(nSS, 1) * sizeof(int32_t)));
     HIP_TRY(cfss_.alloc(std::max<size_t>(16 * nSS, 1) * sizeof(R)));
@@ -208,16 +243,193 @@ class EngineT final : public Engine {
       HIP_TRY(hipMemcpy(sInd_.as<void>(), g.sInd.data(), 4 * nb * sizeof(int32_t), hipMemcpyHostToDevice));
       HIP_TRY(upload<R>(wts_, g.weights));
     }
-    if (ko_) {
-      HIP_TRY(upload<double>(wallD_, g.wallD));
-      HIP_TRY(upload<double>(dX_, g.dX));
+    return 0;
+  }
+
+  // clVariablesFL::update's wall part (clVariablesFL.cpp:745-780):
+  // clVariablesLS::update (clVariablesLS.cpp:1020-1083: updateNType,
+  // updateNodeBoundaryInfo, updateBoundaryNodes), kOmega::updateWallD
+  // (kOmegaKernels.cl:813-840) and the wall-shear refresh
+  // (updateShearArrays: fillShearArray and the shear coefficients), for the
+  // wall points C (same points and links as the built geometry).  The node
+  // types, links, spacings and wall distances are updated on the device;
+  // the wall-node list and the shear coefficients (a host thread in the
+  // reference) are rebuilt on the host from the updated node types.
+  int updateWalls(const std::vector<double>& C, std::string& err) override {
+    if (!haveGeo_) return fail(err, "build the geometry before updating the walls");
+    if (C.size() != wallC0_.size()) {
+      err = "the wall update needs the same number of wall points as the built geometry";
+      return KOFM_ERR_ARG;
     }
-    haveGeo_ = true;
+    const size_t D = size_t(D_);
+    const int nBL = int(geo_.BL.size() / 2);
+    // the populations carry a pending wall correction from the old links:
+    // apply it (FP64 copy) before the links change, store back afterwards
+    Buffer keep;
+    if (haveState_) {
+      HIP_TRY(keep.alloc(9 * D * sizeof(double)));
+      auto p = lbParams(0);
+      hipLaunchKernelGGL((dev::lbExportF<S, R>), gridXY(XF_), dim3(dev::kBlockX), 0, stream_,
+                         cur(), mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(), ibbQ_.as<R>(),
+                         keep.as<double>(), p);
+      HIP_TRY(hipGetLastError());
+    }
+    // segments that can reach each column (x ranges of the links)
+    std::vector<int32_t> start(size_t(nX_) + 1, 0), idx;
+    {
+      std::vector<std::vector<int32_t>> col(static_cast<size_t>(nX_));
+      for (int s = 0; s < nBL; s++) {
+        const double x0 = C[2 * size_t(geo_.BL[2 * s])], x1 = C[2 * size_t(geo_.BL[2 * s + 1])];
+        int lo = int(std::min(x0, x1)), hi = int(std::ceil(std::max(x0, x1)));
+        lo = std::max(lo, 0);
+        hi = std::min(hi, nX_ - 1);
+        for (int i = lo; i <= hi; i++) col[size_t(i)].push_back(s);
+      }
+      for (int i = 0; i < nX_; i++) {
+        start[size_t(i)] = int32_t(idx.size());
+        idx.insert(idx.end(), col[size_t(i)].begin(), col[size_t(i)].end());
+      }
+      start[size_t(nX_)] = int32_t(idx.size());
+    }
+    Buffer dC0, dC, dBL, dStart, dIdx, mark, q8, bits, lsm;
+    HIP_TRY(dC0.alloc(wallC0_.size() * sizeof(double)));
+    HIP_TRY(dC.alloc(C.size() * sizeof(double)));
+    HIP_TRY(dBL.alloc(geo_.BL.size() * sizeof(int32_t)));
+    HIP_TRY(dStart.alloc(start.size() * sizeof(int32_t)));
+    HIP_TRY(dIdx.alloc(std::max<size_t>(idx.size(), 1) * sizeof(int32_t)));
+    HIP_TRY(mark.alloc(D));
+    HIP_TRY(q8.alloc(8 * D * sizeof(double)));
+    HIP_TRY(bits.alloc(D));
+    HIP_TRY(hipMemcpy(dC0.as<void>(), wallC0_.data(), wallC0_.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dC.as<void>(), C.data(), C.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dBL.as<void>(), geo_.BL.data(), geo_.BL.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dStart.as<void>(), start.data(), start.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!idx.empty())
+      HIP_TRY(hipMemcpy(dIdx.as<void>(), idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    dev::WallSegments ws{dC0.as<double>(), dC.as<double>(), dBL.as<int32_t>(), nBL,
+                         dStart.as<int32_t>(), dIdx.as<int32_t>()};
+    dev::WallGrid wg{nX_, nY_, XF_, D_, geo_.input.minDx};
+    const dim3 rows((nX_ + dev::kBlockX - 1) / dev::kBlockX, nY_);
+    const dim3 flat(unsigned((D_ + dev::kBlockX - 1) / dev::kBlockX));
+    hipLaunchKernelGGL(dev::wallFoulMark, dim3((nBL + 63) / 64), dim3(64), 0, stream_, ws, wg,
+                       (const int16_t*)ntype_.as<int16_t>(), mark.as<uint8_t>());
+    hipLaunchKernelGGL((dev::wallFoulApply<R>), flat, dim3(dev::kBlockX), 0, stream_, wg,
+                       ntype_.as<int16_t>(), mark.as<uint8_t>(), rho_.as<R>(), ux_.as<R>(),
+                       uy_.as<R>(), ko_ ? K_.as<double>() : nullptr, ko_ ? O_.as<double>() : nullptr,
+                       ko_ ? nut_.as<double>() : nullptr);
+    hipLaunchKernelGGL(dev::wallBoundaryInfo, rows, dim3(dev::kBlockX), 0, stream_, wg,
+                       ntype_.as<int16_t>(), mask_.as<uint16_t>());
+    // the solver kernels never write solid nodes, and the k and omega rows
+    // keep their coupling to a solid neighbour (whose value is zero): a node
+    // that has turned solid must not keep the Krylov vectors' old values
+    if (ko_)
+      for (int v = 0; v < 2; v++)
+        for (Buffer* b : {&kr_[v], &krh_[v], &kp_[v][0], &kp_[v][1], &kv_[v][0], &kv_[v][1],
+                          &ks_[v], &kt_[v]})
+          HIP_TRY(hipMemsetAsync(b->as<void>(), 0, b->bytes(), stream_));
+    if (ko_)  // dXArr from the clean walls' spacings (clVariablesLS::update)
+      HIP_TRY(hipMemcpyAsync(dX_.as<void>(), dX0_.as<void>(), 4 * D * sizeof(double),
+                             hipMemcpyDeviceToDevice, stream_));
+    hipLaunchKernelGGL(dev::wallLinks, rows, dim3(dev::kBlockX), 0, stream_, ws, wg,
+                       (const int16_t*)ntype_.as<int16_t>(), mask_.as<uint16_t>(), q8.as<double>(),
+                       bits.as<uint8_t>(), ko_ ? dX_.as<double>() : nullptr);
+    hipLaunchKernelGGL(dev::solverPairs, flat, dim3(dev::kBlockX), 0, stream_, mask_.as<uint16_t>(),
+                       nX_, XF_, nY_);
+    if (ko_) {
+      HIP_TRY(lsm.alloc(geo_.lsMap.size() * sizeof(uint16_t)));
+      HIP_TRY(hipMemcpy(lsm.as<void>(), geo_.lsMap.data(), geo_.lsMap.size() * sizeof(uint16_t),
+                        hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(dev::wallDistance, rows, dim3(dev::kBlockX), 0, stream_, ws, wg,
+                         geo_.input.wallDSearchRad, (const uint16_t*)lsm.as<uint16_t>(),
+                         (const int16_t*)ntype_.as<int16_t>(), wallD_.as<double>());
+    }
+    HIP_TRY(hipGetLastError());
+    // the wall nodes in node order, their q rows
+    std::vector<uint8_t> hb(D);
+    HIP_TRY(hipMemcpyAsync(hb.data(), bits.as<void>(), D, hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipMemcpyAsync(geo_.nType.data(), ntype_.as<void>(), D * sizeof(int16_t),
+                           hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    std::vector<int32_t> hIdx(D, -1);
+    ibbNodesH_.clear();
+    for (size_t n = 0; n < D; n++)
+      if (hb[n]) {
+        hIdx[n] = int32_t(ibbNodesH_.size());
+        ibbNodesH_.push_back(int32_t(n));
+      }
+    nIbbNodes_ = int(ibbNodesH_.size());
+    const size_t nr = std::max<size_t>(ibbNodesH_.size(), 1);
+    HIP_TRY(ibbNodes_.alloc(nr * sizeof(int32_t)));
+    HIP_TRY(ibbQ_.alloc(8 * nr * sizeof(R)));
+    Buffer qrows;
+    HIP_TRY(qrows.alloc(8 * nr * sizeof(double)));
+    if (nIbbNodes_) {
+      HIP_TRY(hipMemcpy(ibbNodes_.as<void>(), ibbNodesH_.data(), ibbNodesH_.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice));
+      hipLaunchKernelGGL((dev::wallLinkGather<R>), dim3((nIbbNodes_ + 63) / 64), dim3(64), 0,
+                         stream_, (const double*)q8.as<double>(), D_,
+                         (const int32_t*)ibbNodes_.as<int32_t>(), nIbbNodes_, ibbQ_.as<R>(),
+                         qrows.as<double>());
+    }
+    HIP_TRY(hipMemcpy(ibbIdx_.as<void>(), hIdx.data(), D * sizeof(int32_t), hipMemcpyHostToDevice));
+    std::vector<double> hq(8 * size_t(nIbbNodes_));
+    if (nIbbNodes_)
+      HIP_TRY(hipMemcpyAsync(hq.data(), qrows.as<void>(), hq.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, stream_));
+    if (ko_) {
+      HIP_TRY(hipMemcpyAsync(geo_.dX.data(), dX_.as<void>(), 4 * D * sizeof(double),
+                             hipMemcpyDeviceToHost, stream_));
+      HIP_TRY(hipMemcpyAsync(geo_.wallD.data(), wallD_.as<void>(), D * sizeof(double),
+                             hipMemcpyDeviceToHost, stream_));
+    }
+    HIP_TRY(hipStreamSynchronize(stream_));
+    // the host copy of the link list, in node order (direction order within
+    // a node), and the tables the adjoint would need: the adjoint
+    // differentiates a built geometry only
+    geo_.ibb.clear();
+    geo_.ibbQ.clear();
+    slotLink_.assign(8 * size_t(nIbbNodes_), -1);
+    for (int r = 0; r < nIbbNodes_; r++)
+      for (int d = 0; d < 8; d++) {
+        const double q = hq[8 * size_t(r) + d];
+        if (q < 0.) continue;
+        slotLink_[8 * size_t(r) + d] = int32_t(geo_.ibb.size());
+        geo_.ibb.push_back(int32_t(ibbNodesH_[size_t(r)] + D_ * (d + 1)));
+        geo_.ibbQ.push_back(q);
+      }
+    geo_.ibbSeg.clear();
+    geo_.ibbDqDy.clear();
+    geo_.wallDLink.clear();
+    geo_.wallDCase.clear();
+    geo_.dxSlot.clear();
+    geo_.dxLink.clear();
+    wallsMoved_ = true;
+    // wall nodes and shear coefficients from the updated node types
+    int rc = refreshWallDerived(C, geo_, err);
+    if (!rc) rc = uploadShear(err);
+    if (rc) return rc;
+    gen_++;
+    dropGraphs();
+    run_.valid = false;
+    if (haveState_) {
+      curBuf_ = 0;
+      auto p = lbParams(0);
+      hipLaunchKernelGGL((dev::lbImportF<S, R>), gridXY(XF_), dim3(dev::kBlockX), 0, stream_,
+                         (const double*)keep.as<double>(), fA_.as<S>(), p);
+      HIP_TRY(hipGetLastError());
+      preIbb_ = false;
+    }
+    HIP_TRY(hipStreamSynchronize(stream_));
     return 0;
   }
 
   void setParams(const Params& p) override {
     par_ = p;
+    // kOmega::ini (kOmega.cpp:221-223)
+    stepsPerLB_ = std::max(1, int(1. / p.koDt));
+    dtCur_ = p.koDt;
+    sinceDtChange_ = 0;
+    gen_++;
     dropGraphs();
     run_.valid = false;
   }
@@ -248,11 +460,13 @@ class EngineT final : public Engine {
     HIP_TRY(upload<R>(uy_, v));
     if (ko_) HIP_TRY(hipMemsetAsync(S_.as<void>(), 0, S_.bytes(), stream_));
     preIbb_ = false;
+    prevValid_ = false;
     haveState_ = true;
     return 0;
   }
 
-  int initKOmega(double k, double w, double Re, double pipeRadius, std::string& err) override {
+  int initKOmega(double k, double w, double Re, double geomPenalty, double pipeRadius,
+                 std::string& err) override {
     if (!ko_) return fail(err, "k-omega is off for this context");
     if (!haveState_) return fail(err, "populations must be initialised before k and omega");
     if (k <= 0. || w <= 0.) {
@@ -262,7 +476,7 @@ class EngineT final : public Engine {
       if (!rc) rc = reduce(KOFM_FIELD_RHO, KOFM_RED_SUM, sro, err);
       if (rc) return rc;
       double um = sux / sro;
-      if (um == 0.) um = Re * par_.nu / pipeRadius / 4.;
+      if (um == 0.) um = Re * par_.nu / pipeRadius / 4. * geomPenalty;
       const double reIni = pipeRadius * 4. * um / par_.nu;
       const double ti = 0.16 * std::pow(reIni, -1. / 8.);
       const double lInv = 1. / (0.036 * pipeRadius);
@@ -343,21 +557,37 @@ class EngineT final : public Engine {
       s++;
     }
     HIP_TRY(hipGetLastError());
+    if (badLaunch_) {
+      badLaunch_ = false;
+      return fail(err, "internal: a launch shape this storage type cannot take was selected");
+    }
     return 0;
   }
 
   void setGraphs(bool on) override {
-    useGraphs_ = on;
+    useGraphs_ = on && !kSR;
     dropGraphs();
   }
 
-  int timeSteps(int n, unsigned options, double& msStep, double& msKernel, std::string& err) override {
+  int timeSteps(int n, unsigned options, double& msStep, double* msKernel, std::string& err) override {
     if (!haveState_) return fail(err, "state not initialised");
     if (n <= 0) return fail(err, "nsteps must be positive");
     while (int(evs_.size()) < 2 * n + 2) {
       hipEvent_t e;
       HIP_TRY(hipEventCreate(&e));
       evs_.push_back(e);
+    }
+    if (!msKernel) {
+      // the step alone: no events inside the timed window
+      HIP_TRY(hipEventRecord(evs_[0], stream_));
+      int rc = step(n, options, err);
+      if (rc) return rc;
+      HIP_TRY(hipEventRecord(evs_[1], stream_));
+      HIP_TRY(hipEventSynchronize(evs_[1]));
+      float t = 0.f;
+      HIP_TRY(hipEventElapsedTime(&t, evs_[0], evs_[1]));
+      msStep = double(t) / n;
+      return 0;
     }
     if (ko_ && useGraphs_) {
       // whole captured steps; the collide-stream is not timed on its own
@@ -371,7 +601,7 @@ class EngineT final : public Engine {
       float t = 0.f;
       HIP_TRY(hipEventElapsedTime(&t, evs_[0], evs_[1]));
       msStep = double(t) / n;
-      msKernel = -1.;
+      *msKernel = -1.;
       return 0;
     }
     HIP_TRY(hipEventRecord(evs_[0], stream_));
@@ -400,7 +630,7 @@ class EngineT final : public Engine {
       HIP_TRY(hipEventElapsedTime(&t, evs_[2 + 2 * k], evs_[3 + 2 * k]));
       acc += t;
     }
-    msKernel = acc / n;  // collide-stream time per step
+    *msKernel = acc / n;  // collide-stream time per step
     HIP_TRY(hipGetLastError());
     return 0;
   }
@@ -483,8 +713,10 @@ class EngineT final : public Engine {
       s.res0[v] = st.r0[v];
       s.res[v] = st.rn[v];
     }
-    s.nan = nanSeen_ || st.nan;
-    nanSeen_ = false;
+    s.nan = nanCount_;
+    s.stepsPerLB = stepsPerLB_;
+    s.timeStep = dtCur_;
+    nanCount_ = 0;
     return 0;
   }
 
@@ -493,9 +725,38 @@ class EngineT final : public Engine {
     return 0;
   }
 
+  int komegaTimeStep(int action, int& stepsPerLB, double& dt, std::string& err) override {
+    if (!ko_) return fail(err, "k-omega is off for this context");
+    switch (action) {
+      case KOFM_DT_QUERY: break;
+      case KOFM_DT_HALVE:
+        if (stepsPerLB_ >= kMaxStepsPerLB) return fail(err, "k-omega time step at its minimum");
+        halveTimeStep();
+        HIP_TRY(hipGetLastError());
+        break;
+      case KOFM_DT_DOUBLE:  // kOmega::doubleTimeStep (kOmega.cpp:401-414)
+        sinceDtChange_ = 0;
+        if (stepsPerLB_ > 1) {
+          stepsPerLB_ /= 2;
+          dtCur_ *= 2.;
+        }
+        break;
+      case KOFM_DT_INJECT_NAN: injectNan_ = true; break;
+      default: return fail(err, "unknown k-omega time step action");
+    }
+    stepsPerLB = stepsPerLB_;
+    dt = dtCur_;
+    return 0;
+  }
+
   int objectiveRun(int n, int interval, const Objective& o, double& J, std::string& err) override {
     run_.valid = false;
     if (!haveState_) return fail(err, "state not initialised");
+    if (wallsMoved_)
+      return fail(err, "the adjoint differentiates a built geometry: rebuild it after a wall update");
+    if (kSR)
+      return fail(err, "the adjoint recomputes steps from checkpoints, which the stochastically "
+                       "rounded FP32 form does not reproduce; use f32, f16, bf16 or f64 storage");
     if (n < 1 || interval < 1 || o.window < 1 || o.window > n || o.xIn < 0 || o.xIn >= nX_ ||
         o.xOut < 0 || o.xOut >= nX_) {
       err = "objective run: need nsteps >= 1, interval >= 1, 1 <= window <= nsteps and "
@@ -600,10 +861,25 @@ class EngineT final : public Engine {
   hipStream_t stream_ = nullptr;
   Buffer fA_, fB_, mask_, ibbIdx_, ibbQ_, rho_, ux_, uy_;
   Buffer K_, O_, nut_, S_, wallD_, Dk_, Do_, F1_, dkdo_, dX_, Ak_, Ao_, bk_, bo_;
+  Buffer ntype_, dX0_;  // node types and clean-wall spacings (moving-wall update)
+  std::vector<double> wallC0_;  // the wall points of the built (clean) geometry
+  bool wallsMoved_ = false;     // the walls were updated since the geometry was built
+  Buffer Kprev_, Oprev_;  // k, omega at the start of the step (restart point)
   Buffer kr_[2], krh_[2], kp_[2][2], kv_[2][2], ks_[2], kt_[2];  // FP64 BiCGStab vectors, [0] = k
   Buffer ss_, cfss_, tau_, sInd_, wts_, tauBL_;
   Buffer status_, partial_, redPartial_, redOut_;
   int lastIters_ = 1;  // iterations the previous k-omega solve needed
+  // k-omega time step control (kOmega.cpp:31-73, 387-414): sub-steps per LB
+  // step, their time step, steps since the last change, and non-finite
+  // residuals seen since the last solver-stats query
+  int stepsPerLB_ = 1;
+  double dtCur_ = 1.;
+  int sinceDtChange_ = 0;
+  int nanCount_ = 0;
+  bool injectNan_ = false;  // the next solve reports a non-finite residual (tests)
+  bool prevValid_ = false;  // Kprev_/Oprev_ hold the start of the last k-omega step
+  static constexpr int kDtQuiet = 20;              // kOmega.h:22 timeBtwIncreaseTimeStep
+  static constexpr int kMaxStepsPerLB = 1 << 16;   // halvings give up beyond this
   dev::BicgState* hostStatus_ = nullptr;  // pinned copy of the solver state
   // adjoint: checkpoints of the objective run, the recomputed segment, the
   // two adjoint population levels, per-link buffers and the objective nodes
@@ -627,9 +903,18 @@ class EngineT final : public Engine {
   Geometry geo_;
   Params par_;
   int nIbbNodes_ = 0;
+  // Bumped whenever a device array an ensemble's launch tables point at is
+  // reallocated or a value they copy changes (geometry, plane padding,
+  // parameters, uploaded state): the ensemble then rebuilds its tables and
+  // graphs.
+  uint64_t gen_ = 0;
+  // ensembles this lattice is a member of (told when it is destroyed)
+  std::vector<Ensemble*> users_;
   int curBuf_ = 0;       // 0: fA holds the newest populations
+  uint32_t stepCount_ = 0;  // steps launched (the stochastic rounding's step index)
   bool preIbb_ = false;  // newest populations still need the wall-node correction
-  bool haveGeo_ = false, haveState_ = false, nanSeen_ = false;
+  bool haveGeo_ = false, haveState_ = false;
+  bool badLaunch_ = false;  // a launch shape the storage type cannot take reached a launch
   // captured blocks of plain LB steps, keyed by start buffer and options
   static constexpr int kGraphSteps = 32;
   bool useGraphs_ = true;
@@ -778,8 +1063,19 @@ class EngineT final : public Engine {
     for (int m = 0; m < len; m++) {
       const S* src = cur();
       auto p = lbParams(0);
+      if (stepsPerLB_ != 1) {
+        err = "the adjoint differentiates k-omega steps of one sub-step: the k-omega time step "
+              "must be the LB step (ko_dt = 1) and stay there (no non-finite residual) during "
+              "the objective run";
+        return KOFM_ERR_STATE;
+      }
       int rc = koFullStep(0, err);
       if (rc) return rc;
+      if (stepsPerLB_ != 1) {
+        err = "a non-finite residual halved the k-omega time step during the objective run; "
+              "the adjoint differentiates k-omega steps of one sub-step";
+        return KOFM_ERR_SOLVER_NAN;
+      }
       if (m >= w0)
         hipLaunchKernelGGL((dev::objectiveForward<S, R>), og, dim3(64), 0, stream_, src,
                            (const S*)cur(), mask_.as<uint16_t>(), ibbIdx_.as<int32_t>(),
@@ -870,8 +1166,13 @@ class EngineT final : public Engine {
       const int batch = std::min(std::max(1, lastIters_), 8);
       enqueueKoHead();
       enqueueIters(batch);
-      int rc = koFinish(batch, err);
+      bool nan = false;
+      int rc = koFinish(batch, nan, err);
       if (rc) return rc;
+      if (nan) {
+        err = "non-finite residual in the k-omega solve of the adjoint's recompute";
+        return KOFM_ERR_SOLVER_NAN;
+      }
       recordStep(m);
     }
     auto kp = koParams();
@@ -1258,6 +1559,7 @@ class EngineT final : public Engine {
     p.maxNut = R(par_.maxNut);
     p.saveMacros = (options & KOFM_OPTION_SAVE_MACRO_FIELDS) || ko_ ? 1 : 0;
     p.applyIbb = preIbb_ ? 1 : 0;
+    p.seed = stepCount_;
     return p;
   }
 
@@ -1268,7 +1570,7 @@ class EngineT final : public Engine {
     p.XF = XF_;
     p.D = D_;
     p.nu = par_.nu;
-    p.dt = par_.koDt;
+    p.dt = dtCur_;
     p.tiles.tilesX = (nX_ + dev::kBlockX - 1) / dev::kBlockX;
     p.tiles.nTiles = p.tiles.tilesX * nY_;
     p.tiles.per = (p.tiles.nTiles + 7) / 8;
@@ -1291,6 +1593,8 @@ class EngineT final : public Engine {
       case KOFM_FIELD_NUT: return {ko_ ? nut_.as<void>() : nullptr, true};
       case KOFM_FIELD_SXY: return {ko_ ? S_.as<void>() : nullptr, true};
       case KOFM_FIELD_WALLD: return {ko_ ? wallD_.as<void>() : nullptr, true};
+      case KOFM_FIELD_DIFF_K: return {ko_ ? Dk_.as<void>() : nullptr, true};
+      case KOFM_FIELD_DIFF_OMEGA: return {ko_ ? Do_.as<void>() : nullptr, true};
       case KOFM_FIELD_TAU_NODE: return {tau_.as<void>(), Rd};
       case KOFM_FIELD_TAU_BL: return {tauBL_.as<void>(), Rd};
       default: return {nullptr, false};
@@ -1303,6 +1607,7 @@ class EngineT final : public Engine {
     HIP_TRY(tmp.alloc(9 * D * sizeof(double)));
     HIP_TRY(hipMemcpy(tmp.as<void>(), f, 9 * D * sizeof(double), hipMemcpyHostToDevice));
     curBuf_ = 0;
+    gen_++;
     auto p = lbParams(0);
     hipLaunchKernelGGL((dev::lbImportF<S, R>), gridXY(XF_), dim3(dev::kBlockX), 0, stream_,
                        (const double*)tmp.as<double>(), fA_.as<S>(), p);
@@ -1401,7 +1706,12 @@ class EngineT final : public Engine {
       case 1536: launchT2PT<128, 12>(pA, pB, src, dst); break;
       case 1664: launchT2PT<64, 26>(pA, pB, src, dst); break;
       case 3584:
-        if constexpr (sizeof(S) <= 4) launchT2PT<128, 28>(pA, pB, src, dst);
+        // the 128 x 28 tile's LDS planes exceed a work-group's 160 KB with
+        // FP64 storage: setLaunch and autotune never select it there
+        if constexpr (sizeof(S) <= 4)
+          launchT2PT<128, 28>(pA, pB, src, dst);
+        else
+          badLaunch_ = true;
         break;
       default: launchT2T<64, 8>(pA, pB, src, dst); break;
     }
@@ -1414,9 +1724,11 @@ class EngineT final : public Engine {
     pA.saveMacros = 0;
     auto pB = lbParams(options);
     pB.applyIbb = 1;
+    pB.seed = pA.seed + 1;
     launchT2(lbBlock_, pA, pB, cur(), other());
     curBuf_ ^= 1;
     preIbb_ = true;
+    stepCount_ += 2;
   }
 
   int launchLB(unsigned options) {
@@ -1427,6 +1739,7 @@ class EngineT final : public Engine {
       launchLBBlock<false>(lbBlock_, lbNpt_, p, cur(), other());
     curBuf_ ^= 1;
     preIbb_ = true;
+    stepCount_++;
     return 0;
   }
 
@@ -1447,6 +1760,7 @@ class EngineT final : public Engine {
       HIP_TRY(hipStreamSynchronize(stream_));
     }
     P_ = P;
+    gen_++;
     dropGraphs();
     run_.valid = false;
     ckpt_.clear();
@@ -1472,6 +1786,8 @@ class EngineT final : public Engine {
     if (npt != 1 && npt != 2) return fail(err, "nodes per lane must be 1 or 2");
     if (npt == 2 && nX_ % 2) return fail(err, "two nodes per lane need an even nX");
     if (spp != 1 && spp != 2) return fail(err, "steps per pass must be 1 or 2");
+    if (spp == 2 && kSR)
+      return fail(err, "the FP32 storage form with FP64 collisions takes one step per pass");
     if (spp == 2 && (ko_ || npt != 1 || !t2Tile(block) || nX_ % t2Width(block)))
       return fail(err, "two steps per pass need k-omega off, nodes_per_lane 1 and a tile "
                        "(block_x) of 512, 640, 832, 1856, 1024, 1536, 1664 or (16- and 32-bit "
@@ -1537,7 +1853,7 @@ class EngineT final : public Engine {
             bestPad = pad;
           }
         }
-      if (!ko_)
+      if (!ko_ && !kSR)
         for (int block : {512, 640, 832, 1856, 1024, 1536, 1664, 3584}) {
           if (!t2Tile(block) || nX_ % t2Width(block)) continue;
           auto pA = p, pB = p;
@@ -1590,6 +1906,7 @@ class EngineT final : public Engine {
       }
     }
     V.partial = partial_.as<double>();
+    V.prev[0] = V.prev[1] = nullptr;
     return V;
   }
 
@@ -1597,8 +1914,10 @@ class EngineT final : public Engine {
   int solverBlocks() const { return std::min(8 * koParams().tiles.per, 2048); }
 
   // kOmega::Solve (kOmega.cpp:31-73), first part: diffusivities, then the
-  // coefficients with the first residual and the solver's stage 0.
-  void enqueueKoHead() {
+  // coefficients with the first residual and the solver's stage 0.  The
+  // first sub-step of a step also keeps k and omega as the restart point
+  // (copyToPrevSolution).
+  void enqueueKoHead(bool savePrev = false) {
     auto kp = koParams();
     const dim3 grid(8 * kp.tiles.per), blk(dev::kBlockX);
     const int nsb = solverBlocks();
@@ -1610,10 +1929,15 @@ class EngineT final : public Engine {
     hipLaunchKernelGGL(dev::bicgReset, dim3(1), dim3(64), 0, stream_, st, par_.relTol[0],
                        par_.absTol[0], par_.maxIters[0], par_.relTol[1], par_.absTol[1],
                        par_.maxIters[1]);
+    dev::BicgVecs V = bicgVecs();
+    if (savePrev) {
+      V.prev[0] = Kprev_.as<K>();
+      V.prev[1] = Oprev_.as<K>();
+    }
     hipLaunchKernelGGL((dev::koCoeffs<R>), dim3(nsb), blk, 0, stream_, mask_.as<uint16_t>(),
                        nut_.as<K>(), Dk_.as<K>(), Do_.as<K>(), F1_.as<K>(), dkdo_.as<K>(),
                        S_.as<K>(), rho_.as<R>(), ux_.as<R>(), uy_.as<R>(), dX_.as<K>(),
-                       wallD_.as<K>(), bicgVecs(), kp);
+                       wallD_.as<K>(), V, kp);
     hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_,
                        (const double*)partial_.as<double>(), nsb, st, 0);
   }
@@ -1640,35 +1964,109 @@ class EngineT final : public Engine {
     }
   }
 
-  // One full k-omega step.  The collide-stream, the coefficients and as many
-  // iterations as the previous step needed (at most 8) are replayed from a
-  // captured graph (or launched, with graphs off); then the host polls the
-  // done flags and adds chunks of four iterations until both systems have
-  // converged.
+  // One full k-omega step.  The collide-stream, the first sub-step's
+  // coefficients and as many iterations as the previous solve needed (at
+  // most 8) are replayed from a captured graph (or launched, with graphs
+  // off); then the host polls the done flags and adds chunks of four
+  // iterations until both systems have converged, and runs the remaining
+  // sub-steps of the k-omega time step.
   int koFullStep(unsigned options, std::string& err) {
+    koTimeControl();
     const int batch = std::min(std::max(1, lastIters_), 8);
     if (useGraphs_) {
-      const std::vector<int64_t> key = {2, curBuf_, preIbb_ ? 1 : 0, int64_t(options), batch};
+      const std::vector<int64_t> key = {2, curBuf_, preIbb_ ? 1 : 0, int64_t(options), batch,
+                                        stepsPerLB_};
       int rc = replay(key, [&] {
         launchLB(options);
-        enqueueKoHead();
+        enqueueKoHead(true);
         enqueueIters(batch);
       }, 1, err);
       if (rc) return rc;
     } else {
       launchLB(options);
-      enqueueKoHead();
+      enqueueKoHead(true);
       enqueueIters(batch);
     }
-    return koFinish(batch, err);
+    return koSubSteps(batch, err);
   }
 
   // kOmega::Solve alone (after a collide-stream launched by the caller).
   int koStep(std::string& err) {
+    koTimeControl();
     const int batch = std::min(std::max(1, lastIters_), 8);
-    enqueueKoHead();
+    enqueueKoHead(true);
     enqueueIters(batch);
-    return koFinish(batch, err);
+    return koSubSteps(batch, err);
+  }
+
+  // kOmega::Solve's time step control before the sub-steps: after
+  // kDtQuiet steps without a change the time step doubles back
+  // (doubleTimeStep, kOmega.cpp:401-414).
+  void koTimeControl() {
+    if (++sinceDtChange_ >= kDtQuiet) {
+      sinceDtChange_ = 0;
+      if (stepsPerLB_ > 1) {
+        stepsPerLB_ /= 2;
+        dtCur_ *= 2.;
+      }
+    }
+  }
+
+  // kOmega::halveTimeStep (kOmega.cpp:387-399): twice the sub-steps at half
+  // the time step, k and omega back to the start of the last step.
+  void halveTimeStep() {
+    if (prevValid_)
+      hipLaunchKernelGGL(dev::koRestore, dim3(unsigned((D_ + dev::kBlockX - 1) / dev::kBlockX)),
+                         dim3(dev::kBlockX), 0, stream_, mask_.as<uint16_t>(),
+                         (const double*)Kprev_.as<double>(), (const double*)Oprev_.as<double>(),
+                         K_.as<double>(), O_.as<double>(), D_);
+    stepsPerLB_ *= 2;
+    dtCur_ /= 2.;
+    sinceDtChange_ = 0;
+  }
+
+  // The sub-steps of kOmega::Solve (kOmega.cpp:41-72) after the first one
+  // was enqueued with `launched` iterations.  A non-finite residual in
+  // either solve restores k and omega from the start of the step, halves
+  // the time step and doubles the sub-steps (halveTimeStep,
+  // kOmega.cpp:387-399, via GMRESSolver::checkConvergence,
+  // GMRESSolver.cpp:1535-1546), then starts the sub-steps again.
+  int koSubSteps(int launched, std::string& err) {
+    int sub = 0;
+    while (true) {
+      bool nan = false;
+      int rc = koFinish(launched, nan, err);
+      if (rc) return rc;
+      prevValid_ = true;
+      if (injectNan_) {
+        injectNan_ = false;
+        nan = true;
+      }
+      if (nan) {
+        nanCount_++;
+        if (stepsPerLB_ >= kMaxStepsPerLB) {
+          err = "non-finite residual in the k-omega solve at the smallest time step";
+          return KOFM_ERR_SOLVER_NAN;
+        }
+        halveTimeStep();
+        sub = 0;
+      } else if (++sub >= stepsPerLB_) {
+        return 0;
+      }
+      const int batch = std::min(std::max(1, lastIters_), 8);
+      if (useGraphs_) {
+        const std::vector<int64_t> key = {3, batch, stepsPerLB_};
+        rc = replay(key, [&] {
+          enqueueKoHead(false);
+          enqueueIters(batch);
+        }, 0, err);
+        if (rc) return rc;
+      } else {
+        enqueueKoHead(false);
+        enqueueIters(batch);
+      }
+      launched = batch;
+    }
   }
 
   // Polls a solve on the vectors V until both systems are done (adding
@@ -1696,7 +2094,9 @@ class EngineT final : public Engine {
     return 0;
   }
 
-  int koFinish(int launched, std::string& err) {
+  // Polls the solve until both systems are done (adding chunks of four
+  // iterations); nan reports a non-finite residual.
+  int koFinish(int launched, bool& nan, std::string& err) {
     const int maxIt = std::max(par_.maxIters[0], par_.maxIters[1]);
     int it = launched;
     auto* st = status_.as<dev::BicgState>();
@@ -1704,18 +2104,14 @@ class EngineT final : public Engine {
     while (true) {
       HIP_TRY(hipMemcpyAsync(hostStatus_, st, sizeof(hs), hipMemcpyDeviceToHost, stream_));
       HIP_TRY(hipStreamSynchronize(stream_));
-      if (hs.nan) nanSeen_ = true;
-      if ((hs.done[0] && hs.done[1]) || it >= maxIt) break;
+      if (hs.nan || (hs.done[0] && hs.done[1]) || it >= maxIt) break;
       const int k = std::min(4, maxIt - it);
       enqueueIters(k);
       it += k;
     }
+    nan = hs.nan != 0;
     lastIters_ = std::max(hs.iters[0], hs.iters[1]);
     HIP_TRY(hipGetLastError());
-    if (nanSeen_) {
-      err = "non-finite residual in the k-omega solve";
-      return KOFM_ERR_SOLVER_NAN;
-    }
     return 0;
   }
 };
@@ -1728,8 +2124,14 @@ class EnsembleT final : public Ensemble {
  public:
   explicit EnsembleT(std::vector<EngineT<S, R>*> m) : m_(std::move(m)) {
     ko_ = !m_.empty() && m_[0]->komega();
+    for (auto* e : m_) e->users_.push_back(this);
   }
   ~EnsembleT() override {
+    for (auto* e : m_) {
+      if (!e) continue;
+      auto& u = e->users_;
+      u.erase(std::remove(u.begin(), u.end(), static_cast<Ensemble*>(this)), u.end());
+    }
     if (stream_) (void)hipStreamSynchronize(stream_);
     groups_.clear();
     if (done_) (void)hipEventDestroy(done_);
@@ -1742,7 +2144,17 @@ class EnsembleT final : public Ensemble {
     return regroup(err);
   }
 
+  void memberGone(const Engine* gone) override {
+    for (auto*& e : m_)
+      if (e == gone) e = nullptr;
+    for (auto& g : groups_)
+      for (auto*& e : g->m)
+        if (e == gone) e = nullptr;
+    dead_ = true;
+  }
+
   int64_t fluidNodes() const override {
+    if (dead_) return 0;
     int64_t n = 0;
     for (auto* e : m_)
       for (size_t i = 0; i < e->geo_.nType.size(); i++)
@@ -1758,6 +2170,7 @@ class EnsembleT final : public Ensemble {
   // (128 x 28); every member's nX a multiple of the tile width.
   static int tileWidth(int block) { return (block == 1536 || block == 3584) ? 128 : 64; }
   int setLaunch(int block, int spp, std::string& err) override {
+    if (dead_) return gone(err);
     if (spp == 1 && block != 128 && block != 256 && block != 512 && block != 1024)
       return fail(err, "ensemble block width must be 128, 256, 512 or 1024");
     if (spp == 2) {
@@ -1777,6 +2190,7 @@ class EnsembleT final : public Ensemble {
   }
 
   int setCacheBudget(int64_t bytes, std::string& err) override {
+    if (dead_) return gone(err);
     if (bytes < 0) return fail(err, "the cache budget must be >= 0");
     budget_ = bytes;
     return regroup(err);
@@ -1786,6 +2200,7 @@ class EnsembleT final : public Ensemble {
 
   int autotune(int steps, int& bestBlock, int& bestSpp, int64_t& bestBudget, double& bestMs,
                std::string& err) override {
+    if (dead_) return gone(err);
     for (auto* e : m_)
       if (!e->haveState_) return fail(err, "every member needs an initialised state");
     // save every member's state
@@ -1891,6 +2306,7 @@ class EnsembleT final : public Ensemble {
   // are independent, and a group whose populations fit the last-level cache
   // then streams its steps from there.
   int step(int n, unsigned options, std::string& err) override {
+    if (dead_) return gone(err);
     for (auto* e : m_) {
       if (!e->haveState_) return fail(err, "every member needs an initialised state");
       HIP_TRY(hipStreamSynchronize(e->stream_));
@@ -1911,10 +2327,15 @@ class EnsembleT final : public Ensemble {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(done_, stream_));
     for (auto* e : m_) HIP_TRY(hipStreamWaitEvent(e->stream_, done_, 0));
+    if (badLaunch_) {
+      badLaunch_ = false;
+      return fail(err, "internal: a launch shape this storage type cannot take was selected");
+    }
     return 0;
   }
 
   int timeSteps(int n, unsigned options, double& ms, std::string& err) override {
+    if (dead_) return gone(err);
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a));
     HIP_TRY(hipEventCreate(&b));
@@ -1950,6 +2371,7 @@ class EnsembleT final : public Ensemble {
     bool built = false;
     unsigned builtOptions = 0;
     std::vector<int> builtParity;
+    std::vector<uint64_t> builtGen;  // members' generations when the tables were built
 
     // k-omega members: the solver states, their gathered flags (pinned on
     // the host), and the captured steps keyed by the iteration predictions
@@ -1977,6 +2399,11 @@ class EnsembleT final : public Ensemble {
       for (auto* e : m) v.push_back(e->curBuf_);
       return v;
     }
+    std::vector<uint64_t> gens() const {
+      std::vector<uint64_t> v;
+      for (auto* e : m) v.push_back(e->gen_);
+      return v;
+    }
 
     int step(EnsembleT& en, int n, unsigned options, std::string& err) {
       hipStream_t stream_ = en.stream_;
@@ -1992,7 +2419,7 @@ class EnsembleT final : public Ensemble {
         advance(1);
         s++;
       }
-      if (!built || builtOptions != options || builtParity != parity()) {
+      if (!built || builtOptions != options || builtParity != parity() || builtGen != gens()) {
         int rc = fill(en, ents[0], options, 0, false, per == 2, totalTiles, err);
         if (!rc) rc = fill(en, ents[1], options, 1, false, per == 2, totalTiles, err);
         if (rc) return rc;
@@ -2000,6 +2427,7 @@ class EnsembleT final : public Ensemble {
         built = true;
         builtOptions = options;
         builtParity = parity();
+        builtGen = gens();
       }
       int k = 0;  // launches since the entries were built (selects ents[k & 1])
       while (n - s >= kGraphSteps) {
@@ -2097,6 +2525,11 @@ class EnsembleT final : public Ensemble {
     err = msg;
     return KOFM_ERR_STATE;
   }
+  static int gone(std::string& err) {
+    return fail(err, "an ensemble member was destroyed: the ensemble cannot step any more");
+  }
+  bool dead_ = false;  // a member was destroyed
+  bool badLaunch_ = false;  // a launch shape the storage type cannot take reached a launch
 
   // First-fit decreasing packing of the members into groups under budget_:
   // the population buffers, and with k-omega the FP64 k-omega planes (the
@@ -2144,6 +2577,8 @@ class EnsembleT final : public Ensemble {
         if constexpr (sizeof(S) <= 4)
           hipLaunchKernelGGL((dev::lbCollideStreamBatchT2<S, R, 128, 28, true>), grid,
                              dim3(dev::t2pLanes<128, 28>()), 0, stream_, ents, n, totalTiles);
+        else
+          badLaunch_ = true;  // setLaunch never selects it for FP64
       } else
         hipLaunchKernelGGL((dev::lbCollideStreamBatchT2<S, R, 64, 13, false>), grid,
                            dim3(dev::t2Lanes<64, 13>()), 0, stream_, ents, n, totalTiles);
@@ -2176,6 +2611,10 @@ class EnsembleT final : public Ensemble {
   // chunks of four more iterations for the members not yet converged.
   int stepKO(Group& g, int n, unsigned options, std::string& err) {
     const int nm = int(g.m.size());
+    for (auto* e : g.m)
+      if (e->stepsPerLB_ != 1)
+        return fail(err, "ensemble k-omega members take one k-omega sub-step per LB step "
+                         "(ko_dt = 1, time step not halved)");
     if (g.koStates.bytes() == 0) {
       std::vector<const dev::BicgState*> ptrs;
       for (auto* e : g.m) ptrs.push_back(e->status_.template as<dev::BicgState>());
@@ -2223,7 +2662,8 @@ class EnsembleT final : public Ensemble {
         // parity has flipped g.koShift times since
         std::vector<int> expect(g.builtParity);
         for (int& v : expect) v ^= g.koShift & 1;
-        if (!g.built || g.builtOptions != options || expect != g.parity()) {
+        if (!g.built || g.builtOptions != options || expect != g.parity() ||
+            g.builtGen != g.gens()) {
           int rc = g.fill(*this, g.ents[0], options, 0, false, false, g.totalTiles, err);
           if (!rc) rc = g.fill(*this, g.ents[1], options, 1, false, false, g.totalTiles, err);
           if (rc) return rc;
@@ -2231,6 +2671,7 @@ class EnsembleT final : public Ensemble {
           g.built = true;
           g.builtOptions = options;
           g.builtParity = g.parity();
+          g.builtGen = g.gens();
           g.koShift = 0;
         }
         std::vector<int> key(batch);
@@ -2319,6 +2760,11 @@ std::unique_ptr<Ensemble> makeEnsemble(const std::vector<Engine*>& members, std:
     err = "empty ensemble";
     return nullptr;
   }
+  for (Engine* e : members)
+    if (dynamic_cast<EngineT<float, double>*>(e)) {
+      err = "the FP32 storage form with FP64 collisions (f32x) has no ensemble launch";
+      return nullptr;
+    }
   std::unique_ptr<Ensemble> e;
   if (!(e = tryEnsemble<float, float>(members, err)) && err.empty() &&
       !(e = tryEnsemble<_Float16, float>(members, err)) && err.empty() &&
@@ -2353,6 +2799,12 @@ std::unique_ptr<Engine> makeEngine(int nX, int nY, int storage, bool komega, int
     }
     case KOFM_STORE_F64: {
       auto* p = new EngineT<double, double>(nX, nY, komega);
+      e.reset(p);
+      rc = p->create(err);
+      break;
+    }
+    case KOFM_STORE_F32X: {
+      auto* p = new EngineT<float, double>(nX, nY, komega);
       e.reset(p);
       rc = p->create(err);
       break;

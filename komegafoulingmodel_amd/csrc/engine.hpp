@@ -19,7 +19,9 @@ struct Params {
 struct SolverStats {
   int iters[2] = {0, 0};
   double res0[2] = {0., 0.}, res[2] = {0., 0.};
-  int nan = 0;
+  int nan = 0;          // non-finite residuals since the last query
+  int stepsPerLB = 1;   // k-omega sub-steps per LB step
+  double timeStep = 1.; // their time step
 };
 
 // J = mean over the last `window` steps of a run of
@@ -37,20 +39,32 @@ class Engine {
  public:
   virtual ~Engine() = default;
   virtual int uploadGeometry(const Geometry& g, std::string& err) = 0;
+  // Moves the walls to the points C (the built geometry's points and links)
+  // and updates the node types, links, spacings, wall distances and
+  // wall-shear tables; the updated geometry is returned by geometry().
+  virtual int updateWalls(const std::vector<double>& C, std::string& err) = 0;
+  virtual const Geometry& geometry() const = 0;
   virtual void setParams(const Params& p) = 0;
   virtual int initState(const double* rho, const double* ux, const double* uy,
                         std::string& err) = 0;
-  virtual int initKOmega(double k, double w, double Re, double pipeRadius, std::string& err) = 0;
+  virtual int initKOmega(double k, double w, double Re, double geomPenalty, double pipeRadius,
+                         std::string& err) = 0;
   virtual int uploadState(const double* f, const double* k, const double* w, const double* nut,
                           const double* S, std::string& err) = 0;
   virtual int step(int n, unsigned options, std::string& err) = 0;
-  virtual int timeSteps(int n, unsigned options, double& msStep, double& msKernel,
+  // msKernel null: the steps run as step() does, timed as a whole; otherwise
+  // events also bracket every collide-stream launch (their own cost then
+  // lands in msStep too).
+  virtual int timeSteps(int n, unsigned options, double& msStep, double* msKernel,
                         std::string& err) = 0;
   virtual int wallShear(std::string& err) = 0;
   virtual int reduce(int field, int op, double& out, std::string& err) = 0;
   virtual int download(int field, double* host, std::string& err) = 0;
   virtual int solverStats(SolverStats& s, std::string& err) = 0;
   virtual int synchronize(std::string& err) = 0;
+  // KOFM_DT_* action on the k-omega time step; returns the sub-steps per LB
+  // step and their time step after it.
+  virtual int komegaTimeStep(int action, int& stepsPerLB, double& dt, std::string& err) = 0;
   virtual int64_t fieldSize(int field) const = 0;
   virtual int setLaunch(int block, int npt, int spp, int64_t pad, std::string& err) = 0;
   virtual void setGraphs(bool on) = 0;
@@ -67,6 +81,9 @@ class Engine {
 class Ensemble {
  public:
   virtual ~Ensemble() = default;
+  // A member's destructor calls this; every later call on the ensemble then
+  // fails with KOFM_ERR_STATE.
+  virtual void memberGone(const Engine* e) = 0;
   virtual int step(int n, unsigned options, std::string& err) = 0;
   virtual int timeSteps(int n, unsigned options, double& ms, std::string& err) = 0;
   virtual int setBlock(int block, std::string& err) = 0;

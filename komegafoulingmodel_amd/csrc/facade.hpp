@@ -34,6 +34,12 @@ class clProblem {
   unsigned StopTime = 0, Time = 0, RestartTime = 0;
   unsigned trSteps_wall = 3;
   unsigned saveAvgStartTime = 0, saveTimeStepAvg = 1000, dumpStartTime = 0, dumpTimeStep = 1000;
+  unsigned saveBinStep = 1;  // dumps per restart-file save ("Save Bin Step", clProblem.cpp:515)
+  // The cadence state a restart resumes from (clProblem.cpp:540-550, written
+  // by saveParameters :385-403): the next time-data row and dump, the dumps
+  // so far and the dump count at which the restart files are next written.
+  unsigned nextSaveAvgTime = 0, nextDumpStepTime = 0;
+  int currentDumpStep = 0, nextDumpBin = 1;
   RunParams params;
 
   void loadRunParameters(const std::string& path) {
@@ -52,6 +58,25 @@ class clProblem {
     trSteps_wall = unsigned(params.getInt(d, "LB Steps Per TR Wall", 3));
     cadence("Save Avgs", 2000, saveAvgStartTime, saveTimeStepAvg);
     cadence("Dump", 1000, dumpStartTime, dumpTimeStep);
+    const int bin = params.getInt(d, "Save Bin Step", 1);
+    saveBinStep = unsigned(bin > 0 ? bin : 1);
+    nextSaveAvgTime = unsigned(params.getInt(d, "nextSaveAvgTime", int(saveAvgStartTime + saveTimeStepAvg)));
+    nextDumpStepTime = unsigned(params.getInt(d, "nextDumpStepTime", int(dumpStartTime + dumpTimeStep)));
+    currentDumpStep = params.getInt(d, "currentDumpStep", 0);
+    nextDumpBin = params.getInt(d, "nextDumpBin", int(saveBinStep));
+  }
+
+  // clProblem::saveParameters (clProblem.cpp:256-405): RunParam.yaml with the
+  // current time, marked as a restart, into dir.
+  void saveParameters(const std::string& dir) {
+    const int d = RunParams::kSystem;
+    params.setInt(d, "Current Time", int(Time));
+    params.setBool(d, "Restart Run", true);
+    params.setInt(d, "nextSaveAvgTime", int(nextSaveAvgTime));
+    params.setInt(d, "nextDumpStepTime", int(nextDumpStepTime));
+    params.setInt(d, "currentDumpStep", currentDumpStep);
+    params.setInt(d, "nextDumpBin", nextDumpBin);
+    if (!params.save(dir + "/RunParam.yaml")) throw std::runtime_error("cannot write " + dir + "/RunParam.yaml");
   }
 
  private:
@@ -71,32 +96,49 @@ class clProblem {
 
 // Reducer<...>::reduceSingle / reduce / appendToFileFromDevice
 // (Reducer.h:88-290) for one field and operator, with the TimeData text
-// file "<name>.txt" (time, value rows; Array.h:2854-2904).
+// file "<name>.txt" (time, value rows; Array.h:2818-2904): reduce() keeps
+// one row per call and appends the rows to the file by itself whenever
+// maxLines of them are held (setTimeAndIncrement; the reference holds
+// OUTPUT_MAX_LINES = 128, ConstDef.h:320); appendToFileFromDevice() flushes
+// the rows held so far.
 class Reducer {
  public:
   Reducer() = default;
-  void ini(kofm_ctx* ctx, int field, int op, const std::string& name) {
+  void ini(kofm_ctx* ctx, int field, int op, const std::string& name, const std::string& dir = ".",
+           int maxLines = 128) {
     ctx_ = ctx;
     field_ = field;
     op_ = op;
     name_ = name;
+    dir_ = dir;
+    maxLines_ = maxLines > 0 ? maxLines : 1;
   }
+  void setDir(const std::string& dir) { dir_ = dir; }
   double reduceSingle() {
     double v = 0.;
     check(ctx_, kofm_reduce(ctx_, field_, op_, &v), "reduce");
     return v;
   }
-  void reduce(unsigned time) { rows_.push_back({double(time), reduceSingle()}); }
-  void appendToFileFromDevice(const std::string& dir = ".") {
-    std::ofstream out(dir + "/" + name_ + ".txt", std::ios::app);
+  // returns true when the rows were written out (the buffer was full)
+  bool reduce(unsigned time) {
+    rows_.push_back({double(time), reduceSingle()});
+    if (int(rows_.size()) < maxLines_) return false;
+    appendToFileFromDevice();
+    return true;
+  }
+  void appendToFileFromDevice() {
+    if (rows_.empty()) return;
+    std::ofstream out(dir_ + "/" + name_ + ".txt", std::ios::app);
     for (auto& r : rows_) out << r[0] << "\t" << r[1] << "\n";
     rows_.clear();
   }
+  size_t held() const { return rows_.size(); }
 
  private:
   kofm_ctx* ctx_ = nullptr;
   int field_ = 0, op_ = KOFM_RED_SUM;
-  std::string name_;
+  int maxLines_ = 128;
+  std::string name_, dir_ = ".";
   std::vector<std::vector<double>> rows_;
 };
 
@@ -165,6 +207,12 @@ class kOmega {
   double kappaMaxRelTol = 1e-4, kappaMaxAbsTol = 1e-7, omegaMaxRelTol = 1e-4, omegaMaxAbsTol = 1e-7;
   int kappaMaxIters = 1000, omegaMaxIters = 1000;
   double maxTurbVisc = 10., kIniVal = -1., omegaIniVal = -1.;
+  // initial velocity field of iniDists(Umax) (ConstDef.h:159-190 defaults)
+  bool iniTurbVelocity = true, perturbVelField = true;
+  double perturbDUPlus = 0.025, perturbEpsilon = 250., perturbBeta = 100., perturbAlpha = 55.75,
+         perturbSigma = 0.00055;
+  double roughnessFactor = 9.;  // read only: its use is commented out (kOmega.cpp:736-754)
+  kofm_ctx* ctx = nullptr;
 
   // kOmega::loadParams (kOmega.cpp:352-378).  The reference reads the omega
   // initial value from the key "K Initial Val" (kOmega.cpp:376); here it is
@@ -181,11 +229,25 @@ class kOmega {
     maxTurbVisc = rp.getDouble(d, "Max Turb Visc", 10.);
     kIniVal = rp.getDouble(d, "K Initial Val", -1.);
     omegaIniVal = rp.getDouble(d, "Omega Initial Val", -1.);
+    iniTurbVelocity = rp.getBool(d, "Ini Turb Velocity", true);
+    perturbVelField = rp.getBool(d, "Perturb Vel Field", true);
+    perturbDUPlus = rp.getDouble(d, "DUplus Perturb Multiplier", 0.025);
+    perturbEpsilon = rp.getDouble(d, "Epsilon Perturb Multiplier", 250.);
+    perturbBeta = rp.getDouble(d, "Beta Perturb Multiplier", 100.);
+    perturbAlpha = rp.getDouble(d, "Alpha Perturb Multiplier", 55.75);
+    perturbSigma = rp.getDouble(d, "Sigma Perturb Multiplier", 0.00055);
+    roughnessFactor = rp.getDouble(d, "Wall Roughness Factor", 9.);
   }
-  void ini(kofm_ctx* ctx, double Re) {
-    if (kOmegaSolverFlag) check(ctx, kofm_init_komega(ctx, kIniVal, omegaIniVal, Re), "k-omega init");
+  void ini(kofm_ctx* c, double Re, double geomPenaltyFactor) {
+    ctx = c;
+    if (kOmegaSolverFlag)
+      check(ctx, kofm_init_komega(ctx, kIniVal, omegaIniVal, Re, geomPenaltyFactor), "k-omega init");
   }
   void Solve() {}  // fused into clVariablesLB::Solve
+  // kOmega::halveTimeStep / doubleTimeStep (kOmega.cpp:387-414); kofm_step
+  // calls them itself on a non-finite residual and after 20 quiet steps
+  void halveTimeStep() { check(ctx, kofm_komega_time_step(ctx, KOFM_DT_HALVE, nullptr, nullptr), "halve"); }
+  void doubleTimeStep() { check(ctx, kofm_komega_time_step(ctx, KOFM_DT_DOUBLE, nullptr, nullptr), "double"); }
 };
 
 // clVariablesLB (clVariablesLB.h:40-458): fluid parameters, initial state,
@@ -195,6 +257,11 @@ class clVariablesLB {
   double tau = 1.99, MuVal = 0., Re = 10800., Fval = 0., RhoVal = 1.;
   bool tauDefined = true, reDefined = true;
   int tlbmIniDumpStep = 100000, tlbmNumIniSteps_LB = 1000000;
+  // ConstDef.h:159-192 defaults
+  bool saveAvgInfo = true, loadVelTxtFlag = true, iniPoiseuille = true, saveMacroStart = false;
+  bool turbVelBCFlag = true;  // accepted; its kernel code is commented out (lbKernels.cl:428-460)
+  double geomPenaltyFactor = 1.;
+  int lbOutSize = 128;        // TimeData rows kept before a flush ("lbOut Max Lines")
   int storage = KOFM_STORE_F32;
   kOmega kOmegaClass;
   Reducer sumUx, sumUy, sumRo, sumKappa, sumOmega, minKappa, minOmega;
@@ -204,10 +271,39 @@ class clVariablesLB {
     if (ctx) kofm_destroy(ctx);
   }
 
+  // Fluid-document keys the reference reads or writes (clVariablesLB.cpp:553-639,
+  // 652-700; kOmega.cpp:352-378, 480-520).  Any other key there is an error:
+  // it would otherwise be ignored without notice.
+  static bool knownFluidKey(const std::string& k) {
+    static const char* const keys[] = {
+        "ParamsName", "Use Turb Model", "Save Avg Info", "lbOut Max Lines", "load Vel Text", "Mu",
+        "Tau", "Re", "Fval", "Rho", "Ux inlet", "Nu air", "Rho air", "Save Macros On Start",
+        "Ini Poiseuille", "Run LBFD First", "Ini Dump Step", "Ini LB Steps", "Ini TLBM Steps",
+        "Num Intervals Per Avg", "Time Between Intervals", "Pause Between Adjust",
+        "Flowrate Max Percent Diff", "Use Turb Vel BC", "Geometric Penalty Factor",
+        "Ini Turb Velocity", "Perturb Vel Field", "DUplus Perturb Multiplier",
+        "Epsilon Perturb Multiplier", "Beta Perturb Multiplier", "Alpha Perturb Multiplier",
+        "Sigma Perturb Multiplier", "Wall Roughness Factor", "Kappa Max Rel Tol",
+        "Kappa Max Abs Tol", "Kappa Max Iterations", "Omega Max Rel Tol", "Omega Max Abs Tol",
+        "Omega Max Iterations", "Wall D Search Rad", "K Initial Val", "Omega Initial Val",
+        "Max Turb Visc", "Restart Run",
+        "Population Storage"};  // no reference counterpart: KOFM_STORE_* by name
+    for (const char* s : keys)
+      if (k == s) return true;
+    return false;
+  }
+
   // clVariablesLB::loadParams (clVariablesLB.cpp:553-639)
   void loadParams(const clProblem& p) {
     const RunParams& rp = p.params;
     const int d = RunParams::kFluid;
+    for (const std::string& k : rp.keys(d))
+      if (!knownFluidKey(k))
+        throw std::runtime_error("unsupported fluid parameter \"" + k + "\" in RunParam.yaml");
+    // the thermal (FD) warm-up is not part of this step
+    if (rp.getBool(d, "Run LBFD First", false))
+      throw std::runtime_error("\"Run LBFD First\" needs the thermal solver, which this build "
+                               "does not include");
     double mu = 0., t = 0.;
     switch (rp.either(d, "Mu", "Tau", mu, t)) {
       case 1: MuVal = mu; tau = 1. / (MuVal * 3. + 0.5); tauDefined = false; break;
@@ -224,14 +320,28 @@ class clVariablesLB {
     RhoVal = rp.getDouble(d, "Rho", 1.);
     tlbmIniDumpStep = rp.getInt(d, "Ini Dump Step", 100000);
     tlbmNumIniSteps_LB = rp.getInt(d, "Ini LB Steps", 1000000);
+    saveAvgInfo = rp.getBool(d, "Save Avg Info", true);
+    loadVelTxtFlag = rp.getBool(d, "load Vel Text", true);
+    iniPoiseuille = rp.getBool(d, "Ini Poiseuille", true);
+    saveMacroStart = rp.getBool(d, "Save Macros On Start", false);
+    turbVelBCFlag = rp.getBool(d, "Use Turb Vel BC", true);
+    geomPenaltyFactor = rp.getDouble(d, "Geometric Penalty Factor", 1.);
+    lbOutSize = rp.getInt(d, "lbOut Max Lines", 128);
+    if (lbOutSize < 1) throw std::runtime_error("lbOut Max Lines must be >= 1");
     const std::string st = rp.getString(d, "Population Storage", "f32");
-    storage = st == "f16" ? KOFM_STORE_F16 : st == "bf16" ? KOFM_STORE_BF16 : st == "f64" ? KOFM_STORE_F64 : KOFM_STORE_F32;
+    if (st == "f32") storage = KOFM_STORE_F32;
+    else if (st == "f16") storage = KOFM_STORE_F16;
+    else if (st == "bf16") storage = KOFM_STORE_BF16;
+    else if (st == "f64") storage = KOFM_STORE_F64;
+    else if (st == "f32x") storage = KOFM_STORE_F32X;
+    else throw std::runtime_error("Population Storage must be f32, f16, bf16, f64 or f32x");
     kOmegaClass.loadParams(rp);
   }
 
   // allocation + geometry + initial state (clVariablesLB::ini, :307-365);
   // restart from load/lbf.bin (+ lbkappa.bin, lbomega.bin) when present
-  void ini(const clProblem& p, clVariablesLS& ls, const std::string& loadDir = "load") {
+  void ini(const clProblem& p, clVariablesLS& ls, const std::string& loadDir = "load",
+           const std::string& outDir = ".") {
     kofm_grid g{p.nX, p.nY, storage, kOmegaClass.kOmegaSolverFlag ? 1 : 0};
     if (kofm_create(&g, &ctx) != KOFM_OK) throw std::runtime_error(kofm_last_error(nullptr));
     kofm_params q{};
@@ -251,24 +361,51 @@ class clVariablesLB {
     ls.ini(ctx, p);
     const size_t D = size_t(p.XsizeFull) * p.nY;
     std::vector<double> f;
-    if (readBin(loadDir + "/lbf.bin", 9 * D, f)) {
+    restarted = readBin(loadDir + "/lbf.bin", 9 * D, f);
+    if (restarted) {
       std::vector<double> k, w;
       const bool ko = kOmegaClass.kOmegaSolverFlag && readBin(loadDir + "/lbkappa.bin", D, k) &&
                       readBin(loadDir + "/lbomega.bin", D, w);
       check(ctx, kofm_upload_state(ctx, f.data(), ko ? k.data() : nullptr, ko ? w.data() : nullptr,
                                    nullptr, nullptr), "restart");
+      kOmegaClass.ctx = ctx;
     } else {
-      check(ctx, kofm_init_state(ctx, nullptr, nullptr, nullptr), "iniDists");
-      kOmegaClass.ini(ctx, Re);
+      // clVariablesLB::ini (clVariablesLB.cpp:311-333): velocity text files,
+      // else the Poiseuille / law-of-the-wall field, else rest
+      std::vector<double> ro(D, 0.), ux(D, 0.), uy(D, 0.);
+      const std::string rdir = restartDir(loadDir);
+      if (loadVelTxtFlag && readTxt2D(rdir + "/lbux.txt", p, ux) &&
+          readTxt2D(rdir + "/lbuy.txt", p, uy) && readTxt2D(rdir + "/lbro.txt", p, ro)) {
+        check(ctx, kofm_init_state(ctx, ro.data(), ux.data(), uy.data()), "iniDistsFromVel");
+      } else if (iniPoiseuille) {
+        kofm_ini_flow q{};
+        q.Re = Re;
+        q.nu = MuVal;
+        q.pipe_radius = p.Pipe_radius;
+        q.rho = RhoVal;
+        q.turbulent_profile = kOmegaClass.iniTurbVelocity ? 1 : 0;
+        q.perturb = kOmegaClass.perturbVelField ? 1 : 0;
+        q.duplus_mult = kOmegaClass.perturbDUPlus;
+        q.epsilon_mult = kOmegaClass.perturbEpsilon;
+        q.beta_mult = kOmegaClass.perturbBeta;
+        q.alpha_mult = kOmegaClass.perturbAlpha;
+        q.sigma_mult = kOmegaClass.perturbSigma;
+        q.seed = 1;  // clProblem.cpp:80
+        check(ctx, kofm_ini_poiseuille(ctx, &q, ro.data(), ux.data(), uy.data()), "iniDists(Umax)");
+        check(ctx, kofm_init_state(ctx, ro.data(), ux.data(), uy.data()), "iniDists(Umax)");
+      } else {
+        check(ctx, kofm_init_state(ctx, nullptr, nullptr, nullptr), "iniDists");
+      }
+      kOmegaClass.ini(ctx, Re, geomPenaltyFactor);
     }
-    sumUx.ini(ctx, KOFM_FIELD_UX, KOFM_RED_SUM, "redUx");
-    sumUy.ini(ctx, KOFM_FIELD_UY, KOFM_RED_SUM, "redUy");
-    sumRo.ini(ctx, KOFM_FIELD_RHO, KOFM_RED_SUM, "redRo");
+    sumUx.ini(ctx, KOFM_FIELD_UX, KOFM_RED_SUM, "redUx", outDir, lbOutSize);
+    sumUy.ini(ctx, KOFM_FIELD_UY, KOFM_RED_SUM, "redUy", outDir, lbOutSize);
+    sumRo.ini(ctx, KOFM_FIELD_RHO, KOFM_RED_SUM, "redRo", outDir, lbOutSize);
     if (kOmegaClass.kOmegaSolverFlag) {
-      sumKappa.ini(ctx, KOFM_FIELD_KAPPA, KOFM_RED_SUM, "redKappa");
-      sumOmega.ini(ctx, KOFM_FIELD_OMEGA, KOFM_RED_SUM, "redOmega");
-      minKappa.ini(ctx, KOFM_FIELD_KAPPA, KOFM_RED_MIN, "minKappa");
-      minOmega.ini(ctx, KOFM_FIELD_OMEGA, KOFM_RED_MIN, "minOmega");
+      sumKappa.ini(ctx, KOFM_FIELD_KAPPA, KOFM_RED_SUM, "redKappa", outDir, lbOutSize);
+      sumOmega.ini(ctx, KOFM_FIELD_OMEGA, KOFM_RED_SUM, "redOmega", outDir, lbOutSize);
+      minKappa.ini(ctx, KOFM_FIELD_KAPPA, KOFM_RED_MIN, "minKappa", outDir, lbOutSize);
+      minOmega.ini(ctx, KOFM_FIELD_OMEGA, KOFM_RED_MIN, "minOmega", outDir, lbOutSize);
     }
   }
 
@@ -302,9 +439,17 @@ class clVariablesLB {
   }
 
   void save2file(const std::string& dir) { check(ctx, kofm_save2file(ctx, dir.c_str()), "save2file"); }
-  void saveRestartFiles(const std::string& dir) { save2file(dir); }
+  // clVariablesLB/kOmega/clVariablesLS::saveRestartFiles: lbf.bin,
+  // lbkappa.bin, lbomega.bin, lsc.bin
+  void saveRestartFiles(const std::string& dir) {
+    check(ctx, kofm_save_restart(ctx, dir.c_str()), "saveRestartFiles");
+  }
+  void saveDebug(const std::string& dir) { check(ctx, kofm_save_debug(ctx, dir.c_str()), "saveDebug"); }
 
+  // clVariablesLB::updateTimeData / saveTimeData (clVariablesLB.cpp:768-790,
+  // 926-945): only with "Save Avg Info"
   void updateTimeData(unsigned time) {
+    if (!saveAvgInfo) return;
     sumUx.reduce(time);
     sumUy.reduce(time);
     sumRo.reduce(time);
@@ -313,10 +458,31 @@ class clVariablesLB {
       sumOmega.reduce(time);
     }
   }
-  void saveTimeData(const std::string& dir) {
-    for (Reducer* r : {&sumUx, &sumUy, &sumRo}) r->appendToFileFromDevice(dir);
+  void saveTimeData() {
+    if (!saveAvgInfo) return;
+    for (Reducer* r : {&sumUx, &sumUy, &sumRo}) r->appendToFileFromDevice();
     if (kOmegaClass.kOmegaSolverFlag)
-      for (Reducer* r : {&sumKappa, &sumOmega}) r->appendToFileFromDevice(dir);
+      for (Reducer* r : {&sumKappa, &sumOmega}) r->appendToFileFromDevice();
+  }
+
+  bool restarted = false;  // the state came from load/lbf.bin
+
+  // "restart_files" beside the load directory (the reference opens
+  // restart_files\\lbux relative to its working directory, which holds load\\)
+  static std::string restartDir(const std::string& loadDir) {
+    const size_t s = loadDir.find_last_of('/');
+    return (s == std::string::npos ? std::string(".") : loadDir.substr(0, s)) + "/restart_files";
+  }
+
+  // Array2D::loadtxt (Array.h:2644-2667): nX rows of nY values (the save2file
+  // layout) into the reference layout x + XsizeFull * y.
+  static bool readTxt2D(const std::string& path, const clProblem& p, std::vector<double>& v) {
+    std::ifstream in(path);
+    if (!in) return false;
+    for (int i = 0; i < p.nX; i++)
+      for (int j = 0; j < p.nY; j++)
+        if (!(in >> v[size_t(i) + size_t(p.XsizeFull) * j])) return false;
+    return true;
   }
 
  private:

@@ -72,6 +72,15 @@ class Builder {
     return 0;
   }
 
+  // The wall-node list and the shear tables again, for updated node types.
+  int refresh(std::string& err) {
+    nN_ = int(in_.C0.size() / 2);
+    nBL_ = int(in_.BL.size() / 2);
+    wallNodes();
+    if (!shearSetup(err)) return KOFM_ERR_INITIALIZING_VLS;
+    return 0;
+  }
+
  private:
   const GeometryInput& in_;
   Geometry& g_;
@@ -522,6 +531,13 @@ int buildGeometry(const GeometryInput& in, Geometry& out, std::string& err) {
   }
   Builder b(in, out);
   return b.run(err);
+}
+
+int refreshWallDerived(const std::vector<double>& C, Geometry& g, std::string& err) {
+  GeometryInput in = g.input;
+  in.C0 = C;
+  Builder b(in, g);
+  return b.refresh(err);
 }
 
 void wallFieldGradient(const Geometry& g, const double* wdBar, const double* dxBar,

@@ -96,6 +96,10 @@ struct Geometry {
 // KOFM_ERR_* code with a message in err.
 int buildGeometry(const GeometryInput& in, Geometry& out, std::string& err);
 
+// After a wall update (node types in g updated): the wall-node list and
+// the wall-shear link geometry, normals and weights for the wall points C.
+int refreshWallDerived(const std::vector<double>& C, Geometry& g, std::string& err);
+
 // Adds to dy (one entry per wall point) the y-sensitivity of the
 // wall distance field and of the axis wall spacings, weighted by wdBar (D
 // values, one per node) and dxBar (4*D, planes E, W, N, S):

@@ -33,6 +33,8 @@ enum : uint32_t {
   kSolid = 0x0001,         // M_SOLID_NODE
   kIbbNode = 0x0002,       // node owns at least one interpolated link
   kSolidBoundary = 0x0004, // SOLID_BOUNDARY_NODE (k, omega pinned to 0)
+  kPairUp = 0x0010,        // solver preconditioner: paired with the node above
+  kPairDown = 0x0020,      // solver preconditioner: paired with the node below
   kBoundMask = 0xFF00,     // E_BOUND .. NW_BOUND, reference positions
   kNesw = 0x0F00,          // NESW_BOUNDARY_NODE
 };
@@ -111,8 +113,50 @@ __device__ __forceinline__ R loadDev(const S* __restrict__ p, size_t idx, int di
   if constexpr (Scaled<S>::value) v *= R(unscale16(dir));
   return v;
 }
+// FP32 storage with FP64 collisions rounds its stores stochastically: a
+// forced flow changes each population by a nearly constant amount per step,
+// so round-to-nearest makes nearly the same error every step and the errors
+// add up (1e-4 of the velocity after 1e5 steps); with a rounding error of
+// zero mean that is uncorrelated with the value they only random-walk.
 template <class S, class R>
-__device__ __forceinline__ void storeDev(S* __restrict__ p, size_t idx, int dir, R g) {
+constexpr bool kStochastic = __is_same(S, float) && __is_same(R, double);
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+// floor(|v| + u) in units of the FP32 last place, sign kept, u in [0, 1)
+// from a hash of the destination slot (node and direction, independent of
+// the plane padding) and the step: the 29 mantissa bits a double has beyond
+// FP32 get a uniform addend and are then cleared, so the conversion to FP32
+// is exact.  The same slot and step give the same bits in every kernel and
+// launch shape.
+__device__ __forceinline__ float roundStochastic(double v, uint32_t slot, uint32_t seed) {
+  const uint32_t h = mix32((slot * 0x9E3779B1u) ^ mix32(seed ^ 0x5bd1e995u));
+  unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  b += h & ((1u << 29) - 1u);
+  b &= ~((1ull << 29) - 1ull);
+  return static_cast<float>(__longlong_as_double(static_cast<long long>(b)));
+}
+
+// slot of element idx of direction dir in planes of stride P, numbered
+// without the padding (node + 9 * node-count would not fit; the hash mixes
+// the node and the direction instead)
+__device__ __forceinline__ uint32_t srSlot(size_t idx, int dir, int64_t P) {
+  return (uint32_t(idx - size_t(P) * dir) << 4) ^ uint32_t(dir);
+}
+
+template <class S, class R>
+__device__ __forceinline__ void storeDev(S* __restrict__ p, size_t idx, int dir, R g,
+                                         uint32_t seed = 0, int64_t P = 0) {
+  if constexpr (kStochastic<S, R>) {
+    *popAt(p, idx) = roundStochastic(double(g), srSlot(idx, dir, P), seed);
+    return;
+  }
   if constexpr (Scaled<S>::value) g *= R(scale16(dir));
   *popAt(p, idx) = static_cast<S>(g);
 }
@@ -128,6 +172,7 @@ struct LBParams {
   R maxNut; // MAX_TURB_VISC
   int saveMacros;
   int applyIbb;
+  uint32_t seed;  // the step's index: the stochastic rounding of FP32 stores with FP64 collisions
 };
 
 // 1-D equilibrium deviation: phi(c,u) - w1(c)
@@ -365,12 +410,13 @@ __global__ __launch_bounds__(BX) void lbCollideStream(
   for (int i = 0; i < 9; i++) {
     const int a = cx(i), b = cy(i);
     if (i == 0) {
-      storeDev<S, R>(dst, gi, 0, g[0]);
+      storeDev<S, R>(dst, gi, 0, g[0], p.seed, p.P);
     } else if (m & boundBit(i)) {
-      storeDev<S, R>(dst, gi + size_t(p.P) * rev(i), rev(i), g[i]);
+      storeDev<S, R>(dst, gi + size_t(p.P) * rev(i), rev(i), g[i], p.seed, p.P);
     } else {
       const int tx = a > 0 ? xe : (a < 0 ? xw : x);
-      storeDev<S, R>(dst, size_t(tx) + size_t(p.XF) * (y + b) + size_t(p.P) * i, i, g[i]);
+      storeDev<S, R>(dst, size_t(tx) + size_t(p.XF) * (y + b) + size_t(p.P) * i, i, g[i], p.seed,
+                     p.P);
     }
   }
 }
@@ -488,13 +534,18 @@ __global__ __launch_bounds__(BX) void lbCollideStream2(
         vb *= R(scale16(slot[1]));
       }
       Pair<S> v;
-      v.a = static_cast<S>(va);
-      v.b = static_cast<S>(vb);
+      if constexpr (kStochastic<S, R>) {
+        v.a = roundStochastic(double(va), srSlot(t[0], slot[0], p.P), p.seed);
+        v.b = roundStochastic(double(vb), srSlot(t[1], slot[1], p.P), p.seed);
+      } else {
+        v.a = static_cast<S>(va);
+        v.b = static_cast<S>(vb);
+      }
       *reinterpret_cast<Pair<S>*>(popAt(dst, t[0])) = v;
     } else {
 #pragma unroll
       for (int n = 0; n < 2; n++)
-        if (live[n]) storeDev<S, R>(dst, t[n], slot[n], g[n][i]);
+        if (live[n]) storeDev<S, R>(dst, t[n], slot[n], g[n][i], p.seed, p.P);
     }
   }
 }
@@ -1028,7 +1079,8 @@ __global__ __launch_bounds__(kBlockX) void lbImportF(const double* __restrict__ 
   if (x >= p.XF) return;
   const size_t gi = size_t(x) + size_t(p.XF) * y;
 #pragma unroll
-  for (int i = 0; i < 9; i++) storeDev<S, R>(dst, gi + size_t(p.P) * i, i, R(in[gi + size_t(p.D) * i] - wgt(i)));
+  for (int i = 0; i < 9; i++)
+    storeDev<S, R>(dst, gi + size_t(p.P) * i, i, R(in[gi + size_t(p.D) * i] - wgt(i)), 0, p.P);
 }
 
 // Macros from populations (Calc_Ro_J lbKernels.cl:137-164, used on upload).
@@ -1154,13 +1206,21 @@ struct BicgState {
   double rho[2], rhoNew[2], alpha[2], omega[2], beta[2];
   double r0[2], rn[2];
   double rtol[2], atol[2];
+  double best[2];   // smallest residual so far
   int maxit[2];
   int done[2];
   int iters[2];
   int restart[2];
+  int sinceBest[2]; // iterations since the residual last fell below 0.999 best
   int nan;
   int cur;  // which p/v buffer holds the current iterate
 };
+
+// A solve whose residual has not fallen below 0.999 of its best for this
+// many iterations has reached the rounding floor of FP64 and stops: past
+// that point the recursively updated residual no longer follows b - A x,
+// and further iterations move x without improving it.
+constexpr int kBicgStagnation = 50;
 
 // Solver vectors are FP64 for every storage form: the omega field spans about
 // seven decades (wall rows near 1000, core values near 1e-4) and FP32 Krylov
@@ -1176,6 +1236,7 @@ struct BicgVecs {
   double* s[2];
   double* t[2];
   double* partial;     // 4 doubles per block
+  double* prev[2];     // when set, koCoeffs keeps the initial K, O here (restart point)
 };
 
 __device__ __forceinline__ void nbIndex(const KOParams& p, int x, size_t g, size_t (&nb)[4]) {
@@ -1263,6 +1324,8 @@ __global__ __launch_bounds__(1024) void bicgScalars(const double* __restrict__ p
       st->rhoNew[v] = a;
       st->beta[v] = a;
       st->restart[v] = 1;
+      st->best[v] = r0;
+      st->sinceBest[v] = 0;
       if (!isfinite(r0)) {
         st->nan = 1;
         st->done[v] = 1;
@@ -1293,6 +1356,13 @@ __global__ __launch_bounds__(1024) void bicgScalars(const double* __restrict__ p
         continue;
       }
       if (rn <= st->rtol[v] * st->r0[v] || rn <= st->atol[v] || st->iters[v] >= st->maxit[v]) {
+        st->done[v] = 1;
+        continue;
+      }
+      if (rn < 0.999 * st->best[v]) {
+        st->best[v] = rn;
+        st->sinceBest[v] = 0;
+      } else if (++st->sinceBest[v] >= kBicgStagnation) {
         st->done[v] = 1;
         continue;
       }
@@ -1337,6 +1407,7 @@ __global__ __launch_bounds__(kBlockX) void koCoeffs(
     for (int v = 0; v < 2; v++) {
       const double* A = v == 0 ? o.Ak : o.Ao;
       const double* X = V.x[v];
+      if (V.prev[v]) V.prev[v][g] = X[g];
 #pragma unroll
       for (int d = 0; d < 4; d++) V.A[v][g + size_t(d) * p.D] = A[d];
       const double b = v == 0 ? o.bk : o.bo;
@@ -1352,8 +1423,68 @@ __global__ __launch_bounds__(kBlockX) void koCoeffs(
   blockPartials(a, V.partial);
 }
 
-// p' = r + beta (p - omega v) (neighbour values recomputed), v' = A p';
-// partials (rh, v').  On a restart p' = r and rh is reset to r.
+// K, O of every non-solid node back from the copies koCoeffs kept at the
+// start of the step (kOmega::halveTimeStep's copyFromPrevSolution,
+// kOmega.cpp:387-399, GMRESSolver.cpp:1470-1476).
+__global__ __launch_bounds__(kBlockX) void koRestore(const uint16_t* __restrict__ mask,
+                                                     const double* __restrict__ pk,
+                                                     const double* __restrict__ po,
+                                                     double* __restrict__ K, double* __restrict__ O,
+                                                     int64_t D) {
+  const int64_t i = int64_t(blockIdx.x) * kBlockX + threadIdx.x;
+  if (i >= D || (mask[i] & kSolid)) return;
+  K[i] = pk[i];
+  O[i] = po[i];
+}
+
+// The solver's preconditioner pairs the two fluid nodes next to a wall in
+// each column: the first two above a solid stretch, and the last two below
+// one unless the lower of them is already paired.  Next to a curved or
+// fouled wall the implicit k rows are not diagonally dominant: the solid
+// neighbour's zero diffusivity enters the diffusivity gradient
+// (kOmegaKernels.cl:62-75, 235-257) at a spacing down to 0.1, and the
+// normalised coupling to the inner neighbour reaches hundreds.  Inverting
+// each such 2 x 2 block (M) makes BiCGStab converge; without it the
+// iteration stalls and its iterate drifts.  One thread per node; only the
+// pair bits of the node's own mask change.
+__global__ __launch_bounds__(kBlockX) void solverPairs(uint16_t* __restrict__ mask, int nX, int XF,
+                                                       int nY) {
+  const int64_t i = int64_t(blockIdx.x) * kBlockX + threadIdx.x;
+  if (i >= int64_t(XF) * nY) return;
+  const int x = int(i % XF), y = int(i / XF);
+  uint32_t m = mask[i] & ~uint32_t(kPairUp | kPairDown);
+  if (x < nX && !(m & kSolid)) {
+    auto S = [&](int k) { return y - k < 0 || (mask[i - int64_t(k) * XF] & kSolid) != 0; };
+    auto N = [&](int k) { return y + k >= nY || (mask[i + int64_t(k) * XF] & kSolid) != 0; };
+    if (S(1) && !N(1))
+      m |= kPairUp;  // first fluid node above a wall
+    else if (!S(1) && S(2))
+      m |= kPairDown;  // the second
+    else if (N(1) && !S(1) && !S(2) && !S(3))
+      m |= kPairDown;  // last fluid node below a wall, its partner unpaired
+    else if (!N(1) && N(2) && !S(1) && !S(2))
+      m |= kPairUp;  // the one before it
+  }
+  mask[i] = uint16_t(m);
+}
+
+// (M^-1 u)_i for u given by a function of the node index: with partner q,
+// (u_i - a_iq u_q) / (1 - a_iq a_qi), the couplings taken from the rows in
+// A (for a transposed system A holds the transposed rows, so this is M^T).
+template <class F>
+__device__ __forceinline__ double pairSolve(uint32_t m, const double* __restrict__ A, size_t i,
+                                            const KOParams& p, F u) {
+  if (!(m & (kPairUp | kPairDown))) return u(i);
+  const bool up = (m & kPairUp) != 0;
+  const size_t q = up ? i + size_t(p.XF) : i - size_t(p.XF);
+  const double aiq = A[i + size_t(up ? 2 : 3) * p.D];
+  const double aqi = A[q + size_t(up ? 3 : 2) * p.D];
+  return (u(i) - aiq * u(q)) / (1. - aiq * aqi);
+}
+
+// Right-preconditioned BiCGStab (van der Vorst 1992, Algorithm with M on the
+// right): p' = r + beta (p - omega v), v' = A M^-1 p' (neighbour values
+// recomputed); partials (rh, v').  On a restart p' = r and rh is reset to r.
 __global__ __launch_bounds__(kBlockX) void bicgPV(const uint16_t* __restrict__ mask, BicgVecs V,
                                                   const BicgState* __restrict__ st, KOParams p) {
   if (st->done[0] && st->done[1]) return;
@@ -1374,9 +1505,10 @@ __global__ __launch_bounds__(kBlockX) void bicgPV(const uint16_t* __restrict__ m
       const bool rs = st->restart[v] != 0;
       const double beta = st->beta[v], om = st->omega[v];
       auto pn = [&](size_t i) { return rs ? r[i] : r[i] + beta * (po[i] - om * vo[i]); };
+      auto ph = [&](size_t i) { return pairSolve(mask[i], A, i, p, pn); };
       const double pc = pn(g);
-      const double vc = pc + (A[g] * pn(nb[0]) + A[g + p.D] * pn(nb[1]) +
-                              A[g + 2 * p.D] * pn(nb[2]) + A[g + 3 * p.D] * pn(nb[3]));
+      const double vc = ph(g) + (A[g] * ph(nb[0]) + A[g + p.D] * ph(nb[1]) +
+                                 A[g + 2 * p.D] * ph(nb[2]) + A[g + 3 * p.D] * ph(nb[3]));
       V.p[v][cur ^ 1][g] = pc;
       V.v[v][cur ^ 1][g] = vc;
       double rh = V.rh[v][g];
@@ -1390,7 +1522,7 @@ __global__ __launch_bounds__(kBlockX) void bicgPV(const uint16_t* __restrict__ m
   blockPartials(a, V.partial);
 }
 
-// s = r - alpha v' (neighbours recomputed), t = A s; partials (t, s), (t, t).
+// s = r - alpha v' (neighbours recomputed), t = A M^-1 s; partials (t, s), (t, t).
 __global__ __launch_bounds__(kBlockX) void bicgST(const uint16_t* __restrict__ mask, BicgVecs V,
                                                   const BicgState* __restrict__ st, KOParams p) {
   if (st->done[0] && st->done[1]) return;
@@ -1409,9 +1541,10 @@ __global__ __launch_bounds__(kBlockX) void bicgST(const uint16_t* __restrict__ m
       const double* A = V.A[v];
       const double al = st->alpha[v];
       auto sn = [&](size_t i) { return r[i] - al * vv[i]; };
+      auto sh = [&](size_t i) { return pairSolve(mask[i], A, i, p, sn); };
       const double sc = sn(g);
-      const double tc = sc + (A[g] * sn(nb[0]) + A[g + p.D] * sn(nb[1]) +
-                              A[g + 2 * p.D] * sn(nb[2]) + A[g + 3 * p.D] * sn(nb[3]));
+      const double tc = sh(g) + (A[g] * sh(nb[0]) + A[g + p.D] * sh(nb[1]) +
+                                 A[g + 2 * p.D] * sh(nb[2]) + A[g + 3 * p.D] * sh(nb[3]));
       V.s[v][g] = sc;
       V.t[v][g] = tc;
       a[v][0] += tc * sc;
@@ -1421,7 +1554,7 @@ __global__ __launch_bounds__(kBlockX) void bicgST(const uint16_t* __restrict__ m
   blockPartials(a, V.partial);
 }
 
-// x += alpha p' + omega s, r = s - omega t; partials (rh, r), (r, r).
+// x += alpha M^-1 p' + omega M^-1 s, r = s - omega t; partials (rh, r), (r, r).
 __global__ __launch_bounds__(kBlockX) void bicgXR(const uint16_t* __restrict__ mask, BicgVecs V,
                                                   const BicgState* __restrict__ st, KOParams p) {
   if (st->done[0] && st->done[1]) return;
@@ -1429,13 +1562,18 @@ __global__ __launch_bounds__(kBlockX) void bicgXR(const uint16_t* __restrict__ m
   double a[2][2] = {{0., 0.}, {0., 0.}};
   forOwnTiles(p, [&](int x, int y) {
     const size_t g = size_t(x) + size_t(p.XF) * y;
-    if (mask[g] & kSolid) return;
+    const uint32_t m = mask[g];
+    if (m & kSolid) return;
 #pragma unroll
     for (int v = 0; v < 2; v++) {
       if (st->done[v]) continue;
       const double al = st->alpha[v], om = st->omega[v];
-      const double sc = V.s[v][g];
-      V.x[v][g] += al * V.p[v][nxt][g] + om * sc;
+      const double* pv = V.p[v][nxt];
+      const double* sv = V.s[v];
+      const double sc = sv[g];
+      const double ph = pairSolve(m, V.A[v], g, p, [&](size_t i) { return pv[i]; });
+      const double sh = pairSolve(m, V.A[v], g, p, [&](size_t i) { return sv[i]; });
+      V.x[v][g] += al * ph + om * sh;
       const double r = sc - om * V.t[v][g];
       V.r[v][g] = r;
       a[v][0] += V.rh[v][g] * r;

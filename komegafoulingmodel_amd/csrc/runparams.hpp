@@ -22,11 +22,17 @@ class RunParams {
     std::ifstream in(path);
     if (!in) return false;
     docs_.clear();
-    std::map<std::string, std::string> cur;
+    tags_.clear();
+    std::map<std::string, std::string> cur, curTags;
     std::string line;
     auto flush = [&]() {
-      if (cur.count("ParamsName")) docs_[std::stoi(cur["ParamsName"])] = cur;
+      if (cur.count("ParamsName")) {
+        const int doc = std::stoi(cur["ParamsName"]);
+        docs_[doc] = cur;
+        tags_[doc] = curTags;
+      }
       cur.clear();
+      curTags.clear();
     };
     while (std::getline(in, line)) {
       if (line.compare(0, 3, "---") == 0) {
@@ -39,11 +45,16 @@ class RunParams {
       if (colon == std::string::npos) continue;
       std::string key = trim(line.substr(0, colon));
       std::string val = trim(line.substr(colon + 1));
-      if (val.compare(0, 2, "!!") == 0) {  // drop a type tag
+      std::string tag;
+      if (val.compare(0, 2, "!!") == 0) {  // a type tag, kept for save()
         const size_t sp = val.find(' ');
+        tag = val.substr(0, sp);
         val = sp == std::string::npos ? "" : trim(val.substr(sp + 1));
       }
-      if (!key.empty()) cur[key] = val;
+      if (!key.empty()) {
+        cur[key] = val;
+        if (!tag.empty()) curTags[key] = tag;
+      }
     }
     flush();
     return true;
@@ -55,6 +66,15 @@ class RunParams {
   }
 
   std::string raw(int doc, const std::string& key) const { return docs_.at(doc).at(key); }
+
+  // Keys of one document (empty if the document is absent).
+  std::vector<std::string> keys(int doc) const {
+    std::vector<std::string> k;
+    auto d = docs_.find(doc);
+    if (d != docs_.end())
+      for (const auto& kv : d->second) k.push_back(kv.first);
+    return k;
+  }
 
   double getDouble(int doc, const std::string& key, double def) const {
     return has(doc, key) ? std::stod(raw(doc, key)) : def;
@@ -80,27 +100,41 @@ class RunParams {
     return a ? 1 : (b ? 2 : 0);
   }
 
-  void set(int doc, const std::string& key, const std::string& val) {
+  // Sets a value with its type tag ("!!int", "!!float" or none).
+  void set(int doc, const std::string& key, const std::string& tag, const std::string& val) {
     docs_[doc]["ParamsName"] = std::to_string(doc);
     docs_[doc][key] = val;
+    if (tag.empty())
+      tags_[doc].erase(key);
+    else
+      tags_[doc][key] = tag;
   }
   void setDouble(int doc, const std::string& key, double v) {
     char buf[64];
-    std::snprintf(buf, sizeof buf, "!!float %.8e", v);
-    set(doc, key, buf);
+    std::snprintf(buf, sizeof buf, "%.17g", v);
+    set(doc, key, "!!float", buf);
   }
-  void setInt(int doc, const std::string& key, int v) { set(doc, key, "!!int " + std::to_string(v)); }
-  void setBool(int doc, const std::string& key, bool v) { set(doc, key, v ? "True" : "False"); }
+  void setInt(int doc, const std::string& key, int v) { set(doc, key, "!!int", std::to_string(v)); }
+  void setBool(int doc, const std::string& key, bool v) { set(doc, key, "", v ? "True" : "False"); }
 
-  // Writes the documents back in the reference's layout (clProblem.cpp:256-405).
+  // Writes the documents back in the reference's layout (clProblem.cpp:256-405),
+  // each value with the type tag it was read or set with.
   bool save(const std::string& path) const {
     std::ofstream out(path);
     if (!out) return false;
     out << "# Parameters for Fouling simulation\n";
     for (const auto& d : docs_) {
       out << "---\nParamsName: " << d.first << "\n";
-      for (const auto& kv : d.second)
-        if (kv.first != "ParamsName") out << kv.first << ": " << kv.second << "\n";
+      auto t = tags_.find(d.first);
+      for (const auto& kv : d.second) {
+        if (kv.first == "ParamsName") continue;
+        out << kv.first << ": ";
+        if (t != tags_.end()) {
+          auto tk = t->second.find(kv.first);
+          if (tk != t->second.end()) out << tk->second << " ";
+        }
+        out << kv.second << "\n";
+      }
       out << "\n";
     }
     return bool(out);
@@ -108,6 +142,7 @@ class RunParams {
 
  private:
   std::map<int, std::map<std::string, std::string>> docs_;
+  std::map<int, std::map<std::string, std::string>> tags_;
   static std::string trim(const std::string& s) {
     const size_t a = s.find_first_not_of(" \t\r");
     if (a == std::string::npos) return "";

@@ -15,19 +15,22 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # with different code-generation flags); the default is the in-tree build.
 LIB_PATH = os.environ.get("KOFM_LIBRARY") or os.path.join(HERE, "libkofm.so")
 
-STORE_F32, STORE_F16, STORE_BF16, STORE_F64 = 0, 1, 2, 3
-STORAGE_NAMES = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16, "f64": STORE_F64}
+STORE_F32, STORE_F16, STORE_BF16, STORE_F64, STORE_F32X = 0, 1, 2, 3, 4
+STORAGE_NAMES = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16, "f64": STORE_F64,
+                 "f32x": STORE_F32X}
 OPTION_SAVE_MACRO_FIELDS = 0x2
 
 FIELD_F, FIELD_RHO, FIELD_UX, FIELD_UY = 0, 1, 2, 3
 FIELD_KAPPA, FIELD_OMEGA, FIELD_NUT, FIELD_SXY, FIELD_WALLD = 4, 5, 6, 7, 8
-FIELD_TAU_NODE, FIELD_TAU_BL = 9, 10
+FIELD_TAU_NODE, FIELD_TAU_BL, FIELD_DIFF_K, FIELD_DIFF_OMEGA = 9, 10, 11, 12
 FIELD_NAMES = {
     "f": FIELD_F, "rho": FIELD_RHO, "ux": FIELD_UX, "uy": FIELD_UY, "k": FIELD_KAPPA,
     "omega": FIELD_OMEGA, "nut": FIELD_NUT, "S": FIELD_SXY, "wallD": FIELD_WALLD,
-    "tau_node": FIELD_TAU_NODE, "tau_bl": FIELD_TAU_BL,
+    "tau_node": FIELD_TAU_NODE, "tau_bl": FIELD_TAU_BL, "diff_k": FIELD_DIFF_K,
+    "diff_omega": FIELD_DIFF_OMEGA,
 }
 RED_SUM, RED_MAX, RED_MIN = 1, 2, 4
+DT_QUERY, DT_HALVE, DT_DOUBLE, DT_INJECT_NAN = 0, 1, 2, 3
 
 GEOM = {
     "nType": (0, np.int16), "dX": (1, np.float64), "ibb": (2, np.int32), "ibbQ": (3, np.float64),
@@ -50,7 +53,8 @@ EXPORTS = [
     "kofm_ensemble_autotune",
     "kofm_ensemble_last_error",
     "kofm_ensemble_destroy",
-    "kofm_reduce_device",
+    "kofm_reduce_device", "kofm_komega_time_step", "kofm_ini_poiseuille",
+    "kofm_save_restart", "kofm_save_debug", "kofm_update_walls",
 ]
 
 
@@ -98,10 +102,19 @@ class Objective(C.Structure):
 ADJOINT_FROZEN_TURBULENCE = 1
 
 
+class IniFlow(C.Structure):
+    _fields_ = [("Re", C.c_double), ("nu", C.c_double), ("pipe_radius", C.c_double),
+                ("rho", C.c_double), ("turbulent_profile", C.c_int32), ("perturb", C.c_int32),
+                ("duplus_mult", C.c_double), ("epsilon_mult", C.c_double),
+                ("beta_mult", C.c_double), ("alpha_mult", C.c_double),
+                ("sigma_mult", C.c_double), ("seed", C.c_uint32)]
+
+
 class SolverStats(C.Structure):
     _fields_ = [("kappa_iters", C.c_int32), ("omega_iters", C.c_int32),
                 ("kappa_res0", C.c_double), ("kappa_res", C.c_double),
-                ("omega_res0", C.c_double), ("omega_res", C.c_double), ("nan_flag", C.c_int32)]
+                ("omega_res0", C.c_double), ("omega_res", C.c_double), ("nan_flag", C.c_int32),
+                ("steps_per_lb", C.c_int32), ("time_step", C.c_double)]
 
 
 _lib = None
@@ -127,18 +140,23 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "kofm_build_geometry": (C.c_int, [P, dp, C.c_int32, C.POINTER(C.c_int32), C.c_int32,
                                           C.POINTER(GeomParams)]),
         "kofm_geometry_counts": (C.c_int, [P, C.POINTER(GeomCounts)]),
+        "kofm_update_walls": (C.c_int, [P, dp, C.c_int32]),
         "kofm_geometry_array": (C.c_int, [P, C.c_int32, P]),
         "kofm_init_state": (C.c_int, [P, dp, dp, dp]),
-        "kofm_init_komega": (C.c_int, [P, C.c_double, C.c_double, C.c_double]),
+        "kofm_init_komega": (C.c_int, [P, C.c_double, C.c_double, C.c_double, C.c_double]),
         "kofm_upload_state": (C.c_int, [P, dp, dp, dp, dp, dp]),
         "kofm_step": (C.c_int, [P, C.c_int32, C.c_uint32]),
         "kofm_time_steps": (C.c_int, [P, C.c_int32, C.c_uint32, dp, dp]),
         "kofm_wall_shear": (C.c_int, [P]),
         "kofm_get_solver_stats": (C.c_int, [P, C.POINTER(SolverStats)]),
+        "kofm_komega_time_step": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int32), dp]),
+        "kofm_ini_poiseuille": (C.c_int, [P, C.POINTER(IniFlow), dp, dp, dp]),
         "kofm_reduce": (C.c_int, [P, C.c_int32, C.c_int32, dp]),
         "kofm_reduce_device": (C.c_int, [P, C.c_int32, C.c_int64, C.c_int32, P, P]),
         "kofm_download": (C.c_int, [P, C.c_int32, dp]),
         "kofm_save2file": (C.c_int, [P, C.c_char_p]),
+        "kofm_save_restart": (C.c_int, [P, C.c_char_p]),
+        "kofm_save_debug": (C.c_int, [P, C.c_char_p]),
         "kofm_synchronize": (C.c_int, [P]),
         "kofm_set_launch": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int64]),
         "kofm_set_graphs": (C.c_int, [P, C.c_int32]),
@@ -245,6 +263,13 @@ class Lattice:
             C.byref(p)))
         self._nN = C0.shape[0]
 
+    def update_walls(self, C: np.ndarray):
+        """Moves the walls to the points C [nN, 2] (a fouling layer grows
+        between the built walls and C): node types, links, spacings, wall
+        distances and shear tables are updated."""
+        C = np.ascontiguousarray(C, dtype=np.float64)
+        self._check(self.lib.kofm_update_walls(self.h, _dptr(C), C.shape[0]))
+
     def counts(self) -> GeomCounts:
         c = GeomCounts()
         self._check(self.lib.kofm_geometry_counts(self.h, C.byref(c)))
@@ -261,12 +286,24 @@ class Lattice:
         self._check(self.lib.kofm_geometry_array(self.h, which, out.ctypes.data_as(C.c_void_p)))
         return out[:n]
 
+    def ini_poiseuille(self, Re, nu, pipe_radius, rho=1.0, turbulent=False, perturb=False,
+                       duplus=0.025, epsilon=250.0, beta=100.0, alpha=55.75, sigma=0.00055,
+                       seed=1):
+        """(rho, ux, uy) of clVariablesLB::iniDists(Umax) on this geometry
+        (host only); nodes it does not visit are zero."""
+        out = [np.zeros(self.D) for _ in range(3)]
+        p = IniFlow(Re, nu, pipe_radius, rho, 1 if turbulent else 0, 1 if perturb else 0, duplus,
+                    epsilon, beta, alpha, sigma, seed)
+        self._check(self.lib.kofm_ini_poiseuille(self.h, C.byref(p), _dptr(out[0]),
+                                                 _dptr(out[1]), _dptr(out[2])))
+        return tuple(out)
+
     def init_state(self, rho=None, ux=None, uy=None):
         a = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (rho, ux, uy)]
         self._check(self.lib.kofm_init_state(self.h, *[_dptr(v) for v in a]))
 
-    def init_komega(self, k_ini=-1.0, omega_ini=-1.0, Re=0.0):
-        self._check(self.lib.kofm_init_komega(self.h, k_ini, omega_ini, Re))
+    def init_komega(self, k_ini=-1.0, omega_ini=-1.0, Re=0.0, geom_penalty_factor=1.0):
+        self._check(self.lib.kofm_init_komega(self.h, k_ini, omega_ini, Re, geom_penalty_factor))
 
     def upload_state(self, f, k=None, omega=None, nut=None, S=None):
         arrs = [None if v is None else np.ascontiguousarray(v, dtype=np.float64)
@@ -276,10 +313,14 @@ class Lattice:
     def step(self, n: int = 1, options: int = 0):
         self._check(self.lib.kofm_step(self.h, n, options))
 
-    def time_steps(self, n: int, options: int = 0):
+    def time_steps(self, n: int, options: int = 0, kernel: bool = True):
+        """(ms per step, ms per collide-stream kernel per step).  With
+        kernel=False no events sit inside the timed window and the kernel
+        figure is None."""
         a, b = C.c_double(), C.c_double()
-        self._check(self.lib.kofm_time_steps(self.h, n, options, C.byref(a), C.byref(b)))
-        return a.value, b.value
+        self._check(self.lib.kofm_time_steps(self.h, n, options, C.byref(a),
+                                             C.byref(b) if kernel else None))
+        return a.value, (b.value if kernel else None)
 
     def wall_shear(self):
         self._check(self.lib.kofm_wall_shear(self.h))
@@ -288,6 +329,14 @@ class Lattice:
         s = SolverStats()
         self._check(self.lib.kofm_get_solver_stats(self.h, C.byref(s)))
         return s
+
+    def komega_time_step(self, action: int = 0):
+        """KOFM_DT_* action (0 query, 1 halve, 2 double, 3 inject a
+        non-finite residual into the next solve); returns (sub-steps per LB
+        step, time step) after it."""
+        n, dt = C.c_int32(), C.c_double()
+        self._check(self.lib.kofm_komega_time_step(self.h, action, C.byref(n), C.byref(dt)))
+        return n.value, dt.value
 
     def reduce(self, field: str, op: int = RED_SUM) -> float:
         out = C.c_double()
@@ -303,6 +352,14 @@ class Lattice:
 
     def save2file(self, directory: str):
         self._check(self.lib.kofm_save2file(self.h, directory.encode()))
+
+    def save_restart(self, directory: str):
+        """lbf.bin, lbkappa.bin, lbomega.bin, lsc.bin (saveRestartFiles)."""
+        self._check(self.lib.kofm_save_restart(self.h, directory.encode()))
+
+    def save_debug(self, directory: str):
+        """nType.txt, dXarr_0..3.txt, lbnut.txt, wallD.txt (saveDebug)."""
+        self._check(self.lib.kofm_save_debug(self.h, directory.encode()))
 
     def set_launch(self, block_x: int, nodes_per_lane: int = 1, plane_pad: int = 1088,
                    steps_per_pass: int = 1):

@@ -140,12 +140,20 @@ class WallShapeProblem:
 
     def objective(self, theta) -> float:
         self.build(theta)
+        self._theta = np.array(theta, dtype=np.float64, copy=True)
         return self.lattice().objective_run(self.steps, self.interval, self.window,
                                             shear_weight=self.shear_weight,
                                             frozen_turbulence=self.frozen_turbulence)
 
-    def gradient(self) -> np.ndarray:
-        """dJ/dtheta of the last objective() call."""
+    def gradient(self, theta=None) -> np.ndarray:
+        """dJ/dtheta of the last objective() call; with theta given, of the
+        objective at theta (the objective run is repeated first when the last
+        one was at another theta)."""
+        if theta is not None:
+            theta = np.asarray(theta, dtype=np.float64)
+            last = getattr(self, "_theta", None)
+            if last is None or last.shape != theta.shape or not np.array_equal(last, theta):
+                self.objective(theta)
         dJdy, _ = self.lattice().adjoint()
         return self.dy_dtheta(self._C0).T @ dJdy
 
@@ -163,13 +171,17 @@ def _autograd_function():
 
         @staticmethod
         def forward(ctx, theta, problem):
-            J = problem.objective(theta.detach().to("cpu", torch.float64).numpy())
+            th = theta.detach().to("cpu", torch.float64).numpy().copy()
+            J = problem.objective(th)
             ctx.problem = problem
+            ctx.theta = th
             return theta.new_tensor(J)
 
         @staticmethod
         def backward(ctx, gJ):
-            g = torch.as_tensor(ctx.problem.gradient(), dtype=gJ.dtype, device=gJ.device)
+            # the problem may have evaluated another theta since this forward
+            # (a line search): gradient() then repeats this forward's run
+            g = torch.as_tensor(ctx.problem.gradient(ctx.theta), dtype=gJ.dtype, device=gJ.device)
             return gJ * g, None
 
     return WallObjective

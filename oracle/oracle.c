@@ -937,20 +937,69 @@ static double fluid_dot(const orc_par *p, const int16_t *nType, const double *a,
   return s;
 }
 
+/* The preconditioner's pairing of node (i, j): +1 paired with the node above,
+ * -1 with the node below, 0 unpaired.  In each column the first two fluid
+ * nodes above a solid stretch form a pair, and the last two below one unless
+ * the lower of them is already paired (nodes outside the lattice count as
+ * solid). */
+static int pair_dir(const orc_par *p, const int16_t *nType, int i, int j) {
+  if (nType[i + (long)p->XF * j] & M_SOLID_NODE) return 0;
+#define ORC_S(k) (j - (k) < 0 || (nType[i + (long)p->XF * (j - (k))] & M_SOLID_NODE))
+#define ORC_N(k) (j + (k) >= p->nY || (nType[i + (long)p->XF * (j + (k))] & M_SOLID_NODE))
+  int d = 0;
+  if (ORC_S(1) && !ORC_N(1))
+    d = 1;
+  else if (!ORC_S(1) && ORC_S(2))
+    d = -1;
+  else if (ORC_N(1) && !ORC_S(1) && !ORC_S(2) && !ORC_S(3))
+    d = -1;
+  else if (!ORC_N(1) && ORC_N(2) && !ORC_S(1) && !ORC_S(2))
+    d = 1;
+#undef ORC_S
+#undef ORC_N
+  return d;
+}
+
+/* z = M^-1 u: M is A's 2 x 2 blocks over the pairs (identity elsewhere), so
+ * z_g = (u_g - a_gq u_q) / (1 - a_gq a_qg) with q the partner. */
+static void pair_apply(const orc_par *p, const int16_t *nType, const int8_t *pr, const double *A,
+                       const double *u, double *z) {
+  const long D = (long)p->XF * p->nY;
+  for (long g = 0; g < D; g++) {
+    if (nType[g] & M_SOLID_NODE || (g % p->XF) >= p->nX) {
+      z[g] = u[g];
+      continue;
+    }
+    if (!pr[g]) {
+      z[g] = u[g];
+      continue;
+    }
+    long q = pr[g] > 0 ? g + p->XF : g - p->XF;
+    double agq = A[g + (pr[g] > 0 ? 2 : 3) * D], aqg = A[q + (pr[g] > 0 ? 3 : 2) * D];
+    z[g] = (u[g] - agq * u[q]) / (1. - agq * aqg);
+  }
+}
+
 /* Solves x_g + sum_m A[g+D*m] x_nb(m) = b_g on fluid rows (solid-boundary rows
  * are identity with b = 0, GMRESSolver.cpp:1421-1433, so x = 0 there; interior
- * solid rows are empty) from the current x, with unpreconditioned BiCGStab
- * (van der Vorst 1992) in FP64, until ||r|| <= rtol ||r0|| or ||r|| <= atol
- * (the criterion of GMRESSolver::checkConvergence, GMRESSolver.cpp:1535-1560),
- * or once the residual has stopped falling for 50 iterations (rounding floor).
- * Returns the iteration count, or -1 on a non-finite residual (the reference
- * halves its time step then).  work: 6*D doubles. */
+ * solid rows are empty) from the current x, with BiCGStab (van der Vorst
+ * 1992) right-preconditioned by the wall pairs' 2 x 2 blocks, in FP64, until
+ * ||r|| <= rtol ||r0|| or ||r|| <= atol (the criterion of
+ * GMRESSolver::checkConvergence, GMRESSolver.cpp:1535-1560), or once the
+ * residual has stopped falling for 50 iterations (rounding floor).  The
+ * reference runs GMRES(10) (ConstDef.h:20-21) without a preconditioner;
+ * converged, both give the solution of the same system.  Returns the
+ * iteration count, or -1 on a non-finite residual (the reference halves its
+ * time step then).  work: 8*D doubles plus D bytes. */
 int orc_ko_solve(const orc_par *p, const int16_t *nType, const double *A, const double *b,
                  double *x, double rtol, double atol, int maxit, double *res0, double *res,
                  double *work) {
   const long D = (long)p->XF * p->nY;
   double *r = work, *rh = work + D, *pp = work + 2 * D, *v = work + 3 * D, *s = work + 4 * D,
-         *t = work + 5 * D;
+         *t = work + 5 * D, *ph = work + 6 * D, *sh = work + 7 * D;
+  int8_t *pr = (int8_t *)(work + 8 * D);
+  for (int j = 0; j < p->nY; j++)
+    for (int i = 0; i < p->XF; i++) pr[i + (long)p->XF * j] = i < p->nX ? pair_dir(p, nType, i, j) : 0;
   ko_apply(p, nType, A, x, t);
   for (long g = 0; g < D; g++) {
     int fl = !(nType[g] & M_SOLID_NODE) && (g % p->XF) < p->nX;
@@ -975,16 +1024,18 @@ int orc_ko_solve(const orc_par *p, const int16_t *nType, const double *A, const 
     }
     double beta = (rho1 / rho) * (alpha / om);
     for (long g = 0; g < D; g++) pp[g] = r[g] + beta * (pp[g] - om * v[g]);
-    ko_apply(p, nType, A, pp, v);
+    pair_apply(p, nType, pr, A, pp, ph);
+    ko_apply(p, nType, A, ph, v);
     alpha = rho1 / fluid_dot(p, nType, rh, v);
     for (long g = 0; g < D; g++) s[g] = r[g] - alpha * v[g];
-    ko_apply(p, nType, A, s, t);
+    pair_apply(p, nType, pr, A, s, sh);
+    ko_apply(p, nType, A, sh, t);
     double tt = fluid_dot(p, nType, t, t);
     om = tt > 0. ? fluid_dot(p, nType, t, s) / tt : 0.;
     for (long g = 0; g < D; g++) {
       int fl = !(nType[g] & M_SOLID_NODE) && (g % p->XF) < p->nX;
       if (!fl) continue;
-      x[g] += alpha * pp[g] + om * s[g];
+      x[g] += alpha * ph[g] + om * sh[g];
       r[g] = s[g] - om * t[g];
     }
     rho = rho1;
@@ -1108,7 +1159,7 @@ typedef struct {
  * then start again from the first.  (The reference resumes its loop at the
  * second sub-step after running the k solve on rows assembled with the old
  * time step; here the restart is clean.)  The caller swaps FA/FB.  work:
- * 23*D doubles. */
+ * 26*D doubles. */
 int orc_step(const orc_par *p, const int16_t *nType, const int32_t *ibb, const double *q,
              int nIbb, const double *FA, double *FB, double *rho, double *ux, double *uy,
              double *K, double *O, const double *WallD, double *Nut, double *Sxy,
@@ -1119,8 +1170,8 @@ int orc_step(const orc_par *p, const int16_t *nType, const int32_t *ibb, const d
   if (!komega) return 0;
   double *Dk = work, *Do = work + D, *F1 = work + 2 * D, *dkdo = work + 3 * D;
   double *Ak = work + 5 * D, *Ao = Ak + 4 * D, *bk = Ao + 4 * D, *bo = bk + D;
-  double *sw = bo + D; /* solver work, 6*D */
-  double *Kp = sw + 6 * D, *Op = Kp + D; /* restart point: work must hold 23*D doubles */
+  double *sw = bo + D; /* solver work, 8*D doubles and D bytes */
+  double *Kp = sw + 9 * D, *Op = Kp + D; /* restart point: work must hold 26*D doubles */
   if (sv->steps_per_lb < 1) {
     sv->steps_per_lb = 1;
     sv->dt_cur = p->dt;

@@ -137,7 +137,7 @@ class OracleLattice:
         self.O = np.zeros(D)
         self.nut = np.zeros(D)
         self.S = np.zeros(3 * D)
-        self.work = np.zeros(23 * D)
+        self.work = np.zeros(26 * D)
         self.dX = np.ascontiguousarray(geo["dX"])
         self.wallD = np.ascontiguousarray(geo["wallD"])
         self.nType = geo["nType"]

@@ -283,6 +283,46 @@ def test_ensemble_matches_individual_runs(budget, groups, block, spp, storage):
     ens.close()
 
 
+@pytest.mark.parametrize("block,spp", [(512, 1), (1536, 2)])
+def test_ensemble_follows_member_changes_and_stops_after_a_member_is_destroyed(block, spp):
+    """Between ensemble steps one member is re-laid out with another plane
+    padding (new population arrays), one rebuilds its geometry (new wall-node
+    tables) and one takes new parameters; the ensemble rebuilds its launch
+    tables and graphs and stays bit-identical to the same changes made to solo
+    runs.  After a member is destroyed the ensemble refuses to step."""
+    from komegafoulingmodel_amd import lib
+    ws = [channel.fouled_channel(256, 48, Fval=1e-5, seed=31, amp=(0, 3), width=(10, 40)),
+          channel.fouled_channel(384, 64, Fval=2e-5, seed=32, amp=(0, 3), width=(10, 40)),
+          channel.flat_channel(128, 24, Fval=1e-5, seed=33)]
+    solo = [make_pair(w, "f32")[0] for w in ws]
+    group = [make_pair(w, "f32")[0] for w in ws]
+    ens = lib.Ensemble(group)
+    ens.set_launch(block, spp)
+
+    def run(n):
+        ens.step(n, 2)
+        for L in solo:
+            L.step(n, 2)
+        for i, (a, b) in enumerate(zip(group, solo)):
+            assert np.array_equal(a.download("f"), b.download("f")), (n, i)
+            assert np.array_equal(a.download("ux"), b.download("ux")), (n, i)
+
+    run(40)
+    for L in (group[0], solo[0]):
+        L.set_launch(256, 1, 64)
+    for L, w in ((group[1], ws[1]), (solo[1], ws[1])):
+        L.build_geometry(w.C0, w.BL, **w.geom_params())
+    for L, w in ((group[2], ws[2]), (solo[2], ws[2])):
+        L.set_params(w.tau0, 3e-5, w.nu)
+    run(37)
+    run(32)
+    group[2].close()
+    with pytest.raises(lib.KofmError) as e:
+        ens.step(1, 2)
+    assert e.value.code == -3 and "destroyed" in str(e.value)
+    ens.close()
+
+
 @pytest.mark.parametrize("budget,groups", [(0, 1), (1, 3)])
 def test_komega_ensemble_matches_individual_runs(budget, groups):
     """Three k-omega channels of different sizes stepped by the ensemble (per
@@ -400,3 +440,25 @@ def test_create_rejects_population_arrays_of_4gib_or_more():
         lib.Lattice(16384, 8192, storage="f64")
     assert e.value.code == -1
     assert "4 GiB" in str(e.value)
+
+
+def test_f32x_launch_shapes_and_oracle():
+    """The FP32 storage form with FP64 collisions and stochastically rounded
+    stores: one or two nodes per lane and every work-group width give
+    bit-identical populations (the rounding depends only on the destination
+    slot and the step), and the fouled mini channel stays within 1e-5 of the
+    FP64 oracle after 300 steps."""
+    w = _mini_fouled()
+    L, orc = make_pair(w, "f32x")
+    L.step(300, 2)
+    orc.step(300)
+    _compare_lb(L, orc, TOL32, "f32x fouled 300")
+    ref = L.download("f")
+    for block, npt, pad in ((128, 1, 0), (1024, 2, 64), (512, 2, 1088)):
+        L2, _ = make_pair(w, "f32x")
+        L2.set_launch(block, npt, pad)
+        L2.step(300, 2)
+        assert np.array_equal(L2.download("f"), ref), (block, npt)
+        L2.close()
+    with pytest.raises(Exception):
+        L.set_launch(1536, 1, 64, 2)  # one step per pass only

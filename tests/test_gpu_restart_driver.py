@@ -87,6 +87,8 @@ Tau: !!float {w.tau0!r}
 Fval: !!float {w.Fval!r}
 Use Turb Model: False
 Ini LB Steps: !!int 0
+Ini Poiseuille: False
+load Vel Text: False
 
 --- # Thermal Parameters
 ParamsName: 3
@@ -110,3 +112,168 @@ X Stop Pos: !!float {float(w.nX - 12)!r}
     err = rel_err(lbf, orc.FA)
     print(f"driver lbf vs oracle after 40 steps {err:.2e}", flush=True)
     assert err < 1e-5
+
+
+def test_driver_ini_poiseuille_and_save_macros_on_start(tmp_path):
+    """"Ini Poiseuille" (clVariablesLB.cpp:325-329, iniDists(Umax) :427-496)
+    through kofm_run: with the laminar profile and no perturbation the
+    initial populations are feq(Rho, u(y), 0) with the parabola
+    u(y) = 1.5 Re nu / R (1 - ((y - 0.5) - R)^2 / R^2) of a channel whose walls
+    sit at y = 0.5 and nY - 1.5; "Save Macros On Start" writes lbux.txt
+    (one row per x, nY tab-separated values, 6 significant digits)."""
+    w = channel.flat_channel(256, 40, Fval=1e-6, noise=0.0)
+    load, out = tmp_path / "load", tmp_path / "out"
+    load.mkdir()
+    out.mkdir()
+    channel.write_load_files(str(load), w.C0, w.BL)
+    Re = 5.0  # u_max = 1.5 Re nu / R = 0.039
+    (load / "RunParam.yaml").write_text(f"""# driver test
+--- # System Parameters
+ParamsName: 1
+nX: !!int {w.nX}
+nY: !!int {w.nY}
+Pipe Radius: !!float {w.pipe_radius!r}
+Stop Time: !!int 0
+
+--- # Fluid Parameters
+ParamsName: 2
+Tau: !!float {w.tau0!r}
+Re: !!float {Re!r}
+Use Turb Model: False
+Ini LB Steps: !!int 0
+Ini Poiseuille: True
+Ini Turb Velocity: False
+Perturb Vel Field: False
+load Vel Text: False
+Save Macros On Start: True
+
+--- # Particle Parameters
+ParamsName: 6
+X Release Pos: !!float 10.0
+X Stop Pos: !!float {float(w.nX - 12)!r}
+""")
+    exe = os.path.join(ROOT, "komegafoulingmodel_amd", "kofm_run")
+    res = subprocess.run([exe, str(load), str(out)], capture_output=True, text=True, timeout=120)
+    print(res.stdout[-400:], res.stderr[-400:], flush=True)
+    assert res.returncode == 0
+    R, nu = w.pipe_radius, w.nu
+    umax = 1.5 * Re * nu / R
+    XF, D = w.XF, w.D
+    lbf = np.fromfile(out / "lbf.bin", dtype=np.float64).reshape(9, D)
+    ux_txt = np.loadtxt(out / "lbux.txt")
+    assert ux_txt.shape == (w.nX, w.nY)
+    for y in range(1, w.nY - 1):
+        s = (y - 0.5) - R
+        u = umax * (1.0 - s * s / (R * R))
+        fe = oracle.feq(1.0, u, 0.0)
+        got = lbf[:, y * XF:y * XF + w.nX]
+        assert np.max(np.abs(got - fe[:, None])) < 1e-8, y
+        assert np.allclose(ux_txt[:, y], u, rtol=1e-5, atol=1e-12), y
+    assert np.all(ux_txt[:, 0] == 0.0) and np.all(ux_txt[:, -1] == 0.0)
+
+
+def _driver_yaml(w, stop, extra_sys="", extra_fluid=""):
+    return f"""# driver test
+--- # System Parameters
+ParamsName: 1
+nX: !!int {w.nX}
+nY: !!int {w.nY}
+Pipe Radius: !!float {w.pipe_radius!r}
+Stop Time: !!int {stop}
+{extra_sys}
+--- # LS Parameters
+ParamsName: 5
+LS Spacing: !!float 1.0
+nN: !!int {w.C0.shape[0]}
+
+--- # Fluid Parameters
+ParamsName: 2
+Tau: !!float {w.tau0!r}
+Fval: !!float {w.Fval!r}
+Use Turb Model: False
+Ini LB Steps: !!int 0
+Ini Poiseuille: False
+load Vel Text: False
+{extra_fluid}
+--- # Particle Parameters
+ParamsName: 6
+X Release Pos: !!float 10.0
+X Stop Pos: !!float {float(w.nX - 12)!r}
+"""
+
+
+def _run_driver(load, out):
+    exe = os.path.join(ROOT, "komegafoulingmodel_amd", "kofm_run")
+    res = subprocess.run([exe, str(load), str(out)], capture_output=True, text=True, timeout=120)
+    print(res.stdout[-300:], res.stderr[-300:], flush=True)
+    assert res.returncode == 0
+    return res
+
+
+def test_driver_dumps_restart_files_and_restarts_from_them(tmp_path):
+    """The dump sequence of clProblem::step (clProblem.cpp:130-168): every
+    dump writes the save2file text, the held time data and RunParam.yaml
+    (Current Time, Restart Run) into the output directory; every "Save Bin
+    Step" dumps the restart files (lbf.bin, lsc.bin, saveRestartFiles) and
+    RunParam.yaml go to the load directory.  Time data rows are held and
+    written every "lbOut Max Lines" rows.  A run of 40 steps restarted from
+    its load directory to 60 steps ends bit-identical to 60 steps in one
+    run."""
+    w = _mini()
+    sys_extra = "Dump Timestep: !!int 10\nSave Avgs Timestep: !!int 5\nSave Bin Step: !!int 2\n"
+    fl_extra = "lbOut Max Lines: !!int 3\n"
+    # one run of 60 steps
+    la, oa = tmp_path / "la", tmp_path / "oa"
+    la.mkdir(), oa.mkdir()
+    channel.write_load_files(str(la), w.C0, w.BL)
+    (la / "RunParam.yaml").write_text(_driver_yaml(w, 60, sys_extra, fl_extra))
+    _run_driver(la, oa)
+    # 40 steps, then a restart from the load directory to 60
+    lb, ob = tmp_path / "lb", tmp_path / "ob"
+    lb.mkdir(), ob.mkdir()
+    channel.write_load_files(str(lb), w.C0, w.BL)
+    (lb / "RunParam.yaml").write_text(_driver_yaml(w, 40, sys_extra, fl_extra))
+    _run_driver(lb, ob)
+    y = (ob / "RunParam.yaml").read_text()
+    assert "Current Time: !!int 40" in y and "Restart Run: True" in y
+    assert (lb / "RunParam.yaml").read_text() == y
+    assert np.array_equal(np.fromfile(lb / "lbf.bin"), np.fromfile(ob / "lbf.bin"))
+    # lsc.bin holds the wall points as read from lsc0.txt
+    assert np.array_equal(np.fromfile(lb / "lsc.bin").reshape(-1, 2), np.loadtxt(lb / "lsc0.txt"))
+    rows = np.loadtxt(ob / "redUx.txt")
+    assert rows.shape == (8, 2) and list(rows[:, 0]) == [5, 10, 15, 20, 25, 30, 35, 40]
+    (lb / "RunParam.yaml").write_text(y.replace("Stop Time: !!int 40", "Stop Time: !!int 60"))
+    ob2 = tmp_path / "ob2"
+    ob2.mkdir()
+    _run_driver(lb, ob2)
+    assert np.array_equal(np.fromfile(ob2 / "lbf.bin"), np.fromfile(oa / "lbf.bin"))
+    assert "Current Time: !!int 60" in (ob2 / "RunParam.yaml").read_text()
+
+
+def test_save_debug_and_restart_file_layouts(tmp_path):
+    """kofm_save_debug writes nType.txt (integers), dXarr_0..3.txt, lbnut.txt
+    and wallD.txt in the save2file layout (one row per x, nY values);
+    kofm_save_restart writes lbf.bin, lbkappa.bin, lbomega.bin and lsc.bin."""
+    nu, Re, H = 0.01, 500.0, 56.0
+    w = channel.wavy_channel(256, 72, H=H, amp=4.0, period=128.0, nu=nu, Re=Re,
+                             Fval=channel.turbulent_forcing(H, nu, Re), seed=7)
+    L, _ = make_pair(w, "f32")
+    L.step(3, 2)
+    L.save_debug(str(tmp_path))
+    L.save_restart(str(tmp_path))
+    nx, ny, XF = w.nX, w.nY, w.XF
+
+    def grid(a):
+        return np.asarray(a).reshape(-1, XF)[:ny, :nx].T
+
+    nt = np.loadtxt(tmp_path / "nType.txt", dtype=np.int64)
+    assert np.array_equal(nt, grid(L.geometry("nType").astype(np.int64)))
+    dX = L.geometry("dX").reshape(4, -1)
+    for k in range(4):
+        assert np.allclose(np.loadtxt(tmp_path / f"dXarr_{k}.txt"), grid(dX[k]), rtol=1e-5, atol=0)
+    assert np.allclose(np.loadtxt(tmp_path / "lbnut.txt"), grid(L.download("nut")), rtol=1e-5, atol=1e-300)
+    assert np.allclose(np.loadtxt(tmp_path / "wallD.txt"), grid(L.download("wallD")), rtol=1e-5, atol=0)
+    assert np.array_equal(np.fromfile(tmp_path / "lbf.bin"), L.download("f"))
+    assert np.array_equal(np.fromfile(tmp_path / "lbkappa.bin"), L.download("k"))
+    assert np.array_equal(np.fromfile(tmp_path / "lbomega.bin"), L.download("omega"))
+    assert np.array_equal(np.fromfile(tmp_path / "lsc.bin").reshape(-1, 2), w.C0)

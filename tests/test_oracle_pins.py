@@ -674,3 +674,61 @@ def test_solid_fluid_map_brute_force(shape):
                 continue
             fluid = lo < y < hi
             assert bool(nt[y, x] & 1) != fluid, (x, y, lo, hi)
+
+
+def test_komega_solve_matches_a_direct_solve_with_non_dominant_rows():
+    """The k system of a wavy k-omega channel at Re_H = 10,000 has rows next
+    to the walls whose normalised off-diagonal couplings sum to hundreds
+    (the solid neighbour's zero diffusivity enters the diffusivity gradient
+    at spacings down to 0.1, kOmegaKernels.cl:62-75, 235-257).  There
+    unpreconditioned BiCGStab stalls and its iterate drifts; the oracle's
+    solve, preconditioned by the wall pairs' 2 x 2 blocks, must reach the
+    solution of a sparse direct solve (scipy's SuperLU) of the same rows."""
+    import ctypes as C
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    nu, Re, H = 0.005, 10000.0, 56.0
+    w = channel.wavy_channel(256, 72, H=H, amp=4.0, period=128.0, nu=nu, Re=Re,
+                             Fval=channel.turbulent_forcing(H, nu, Re), seed=3)
+    g, orc = _lattice(w)
+    rho, ux, uy = w.initial_velocity(g["nType"])
+    orc.init_state(rho, ux, uy)
+    orc.init_komega(w.k_ini, w.omega_ini)
+    lib, D, XF, nX = orc.lib, g["D"], g["XF"], w.nX
+    P = oracle.p
+    Dk, Do, F1, dkdo = np.zeros(D), np.zeros(D), np.zeros(D), np.zeros(2 * D)
+    lib.orc_ko_diffusivities(C.byref(orc.par), P(orc.nType), P(orc.K), P(orc.O), P(orc.nut),
+                             P(orc.wallD), P(Dk), P(Do), P(F1), P(dkdo), P(orc.rho), P(orc.dX))
+    Ak, Ao, bk, bo = np.zeros(4 * D), np.zeros(4 * D), np.zeros(D), np.zeros(D)
+    lib.orc_ko_coeffs(C.byref(orc.par), P(orc.nType), P(orc.K), P(orc.O), P(orc.nut), P(Dk),
+                      P(Do), P(F1), P(dkdo), P(orc.S), P(orc.rho), P(orc.ux), P(orc.uy),
+                      P(orc.dX), P(orc.wallD), P(Ak), P(Ao), P(bk), P(bo))
+    fl = (g["nType"].astype(np.int32) & 1) == 0
+    A4 = Ak.reshape(4, D)
+    off = np.abs(A4).sum(axis=0)
+    assert np.sum(off[fl] > 1.0) > 50 and off[fl].max() > 100.0
+    # the same rows as a sparse matrix (solid neighbours hold x = 0)
+    rows = np.where(fl)[0]
+    idx = np.full(D, -1)
+    idx[rows] = np.arange(len(rows))
+    xs = rows % XF
+    nbs = [np.where(xs < nX - 1, rows + 1, rows - xs), np.where(xs > 0, rows - 1, rows + nX - 1),
+           rows + XF, rows - XF]
+    I, J, V = [np.arange(len(rows))], [np.arange(len(rows))], [np.ones(len(rows))]
+    for m in range(4):
+        col = idx[nbs[m]]
+        ok = col >= 0
+        I.append(np.arange(len(rows))[ok])
+        J.append(col[ok])
+        V.append(A4[m, rows][ok])
+    M = sp.csc_matrix((np.concatenate(V), (np.concatenate(I), np.concatenate(J))),
+                      shape=(len(rows), len(rows)))
+    xd = spl.spsolve(M, bk[rows])
+    x = orc.K.copy()
+    work = np.zeros(9 * D)
+    it = lib.orc_ko_solve(C.byref(orc.par), P(orc.nType), P(Ak), P(bk), P(x), C.c_double(1e-12),
+                          C.c_double(0.0), C.c_int(400), None, None, P(work))
+    err = np.max(np.abs(x[rows] - xd)) / np.max(np.abs(xd))
+    print(f"{int(np.sum(off[fl] > 1.0))} non-dominant rows, {it} iterations, {err:.2e} from the direct solve")
+    assert 0 < it < 200
+    assert err < 1e-9
