@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -28,7 +29,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-B_LU = {"f32": 74.0, "f16": 38.0, "bf16": 38.0, "f64": 146.0, "k32": 94.0, "k16": 58.0}
+B_LU = {"f32": 74.0, "f16": 38.0, "bf16": 38.0, "f64": 146.0, "f32x": 74.0, "k32": 94.0,
+        "k16": 58.0}
 
 
 def log(msg):
@@ -52,13 +54,19 @@ def setup(w, storage, komega=False, rtol=1e-4, maxit=1000, tune=True):
 
 
 def measure(L, fluid, steps, warmup, blu, options=0):
+    """Times exactly `steps` steps after `warmup` untimed ones: the window is
+    the step alone (no events inside it), bracketed by stream
+    synchronisation.  A second, separate pass of the same length brackets
+    every collide-stream launch with events for the kernel's own time."""
     L.step(warmup, options)
     L.synchronize()
     t0 = time.perf_counter()
-    ms_step, ms_kernel = L.time_steps(steps, options)
+    L.step(steps, options)
     L.synchronize()
     wall = time.perf_counter() - t0
     mlups = fluid * steps / wall / 1e6
+    ms_step, _ = L.time_steps(steps, options, kernel=False)
+    _, ms_kernel = L.time_steps(steps, options, kernel=True)
     # with k-omega the captured step is timed as a whole (ms_kernel < 0): the
     # algorithmic bytes are then set against the whole step
     achieved = blu * fluid / ((ms_kernel if ms_kernel > 0 else ms_step) * 1e-3) / 1e9
@@ -170,7 +178,7 @@ def measure_precision(name="c3", steps=100_000):
 
     w = channel.config(name)
     runs = {}
-    for st in ("f64", "f32", "f16", "bf16"):
+    for st in ("f64", "f32", "f32x", "f16", "bf16"):
         t0 = time.perf_counter()
         L, _ = setup(w, st)
         L.step(steps - 1)
@@ -185,7 +193,7 @@ def measure_precision(name="c3", steps=100_000):
     umax = float(np.max(np.hypot(ref["ux"], ref["uy"])))
     out = {"workload": f"{name}: {w.nX}x{w.nY}, k-omega off, seeded state", "steps": steps,
            "reference": "FP64 populations and arithmetic on the GPU", "max_u_fp64": umax}
-    for st in ("f32", "f16", "bf16"):
+    for st in ("f32", "f32x", "f16", "bf16"):
         r = runs[st]
         out[st] = {
             "u": float(np.max(np.hypot(r["ux"] - ref["ux"], r["uy"] - ref["uy"]))) / umax,
@@ -242,6 +250,29 @@ def measure_ensemble_ko(storage, steps=40):
     return out
 
 
+def measure_reference_step(L, fluid, steps=240, warmup=60):
+    """The reference's production step semantics: rho and u written every
+    step (OPTION_SAVE_MACRO_FIELDS, LBLS.cpp:348-349) and the wall shear of
+    nodes and boundary links every 3 steps (LBLS.cpp:237-241, 322-326),
+    timed over `steps` steps with no events inside the window."""
+    from komegafoulingmodel_amd import lib
+
+    def run(n):
+        for _ in range(n // 3):
+            L.step(3, lib.OPTION_SAVE_MACRO_FIELDS)
+            L.wall_shear()
+
+    run(warmup)
+    L.synchronize()
+    t0 = time.perf_counter()
+    run(steps)
+    L.synchronize()
+    wall = time.perf_counter() - t0
+    return {"mlups": fluid * steps / wall / 1e6, "ms_per_step": wall * 1e3 / steps,
+            "steps": steps, "fluid_nodes": fluid,
+            "semantics": "macros every step, wall shear every 3 steps"}
+
+
 def kernel_name(launch, storage="float"):
     if launch.get("steps_per_pass", 1) == 2:
         tile = {512: (64, 8), 640: (128, 5), 832: (64, 13), 1856: (64, 29), 1024: (64, 16),
@@ -288,7 +319,18 @@ def cpu_baseline(w, seconds=15.0, max_steps=40):
     orc.step(n)
     dt = time.perf_counter() - t0
     return dict(value=fluid * n / dt / 1e6, unit="MLUPS", cores=int(threads), kind="port",
-                sample=f"{n} FP64 steps of the {w.nX}x{w.nY} fouled channel (oracle, OpenMP)")
+                sample=f"{n} FP64 steps of the {w.nX}x{w.nY} fouled channel (oracle, OpenMP)",
+                cpu_model=cpu_model())
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def copy_calibration(nbytes=2 << 30, reps=20):
@@ -318,7 +360,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--forms", default="f16,bf16,prec,k32,k16,adj32,adj32_exact,adj32_exact_smooth,adj32_c3,ens32,ens16,ens32ko,ens16ko")
+    ap.add_argument("--forms", default="p32_ref,f32x,f16,bf16,prec,k32,k16,k32_c3,adj32,adj32_exact,adj32_exact_smooth,adj32_c3,ens32,ens16,ens32ko,ens16ko")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true")
@@ -348,12 +390,20 @@ def main():
         L.set_launch(b, n, pad, sp)
         L.launch = (b, n, pad, 0.0, sp)
     r = measure(L, fluid, args.steps, args.warmup, B_LU["f32"])
-    L.close()
     log(f"p32: {r}")
-
     forms = {"p32": r}
+    if "p32_ref" in args.forms.split(","):
+        forms["p32_ref"] = measure_reference_step(L, fluid)
+        forms["p32_ref"]["vs_p32"] = forms["p32_ref"]["mlups"] / r["mlups"]
+        log(f"p32_ref: {forms['p32_ref']}")
+    L.close()
     for form in [f for f in args.forms.split(",") if f]:
-        if form in ("f16", "bf16"):
+        if form == "f32x":
+            # FP32 storage, FP64 collisions, stochastically rounded stores
+            Lf, nf = setup(w, "f32x")
+            forms["p32x"] = measure(Lf, nf, args.steps, args.warmup, B_LU["f32x"])
+            Lf.close()
+        elif form in ("f16", "bf16"):
             Lf, nf = setup(w, form, tune=not args.launch_16bit)
             if args.launch_16bit:
                 b, n, sp, pad = (int(v) for v in args.launch_16bit.split(","))
@@ -362,15 +412,22 @@ def main():
             forms["p" + form[1:] if form == "f16" else "pbf16"] = measure(
                 Lf, nf, args.steps, args.warmup, B_LU[form])
             Lf.close()
-        elif form in ("k32", "k16"):
-            wk = channel.config("c2")
-            log(f"{form} on c2 {wk.nX}x{wk.nY}")
-            Lk, nk = setup(wk, "f32" if form == "k32" else "f16", komega=True)
+        elif form in ("k32", "k16", "k32_c3"):
+            wk = channel.config("c3" if form == "k32_c3" else "c2")
+            if form == "k32_c3":  # the roofline shape with k-omega on
+                wk.komega, wk.Re = True, 10000.0
+                wk.k_ini = 1.5 * (0.05 * 0.05) ** 2
+                wk.omega_ini = math.sqrt(wk.k_ini) / (0.09 ** 0.25 * 0.07 * (wk.nY - 2))
+            log(f"{form} on {wk.nX}x{wk.nY}")
+            Lk, nk = setup(wk, "f16" if form == "k16" else "f32", komega=True)
             # past the start-up transient (the first steps need tens of
             # Krylov iterations), then a timed window at the reference tolerances
-            forms[form] = measure(Lk, nk, 50, 200, B_LU[form])
+            forms[form] = measure(Lk, nk, 50, 200, B_LU["k16" if form == "k16" else "k32"])
             st = Lk.solver_stats()
             forms[form]["solver_sweeps_last"] = [st.kappa_iters, st.omega_iters]
+            # the achieved relative residuals of the last solves (tolerance 1e-4)
+            forms[form]["residual_last"] = [st.kappa_res / st.kappa_res0 if st.kappa_res0 else 0.0,
+                                            st.omega_res / st.omega_res0 if st.omega_res0 else 0.0]
             Lk.close()
         elif form in ("ens32", "ens16"):
             forms[form] = measure_ensemble("f32" if form == "ens32" else "f16")
@@ -403,6 +460,8 @@ def main():
         forms[k] = {a: (round(b, 6) if isinstance(b, float) else b) for a, b in v.items()}
     if "p16" in forms:
         forms["p16"]["speedup_vs_p32"] = round(forms["p16"]["mlups"] / r["mlups"], 3)
+    if "p32x" in forms:
+        forms["p32x"]["speedup_vs_p32"] = round(forms["p32x"]["mlups"] / r["mlups"], 3)
     if "pbf16" in forms:
         forms["pbf16"]["speedup_vs_p32"] = round(forms["pbf16"]["mlups"] / r["mlups"], 3)
 
@@ -412,6 +471,8 @@ def main():
         cpu = cpu_baseline(w, seconds=args.cpu_seconds)
 
     traffic = read_traffic("c3_p32", kernel_name(r["launch"]))
+    # the counters give bytes per launch, a launch makes steps_per_pass steps
+    launch_ms = r["ms_per_kernel"] * r["launch"]["steps_per_pass"]
     out = {
         "metric": "MLUPS + % HBM roofline: FP32 vs FP16-store fwd, adjoint cost ratio",
         "value": round(r["mlups"], 3),
@@ -431,8 +492,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 3),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(r["frac"], 4),
                      "traffic": traffic, "bytes_per_lu": B_LU["f32"],
+                     # HBM bytes the counters saw per launch over the launch time
+                     "frac_counters": (round(traffic / (launch_ms * 1e-3) / 1e9
+                                             / HBM_PEAK_GBS, 4) if traffic else None),
                      "kernel": kernel_name(r["launch"]),
-                     "kernel_ms": round(r["ms_per_kernel"], 6)},
+                     "kernel_ms": round(r["ms_per_kernel"], 6),
+                     "kernel_ms_per_launch": round(launch_ms, 6)},
         "cpu_baseline": cpu,
         "forms": forms,
     }
