@@ -147,6 +147,11 @@ typedef struct kofm_solver_stats {
                            halved the k-omega time step) */
   int32_t steps_per_lb; /* k-omega sub-steps per LB step (kOmega::stepsPerLB) */
   double time_step;     /* their time step (kOmega::timeStep) */
+  int32_t batches;      /* batches of 8 k-omega steps replayed from one graph, with one
+                           poll of the solver state, since the last query */
+  int32_t batch_rollbacks; /* of those, batches in which a solve needed more
+                              iterations than provisioned or met a non-finite
+                              residual: they were rerun one step at a time */
 } kofm_solver_stats;
 
 /* --- lifecycle ------------------------------------------------------------ */

@@ -314,6 +314,8 @@ int kofm_get_solver_stats(kofm_ctx* ctx, kofm_solver_stats* out) {
   out->nan_flag = s.nan;
   out->steps_per_lb = s.stepsPerLB;
   out->time_step = s.timeStep;
+  out->batches = s.batches;
+  out->batch_rollbacks = s.rollbacks;
   return KOFM_OK;
 }
 

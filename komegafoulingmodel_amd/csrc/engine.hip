@@ -97,6 +97,7 @@ class EngineT final : public Engine {
     dropGraphs();
     for (auto& e : evs_) (void)hipEventDestroy(e);
     if (hostStatus_) (void)hipHostFree(hostStatus_);
+    if (hostFlags_) (void)hipHostFree(hostFlags_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -552,8 +553,17 @@ class EngineT final : public Engine {
         s += 2;
         continue;
       }
+      if (ko_ && n - s >= kKoBatch) {
+        const int r = koBatch(options, err);
+        if (r < 0) return r;
+        if (r == 1) {
+          s += kKoBatch;
+          continue;
+        }
+      }
       int rc = ko_ ? koFullStep(options, err) : launchLB(options);
       if (rc) return rc;
+      if (ko_ && koBatchHold_ > 0) koBatchHold_--;
       s++;
     }
     HIP_TRY(hipGetLastError());
@@ -716,7 +726,10 @@ class EngineT final : public Engine {
     s.nan = nanCount_;
     s.stepsPerLB = stepsPerLB_;
     s.timeStep = dtCur_;
+    s.batches = batchCount_;
+    s.rollbacks = rollbackCount_;
     nanCount_ = 0;
+    batchCount_ = rollbackCount_ = 0;
     return 0;
   }
 
@@ -878,6 +891,14 @@ class EngineT final : public Engine {
   int nanCount_ = 0;
   bool injectNan_ = false;  // the next solve reports a non-finite residual (tests)
   bool prevValid_ = false;  // Kprev_/Oprev_ hold the start of the last k-omega step
+  // batches of k-omega steps (koBatch): the checkpoint of populations and
+  // fields, the per-step done flags and iteration counts (device, pinned
+  // host copy), and the steps still to run one at a time after a rollback
+  static constexpr int kKoBatch = 8, kKoSlack = 2, kKoBatchMaxIters = 16;
+  Buffer batchF_, batchKO_, batchM_, batchFlags_;
+  int* hostFlags_ = nullptr;
+  int koBatchHold_ = 0;
+  int batchCount_ = 0, rollbackCount_ = 0;
   static constexpr int kDtQuiet = 20;              // kOmega.h:22 timeBtwIncreaseTimeStep
   static constexpr int kMaxStepsPerLB = 1 << 16;   // halvings give up beyond this
   dev::BicgState* hostStatus_ = nullptr;  // pinned copy of the solver state
@@ -1988,6 +2009,101 @@ class EngineT final : public Engine {
       enqueueIters(batch);
     }
     return koSubSteps(batch, err);
+  }
+
+  // kKoBatch k-omega steps from one captured graph, the host polling once
+  // for all of them.  Each step gets the iterations the last solve needed
+  // plus kKoSlack (launches past convergence return at once) and records
+  // whether its solves finished.  When every one did, the result is the
+  // same as kKoBatch calls of koFullStep; otherwise (a solve needing more
+  // iterations, a non-finite residual) the populations and fields go back
+  // to the checkpoint taken before the batch and the caller runs the steps
+  // one at a time, which polls each solve to its end and applies the time
+  // step control.  Returns 1 when the batch ran, 0 when the caller should
+  // step one at a time, or a negative error code.
+  int koBatch(unsigned options, std::string& err) {
+    const int B = std::max(1, lastIters_) + kKoSlack;
+    if (!useGraphs_ || stepsPerLB_ != 1 || injectNan_ || koBatchHold_ > 0 || B > kKoBatchMaxIters)
+      return 0;
+    const size_t D = size_t(D_);
+    const size_t fBytes = 9 * size_t(P_) * sizeof(S);
+    if (batchF_.bytes() != fBytes) {
+      HIP_TRY(batchF_.alloc(fBytes));
+      HIP_TRY(batchKO_.alloc(6 * D * sizeof(double)));
+      HIP_TRY(batchM_.alloc(3 * D * sizeof(R)));
+      HIP_TRY(batchFlags_.alloc(2 * kKoBatch * sizeof(int)));
+      if (!hostFlags_)
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&hostFlags_), 2 * kKoBatch * sizeof(int),
+                              hipHostMallocDefault));
+    }
+    auto copy = [&](void* dst, const void* src, size_t bytes) {
+      return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_);
+    };
+    double* ck = batchKO_.as<double>();
+    R* cm = batchM_.as<R>();
+    auto fields = [&](bool save) -> hipError_t {
+      const std::pair<Buffer*, size_t> f64[] = {{&K_, 0}, {&O_, D}, {&nut_, 2 * D}, {&S_, 3 * D}};
+      for (const auto& b : f64) {
+        const size_t n = b.first == &S_ ? 3 * D : D;
+        hipError_t e = save ? copy(ck + b.second, b.first->as<void>(), n * sizeof(double))
+                            : copy(b.first->as<void>(), ck + b.second, n * sizeof(double));
+        if (e != hipSuccess) return e;
+      }
+      Buffer* m[] = {&rho_, &ux_, &uy_};
+      for (int i = 0; i < 3; i++) {
+        hipError_t e = save ? copy(cm + i * D, m[i]->as<void>(), D * sizeof(R))
+                            : copy(m[i]->as<void>(), cm + i * D, D * sizeof(R));
+        if (e != hipSuccess) return e;
+      }
+      return hipSuccess;
+    };
+    const int buf0 = curBuf_;
+    const bool pre0 = preIbb_;
+    const uint32_t count0 = stepCount_;
+    const int quiet0 = sinceDtChange_;
+    HIP_TRY(copy(batchF_.as<void>(), cur(), fBytes));
+    HIP_TRY(fields(true));
+    const std::vector<int64_t> key = {4, curBuf_, preIbb_ ? 1 : 0, int64_t(options), B, kKoBatch};
+    int rc = replay(key, [&] {
+      for (int i = 0; i < kKoBatch; i++) {
+        launchLB(options);
+        enqueueKoHead(true);
+        enqueueIters(B);
+        hipLaunchKernelGGL(dev::koBatchFlag, dim3(1), dim3(64), 0, stream_,
+                           (const dev::BicgState*)status_.as<dev::BicgState>(), i,
+                           batchFlags_.as<int>());
+      }
+    }, kKoBatch, err);
+    if (rc) return rc;
+    batchCount_++;
+    // koTimeControl for each step: with one sub-step per LB step only the
+    // count of quiet steps changes
+    sinceDtChange_ = (sinceDtChange_ + kKoBatch) % kDtQuiet;
+    HIP_TRY(hipMemcpyAsync(hostFlags_, batchFlags_.as<void>(), 2 * kKoBatch * sizeof(int),
+                           hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    bool ok = true;
+    int iters = 1;
+    for (int i = 0; i < kKoBatch; i++) {
+      ok = ok && hostFlags_[2 * i] == 1;
+      iters = std::max(iters, hostFlags_[2 * i + 1]);
+    }
+    if (ok) {
+      lastIters_ = iters;
+      prevValid_ = true;
+      return 1;
+    }
+    // roll back to the checkpoint
+    HIP_TRY(copy(buf0 == 0 ? fA_.as<void>() : fB_.as<void>(), batchF_.as<void>(), fBytes));
+    HIP_TRY(fields(false));
+    curBuf_ = buf0;
+    preIbb_ = pre0;
+    stepCount_ = count0;
+    sinceDtChange_ = quiet0;
+    lastIters_ = std::max(lastIters_, iters);
+    koBatchHold_ = kKoBatch;
+    rollbackCount_++;
+    return 0;
   }
 
   // kOmega::Solve alone (after a collide-stream launched by the caller).

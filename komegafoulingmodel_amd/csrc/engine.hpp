@@ -22,6 +22,8 @@ struct SolverStats {
   int nan = 0;          // non-finite residuals since the last query
   int stepsPerLB = 1;   // k-omega sub-steps per LB step
   double timeStep = 1.; // their time step
+  int batches = 0;      // batches of k-omega steps run from one graph since the last query
+  int rollbacks = 0;    // of those, batches rerun one step at a time
 };
 
 // J = mean over the last `window` steps of a run of

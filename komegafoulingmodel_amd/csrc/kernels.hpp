@@ -1583,6 +1583,14 @@ __global__ __launch_bounds__(kBlockX) void bicgXR(const uint16_t* __restrict__ m
   blockPartials(a, V.partial);
 }
 
+// Step i of a batch of k-omega steps: out[2 i] = 1 when both solves have
+// finished with finite residuals, out[2 i + 1] = the iterations they took.
+__global__ void koBatchFlag(const BicgState* __restrict__ st, int i, int* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  out[2 * i] = (st->done[0] && st->done[1] && !st->nan) ? 1 : 0;
+  out[2 * i + 1] = st->iters[0] > st->iters[1] ? st->iters[0] : st->iters[1];
+}
+
 // Done flags, iteration count and NaN flag of several solver states (an
 // ensemble's members) in one array: out[4 i .. 4 i + 3].
 __global__ void bicgGather(const BicgState* const* __restrict__ st, int n, int* __restrict__ out) {

@@ -114,7 +114,8 @@ class SolverStats(C.Structure):
     _fields_ = [("kappa_iters", C.c_int32), ("omega_iters", C.c_int32),
                 ("kappa_res0", C.c_double), ("kappa_res", C.c_double),
                 ("omega_res0", C.c_double), ("omega_res", C.c_double), ("nan_flag", C.c_int32),
-                ("steps_per_lb", C.c_int32), ("time_step", C.c_double)]
+                ("steps_per_lb", C.c_int32), ("time_step", C.c_double),
+                ("batches", C.c_int32), ("batch_rollbacks", C.c_int32)]
 
 
 _lib = None

@@ -196,6 +196,31 @@ def test_komega_graph_steps_match_eager_launches():
             assert np.array_equal(La.download(f), Lb.download(f)), (n, f)
 
 
+def test_komega_batches_and_their_rollback_match_eager_launches():
+    """Batches of 8 k-omega steps replayed from one graph with one poll of
+    the solver state: each step is provisioned the last solve's iterations
+    plus 2.  From the initial state the first solves need more than that, so
+    the first batch rolls back to its checkpoint and runs one step at a time;
+    later batches run whole.  Both end bit-identical to eager launches."""
+    w = _mini_wavy_ko()
+    La, _ = make_pair(w, "f32")
+    Lb, _ = make_pair(w, "f32")
+    Lb.set_graphs(False)
+    La.solver_stats()
+    for n in (16, 24, 11):
+        La.step(n, 2)
+        Lb.step(n, 2)
+        st = La.solver_stats()
+        print(f"{n} steps: {st.batches} batches, {st.batch_rollbacks} rolled back, "
+              f"last iterations {st.kappa_iters} {st.omega_iters}", flush=True)
+        if n == 16:
+            assert st.batches == 2 and st.batch_rollbacks >= 1
+        if n == 24:
+            assert st.batches >= 2
+        for f in ("f", "k", "omega", "nut", "ux"):
+            assert np.array_equal(La.download(f), Lb.download(f)), (n, f)
+
+
 @pytest.mark.parametrize("block,npt,pad,spp", [(128, 1, 0, 1), (1024, 1, 64, 1), (256, 2, 1088, 1),
                                                (512, 2, 0, 1), (512, 1, 1088, 2), (640, 1, 0, 2),
                                                (832, 1, 64, 2), (1024, 1, 64, 2),
