@@ -37,10 +37,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+GRAPHS = True  # --no-graphs: every kernel launched on its own (for kernel traces)
+
+
 def setup(w, storage, komega=False, rtol=1e-4, maxit=1000, tune=True):
     from komegafoulingmodel_amd import lib
 
     L = lib.Lattice(w.nX, w.nY, storage=storage, komega=komega)
+    if not GRAPHS:
+        L.set_graphs(False)
     L.set_params(w.tau0, w.Fval, w.nu, kappa_rel_tol=rtol, kappa_max_iters=maxit)
     L.build_geometry(w.C0, w.BL, **w.geom_params())
     L.launch = L.autotune(40) if tune else None
@@ -369,9 +374,14 @@ def main():
                          "form instead of the autotuner's choice (for profiling one kernel)")
     ap.add_argument("--prec-steps", type=int, default=100_000,
                     help="steps of the precision form (errors against FP64 at 8192x1024)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every kernel on its own instead of replaying captured graphs "
+                         "(the same kernels and work; for per-kernel traces)")
     ap.add_argument("--launch-16bit", default="",
                     help="the same for the f16 and bf16 forms")
     args = ap.parse_args()
+    global GRAPHS
+    GRAPHS = not args.no_graphs
     if args.copy_calibration:
         copy_calibration()
         return
