@@ -70,6 +70,7 @@ typedef struct kofm_ctx kofm_ctx;
 #define KOFM_FIELD_DIFF_K 11  /* D   kOmega::Diff_K (k diffusivity; a node keeps its
                                  last value when a wall update makes it solid)   */
 #define KOFM_FIELD_DIFF_OMEGA 12 /* D kOmega::Diff_Omega                           */
+#define KOFM_FIELD_TEMP 13    /* D   clVariablesFD::tempArray (temperature)      */
 
 /* reduction operators (ReduceGenerator.cpp:219-305) */
 #define KOFM_RED_SUM 1
@@ -152,7 +153,24 @@ typedef struct kofm_solver_stats {
   int32_t batch_rollbacks; /* of those, batches in which a solve needed more
                               iterations than provisioned or met a non-finite
                               residual: they were rerun one step at a time */
+  int32_t temp_iters;      /* temperature: iterations of the last solve */
+  double temp_res0, temp_res; /* its first and last residual norms */
+  int32_t temp_steps_per_lb;  /* clVariablesFD::stepsPerLB */
+  double temp_time_step;      /* clVariablesFD::timeStep */
 } kofm_solver_stats;
+
+/* The temperature solve's constants (clVariablesFD::loadParams,
+ * clVariablesFD.cpp:445-497; the facade derives them from RunParam.yaml).
+ * alpha_fluid = MuVal / Pr, alpha_foul = (k_soot / rho_soot / cp_soot)
+ * * DELTA_L^2 / DELTA_T (the fouling layer), pr_turb the turbulent Prandtl
+ * number, t_in the inlet temperature "Tinlet Sim", t_initial "Tinitial",
+ * dt = dTtfd ("LB Steps Per T"), and the BiCGStab tolerances: converged at
+ * ||r|| <= rel_tol ||b|| or ||r|| <= abs_tol ||r0||. */
+typedef struct kofm_thermal_params {
+  double alpha_fluid, alpha_foul, pr_turb, t_in, t_initial, dt;
+  double rel_tol, abs_tol;
+  int32_t max_iters;
+} kofm_thermal_params;
 
 /* --- lifecycle ------------------------------------------------------------ */
 
@@ -347,6 +365,34 @@ int kofm_reduce(kofm_ctx* ctx, int32_t field, int32_t op, double* out);
 #define KOFM_REDUCE_DEVICE_MAX_N (((int64_t)1) << 40)
 int kofm_reduce_device(const void* field, int32_t elem_bytes, int64_t n, int32_t op, double* out,
                        void* stream);
+
+/* clVariablesFD::ini (clVariablesFD.cpp:230-283) for k-omega contexts with
+ * a built geometry: T = t_initial on the nodes fluid in the clean geometry
+ * (fouling layer included), 0 on the clean walls; stepsPerLB =
+ * (int)(1 / dt).  Building the geometry again turns the temperature off
+ * until the next call.  KOFM_ERR_ARG without k-omega (the rows take nu_t,
+ * Update_T_Coeffs_Implicit tfdKernels.cl:122-280). */
+int kofm_thermal_init(kofm_ctx* ctx, const kofm_thermal_params* params);
+
+/* n calls of clVariablesFD::Solve (clVariablesFD.h:343-358) with the
+ * current u and nu_t: per call, the time step control (doubling back after
+ * 20 calls without a change), then stepsPerLB sub-steps of the implicit
+ * temperature rows (Update_T_Coeffs_Implicit: rows on the nodes fluid in
+ * the clean geometry, the fouling layer with its own diffusivity, inlet at
+ * t_in, zero-gradient outlet) solved by BiCGStab; a non-finite or jumping
+ * residual restores T, halves the time step and restarts the sub-steps
+ * (halveTimeStep, clVariablesFD.cpp:202-213).  The reference calls it once
+ * per "LB Steps Per T" LB steps, after the k-omega solve (LBLS.cpp:197,
+ * 230, 315). */
+int kofm_thermal_step(kofm_ctx* ctx, int32_t n);
+
+/* KOFM_DT_* action on the temperature time step (clVariablesFD::
+ * halveTimeStep / doubleTimeStep); reports the sub-steps and time step. */
+int kofm_thermal_time_step(kofm_ctx* ctx, int32_t action, int32_t* steps_per_lb, double* time_step);
+
+/* Replaces T (D doubles, reference layout), e.g. from load/temp.bin on a
+ * restart (clVariablesFD.cpp:797-806). */
+int kofm_set_temperature(kofm_ctx* ctx, const double* T);
 
 /* Copies one field (KOFM_FIELD_*) to host FP64 in the reference layout. */
 int kofm_download(kofm_ctx* ctx, int32_t field, double* host);

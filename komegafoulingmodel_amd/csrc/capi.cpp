@@ -298,6 +298,53 @@ int kofm_komega_time_step(kofm_ctx* ctx, int32_t action, int32_t* steps_per_lb,
   return KOFM_OK;
 }
 
+int kofm_thermal_init(kofm_ctx* ctx, const kofm_thermal_params* q) {
+  if (!ctx || !q) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  kofm::ThermalParams t;
+  t.alphaFluid = q->alpha_fluid;
+  t.alphaFoul = q->alpha_foul;
+  t.prTurb = q->pr_turb;
+  t.tIn = q->t_in;
+  t.tInitial = q->t_initial;
+  t.dt = q->dt;
+  t.relTol = q->rel_tol;
+  t.absTol = q->abs_tol;
+  t.maxIters = q->max_iters;
+  std::string err;
+  int rc = ctx->engine->thermalInit(t, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_thermal_step(kofm_ctx* ctx, int32_t n) {
+  if (!ctx || n < 0) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int rc = ctx->engine->thermalStep(n, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_thermal_time_step(kofm_ctx* ctx, int32_t action, int32_t* steps_per_lb, double* time_step) {
+  if (!ctx) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int spl = 0;
+  double dt = 0.;
+  int rc = ctx->engine->thermalTimeStep(action, spl, dt, err);
+  if (rc) return setErr(ctx, rc, err);
+  if (steps_per_lb) *steps_per_lb = spl;
+  if (time_step) *time_step = dt;
+  return KOFM_OK;
+}
+
+int kofm_set_temperature(kofm_ctx* ctx, const double* T) {
+  if (!ctx || !T) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int rc = ctx->engine->setTemperature(T, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
 int kofm_get_solver_stats(kofm_ctx* ctx, kofm_solver_stats* out) {
   if (!ctx || !out) return KOFM_ERR_ARG;
   NEED_ENGINE(ctx);
@@ -316,6 +363,11 @@ int kofm_get_solver_stats(kofm_ctx* ctx, kofm_solver_stats* out) {
   out->time_step = s.timeStep;
   out->batches = s.batches;
   out->batch_rollbacks = s.rollbacks;
+  out->temp_iters = s.tIters;
+  out->temp_res0 = s.tRes0;
+  out->temp_res = s.tRes;
+  out->temp_steps_per_lb = s.tStepsPerLB;
+  out->temp_time_step = s.tTimeStep;
   return KOFM_OK;
 }
 

@@ -98,6 +98,7 @@ class EngineT final : public Engine {
     for (auto& e : evs_) (void)hipEventDestroy(e);
     if (hostStatus_) (void)hipHostFree(hostStatus_);
     if (hostFlags_) (void)hipHostFree(hostFlags_);
+    if (hostStatusT_) (void)hipHostFree(hostStatusT_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -162,6 +163,7 @@ class EngineT final : public Engine {
   int uploadGeometry(const Geometry& g, std::string& err) override {
     geo_ = g;
     wallsMoved_ = false;
+    tOn_ = false;  // the temperature mask and field follow the clean geometry: init again
     gen_++;
     dropGraphs();
     const size_t D = size_t(D_);
@@ -728,6 +730,17 @@ class EngineT final : public Engine {
     s.timeStep = dtCur_;
     s.batches = batchCount_;
     s.rollbacks = rollbackCount_;
+    if (tOn_) {
+      HIP_TRY(hipMemcpyAsync(&st, statusT_.as<void>(), sizeof(st), hipMemcpyDeviceToHost, stream_));
+      HIP_TRY(hipStreamSynchronize(stream_));
+      s.tIters = st.iters[0];
+      s.tRes0 = st.r0[0];
+      s.tRes = st.rn[0];
+      s.tStepsPerLB = tSteps_;
+      s.tTimeStep = tDt_;
+      s.nan += tNan_;
+      tNan_ = 0;
+    }
     nanCount_ = 0;
     batchCount_ = rollbackCount_ = 0;
     return 0;
@@ -735,6 +748,150 @@ class EngineT final : public Engine {
 
   int synchronize(std::string& err) override {
     HIP_TRY(hipStreamSynchronize(stream_));
+    return 0;
+  }
+
+  // clVariablesFD::ini (clVariablesFD.cpp:230-283): T = Tinitial on the
+  // nodes that are fluid in the clean geometry (fillByNodeType with
+  // M0_FLUID_NODE), 0 elsewhere (the clean walls hold T = 0); stepsPerLB =
+  // (int)(1 / dTtfd).  The rows take nu_t, so k-omega must be on
+  // (Update_T_Coeffs_Implicit reads Nut_array).
+  int thermalInit(const ThermalParams& q, std::string& err) override {
+    if (!ko_) return fail(err, "the temperature solve needs k-omega (its rows take the eddy viscosity)");
+    if (!haveGeo_) return fail(err, "build the geometry first");
+    if (!(q.prTurb > 0.) || !(q.dt > 0.) || q.maxIters < 1)
+      return fail(err, "thermal parameters: need Pr_t > 0, dt > 0 and max iterations >= 1");
+    const size_t D = size_t(D_);
+    for (Buffer* b : {&T_, &Tprev_, &bt_, &tr_, &trh_, &tp_[0], &tp_[1], &tv_[0], &tv_[1], &ts_, &tt_})
+      HIP_TRY(b->alloc(D * sizeof(double)));
+    HIP_TRY(At_.alloc(4 * D * sizeof(double)));
+    HIP_TRY(tmask_.alloc(D * sizeof(uint16_t)));
+    HIP_TRY(statusT_.alloc(sizeof(dev::BicgState)));
+    if (!hostStatusT_)
+      HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&hostStatusT_), sizeof(dev::BicgState),
+                            hipHostMallocDefault));
+    std::vector<uint16_t> m(D, 0);
+    std::vector<double> T(D, 0.);
+    for (size_t i = 0; i < D; i++) {
+      const int x = int(i % size_t(XF_));
+      const uint16_t t = uint16_t(geo_.nType[i]);
+      if (x >= nX_ || (t & kM0Solid)) m[i] = dev::kSolid;
+      if (x < nX_ && (t & kM0Fluid)) T[i] = q.tInitial;
+    }
+    HIP_TRY(hipMemcpy(tmask_.as<void>(), m.data(), D * sizeof(uint16_t), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(dev::solverPairs, dim3(unsigned((D_ + dev::kBlockX - 1) / dev::kBlockX)),
+                       dim3(dev::kBlockX), 0, stream_, tmask_.as<uint16_t>(), nX_, XF_, nY_);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(T_.as<void>(), T.data(), D * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    tpar_ = dev::TParams{q.alphaFluid, q.alphaFoul, 1. / q.prTurb, q.tIn};
+    tRel_ = q.relTol;
+    tAbs_ = q.absTol;
+    tMaxIt_ = q.maxIters;
+    tSteps_ = int(1. / q.dt);
+    tDt_ = q.dt;
+    tQuiet_ = 0;
+    tIters_ = 1;
+    tNan_ = 0;
+    tOn_ = true;
+    return 0;
+  }
+
+  int setTemperature(const double* T, std::string& err) override {
+    if (!tOn_) return fail(err, "initialise the temperature solve first");
+    HIP_TRY(hipMemcpy(T_.as<void>(), T, size_t(D_) * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+  }
+
+  // clVariablesFD::Solve (clVariablesFD.h:343-358), n calls: after 20 calls
+  // without a change the time step doubles back (doubleTimeStep,
+  // clVariablesFD.cpp:215-228); each sub-step assembles the rows and solves
+  // with BiCGStabSolver's stopping rule; a non-finite or jumping residual
+  // restores T to the start of the call, halves the time step, doubles the
+  // sub-steps and starts them again (halveTimeStep, :202-213; the restart is
+  // clean, as for k-omega).  With dTtfd > 1, (int)(1 / dTtfd) is 0 and the
+  // call does nothing, as in the reference.
+  int thermalStep(int n, std::string& err) override {
+    if (!tOn_) return fail(err, "initialise the temperature solve first");
+    if (!haveState_) return fail(err, "state not initialised");
+    auto kp = koParams();
+    const dim3 blk(dev::kBlockX);
+    const int nsb = solverBlocks();
+    auto* st = statusT_.as<dev::BicgState>();
+    const dev::BicgVecs V = thermalVecs();
+    for (int c = 0; c < n; c++) {
+      if (++tQuiet_ >= kDtQuiet) {
+        tQuiet_ = 0;
+        if (tSteps_ > 1) {
+          tSteps_ /= 2;
+          tDt_ *= 2.;
+        }
+      }
+      bool first = true;
+      int sub = 0;
+      while (sub < tSteps_) {
+        hipLaunchKernelGGL(dev::bicgResetT, dim3(1), dim3(64), 0, stream_, st, tRel_, tAbs_, tMaxIt_);
+        hipLaunchKernelGGL((dev::thermalCoeffs<R>), dim3(nsb), blk, 0, stream_,
+                           (const uint16_t*)tmask_.as<uint16_t>(),
+                           (const uint16_t*)mask_.as<uint16_t>(), (const double*)dX0_.as<double>(),
+                           (const double*)nut_.as<double>(), (const R*)ux_.as<R>(),
+                           (const R*)uy_.as<R>(), first ? Tprev_.as<double>() : nullptr, V, tpar_,
+                           tDt_, kp);
+        hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_,
+                           (const double*)partial_.as<double>(), nsb, st, 0);
+        const int batch = std::min(std::max(1, tIters_), 8);
+        enqueueIters(batch, V, tmask_.as<uint16_t>(), st);
+        bool nan = false;
+        int rc = thermalFinish(batch, V, nan, err);
+        if (rc) return rc;
+        first = false;
+        if (tInjectNan_) {
+          tInjectNan_ = false;
+          nan = true;
+        }
+        if (nan) {
+          tNan_++;
+          if (tSteps_ >= kMaxStepsPerLB)
+            return fail(err, "non-finite residual in the temperature solve at the smallest time step");
+          tSteps_ *= 2;
+          tDt_ /= 2.;
+          tQuiet_ = 0;
+          HIP_TRY(hipMemcpyAsync(T_.as<void>(), Tprev_.as<void>(), size_t(D_) * sizeof(double),
+                                 hipMemcpyDeviceToDevice, stream_));
+          sub = 0;
+          continue;
+        }
+        sub++;
+      }
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+  }
+
+  int thermalTimeStep(int action, int& stepsPerLB, double& dt, std::string& err) override {
+    if (!tOn_) return fail(err, "initialise the temperature solve first");
+    switch (action) {
+      case KOFM_DT_QUERY: break;
+      case KOFM_DT_HALVE:  // clVariablesFD::halveTimeStep: T back to the start of the last call
+        if (tSteps_ >= kMaxStepsPerLB) return fail(err, "temperature time step at its minimum");
+        tSteps_ = std::max(1, tSteps_) * 2;
+        tDt_ /= 2.;
+        tQuiet_ = 0;
+        HIP_TRY(hipMemcpy(T_.as<void>(), Tprev_.as<void>(), size_t(D_) * sizeof(double),
+                          hipMemcpyDeviceToDevice));
+        break;
+      case KOFM_DT_DOUBLE:  // clVariablesFD::doubleTimeStep
+        tQuiet_ = 0;
+        if (tSteps_ > 1) {
+          tSteps_ /= 2;
+          tDt_ *= 2.;
+        }
+        break;
+      case KOFM_DT_INJECT_NAN: tInjectNan_ = true; break;
+      default: return fail(err, "unknown temperature time step action");
+    }
+    stepsPerLB = tSteps_;
+    dt = tDt_;
     return 0;
   }
 
@@ -899,6 +1056,16 @@ class EngineT final : public Engine {
   int* hostFlags_ = nullptr;
   int koBatchHold_ = 0;
   int batchCount_ = 0, rollbackCount_ = 0;
+  // temperature (clVariablesFD): T and the start of the step, the rows, the
+  // mask of the clean geometry (solid = not a row), the solver's own vectors
+  // (the fouling layer is a row here but solid for k-omega, whose vectors
+  // must read zero there) and state, the time step control
+  Buffer T_, Tprev_, At_, bt_, tmask_, statusT_, tr_, trh_, tp_[2], tv_[2], ts_, tt_;
+  dev::BicgState* hostStatusT_ = nullptr;
+  dev::TParams tpar_{};
+  double tRel_ = 1e-4, tAbs_ = 1e-7, tDt_ = 1.;
+  int tMaxIt_ = 1000, tSteps_ = 1, tQuiet_ = 0, tIters_ = 1, tNan_ = 0;
+  bool tOn_ = false, tInjectNan_ = false;
   static constexpr int kDtQuiet = 20;              // kOmega.h:22 timeBtwIncreaseTimeStep
   static constexpr int kMaxStepsPerLB = 1 << 16;   // halvings give up beyond this
   dev::BicgState* hostStatus_ = nullptr;  // pinned copy of the solver state
@@ -1616,6 +1783,7 @@ class EngineT final : public Engine {
       case KOFM_FIELD_WALLD: return {ko_ ? wallD_.as<void>() : nullptr, true};
       case KOFM_FIELD_DIFF_K: return {ko_ ? Dk_.as<void>() : nullptr, true};
       case KOFM_FIELD_DIFF_OMEGA: return {ko_ ? Do_.as<void>() : nullptr, true};
+      case KOFM_FIELD_TEMP: return {tOn_ ? T_.as<void>() : nullptr, true};
       case KOFM_FIELD_TAU_NODE: return {tau_.as<void>(), Rd};
       case KOFM_FIELD_TAU_BL: return {tauBL_.as<void>(), Rd};
       default: return {nullptr, false};
@@ -1931,6 +2099,45 @@ class EngineT final : public Engine {
     return V;
   }
 
+  // The temperature system as system 0 of the solver (system 1 is never
+  // touched: its zero partials end it at stage 0).
+  dev::BicgVecs thermalVecs() const {
+    dev::BicgVecs V = bicgVecs();
+    V.x[0] = V.x[1] = T_.as<double>();
+    V.A[0] = V.A[1] = At_.as<double>();
+    V.b[0] = V.b[1] = bt_.as<double>();
+    V.r[0] = V.r[1] = tr_.as<double>();
+    V.rh[0] = V.rh[1] = trh_.as<double>();
+    V.s[0] = V.s[1] = ts_.as<double>();
+    V.t[0] = V.t[1] = tt_.as<double>();
+    for (int c = 0; c < 2; c++) {
+      V.p[0][c] = V.p[1][c] = tp_[c].as<double>();
+      V.v[0][c] = V.v[1][c] = tv_[c].as<double>();
+    }
+    V.prev[0] = V.prev[1] = nullptr;
+    return V;
+  }
+
+  // Polls the temperature solve until it is done (adding chunks of four
+  // iterations); nan reports a non-finite or jumping residual.
+  int thermalFinish(int launched, const dev::BicgVecs& V, bool& nan, std::string& err) {
+    int it = launched;
+    auto* st = statusT_.as<dev::BicgState>();
+    const dev::BicgState& hs = *hostStatusT_;
+    while (true) {
+      HIP_TRY(hipMemcpyAsync(hostStatusT_, st, sizeof(hs), hipMemcpyDeviceToHost, stream_));
+      HIP_TRY(hipStreamSynchronize(stream_));
+      if (hs.nan || (hs.done[0] && hs.done[1]) || it >= tMaxIt_) break;
+      const int k = std::min(4, tMaxIt_ - it);
+      enqueueIters(k, V, tmask_.as<uint16_t>(), st);
+      it += k;
+    }
+    nan = hs.nan != 0;
+    tIters_ = hs.iters[0];
+    HIP_TRY(hipGetLastError());
+    return 0;
+  }
+
   // the solver kernels walk their XCD's tiles with a fixed block count
   int solverBlocks() const { return std::min(8 * koParams().tiles.per, 2048); }
 
@@ -1967,20 +2174,19 @@ class EngineT final : public Engine {
   // at once).
   void enqueueIters(int k) { enqueueIters(k, bicgVecs()); }
   void enqueueIters(int k, const dev::BicgVecs& V) {
+    enqueueIters(k, V, mask_.as<uint16_t>(), status_.as<dev::BicgState>());
+  }
+  void enqueueIters(int k, const dev::BicgVecs& V, const uint16_t* mask, dev::BicgState* st) {
     auto kp = koParams();
     const dim3 sgrid(solverBlocks()), blk(dev::kBlockX);
     const int nsb = solverBlocks();
-    auto* st = status_.as<dev::BicgState>();
     const double* part = partial_.as<double>();
     for (int c = 0; c < k; c++) {
-      hipLaunchKernelGGL(dev::bicgPV, sgrid, blk, 0, stream_, mask_.as<uint16_t>(), V,
-                         (const dev::BicgState*)st, kp);
+      hipLaunchKernelGGL(dev::bicgPV, sgrid, blk, 0, stream_, mask, V, (const dev::BicgState*)st, kp);
       hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, part, nsb, st, 1);
-      hipLaunchKernelGGL(dev::bicgST, sgrid, blk, 0, stream_, mask_.as<uint16_t>(), V,
-                         (const dev::BicgState*)st, kp);
+      hipLaunchKernelGGL(dev::bicgST, sgrid, blk, 0, stream_, mask, V, (const dev::BicgState*)st, kp);
       hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, part, nsb, st, 2);
-      hipLaunchKernelGGL(dev::bicgXR, sgrid, blk, 0, stream_, mask_.as<uint16_t>(), V,
-                         (const dev::BicgState*)st, kp);
+      hipLaunchKernelGGL(dev::bicgXR, sgrid, blk, 0, stream_, mask, V, (const dev::BicgState*)st, kp);
       hipLaunchKernelGGL(dev::bicgScalars, dim3(1), dim3(1024), 0, stream_, part, nsb, st, 3);
     }
   }

@@ -24,6 +24,19 @@ struct SolverStats {
   double timeStep = 1.; // their time step
   int batches = 0;      // batches of k-omega steps run from one graph since the last query
   int rollbacks = 0;    // of those, batches rerun one step at a time
+  int tIters = 0;       // temperature: iterations of the last solve, its residuals,
+  double tRes0 = 0., tRes = 0.;  // sub-steps per call and their time step
+  int tStepsPerLB = 0;
+  double tTimeStep = 0.;
+};
+
+// The temperature solve (clVariablesFD): diffusivities, turbulent Prandtl
+// number, inlet and initial temperatures, the time step dTtfd and the
+// solver tolerances with BiCGStabSolver's meaning.
+struct ThermalParams {
+  double alphaFluid = 0., alphaFoul = 0., prTurb = 0.9, tIn = 2., tInitial = 0.5, dt = 1.;
+  double relTol = 1e-4, absTol = 1e-7;
+  int maxIters = 1000;
 };
 
 // J = mean over the last `window` steps of a run of
@@ -67,6 +80,12 @@ class Engine {
   // KOFM_DT_* action on the k-omega time step; returns the sub-steps per LB
   // step and their time step after it.
   virtual int komegaTimeStep(int action, int& stepsPerLB, double& dt, std::string& err) = 0;
+  // clVariablesFD::ini, ::Solve (n calls), halveTimeStep/doubleTimeStep, and
+  // the temperature field
+  virtual int thermalInit(const ThermalParams& q, std::string& err) = 0;
+  virtual int thermalStep(int n, std::string& err) = 0;
+  virtual int thermalTimeStep(int action, int& stepsPerLB, double& dt, std::string& err) = 0;
+  virtual int setTemperature(const double* T, std::string& err) = 0;
   virtual int64_t fieldSize(int field) const = 0;
   virtual int setLaunch(int block, int npt, int spp, int64_t pad, std::string& err) = 0;
   virtual void setGraphs(bool on) = 0;

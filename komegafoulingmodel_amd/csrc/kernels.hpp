@@ -1207,6 +1207,9 @@ struct BicgState {
   double r0[2], rn[2];
   double rtol[2], atol[2];
   double best[2];   // smallest residual so far
+  double bnorm[2];  // ||b|| (stopping rule 1)
+  double prevRes[2];  // the previous iteration's residual (stopping rule 1)
+  int crit[2];      // stopping rule: 0 as GMRESSolver, 1 as BiCGStabSolver
   int maxit[2];
   int done[2];
   int iters[2];
@@ -1315,7 +1318,8 @@ __global__ __launch_bounds__(1024) void bicgScalars(const double* __restrict__ p
     if (st->done[v]) continue;
     const double a = s[2 * v], b = s[2 * v + 1];
     if (stage == 0) {
-      // a = (r, r); the first bicgPV takes p = r (restart form), rh = r
+      // a = (r, r), b = (b, b) (rule 1); the first bicgPV takes p = r
+      // (restart form), rh = r
       const double r0 = sqrt(a);
       st->r0[v] = r0;
       st->rn[v] = r0;
@@ -1326,7 +1330,20 @@ __global__ __launch_bounds__(1024) void bicgScalars(const double* __restrict__ p
       st->restart[v] = 1;
       st->best[v] = r0;
       st->sinceBest[v] = 0;
-      if (!isfinite(r0)) {
+      st->prevRes[v] = r0;
+      if (st->crit[v] == 1) {
+        // BiCGStabSolver::solve (BiCGStabSolver.cpp:648-658): ||b|| <= 1e-16
+        // ends the solve (the kernel leaves x; the reference copies b, the
+        // same zero field up to 1e-16)
+        const double bn = sqrt(b);
+        st->bnorm[v] = bn;
+        if (!isfinite(r0) || !isfinite(bn)) {
+          st->nan = 1;
+          st->done[v] = 1;
+        } else if (bn <= 1.0e-16 || r0 == 0. || r0 <= st->rtol[v] * bn) {
+          st->done[v] = 1;
+        }
+      } else if (!isfinite(r0)) {
         st->nan = 1;
         st->done[v] = 1;
       } else if (r0 == 0. || r0 <= st->atol[v]) {
@@ -1355,7 +1372,24 @@ __global__ __launch_bounds__(1024) void bicgScalars(const double* __restrict__ p
         st->done[v] = 1;
         continue;
       }
-      if (rn <= st->rtol[v] * st->r0[v] || rn <= st->atol[v] || st->iters[v] >= st->maxit[v]) {
+      if (st->crit[v] == 1) {
+        // BiCGStabSolver::reduceAndCheckConvergence (BiCGStabSolver.cpp:509-545):
+        // a residual above 10 times the last one after the first iteration
+        // fails like a non-finite one (the caller halves the time step);
+        // converged at ||r|| <= rtol ||b|| or ||r|| <= atol ||r0||
+        if (st->iters[v] > 1 && rn > 10. * st->prevRes[v]) {
+          st->nan = 1;
+          st->done[v] = 1;
+          continue;
+        }
+        st->prevRes[v] = rn;
+        if (rn <= st->rtol[v] * st->bnorm[v] || rn <= st->atol[v] * st->r0[v] ||
+            st->iters[v] >= st->maxit[v]) {
+          st->done[v] = 1;
+          continue;
+        }
+      } else if (rn <= st->rtol[v] * st->r0[v] || rn <= st->atol[v] ||
+                 st->iters[v] >= st->maxit[v]) {
         st->done[v] = 1;
         continue;
       }
@@ -1616,6 +1650,113 @@ __global__ void bicgReset(BicgState* __restrict__ st, double relK, double absK, 
   z.atol[1] = absO;
   z.maxit[1] = maxO;
   *st = z;
+}
+
+// The solver state for the temperature system: system 0 with
+// BiCGStabSolver's stopping rule, system 1 empty (its partials are zero, so
+// it is done at stage 0).
+__global__ void bicgResetT(BicgState* __restrict__ st, double rel, double abs_, int maxit) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  BicgState z = {};
+  z.rtol[0] = rel;
+  z.atol[0] = abs_;
+  z.maxit[0] = maxit;
+  z.crit[0] = 1;
+  *st = z;
+}
+
+// ---------------------------------------------------------------------------
+// temperature (clVariablesFD)
+// ---------------------------------------------------------------------------
+
+struct TParams {
+  double alphaFluid, alphaFoul, prTurbInv, tIn;
+};
+
+// The three-point weights of Update_T_Coeffs_Implicit (tfdKernels.cl:154-181):
+// spacings held at 0.1 or more; the first derivative falls back to the
+// two-point central difference where |the tested weight| > 10 (the x row
+// tests its centre weight, the y row its north weight, as the kernel does).
+__device__ __forceinline__ void tWeights(double dp, double dm, bool testCentre, double (&d1)[3],
+                                         double (&d2)[3]) {
+  dp = fmax(dp, 0.1);
+  dm = fmax(dm, 0.1);
+  const double dc = fmax(dp + dm, 0.1);
+  d1[0] = dm / (dp * dc);
+  d1[1] = -dp / (dm * dc);
+  d1[2] = (dp - dm) / (dp * dm);
+  d2[0] = 2. / (dp * dc);
+  d2[1] = 2. / (dm * dc);
+  d2[2] = -2. / (dp * dm);
+  if (fabs(testCentre ? d1[2] : d1[0]) > 10.) {
+    d1[0] = 1. / dc;
+    d1[1] = -1. / dc;
+    d1[2] = 0.;
+  }
+}
+
+// Update_T_Coeffs_Implicit (tfdKernels.cl:122-280) in the five-point row
+// form of the k-omega systems (diagonal 1; planes E, W, N, S), then the first
+// residual r = b - A T from the current T; partials (r, r) and (b, b) of
+// system 0 (system 1 gets zeros).  Rows on the nodes of tmask that are not
+// solid (solid in the clean geometry); a node solid in the LB mask (the
+// fouling layer) takes alphaFoul.  nu_t / Pr_t adds to alpha and the x and y
+// differences of nu_t to the advection (the kernel multiplies the centre
+// term by 1 / Pr_t twice, which is kept).  The source is T clamped to
+// [0, T_in]; the inlet column's W coefficient moves to the right-hand side
+// with T_in and the outlet column's E coefficient into the diagonal.  When
+// Tprev is set, T is also copied there (the restart point of the step).
+template <class M>
+__global__ __launch_bounds__(kBlockX) void thermalCoeffs(
+    const uint16_t* __restrict__ tmask, const uint16_t* __restrict__ lbmask,
+    const double* __restrict__ dX0, const double* __restrict__ Nut, const M* __restrict__ uxa,
+    const M* __restrict__ uya, double* __restrict__ Tprev, BicgVecs V, TParams tp, double dt,
+    KOParams p) {
+  double a[2][2] = {{0., 0.}, {0., 0.}};
+  forOwnTiles(p, [&](int x, int y) {
+    const size_t g = size_t(x) + size_t(p.XF) * y;
+    if (tmask[g] & kSolid) return;
+    const double* T = V.x[0];
+    double dx1[3], dx2[3], dy1[3], dy2[3];
+    tWeights(dX0[g], dX0[g + p.D], true, dx1, dx2);
+    tWeights(dX0[g + 2 * p.D], dX0[g + 3 * p.D], false, dy1, dy2);
+    size_t nb[4];
+    nbIndex(p, x, g, nb);
+    double alpha = (lbmask[g] & kSolid) ? tp.alphaFoul : tp.alphaFluid;
+    const double at = Nut[g] * tp.prTurbInv;
+    const double Jx = double(uxa[g]) -
+                      tp.prTurbInv * (dx1[0] * Nut[nb[0]] + dx1[1] * Nut[nb[1]] + dx1[2] * at);
+    const double Jy = double(uya[g]) -
+                      tp.prTurbInv * (dy1[0] * Nut[nb[2]] + dy1[1] * Nut[nb[3]] + dy1[2] * at);
+    alpha += at;
+    const double Tg = T[g];
+    double src = fmin(fmax(0., Tg), tp.tIn);
+    double C = 1. + dt * (Jx * dx1[2] + Jy * dy1[2] - alpha * (dx2[2] + dy2[2]));
+    const double S = dt * (Jy * dy1[1] - alpha * dy2[1]);
+    const double W = dt * (Jx * dx1[1] - alpha * dx2[1]);
+    if (x == 0) src -= tp.tIn * W;
+    const double E = dt * (Jx * dx1[0] - alpha * dx2[0]);
+    if (x == p.nX - 1) C += E;
+    const double N = dt * (Jy * dy1[0] - alpha * dy2[0]);
+    const double inv = 1. / C;
+    double A[4];
+    A[0] = x < p.nX - 1 ? E * inv : 0.;
+    A[1] = x > 0 ? W * inv : 0.;
+    A[2] = N * inv;
+    A[3] = S * inv;
+    const double b = src * inv;
+#pragma unroll
+    for (int d = 0; d < 4; d++) V.A[0][g + size_t(d) * p.D] = A[d];
+    V.b[0][g] = b;
+    const double ax = Tg + (A[0] * T[nb[0]] + A[1] * T[nb[1]] + A[2] * T[nb[2]] + A[3] * T[nb[3]]);
+    const double r = b - ax;
+    V.r[0][g] = r;
+    V.rh[0][g] = r;
+    if (Tprev) Tprev[g] = Tg;
+    a[0][0] += r * r;
+    a[0][1] += b * b;
+  });
+  blockPartials(a, V.partial);
 }
 
 // ---------------------------------------------------------------------------

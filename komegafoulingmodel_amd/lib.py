@@ -22,12 +22,12 @@ OPTION_SAVE_MACRO_FIELDS = 0x2
 
 FIELD_F, FIELD_RHO, FIELD_UX, FIELD_UY = 0, 1, 2, 3
 FIELD_KAPPA, FIELD_OMEGA, FIELD_NUT, FIELD_SXY, FIELD_WALLD = 4, 5, 6, 7, 8
-FIELD_TAU_NODE, FIELD_TAU_BL, FIELD_DIFF_K, FIELD_DIFF_OMEGA = 9, 10, 11, 12
+FIELD_TAU_NODE, FIELD_TAU_BL, FIELD_DIFF_K, FIELD_DIFF_OMEGA, FIELD_TEMP = 9, 10, 11, 12, 13
 FIELD_NAMES = {
     "f": FIELD_F, "rho": FIELD_RHO, "ux": FIELD_UX, "uy": FIELD_UY, "k": FIELD_KAPPA,
     "omega": FIELD_OMEGA, "nut": FIELD_NUT, "S": FIELD_SXY, "wallD": FIELD_WALLD,
     "tau_node": FIELD_TAU_NODE, "tau_bl": FIELD_TAU_BL, "diff_k": FIELD_DIFF_K,
-    "diff_omega": FIELD_DIFF_OMEGA,
+    "diff_omega": FIELD_DIFF_OMEGA, "T": FIELD_TEMP,
 }
 RED_SUM, RED_MAX, RED_MIN = 1, 2, 4
 DT_QUERY, DT_HALVE, DT_DOUBLE, DT_INJECT_NAN = 0, 1, 2, 3
@@ -55,6 +55,7 @@ EXPORTS = [
     "kofm_ensemble_destroy",
     "kofm_reduce_device", "kofm_komega_time_step", "kofm_ini_poiseuille",
     "kofm_save_restart", "kofm_save_debug", "kofm_update_walls",
+    "kofm_thermal_init", "kofm_thermal_step", "kofm_thermal_time_step", "kofm_set_temperature",
 ]
 
 
@@ -115,7 +116,15 @@ class SolverStats(C.Structure):
                 ("kappa_res0", C.c_double), ("kappa_res", C.c_double),
                 ("omega_res0", C.c_double), ("omega_res", C.c_double), ("nan_flag", C.c_int32),
                 ("steps_per_lb", C.c_int32), ("time_step", C.c_double),
-                ("batches", C.c_int32), ("batch_rollbacks", C.c_int32)]
+                ("batches", C.c_int32), ("batch_rollbacks", C.c_int32),
+                ("temp_iters", C.c_int32), ("temp_res0", C.c_double), ("temp_res", C.c_double),
+                ("temp_steps_per_lb", C.c_int32), ("temp_time_step", C.c_double)]
+
+
+class ThermalParams(C.Structure):
+    _fields_ = [("alpha_fluid", C.c_double), ("alpha_foul", C.c_double), ("pr_turb", C.c_double),
+                ("t_in", C.c_double), ("t_initial", C.c_double), ("dt", C.c_double),
+                ("rel_tol", C.c_double), ("abs_tol", C.c_double), ("max_iters", C.c_int32)]
 
 
 _lib = None
@@ -151,6 +160,10 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "kofm_wall_shear": (C.c_int, [P]),
         "kofm_get_solver_stats": (C.c_int, [P, C.POINTER(SolverStats)]),
         "kofm_komega_time_step": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int32), dp]),
+        "kofm_thermal_init": (C.c_int, [P, C.POINTER(ThermalParams)]),
+        "kofm_thermal_step": (C.c_int, [P, C.c_int32]),
+        "kofm_thermal_time_step": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int32), dp]),
+        "kofm_set_temperature": (C.c_int, [P, dp]),
         "kofm_ini_poiseuille": (C.c_int, [P, C.POINTER(IniFlow), dp, dp, dp]),
         "kofm_reduce": (C.c_int, [P, C.c_int32, C.c_int32, dp]),
         "kofm_reduce_device": (C.c_int, [P, C.c_int32, C.c_int64, C.c_int32, P, P]),
@@ -338,6 +351,31 @@ class Lattice:
         n, dt = C.c_int32(), C.c_double()
         self._check(self.lib.kofm_komega_time_step(self.h, action, C.byref(n), C.byref(dt)))
         return n.value, dt.value
+
+    def thermal_init(self, *, alpha_fluid: float, alpha_foul: float, pr_turb: float = 0.9,
+                     t_in: float = 2.0, t_initial: float = 0.5, dt: float = 1.0,
+                     rel_tol: float = 1e-4, abs_tol: float = 1e-7, max_iters: int = 1000):
+        """kofm_thermal_init (clVariablesFD::ini): T = t_initial on the nodes
+        fluid in the clean geometry; needs k-omega and a built geometry."""
+        q = ThermalParams(alpha_fluid, alpha_foul, pr_turb, t_in, t_initial, dt, rel_tol, abs_tol,
+                          max_iters)
+        self._check(self.lib.kofm_thermal_init(self.h, C.byref(q)))
+
+    def thermal_step(self, n: int = 1):
+        """n calls of clVariablesFD::Solve with the current u and nu_t."""
+        self._check(self.lib.kofm_thermal_step(self.h, n))
+
+    def thermal_time_step(self, action: int = 0):
+        """KOFM_DT_* action on the temperature time step; returns (sub-steps
+        per call, time step) after it."""
+        n, dt = C.c_int32(), C.c_double()
+        self._check(self.lib.kofm_thermal_time_step(self.h, action, C.byref(n), C.byref(dt)))
+        return n.value, dt.value
+
+    def set_temperature(self, T: np.ndarray):
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        assert T.size == self.D
+        self._check(self.lib.kofm_set_temperature(self.h, _dptr(T)))
 
     def reduce(self, field: str, op: int = RED_SUM) -> float:
         out = C.c_double()

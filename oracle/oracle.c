@@ -912,14 +912,14 @@ void orc_ko_coeffs(const orc_par *p, const int16_t *nType, const double *K, cons
 
 /* y = A x on fluid rows (diagonal 1, four neighbours with the x wrap at nX);
  * other rows give y = x, which keeps solid-boundary entries (x = 0) fixed. */
-static void ko_apply(const orc_par *p, const int16_t *nType, const double *A, const double *x,
-                     double *y) {
+static void ko_apply(const orc_par *p, const int16_t *nType, int solid, const double *A,
+                     const double *x, double *y) {
   const long D = (long)p->XF * p->nY;
   for (long g = 0; g < D; g++) y[g] = x[g];
   for (int j = 0; j < p->nY; j++)
     for (int i = 0; i < p->nX; i++) {
       long g = i + (long)p->XF * j;
-      if (nType[g] & M_SOLID_NODE) continue;
+      if (nType[g] & solid) continue;
       long gw = i > 0 ? g - 1 : p->nX - 1 + (long)j * p->XF;
       long ge = i < p->nX - 1 ? g + 1 : (long)j * p->XF;
       y[g] = x[g] + (A[g] * x[ge] + A[g + D] * x[gw] + A[g + 2 * D] * x[g + p->XF] +
@@ -927,12 +927,13 @@ static void ko_apply(const orc_par *p, const int16_t *nType, const double *A, co
     }
 }
 
-static double fluid_dot(const orc_par *p, const int16_t *nType, const double *a, const double *b) {
+static double fluid_dot(const orc_par *p, const int16_t *nType, int solid, const double *a,
+                        const double *b) {
   double s = 0.;
   for (int j = 0; j < p->nY; j++)
     for (int i = 0; i < p->nX; i++) {
       long g = i + (long)p->XF * j;
-      if (!(nType[g] & M_SOLID_NODE)) s += a[g] * b[g];
+      if (!(nType[g] & solid)) s += a[g] * b[g];
     }
   return s;
 }
@@ -942,10 +943,10 @@ static double fluid_dot(const orc_par *p, const int16_t *nType, const double *a,
  * nodes above a solid stretch form a pair, and the last two below one unless
  * the lower of them is already paired (nodes outside the lattice count as
  * solid). */
-static int pair_dir(const orc_par *p, const int16_t *nType, int i, int j) {
-  if (nType[i + (long)p->XF * j] & M_SOLID_NODE) return 0;
-#define ORC_S(k) (j - (k) < 0 || (nType[i + (long)p->XF * (j - (k))] & M_SOLID_NODE))
-#define ORC_N(k) (j + (k) >= p->nY || (nType[i + (long)p->XF * (j + (k))] & M_SOLID_NODE))
+static int pair_dir(const orc_par *p, const int16_t *nType, int solid, int i, int j) {
+  if (nType[i + (long)p->XF * j] & solid) return 0;
+#define ORC_S(k) (j - (k) < 0 || (nType[i + (long)p->XF * (j - (k))] & solid))
+#define ORC_N(k) (j + (k) >= p->nY || (nType[i + (long)p->XF * (j + (k))] & solid))
   int d = 0;
   if (ORC_S(1) && !ORC_N(1))
     d = 1;
@@ -962,11 +963,11 @@ static int pair_dir(const orc_par *p, const int16_t *nType, int i, int j) {
 
 /* z = M^-1 u: M is A's 2 x 2 blocks over the pairs (identity elsewhere), so
  * z_g = (u_g - a_gq u_q) / (1 - a_gq a_qg) with q the partner. */
-static void pair_apply(const orc_par *p, const int16_t *nType, const int8_t *pr, const double *A,
-                       const double *u, double *z) {
+static void pair_apply(const orc_par *p, const int16_t *nType, int solid, const int8_t *pr,
+                       const double *A, const double *u, double *z) {
   const long D = (long)p->XF * p->nY;
   for (long g = 0; g < D; g++) {
-    if (nType[g] & M_SOLID_NODE || (g % p->XF) >= p->nX) {
+    if (nType[g] & solid || (g % p->XF) >= p->nX) {
       z[g] = u[g];
       continue;
     }
@@ -991,58 +992,80 @@ static void pair_apply(const orc_par *p, const int16_t *nType, const int8_t *pr,
  * converged, both give the solution of the same system.  Returns the
  * iteration count, or -1 on a non-finite residual (the reference halves its
  * time step then).  work: 8*D doubles plus D bytes. */
-int orc_ko_solve(const orc_par *p, const int16_t *nType, const double *A, const double *b,
-                 double *x, double rtol, double atol, int maxit, double *res0, double *res,
-                 double *work) {
+/* mode 0: GMRESSolver::checkConvergence (GMRESSolver.cpp:1535-1560):
+ *   ||r|| <= rtol ||r0|| or ||r|| <= atol, non-finite residual fails;
+ * mode 1: BiCGStabSolver (BiCGStabSolver.cpp:509-545, 628-660): with
+ *   e = ||r|| / ||b||, e <= rtol or ||r|| <= atol ||r0||; ||b|| <= 1e-16 gives
+ *   x = b; a non-finite e, or e above 10 times the previous iteration's after
+ *   the first, fails (the solver then halves the time step). */
+static int solve5(const orc_par *p, const int16_t *nType, int solid, int mode, const double *A,
+                  const double *b, double *x, double rtol, double atol, int maxit, double *res0,
+                  double *res, double *work) {
   const long D = (long)p->XF * p->nY;
   double *r = work, *rh = work + D, *pp = work + 2 * D, *v = work + 3 * D, *s = work + 4 * D,
          *t = work + 5 * D, *ph = work + 6 * D, *sh = work + 7 * D;
   int8_t *pr = (int8_t *)(work + 8 * D);
   for (int j = 0; j < p->nY; j++)
-    for (int i = 0; i < p->XF; i++) pr[i + (long)p->XF * j] = i < p->nX ? pair_dir(p, nType, i, j) : 0;
-  ko_apply(p, nType, A, x, t);
+    for (int i = 0; i < p->XF; i++) pr[i + (long)p->XF * j] = i < p->nX ? pair_dir(p, nType, solid, i, j) : 0;
+  double bn = 0.;
+  if (mode == 1) {
+    bn = sqrt(fluid_dot(p, nType, solid, b, b));
+    if (bn <= 1.0e-16) {
+      for (long g = 0; g < D; g++)
+        if (!(nType[g] & solid) && (g % p->XF) < p->nX) x[g] = b[g];
+      return 0;
+    }
+  }
+  ko_apply(p, nType, solid, A, x, t);
   for (long g = 0; g < D; g++) {
-    int fl = !(nType[g] & M_SOLID_NODE) && (g % p->XF) < p->nX;
+    int fl = !(nType[g] & solid) && (g % p->XF) < p->nX;
     r[g] = fl ? b[g] - t[g] : 0.;
     rh[g] = r[g];
     pp[g] = 0.;
     v[g] = 0.;
   }
-  double r0 = sqrt(fluid_dot(p, nType, r, r));
+  double r0 = sqrt(fluid_dot(p, nType, solid, r, r));
   if (res0) *res0 = r0;
   if (res) *res = r0;
   if (!isfinite(r0)) return -1;
-  if (r0 <= atol || r0 == 0.) return 0;
+  if (mode == 0 ? (r0 <= atol || r0 == 0.) : (r0 <= rtol * bn || r0 == 0.)) return 0;
+  double prev = r0;
   double rho = 1., alpha = 1., om = 1., best = r0;
   int since_best = 0;
   for (int it = 1; it <= maxit; it++) {
-    double rho1 = fluid_dot(p, nType, rh, r);
+    double rho1 = fluid_dot(p, nType, solid, rh, r);
     if (rho1 == 0.) { /* breakdown: restart from the current residual */
       for (long g = 0; g < D; g++) { rh[g] = r[g]; pp[g] = 0.; v[g] = 0.; }
       rho = alpha = om = 1.;
-      rho1 = fluid_dot(p, nType, rh, r);
+      rho1 = fluid_dot(p, nType, solid, rh, r);
     }
     double beta = (rho1 / rho) * (alpha / om);
     for (long g = 0; g < D; g++) pp[g] = r[g] + beta * (pp[g] - om * v[g]);
-    pair_apply(p, nType, pr, A, pp, ph);
-    ko_apply(p, nType, A, ph, v);
-    alpha = rho1 / fluid_dot(p, nType, rh, v);
+    pair_apply(p, nType, solid, pr, A, pp, ph);
+    ko_apply(p, nType, solid, A, ph, v);
+    alpha = rho1 / fluid_dot(p, nType, solid, rh, v);
     for (long g = 0; g < D; g++) s[g] = r[g] - alpha * v[g];
-    pair_apply(p, nType, pr, A, s, sh);
-    ko_apply(p, nType, A, sh, t);
-    double tt = fluid_dot(p, nType, t, t);
-    om = tt > 0. ? fluid_dot(p, nType, t, s) / tt : 0.;
+    pair_apply(p, nType, solid, pr, A, s, sh);
+    ko_apply(p, nType, solid, A, sh, t);
+    double tt = fluid_dot(p, nType, solid, t, t);
+    om = tt > 0. ? fluid_dot(p, nType, solid, t, s) / tt : 0.;
     for (long g = 0; g < D; g++) {
-      int fl = !(nType[g] & M_SOLID_NODE) && (g % p->XF) < p->nX;
+      int fl = !(nType[g] & solid) && (g % p->XF) < p->nX;
       if (!fl) continue;
       x[g] += alpha * ph[g] + om * sh[g];
       r[g] = s[g] - om * t[g];
     }
     rho = rho1;
-    double rn = sqrt(fluid_dot(p, nType, r, r));
+    double rn = sqrt(fluid_dot(p, nType, solid, r, r));
     if (res) *res = rn;
     if (!isfinite(rn)) return -1;
-    if (rn <= rtol * r0 || rn <= atol) return it;
+    if (mode == 0) {
+      if (rn <= rtol * r0 || rn <= atol) return it;
+    } else {
+      if (it > 1 && rn > 10. * prev) return -1;
+      if (rn <= rtol * bn || rn <= atol * r0) return it;
+      prev = rn;
+    }
     /* converged to the rounding floor: no progress in 50 iterations */
     if (rn < 0.999 * best) {
       best = rn;
@@ -1056,6 +1079,22 @@ int orc_ko_solve(const orc_par *p, const int16_t *nType, const double *A, const 
     }
   }
   return maxit;
+}
+
+/* The k and omega systems: rows on the nodes that are not solid. */
+int orc_ko_solve(const orc_par *p, const int16_t *nType, const double *A, const double *b,
+                 double *x, double rtol, double atol, int maxit, double *res0, double *res,
+                 double *work) {
+  return solve5(p, nType, M_SOLID_NODE, 0, A, b, x, rtol, atol, maxit, res0, res, work);
+}
+
+/* The temperature system: rows on the nodes that are not solid in the clean
+ * geometry (fluid and the fouling layer, clVariablesFD's CSR_Inds built on
+ * the M0 flags), with BiCGStabSolver's stopping rule. */
+int orc_t_solve(const orc_par *p, const int16_t *nType, const double *A, const double *b,
+                double *x, double rtol, double atol, int maxit, double *res0, double *res,
+                double *work) {
+  return solve5(p, nType, M0_SOLID_NODE, 1, A, b, x, rtol, atol, maxit, res0, res, work);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1240,4 +1279,138 @@ int orc_max_threads(void) {
 #else
   return 1;
 #endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* Temperature (clVariablesFD)                                               */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  double alpha_fluid; /* Alpha_fluid = MuVal / Pr (clVariablesFD.cpp:458) */
+  double alpha_foul;  /* Alpha_foul, fouling-layer nodes (:463) */
+  double pr_turb_inv; /* 1 / PrTurbNum (PR_TURB_NUMBER_INV) */
+  double t_in;        /* TFD_X_IN_VAL, the inlet temperature "Tinlet Sim" */
+  double rtol, atol;  /* "Temp Max Rel Tol", "Temp Max Abs Tol" */
+  int maxit;          /* "Temp Max Iterations" */
+  int steps_per_lb;   /* stepsPerLB = (int)(1 / dTtfd) */
+  double dt;          /* timeStep */
+  int since_change;   /* timeSinceTimeStepChange */
+  int last_iters, halvings, inject_nan;
+} orc_tsolver;
+
+/* One row of the nonuniform three-point differences on the clean-wall
+ * spacings, as Update_T_Coeffs_Implicit forms them (tfdKernels.cl:154-181):
+ * spacings held at 0.1 or more, the x first derivative falling back to the
+ * two-point central difference where |its centre weight| > 10, and the y
+ * first derivative where |its north weight| > 10 (the kernel tests dYn). */
+static void t_fd(double dp, double dm, int clamp, int test_c, double *d1, double *d2) {
+  if (clamp) {
+    dp = dp > 0.1 ? dp : 0.1;
+    dm = dm > 0.1 ? dm : 0.1;
+  }
+  double dc = dp + dm;
+  if (clamp) dc = dc > 0.1 ? dc : 0.1;
+  d1[0] = dm / (dp * dc);          /* +: east / north */
+  d1[1] = -dp / (dm * dc);         /* -: west / south */
+  d1[2] = (dp - dm) / (dp * dm);   /* centre */
+  d2[0] = 2. / (dp * dc);
+  d2[1] = 2. / (dm * dc);
+  d2[2] = -2. / (dp * dm);
+  if (clamp && fabs(test_c ? d1[2] : d1[0]) > 10.) {
+    d1[0] = 1. / dc;
+    d1[1] = -1. / dc;
+    d1[2] = 0.;
+  }
+}
+
+/* Update_T_Coeffs_Implicit (tfdKernels.cl:122-280) in the five-point row form
+ * of the k-omega systems (diagonal 1; planes E, W, N, S): rows on the nodes
+ * that are not solid in the clean geometry; a fouling-layer node (solid now,
+ * fluid in the clean geometry) takes alpha_foul; nu_t / Pr_t adds to alpha and
+ * the x and y differences of nu_t to the advection (the kernel multiplies the
+ * centre term by 1 / Pr_t twice, which is kept); the source is T clamped to
+ * [0, T_in]; the inlet column's W coefficient moves to the right-hand side
+ * with T_in and the outlet column's E coefficient into the diagonal (zero
+ * gradient). */
+void orc_t_coeffs(const orc_par *p, const int16_t *nType, const double *dX0, const double *T,
+                  const double *Nut, const double *ux, const double *uy, const orc_tsolver *tp,
+                  double dt, double *A, double *b) {
+  const long D = (long)p->XF * p->nY;
+  for (int j = 0; j < p->nY; j++)
+    for (int i = 0; i < p->nX; i++) {
+      long g = i + (long)p->XF * j;
+      int t = nType[g];
+      if (t & M0_SOLID_NODE) continue;
+      double dx1[3], dx2[3], dy1[3], dy2[3];
+      t_fd(dX0[g], dX0[g + D], 1, 1, dx1, dx2);
+      t_fd(dX0[g + 2 * D], dX0[g + 3 * D], 1, 0, dy1, dy2);
+      long gw = i > 0 ? g - 1 : p->nX - 1 + (long)j * p->XF;
+      long ge = i < p->nX - 1 ? g + 1 : (long)j * p->XF;
+      double alpha = (t & M_SOLID_NODE) ? tp->alpha_foul : tp->alpha_fluid;
+      double at = Nut[g] * tp->pr_turb_inv;
+      double Jx = ux[g] - tp->pr_turb_inv * (dx1[0] * Nut[ge] + dx1[1] * Nut[gw] + dx1[2] * at);
+      double Jy = uy[g] - tp->pr_turb_inv * (dy1[0] * Nut[g + p->XF] + dy1[1] * Nut[g - p->XF] +
+                                             dy1[2] * at);
+      alpha += at;
+      double src = T[g] > 0. ? T[g] : 0.;
+      src = src < tp->t_in ? src : tp->t_in;
+      double C = 1. + dt * (Jx * dx1[2] + Jy * dy1[2] - alpha * (dx2[2] + dy2[2]));
+      double S = dt * (Jy * dy1[1] - alpha * dy2[1]);
+      double W = dt * (Jx * dx1[1] - alpha * dx2[1]);
+      if (i == 0) src -= tp->t_in * W;
+      double E = dt * (Jx * dx1[0] - alpha * dx2[0]);
+      if (i == p->nX - 1) C += E;
+      double N = dt * (Jy * dy1[0] - alpha * dy2[0]);
+      double inv = 1. / C;
+      A[g] = i < p->nX - 1 ? E * inv : 0.;
+      A[g + D] = i > 0 ? W * inv : 0.;
+      A[g + 2 * D] = N * inv;
+      A[g + 3 * D] = S * inv;
+      b[g] = src * inv;
+    }
+}
+
+#define ORC_T_QUIET 20 /* clVariablesFD.h:24 timeBtwIncreaseTimeStep */
+
+/* clVariablesFD::Solve (clVariablesFD.h:343-358) with halveTimeStep and
+ * doubleTimeStep (clVariablesFD.cpp:202-228): after 20 calls without a change
+ * the time step doubles back (never past one sub-step); the start is kept;
+ * each sub-step assembles the rows and solves; a failed solve restores the
+ * start, halves the time step, doubles the sub-steps and starts them again.
+ * work: 4*D + 9*D doubles plus D bytes (rows, right-hand side, start, solver). */
+int orc_t_step(const orc_par *p, const int16_t *nType, const double *dX0, double *T,
+               const double *Nut, const double *ux, const double *uy, orc_tsolver *tp,
+               double *work) {
+  const long D = (long)p->XF * p->nY;
+  double *A = work, *b = work + 4 * D, *Tp = work + 5 * D, *sw = work + 6 * D;
+  if (++tp->since_change >= ORC_T_QUIET) {
+    tp->since_change = 0;
+    if (tp->steps_per_lb > 1) {
+      tp->steps_per_lb /= 2;
+      tp->dt *= 2.;
+    }
+  }
+  memcpy(Tp, T, sizeof(double) * D);
+  int sub = 0;
+  while (sub < tp->steps_per_lb) {
+    orc_t_coeffs(p, nType, dX0, T, Nut, ux, uy, tp, tp->dt, A, b);
+    int it = orc_t_solve(p, nType, A, b, T, tp->rtol, tp->atol, tp->maxit, 0, 0, sw);
+    if (tp->inject_nan) {
+      tp->inject_nan = 0;
+      it = -1;
+    }
+    if (it < 0) {
+      if (tp->steps_per_lb >= ORC_MAX_STEPS_PER_LB) return -2;
+      tp->steps_per_lb *= 2;
+      tp->dt /= 2.;
+      tp->since_change = 0;
+      tp->halvings++;
+      memcpy(T, Tp, sizeof(double) * D);
+      sub = 0;
+      continue;
+    }
+    tp->last_iters = it;
+    sub++;
+  }
+  return 0;
 }

@@ -30,6 +30,14 @@ class OrcSolver(C.Structure):
                 ("inject_nan", C.c_int), ("halvings", C.c_int)]
 
 
+class OrcTSolver(C.Structure):
+    _fields_ = [("alpha_fluid", C.c_double), ("alpha_foul", C.c_double),
+                ("pr_turb_inv", C.c_double), ("t_in", C.c_double), ("rtol", C.c_double),
+                ("atol", C.c_double), ("maxit", C.c_int), ("steps_per_lb", C.c_int),
+                ("dt", C.c_double), ("since_change", C.c_int), ("last_iters", C.c_int),
+                ("halvings", C.c_int), ("inject_nan", C.c_int)]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
@@ -186,3 +194,28 @@ class OracleLattice:
 
     def reduce(self, a, op=1):
         return self.lib.orc_reduce(p(np.ascontiguousarray(a)), len(a), op)
+
+    # --- temperature (clVariablesFD) ---------------------------------------
+    def init_thermal(self, *, alpha_fluid, alpha_foul, pr_turb, t_in, t_initial, dt=1.0,
+                     rtol=1e-4, atol=1e-7, maxit=1000, nType=None):
+        """clVariablesFD::ini: T = Tinitial on the nodes that are fluid in the
+        clean geometry, 0 elsewhere; stepsPerLB = (int)(1 / dTtfd).  nType
+        (default the geometry's) carries the fouling layer: solid now, fluid
+        in the clean geometry."""
+        self.tnType = np.ascontiguousarray(self.nType if nType is None else nType, dtype=np.int16)
+        m0fluid = (self.tnType.astype(np.int32) & 0x8) != 0
+        self.T = np.where(m0fluid, float(t_initial), 0.0)
+        self.ts = OrcTSolver(alpha_fluid, alpha_foul, 1.0 / pr_turb, t_in, rtol, atol, maxit,
+                             int(1.0 / dt), dt, 0, 0, 0, 0)
+        self.twork = np.zeros(16 * self.D)
+        self.dX0 = self.dX.copy()
+
+    def t_step(self, n=1):
+        """n calls of clVariablesFD::Solve with the current u and nu_t."""
+        nut = self.nut if self.komega else np.zeros(self.D)
+        for _ in range(n):
+            rc = self.lib.orc_t_step(C.byref(self.par), p(self.tnType), p(self.dX0), p(self.T),
+                                     p(nut), p(self.ux), p(self.uy), C.byref(self.ts),
+                                     p(self.twork))
+            if rc != 0:
+                raise RuntimeError(f"oracle temperature step failed ({rc})")

@@ -732,3 +732,119 @@ def test_komega_solve_matches_a_direct_solve_with_non_dominant_rows():
     print(f"{int(np.sum(off[fl] > 1.0))} non-dominant rows, {it} iterations, {err:.2e} from the direct solve")
     assert 0 < it < 200
     assert err < 1e-9
+
+
+def _thermal_1d(y_nodes, wall_lo, wall_hi, alpha, nut, uy, prt, dt, rhs):
+    """One implicit step of T_t + uy T_y = d/dy((alpha + nut/Pr_t) T_y)-type
+    rows in y alone, with the textbook nonuniform three-point differences
+    between Dirichlet walls (T = 0) at wall_lo and wall_hi: spacing to a wall
+    is the distance to it, to a node 1.  The eddy-diffusivity gradient enters
+    as advection -(1/Pr_t) d(nu_t)/dy with the centre weight taken once more
+    by 1/Pr_t, as Update_T_Coeffs_Implicit does.  Returns T1 at y_nodes."""
+    n = len(y_nodes)
+    M = np.zeros((n, n))
+    for k, y in enumerate(y_nodes):
+        dn = min(1.0, wall_hi - y)
+        ds = min(1.0, y - wall_lo)
+        dn, ds = max(dn, 0.1), max(ds, 0.1)
+        dc = dn + ds
+        d1 = [ds / (dn * dc), -dn / (ds * dc), (dn - ds) / (dn * ds)]
+        d2 = [2 / (dn * dc), 2 / (ds * dc), -2 / (dn * ds)]
+        if abs(d1[0]) > 10:
+            d1 = [1 / dc, -1 / dc, 0.0]
+        nut_n = nut[k + 1] if k + 1 < n else 0.0
+        nut_s = nut[k - 1] if k > 0 else 0.0
+        at = nut[k] / prt
+        Jy = uy - (d1[0] * nut_n + d1[1] * nut_s + d1[2] * at) / prt
+        a = alpha[k] + at
+        M[k, k] = 1 + dt * (Jy * d1[2] - a * d2[2])
+        if k + 1 < n:
+            M[k, k + 1] = dt * (Jy * d1[0] - a * d2[0])
+        if k > 0:
+            M[k, k - 1] = dt * (Jy * d1[1] - a * d2[1])
+    return np.linalg.solve(M, rhs)
+
+
+@pytest.mark.parametrize("fouled", [False, True])
+def test_thermal_step_reduces_to_one_dimensional_rows_in_y(fouled):
+    """Update_T_Coeffs_Implicit (tfdKernels.cl:122-280) and the solve: on a
+    flat channel with clean walls at y = 0.5 and y = nY - 1.3 (wall spacings
+    0.5 and 0.3), uniform u = (U, 0), nu_t linear in y and
+    T0 = (a + b x) phi(y), one implicit step away from the inlet and outlet
+    columns is T1 = c(y) + x d(y) (central differences are exact for linear
+    x), where (I - dt L_y) d = b phi and (I - dt L_y) c = a phi - dt U d with
+    L_y the one-dimensional rows of the two-point-per-side nonuniform
+    differences.  With fouled = True the two rows above the bottom wall are a
+    fouling layer (solid now, fluid in the clean geometry): alpha_foul there,
+    u = nu_t = 0 there, the wall spacing still measured to the clean wall."""
+    nX, nY = 96, 24
+    lo, hi = 0.5, nY - 1.3
+    C0, BL = channel.wall_points(nX, lambda x: np.full_like(x, lo), lambda x: np.full_like(x, hi))
+    g = oracle.geometry(nX, nY, C0, BL, x_release_pos=10.0, x_stop_pos=nX - 12.0)
+    orc = oracle.OracleLattice(g, tau0=1.25, Fval=0.0, nu=0.1, komega=True)
+    XF, D = g["XF"], g["D"]
+    nType = g["nType"].astype(np.int32).copy()
+    ys = np.arange(nY)
+    rows = [j for j in range(nY) if not (nType[nX // 2 + XF * j] & 0x4)]
+    foul_rows = rows[:2] if fouled else []
+    for j in foul_rows:
+        for i in range(nX):
+            nType[i + XF * j] = (nType[i + XF * j] & ~0x2) | 0x1  # solid now, M0 fluid kept
+    U, prt, dt, a0, b0, al_f, al_s = 0.01, 0.9, 1.0, 0.3, 0.004, 0.02, 0.005
+    nut_y = np.where(np.isin(ys, rows) & ~np.isin(ys, foul_rows), 0.002 + 0.0004 * ys, 0.0)
+    phi = np.where(np.isin(ys, rows), 0.2 + 0.8 * np.sin(np.pi * (ys - lo) / (hi - lo)) ** 2
+                   + 0.01 * ys, 0.0)
+    X = np.arange(XF)[None, :]
+    orc.init_thermal(alpha_fluid=al_f, alpha_foul=al_s, pr_turb=prt, t_in=2.0, t_initial=0.0,
+                     rtol=1e-14, atol=0.0, maxit=500, nType=nType.astype(np.int16))
+    fluid_now = np.isin(ys, rows) & ~np.isin(ys, foul_rows)
+    orc.T[:] = ((a0 + b0 * X) * phi[:, None]).reshape(-1) * (np.arange(D) % XF < nX)
+    orc.ux[:] = np.where(np.repeat(fluid_now, XF), U, 0.0)
+    orc.uy[:] = 0.0
+    orc.nut[:] = np.repeat(nut_y, XF)
+    orc.komega = True
+    orc.t_step(1)
+    alpha = np.array([al_s if j in foul_rows else al_f for j in rows])
+    yr = np.array(rows, dtype=float)
+    d = _thermal_1d(yr, lo, hi, alpha, nut_y[rows], 0.0, prt, dt, b0 * phi[rows])
+    # the x-linear part advected at U only where the fluid moves
+    Uk = np.where(fluid_now[rows], U, 0.0)
+    c = _thermal_1d(yr, lo, hi, alpha, nut_y[rows], 0.0, prt, dt, a0 * phi[rows] - dt * Uk * d)
+    T1 = orc.T.reshape(nY, XF)
+    worst = 0.0
+    for x in range(30, 66):
+        expect = c + x * d
+        worst = max(worst, np.max(np.abs(T1[rows, x] - expect)) / np.max(np.abs(expect)))
+    print(f"fouled={fouled}: max relative deviation {worst:.2e}")
+    assert worst < 1e-10
+
+
+def test_thermal_outlet_column_has_zero_gradient():
+    """The outlet column's east neighbour is itself (zero gradient: its
+    coefficient joins the diagonal, tfdKernels.cl:246-250).  With u = 0 and
+    T0 uniform in x, every column beyond the inlet's reach, the outlet
+    included, must take the same one-dimensional step in y; dropping the
+    outlet's east coefficient instead of folding it in would leave the last
+    column off.  (The inlet column sees T_in as its west neighbour, which the
+    interior does not, so it is left out.)"""
+    nX, nY = 64, 20
+    lo, hi = 0.5, nY - 1.3
+    C0, BL = channel.wall_points(nX, lambda x: np.full_like(x, lo), lambda x: np.full_like(x, hi))
+    g = oracle.geometry(nX, nY, C0, BL, x_release_pos=10.0, x_stop_pos=nX - 12.0)
+    orc = oracle.OracleLattice(g, tau0=1.25, Fval=0.0, nu=0.1, komega=True)
+    XF = g["XF"]
+    t_in = 1.7
+    orc.init_thermal(alpha_fluid=0.03, alpha_foul=0.01, pr_turb=0.9, t_in=t_in, t_initial=t_in,
+                     rtol=1e-14, atol=0.0, maxit=500)
+    orc.ux[:] = 0.0
+    orc.uy[:] = 0.0
+    orc.nut[:] = 0.0
+    orc.t_step(1)
+    T1 = orc.T.reshape(nY, XF)[:, :nX]
+    rows = [j for j in range(nY) if not (g["nType"][nX // 2 + XF * j] & 0x4)]
+    y = np.array(rows, dtype=float)
+    expect = _thermal_1d(y, lo, hi, np.full(len(rows), 0.03), np.zeros(len(rows)), 0.0, 0.9, 1.0,
+                         np.full(len(rows), t_in))
+    dev = np.max(np.abs(T1[rows, 16:] - expect[:, None])) / t_in
+    print(f"columns 16..{nX - 1} against the 1-D step: {dev:.2e}")
+    assert dev < 1e-12
