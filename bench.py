@@ -30,7 +30,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 B_LU = {"f32": 74.0, "f16": 38.0, "bf16": 38.0, "f64": 146.0, "f32x": 74.0, "k32": 94.0,
-        "k16": 58.0}
+        "k16": 58.0, "t": 68.0}
+# t: one temperature call per node: T read and written (16), u (8, FP32),
+# nu_t (8), the clean-wall spacings (32), the two masks (4)
 
 
 def log(msg):
@@ -278,6 +280,46 @@ def measure_reference_step(L, fluid, steps=240, warmup=60):
             "semantics": "macros every step, wall shear every 3 steps"}
 
 
+def measure_thermal(calls=50):
+    """The temperature solve (clVariablesFD::Solve) on the 4096x512 k-omega
+    channel after 200 k-omega steps: the time of one call with the flow
+    held (every call assembles the rows and solves to the reference's
+    tolerances) and of the reference's step, LB + k-omega + temperature."""
+    from komegafoulingmodel_amd import channel, lib
+
+    wk = channel.config("c2")
+    log(f"t32 on {wk.nX}x{wk.nY}")
+    L, n = setup(wk, "f32", komega=True)
+    rows = int(np.count_nonzero((L.geometry("nType").astype(np.int32) & 0x4) == 0))
+    L.thermal_init(alpha_fluid=wk.nu / 0.708, alpha_foul=(0.057 / 35.0 / 866.8) * wk.nu / 4.1e-5,
+                   pr_turb=0.9, t_in=2.0, t_initial=0.5)
+    opt = lib.OPTION_SAVE_MACRO_FIELDS
+    L.step(200, opt)
+    for _ in range(20):
+        L.step(1, opt)
+        L.thermal_step(1)
+    L.synchronize()
+    t0 = time.perf_counter()
+    L.thermal_step(calls)
+    L.synchronize()
+    ms_t = (time.perf_counter() - t0) * 1e3 / calls
+    st = L.solver_stats()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        L.step(1, opt)
+        L.thermal_step(1)
+    L.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / calls
+    L.close()
+    achieved = B_LU["t"] * rows / (ms_t * 1e-3) / 1e9
+    return {"rows": rows, "ms_per_call": ms_t, "mrows_per_s": rows / ms_t / 1e3,
+            "achieved_gbs": achieved, "frac": achieved / HBM_PEAK_GBS, "bytes_per_row": B_LU["t"],
+            "iterations_last": st.temp_iters,
+            "residual_last": st.temp_res / st.temp_res0 if st.temp_res0 else 0.0,
+            "ms_per_step_lb_komega_temperature": ms_step,
+            "mlups_with_temperature": n / ms_step / 1e3}
+
+
 def kernel_name(launch, storage="float"):
     if launch.get("steps_per_pass", 1) == 2:
         tile = {512: (64, 8), 640: (128, 5), 832: (64, 13), 1856: (64, 29), 1024: (64, 16),
@@ -365,7 +407,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--forms", default="p32_ref,f32x,f16,bf16,prec,k32,k16,k32_c3,adj32,adj32_exact,adj32_exact_smooth,adj32_c3,ens32,ens16,ens32ko,ens16ko")
+    ap.add_argument("--forms", default="p32_ref,f32x,f16,bf16,prec,k32,k16,k32_c3,t32,adj32,adj32_exact,adj32_exact_smooth,adj32_c3,ens32,ens16,ens32ko,ens16ko")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true")
@@ -439,6 +481,8 @@ def main():
             forms[form]["residual_last"] = [st.kappa_res / st.kappa_res0 if st.kappa_res0 else 0.0,
                                             st.omega_res / st.omega_res0 if st.omega_res0 else 0.0]
             Lk.close()
+        elif form == "t32":
+            forms["t32"] = measure_thermal()
         elif form in ("ens32", "ens16"):
             forms[form] = measure_ensemble("f32" if form == "ens32" else "f16")
         elif form in ("ens32ko", "ens16ko"):

@@ -29,6 +29,11 @@ static int printParams(const std::string& loadDir) {
               vlb.Re, vlb.Fval, vlb.RhoVal);
   std::printf("UseTurbModel %d\nStopTime %u\ntrStepsWall %u\n",
               vlb.kOmegaClass.kOmegaSolverFlag ? 1 : 0, p.StopTime, p.trSteps_wall);
+  clVariablesFD vfd;
+  vfd.loadParams(p, vlb);
+  std::printf("ThermalSolver %d\nAlphaFluid %.17g\nAlphaFoul %.17g\nPrTurb %.17g\nTin %.17g\n"
+              "Tinitial %.17g\ntfdSteps %u\n", vfd.thermalSolverFlag ? 1 : 0, vfd.Alpha_fluid,
+              vfd.Alpha_foul, vfd.PrTurbNum, vfd.ROE_INX, vfd.ROE0, p.tfdSteps);
   std::printf("IniPoiseuille %d\nIniTurbVelocity %d\nPerturbVelField %d\nSaveAvgInfo %d\n"
               "LoadVelText %d\nUseTurbVelBC %d\nGeomPenaltyFactor %.17g\nlbOutMaxLines %d\n",
               vlb.iniPoiseuille, vlb.kOmegaClass.iniTurbVelocity, vlb.kOmegaClass.perturbVelField,
@@ -60,11 +65,18 @@ int main(int argc, char** argv) {
     vls.loadParams(p, loadDir);
     clVariablesLB vlb;
     vlb.loadParams(p);
+    clVariablesFD vfd;
+    vfd.loadParams(p, vlb);
     vlb.ini(p, vls, loadDir, outDir);
     // "Save Macros On Start" (clVariablesLB.cpp:362-363)
     if (vlb.saveMacroStart && !vlb.restarted) vlb.save2file(outDir);
-    std::printf("nX %d nY %d tau0 %.6f nu %.6g Fval %.6g k-omega %s\n", p.nX, p.nY, vlb.tau,
-                vlb.MuVal, vlb.Fval, vlb.kOmegaClass.kOmegaSolverFlag ? "on" : "off");
+    vfd.ini(p, vlb.ctx, loadDir);
+    // clVariablesFD::ini's "Save Macros On Start" (clVariablesFD.cpp:279-280):
+    // the dump then carries the initial temperature too
+    if (vfd.thermalSolverFlag && vfd.saveMacroStart && !vlb.restarted) vlb.save2file(outDir);
+    std::printf("nX %d nY %d tau0 %.6f nu %.6g Fval %.6g k-omega %s temperature %s\n", p.nX, p.nY,
+                vlb.tau, vlb.MuVal, vlb.Fval, vlb.kOmegaClass.kOmegaSolverFlag ? "on" : "off",
+                vfd.thermalSolverFlag ? "on" : "off");
     if (p.RestartTime == 0 && vlb.tlbmNumIniSteps_LB > 0) vlb.getInitialFlow(p, outDir);
     shearStress ss;
     // a run restarted from a RunParam.yaml without the cadence keys resumes
@@ -75,8 +87,12 @@ int main(int argc, char** argv) {
       unsigned n = std::min(p.StopTime - p.Time, p.trSteps_wall);
       n = std::min(n, p.nextSaveAvgTime - p.Time);
       n = std::min(n, p.nextDumpStepTime - p.Time);
+      // the temperature solve follows the LB step every "LB Steps Per T"
+      // steps (LBLS.cpp:197-200, 230-233, 315-318)
+      if (vfd.thermalSolverFlag) n = std::min(n, p.tfdSteps - p.Time % p.tfdSteps);
       vlb.Solve(int(n));
       p.Time += n;
+      if (vfd.thermalSolverFlag && p.Time % p.tfdSteps == 0) vfd.Solve();
       if (p.Time % p.trSteps_wall == 0) ss.calculateShear(vlb.ctx);
       if (p.Time >= p.nextSaveAvgTime) {
         // rows held in memory, written out every "lbOut Max Lines" rows

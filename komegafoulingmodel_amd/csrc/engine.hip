@@ -797,6 +797,8 @@ class EngineT final : public Engine {
     return 0;
   }
 
+  bool thermalOn() const override { return tOn_; }
+
   int setTemperature(const double* T, std::string& err) override {
     if (!tOn_) return fail(err, "initialise the temperature solve first");
     HIP_TRY(hipMemcpy(T_.as<void>(), T, size_t(D_) * sizeof(double), hipMemcpyHostToDevice));

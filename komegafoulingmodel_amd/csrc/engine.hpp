@@ -86,6 +86,7 @@ class Engine {
   virtual int thermalStep(int n, std::string& err) = 0;
   virtual int thermalTimeStep(int action, int& stepsPerLB, double& dt, std::string& err) = 0;
   virtual int setTemperature(const double* T, std::string& err) = 0;
+  virtual bool thermalOn() const = 0;
   virtual int64_t fieldSize(int field) const = 0;
   virtual int setLaunch(int block, int npt, int spp, int64_t pad, std::string& err) = 0;
   virtual void setGraphs(bool on) = 0;

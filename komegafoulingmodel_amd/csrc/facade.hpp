@@ -33,6 +33,8 @@ class clProblem {
   double Pipe_radius = 0.;
   unsigned StopTime = 0, Time = 0, RestartTime = 0;
   unsigned trSteps_wall = 3;
+  unsigned tfdSteps = 1;  // "LB Steps Per T" (clProblem.cpp:506); dTtfd = tfdSteps
+  double dTtfd = 1.;
   unsigned saveAvgStartTime = 0, saveTimeStepAvg = 1000, dumpStartTime = 0, dumpTimeStep = 1000;
   unsigned saveBinStep = 1;  // dumps per restart-file save ("Save Bin Step", clProblem.cpp:515)
   // The cadence state a restart resumes from (clProblem.cpp:540-550, written
@@ -56,6 +58,10 @@ class clProblem {
     Time = RestartTime;
     StopTime = unsigned(params.getInt(d, "Stop Time", 200000000));
     trSteps_wall = unsigned(params.getInt(d, "LB Steps Per TR Wall", 3));
+    const int tfd = params.getInt(d, "LB Steps Per T", 1);
+    if (tfd < 1) throw std::runtime_error("LB Steps Per T must be >= 1");
+    tfdSteps = unsigned(tfd);
+    dTtfd = double(tfdSteps);
     cadence("Save Avgs", 2000, saveAvgStartTime, saveTimeStepAvg);
     cadence("Dump", 1000, dumpStartTime, dumpTimeStep);
     const int bin = params.getInt(d, "Save Bin Step", 1);
@@ -255,6 +261,7 @@ class kOmega {
 class clVariablesLB {
  public:
   double tau = 1.99, MuVal = 0., Re = 10800., Fval = 0., RhoVal = 1.;
+  double nuAir = 4.1e-5;  // "Nu air": DELTA_L^2 / DELTA_T = MuVal / nuAir (clVariablesLB.cpp:608)
   bool tauDefined = true, reDefined = true;
   int tlbmIniDumpStep = 100000, tlbmNumIniSteps_LB = 1000000;
   // ConstDef.h:159-192 defaults
@@ -300,10 +307,11 @@ class clVariablesLB {
     for (const std::string& k : rp.keys(d))
       if (!knownFluidKey(k))
         throw std::runtime_error("unsupported fluid parameter \"" + k + "\" in RunParam.yaml");
-    // the thermal (FD) warm-up is not part of this step
+    // the combined LB / thermal warm-up of clVariablesLB::ini is not built
     if (rp.getBool(d, "Run LBFD First", false))
-      throw std::runtime_error("\"Run LBFD First\" needs the thermal solver, which this build "
-                               "does not include");
+      throw std::runtime_error("\"Run LBFD First\" (the combined LB and thermal warm-up) is "
+                               "not part of this build");
+    nuAir = rp.getDouble(d, "Nu air", 4.1e-5);  // VISC_AIR, ConstDef.h:59
     double mu = 0., t = 0.;
     switch (rp.either(d, "Mu", "Tau", mu, t)) {
       case 1: MuVal = mu; tau = 1. / (MuVal * 3. + 0.5); tauDefined = false; break;
@@ -485,7 +493,7 @@ class clVariablesLB {
     return true;
   }
 
- private:
+  // a .bin file of n doubles (Array.h's binary layout)
   static bool readBin(const std::string& path, size_t n, std::vector<double>& v) {
     FILE* f = std::fopen(path.c_str(), "rb");
     if (!f) return false;
@@ -493,6 +501,112 @@ class clVariablesLB {
     const size_t got = std::fread(v.data(), sizeof(double), n, f);
     std::fclose(f);
     return got == n;
+  }
+};
+
+// clVariablesFD (clVariablesFD.h/.cpp): the temperature solve after the
+// k-omega step.  Its keys are in the thermal document (ParamsName 3);
+// without that document the solve is off.  Update_T_Coeffs_Implicit reads
+// the eddy viscosity, so the reference's temperature kernels need k-omega:
+// the solve on with k-omega off is an error.
+class clVariablesFD {
+ public:
+  bool thermalSolverFlag = false, calcNuFlag = true, saveMacroStart = false, calcSSTempFlag = true;
+  bool chtCorrectionFlag = true;
+  // ConstDef.h:59-63, 232-249, 299-304
+  double PrNum = 0.708, PrTurbNum = 0.9, kSoot = 0.057, kAir = 0.042, rhoSoot = 35.,
+         cpSoot = 866.8;
+  double T_Actual_Max = 400. + 273., T_Actual_Min = 90. + 273., ROE0 = 0.5, ROE_INX = 2.0;
+  double tempMaxRelTol = 1e-4, tempMaxAbsTol = 1e-7;
+  int tempMaxIters = 1000;
+  double nuCutoffRadius = 5.;
+  int maxOutputLinesNu = 10;
+  double Alpha_fluid = 0., Alpha_foul = 0.;
+  bool tempLoaded = false;
+  kofm_ctx* ctx = nullptr;
+
+  static bool knownKey(const std::string& k) {
+    static const char* const keys[] = {
+        "ParamsName", "Dummy", "Restart Run", "Thermal Solver", "Calculate Nu",
+        "Save Macros On Start", "Solve Steady Temp", "Use CHT Correction", "Pr Number", "K Soot",
+        "K air", "Rho Soot", "Cp Soot", "Tinlet Actual", "Twalls Actual", "Tinitial",
+        "Tinlet Sim", "Temp Max Rel Tol", "Temp Max Abs Tol", "Temp Max Iterations",
+        "Nu Cutoff Radius", "Max Nu Output Lines"};
+    for (const char* s : keys)
+      if (k == s) return true;
+    return false;
+  }
+
+  // clVariablesFD::loadParams (clVariablesFD.cpp:445-497).  "Pr Number" sets
+  // both Pr and the turbulent Pr (the reference reads PrTurbNum from the same
+  // key, :456), which is kept.  "Calculate Nu" is read, but the Nusselt
+  // number is not built; "Solve Steady Temp" is read and, as in the reference
+  // (whose only call of solveSSThermal is commented out,
+  // clVariablesLB.cpp:237), changes nothing; "Use CHT Correction" only adds an
+  // argument that the active kernel does not use.
+  void loadParams(const clProblem& p, const clVariablesLB& vlb) {
+    const RunParams& rp = p.params;
+    const int d = RunParams::kThermal;
+    bool present = false;
+    for (const std::string& k : rp.keys(d)) {
+      present = true;
+      if (!knownKey(k))
+        throw std::runtime_error("unsupported thermal parameter \"" + k + "\" in RunParam.yaml");
+    }
+    thermalSolverFlag = present && rp.getBool(d, "Thermal Solver", true);
+    calcNuFlag = rp.getBool(d, "Calculate Nu", true);
+    saveMacroStart = rp.getBool(d, "Save Macros On Start", false);
+    calcSSTempFlag = rp.getBool(d, "Solve Steady Temp", true);
+    chtCorrectionFlag = rp.getBool(d, "Use CHT Correction", true);
+    PrNum = rp.getDouble(d, "Pr Number", 0.708);
+    PrTurbNum = rp.getDouble(d, "Pr Number", 0.9);
+    Alpha_fluid = vlb.MuVal / PrNum;
+    kSoot = rp.getDouble(d, "K Soot", 0.057);
+    kAir = rp.getDouble(d, "K air", 0.042);
+    rhoSoot = rp.getDouble(d, "Rho Soot", 35.);
+    cpSoot = rp.getDouble(d, "Cp Soot", 866.8);
+    // (kSoot / rhoSoot / cpSoot) DELTA_L^2 / DELTA_T with DELTA_T =
+    // nuAir DELTA_L^2 / MuVal (clVariablesLB.cpp:608)
+    Alpha_foul = (kSoot / rhoSoot / cpSoot) * (vlb.MuVal / vlb.nuAir);
+    T_Actual_Max = rp.getDouble(d, "Tinlet Actual", 400. + 273.);
+    T_Actual_Min = rp.getDouble(d, "Twalls Actual", 90. + 273.);
+    ROE0 = rp.getDouble(d, "Tinitial", 0.5);
+    ROE_INX = rp.getDouble(d, "Tinlet Sim", 2.0);
+    tempMaxRelTol = rp.getDouble(d, "Temp Max Rel Tol", 1e-4);
+    tempMaxAbsTol = rp.getDouble(d, "Temp Max Abs Tol", 1e-7);
+    tempMaxIters = rp.getInt(d, "Temp Max Iterations", 1000);
+    nuCutoffRadius = rp.getDouble(d, "Nu Cutoff Radius", 5.);
+    maxOutputLinesNu = rp.getInt(d, "Max Nu Output Lines", 10);
+    if (thermalSolverFlag && !vlb.kOmegaClass.kOmegaSolverFlag)
+      throw std::runtime_error("the temperature solve needs k-omega (\"Use Turb Model\"): its rows "
+                               "take the eddy viscosity");
+  }
+
+  // clVariablesFD::ini (clVariablesFD.cpp:230-283) with testRestartRun
+  // (:790-806): load/temp.bin, when present, replaces the initial field.
+  void ini(const clProblem& p, kofm_ctx* c, const std::string& loadDir = "load") {
+    ctx = c;
+    if (!thermalSolverFlag) return;
+    kofm_thermal_params q{};
+    q.alpha_fluid = Alpha_fluid;
+    q.alpha_foul = Alpha_foul;
+    q.pr_turb = PrTurbNum;
+    q.t_in = ROE_INX;
+    q.t_initial = ROE0;
+    q.dt = p.dTtfd;
+    q.rel_tol = tempMaxRelTol;
+    q.abs_tol = tempMaxAbsTol;
+    q.max_iters = tempMaxIters;
+    check(ctx, kofm_thermal_init(ctx, &q), "clVariablesFD::ini");
+    const size_t D = size_t(p.XsizeFull) * p.nY;
+    std::vector<double> T;
+    tempLoaded = clVariablesLB::readBin(loadDir + "/temp.bin", D, T);
+    if (tempLoaded) check(ctx, kofm_set_temperature(ctx, T.data()), "load temp.bin");
+  }
+
+  // clVariablesFD::Solve (clVariablesFD.h:343-358)
+  void Solve() {
+    if (thermalSolverFlag) check(ctx, kofm_thermal_step(ctx, 1), "clVariablesFD::Solve");
   }
 };
 

@@ -93,6 +93,12 @@ int saveFields(Engine& e, int nX, int nY, bool komega, const std::string& dir, s
     if (!rc) rc = writeTxt2D(dir + "/" + it.txt + ".txt", a, nX, XF, nY, err);
     if (!rc && it.bin) rc = writeBin(dir + "/" + it.bin + ".bin", a, err);
   }
+  // clVariablesFD::save2file (clVariablesFD.cpp:545-548): the temperature
+  // array "temp" in the same text layout
+  if (!rc && e.thermalOn()) {
+    rc = e.download(KOFM_FIELD_TEMP, a.data(), err);
+    if (!rc) rc = writeTxt2D(dir + "/temp.txt", a, nX, XF, nY, err);
+  }
   return rc;
 }
 
@@ -109,6 +115,13 @@ int saveRestart(Engine& e, int nX, int nY, bool komega, const std::vector<double
     if (!rc) rc = writeBin(dir + "/lbkappa.bin", a, err);
     if (!rc) rc = e.download(KOFM_FIELD_OMEGA, a.data(), err);
     if (!rc) rc = writeBin(dir + "/lbomega.bin", a, err);
+  }
+  // clVariablesFD::saveRestartFiles (clVariablesFD.cpp:550-553): temp.bin,
+  // which clVariablesFD::testRestartRun reads back (:797-806)
+  if (!rc && e.thermalOn()) {
+    std::vector<double> t(D);
+    rc = e.download(KOFM_FIELD_TEMP, t.data(), err);
+    if (!rc) rc = writeBin(dir + "/temp.bin", t, err);
   }
   if (!rc) rc = writeBin(dir + "/lsc.bin", C, err);
   return rc;

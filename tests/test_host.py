@@ -211,15 +211,18 @@ def test_ini_poiseuille_perturbation_uses_the_reference_generator():
     assert np.any(v1 != 0.0)
 
 
-def _kofm_params(tmp_path, fluid_extra):
+def _kofm_params(tmp_path, fluid_extra, thermal=None, system_extra=""):
     load = tmp_path / "load"
     load.mkdir(exist_ok=True)
+    thermal_doc = f"--- # Thermal Parameters\nParamsName: 3\n{thermal}\n" if thermal is not None else ""
     (load / "RunParam.yaml").write_text(f"""# test
 --- # System Parameters
 ParamsName: 1
 nX: !!int 256
 nY: !!int 40
 Pipe Radius: !!float 19.0
+{system_extra}
+{thermal_doc}
 
 --- # Fluid Parameters
 ParamsName: 2
@@ -258,3 +261,41 @@ def test_runparam_rejects_unsupported_fluid_keys(tmp_path):
     assert res.returncode != 0 and "thermal" in res.stderr
     res = _kofm_params(tmp_path, "Mu: !!float 0.1")  # Mu and Tau together
     assert res.returncode != 0 and "Cannot define both" in res.stderr
+
+
+def test_runparam_thermal_keys(tmp_path):
+    """clVariablesFD::loadParams (clVariablesFD.cpp:445-497) through the
+    facade.  Alpha_fluid = MuVal / Pr; Alpha_foul = (k_soot / rho_soot /
+    cp_soot) DELTA_L^2 / DELTA_T with DELTA_T = nu_air DELTA_L^2 / MuVal
+    (clVariablesLB.cpp:608), i.e. (k / rho / cp) MuVal / nu_air; the
+    turbulent Pr is read from the same "Pr Number" key (:456), so it equals
+    Pr when the key is given and 0.9 otherwise; dTtfd = "LB Steps Per T"."""
+    mu = (1 / 1.25 - 0.5) / 3
+    res = _kofm_params(tmp_path, "Use Turb Model: True", thermal="Dummy: !!int 0")
+    assert res.returncode == 0, res.stderr
+    v = dict(line.split() for line in res.stdout.strip().splitlines())
+    assert v["ThermalSolver"] == "1" and v["tfdSteps"] == "1"
+    assert abs(float(v["AlphaFluid"]) - mu / 0.708) < 1e-17
+    assert abs(float(v["AlphaFoul"]) - (0.057 / 35.0 / 866.8) * mu / 4.1e-5) < 1e-17
+    assert float(v["PrTurb"]) == 0.9 and float(v["Tin"]) == 2.0 and float(v["Tinitial"]) == 0.5
+    res = _kofm_params(tmp_path, "Use Turb Model: True\nNu air: !!float 2e-5",
+                       thermal="Pr Number: !!float 0.71\nK Soot: !!float 0.1\n"
+                               "Tinlet Sim: !!float 1.5\nTinitial: !!float 0.25",
+                       system_extra="LB Steps Per T: !!int 4")
+    assert res.returncode == 0, res.stderr
+    v = dict(line.split() for line in res.stdout.strip().splitlines())
+    assert abs(float(v["AlphaFluid"]) - mu / 0.71) < 1e-17 and float(v["PrTurb"]) == 0.71
+    assert abs(float(v["AlphaFoul"]) - (0.1 / 35.0 / 866.8) * mu / 2e-5) < 1e-17
+    assert float(v["Tin"]) == 1.5 and float(v["Tinitial"]) == 0.25 and v["tfdSteps"] == "4"
+    # no thermal document: off; "Thermal Solver: False": off
+    res = _kofm_params(tmp_path, "Use Turb Model: True")
+    assert dict(line.split() for line in res.stdout.strip().splitlines())["ThermalSolver"] == "0"
+    res = _kofm_params(tmp_path, "Use Turb Model: True", thermal="Thermal Solver: False")
+    assert dict(line.split() for line in res.stdout.strip().splitlines())["ThermalSolver"] == "0"
+
+
+def test_runparam_thermal_rejections(tmp_path):
+    res = _kofm_params(tmp_path, "Use Turb Model: True", thermal="Pr Numbr: !!float 0.7")
+    assert res.returncode != 0 and "unsupported thermal parameter" in res.stderr
+    res = _kofm_params(tmp_path, "Use Turb Model: False", thermal="Thermal Solver: True")
+    assert res.returncode != 0 and "needs k-omega" in res.stderr
