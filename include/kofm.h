@@ -172,6 +172,16 @@ typedef struct kofm_thermal_params {
   int32_t max_iters;
 } kofm_thermal_params;
 
+/* The Nusselt-number output (kofm_nusselt_init): height = the rows the bulk
+ * temperature sums (FULLSIZEY = 2 * (int)"Pipe Radius"); the wavy section
+ * ("Wavy Start Loc", "Period Length", "Num Periods"); the conductivities of
+ * soot and air in lattice units (KSOOT, KAIR = k * DELTA_F / DELTA_T,
+ * clVariablesTR.cpp:1912-1913). */
+typedef struct kofm_nusselt_params {
+  int32_t height, wavy_start, wavy_period, wavy_periods;
+  double k_soot, k_air;
+} kofm_nusselt_params;
+
 /* --- lifecycle ------------------------------------------------------------ */
 
 /* Allocates device storage for an nX x nY lattice on the current HIP device.
@@ -393,6 +403,38 @@ int kofm_thermal_time_step(kofm_ctx* ctx, int32_t action, int32_t* steps_per_lb,
 /* Replaces T (D doubles, reference layout), e.g. from load/temp.bin on a
  * restart (clVariablesFD.cpp:797-806). */
 int kofm_set_temperature(kofm_ctx* ctx, const double* T);
+
+/* clVariablesFD::iniNuCoeffs (clVariablesFD.cpp:362-442), after
+ * kofm_thermal_init: the Nusselt number over the columns of the particle
+ * domain of kofm_build_geometry's parameters, TR_X_IND_START =
+ * floor(x_min_val) to TR_X_IND_STOP = floor(x_stop_pos) (clVariablesTR.cpp:
+ * 1373), which must lie in [1, nX - 1].  Each column searches the boundary
+ * links whose box reaches it: the first min(MAX_BL_BOT - MIN_BL_BOT rounded
+ * up to 256, columns) links of each wall from MIN_BL_BOT / MIN_BL_TOP
+ * (clVariablesTR.cpp:497-530, updateNuCoeff1's launch, clVariablesFD.cpp:
+ * 164-165).  The wavy section is required, as in the reference (:364-366).
+ * kofm_update_walls keeps it set up; building the geometry again turns it
+ * off.  KOFM_ERR_ARG for parameters outside these ranges. */
+int kofm_nusselt_init(kofm_ctx* ctx, const kofm_nusselt_params* params);
+
+/* The sizes of kofm_nusselt's outputs: columns (TR_X_IND_STOP -
+ * TR_X_IND_START) and period boundaries (Num Periods + 1). */
+int kofm_nusselt_sizes(kofm_ctx* ctx, int32_t* columns, int32_t* locations);
+
+/* One Nusselt record (clVariablesFD::updateTimeData, clVariablesFD.cpp:
+ * 838-869) from the current T and the last saved velocity, all in FP64:
+ * the nearest node of each column to each wall along the wall normal
+ * (updateNuCoeff1/2, tfdKernels.cl:759-856; ties go to the lower link, then
+ * the lower row), then FD_calc_Nu (:919-982) into nu[5 * columns]: per
+ * column the bulk temperature (getTbm, :858-876, whose rows x + j * nY are
+ * kept), then the wall-normal gradient and the wall temperature (getTandDT,
+ * :878-917) of the bottom and of the top wall, -1000 where a wall has no
+ * node; and getNuMean (:984-1033, the OpenCL 1.2 form) into
+ * mean[2 * locations]: sum |u| T and sum |u| over the column at each
+ * period boundary.  When non-NULL, nodes[2 * columns] receives each slot's
+ * (row << 2) | type (bottom wall first; row -1 when none) and
+ * vec[4 * columns] its vector from the wall (x y interleaved). */
+int kofm_nusselt(kofm_ctx* ctx, double* nu, double* mean, int32_t* nodes, double* vec);
 
 /* Copies one field (KOFM_FIELD_*) to host FP64 in the reference layout. */
 int kofm_download(kofm_ctx* ctx, int32_t field, double* host);

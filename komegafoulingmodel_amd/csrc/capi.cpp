@@ -345,6 +345,41 @@ int kofm_set_temperature(kofm_ctx* ctx, const double* T) {
   return rc ? setErr(ctx, rc, err) : KOFM_OK;
 }
 
+int kofm_nusselt_init(kofm_ctx* ctx, const kofm_nusselt_params* q) {
+  if (!ctx || !q) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  kofm::NusseltParams p;
+  p.height = q->height;
+  p.wavyStart = q->wavy_start;
+  p.wavyPeriod = q->wavy_period;
+  p.wavyPeriods = q->wavy_periods;
+  p.kSoot = q->k_soot;
+  p.kAir = q->k_air;
+  std::string err;
+  int rc = ctx->engine->nusseltInit(p, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_nusselt_sizes(kofm_ctx* ctx, int32_t* columns, int32_t* locations) {
+  if (!ctx) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int c = 0, l = 0;
+  int rc = ctx->engine->nusseltSizes(c, l, err);
+  if (rc) return setErr(ctx, rc, err);
+  if (columns) *columns = c;
+  if (locations) *locations = l;
+  return KOFM_OK;
+}
+
+int kofm_nusselt(kofm_ctx* ctx, double* nu, double* mean, int32_t* nodes, double* vec) {
+  if (!ctx) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int rc = ctx->engine->nusselt(nu, mean, nodes, vec, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
 int kofm_get_solver_stats(kofm_ctx* ctx, kofm_solver_stats* out) {
   if (!ctx || !out) return KOFM_ERR_ARG;
   NEED_ENGINE(ctx);

@@ -7,6 +7,7 @@
 // lbkappa.bin, lbomega.bin for a restart), runs "Ini LB Steps" warm-up steps
 // and then steps to "Stop Time", computing the wall shear every
 // "LB Steps Per TR Wall" steps, appending the reductions every "Save Avgs"
+// interval, the Nusselt record (with the temperature solve) every "Save Nu"
 // interval and writing the save2file dumps every "Dump" interval.
 #include <algorithm>
 #include <cstdio>
@@ -34,6 +35,10 @@ static int printParams(const std::string& loadDir) {
   std::printf("ThermalSolver %d\nAlphaFluid %.17g\nAlphaFoul %.17g\nPrTurb %.17g\nTin %.17g\n"
               "Tinitial %.17g\ntfdSteps %u\n", vfd.thermalSolverFlag ? 1 : 0, vfd.Alpha_fluid,
               vfd.Alpha_foul, vfd.PrTurbNum, vfd.ROE_INX, vfd.ROE0, p.tfdSteps);
+  std::printf("CalcNu %d\nKSootLattice %.17g\nKAirLattice %.17g\nWavyStart %d\nPeriodLength %d\n"
+              "NumPeriods %d\nSaveNuTimestep %u\nnextSaveNuTime %u\nnextSaveAvgTime %u\n",
+              vfd.nuOn() ? 1 : 0, vfd.kSootLattice, vfd.kAirLattice, p.xWavyStart, p.wavyPeriodLen,
+              p.numWavyPeriods, p.saveTimeStepNu, p.nextSaveNuTime, p.nextSaveAvgTime);
   std::printf("IniPoiseuille %d\nIniTurbVelocity %d\nPerturbVelField %d\nSaveAvgInfo %d\n"
               "LoadVelText %d\nUseTurbVelBC %d\nGeomPenaltyFactor %.17g\nlbOutMaxLines %d\n",
               vlb.iniPoiseuille, vlb.kOmegaClass.iniTurbVelocity, vlb.kOmegaClass.perturbVelField,
@@ -71,22 +76,25 @@ int main(int argc, char** argv) {
     // "Save Macros On Start" (clVariablesLB.cpp:362-363)
     if (vlb.saveMacroStart && !vlb.restarted) vlb.save2file(outDir);
     vfd.ini(p, vlb.ctx, loadDir);
+    vfd.iniNuCoeffs(p, vlb.restarted, outDir);
     // clVariablesFD::ini's "Save Macros On Start" (clVariablesFD.cpp:279-280):
     // the dump then carries the initial temperature too
     if (vfd.thermalSolverFlag && vfd.saveMacroStart && !vlb.restarted) vlb.save2file(outDir);
-    std::printf("nX %d nY %d tau0 %.6f nu %.6g Fval %.6g k-omega %s temperature %s\n", p.nX, p.nY,
-                vlb.tau, vlb.MuVal, vlb.Fval, vlb.kOmegaClass.kOmegaSolverFlag ? "on" : "off",
-                vfd.thermalSolverFlag ? "on" : "off");
+    std::printf("nX %d nY %d tau0 %.6f nu %.6g Fval %.6g k-omega %s temperature %s Nu %s\n", p.nX,
+                p.nY, vlb.tau, vlb.MuVal, vlb.Fval, vlb.kOmegaClass.kOmegaSolverFlag ? "on" : "off",
+                vfd.thermalSolverFlag ? "on" : "off", vfd.nuOn() ? "on" : "off");
     if (p.RestartTime == 0 && vlb.tlbmNumIniSteps_LB > 0) vlb.getInitialFlow(p, outDir);
     shearStress ss;
     // a run restarted from a RunParam.yaml without the cadence keys resumes
     // at the next cadence point after its current time
     while (p.nextSaveAvgTime <= p.Time) p.nextSaveAvgTime += p.saveTimeStepAvg;
+    while (p.nextSaveNuTime <= p.Time) p.nextSaveNuTime += p.saveTimeStepNu;
     while (p.nextDumpStepTime <= p.Time) p.nextDumpStepTime += p.dumpTimeStep;
     while (p.Time < p.StopTime) {
       unsigned n = std::min(p.StopTime - p.Time, p.trSteps_wall);
       n = std::min(n, p.nextSaveAvgTime - p.Time);
       n = std::min(n, p.nextDumpStepTime - p.Time);
+      if (vfd.nuOn()) n = std::min(n, p.nextSaveNuTime - p.Time);
       // the temperature solve follows the LB step every "LB Steps Per T"
       // steps (LBLS.cpp:197-200, 230-233, 315-318)
       if (vfd.thermalSolverFlag) n = std::min(n, p.tfdSteps - p.Time % p.tfdSteps);
@@ -100,6 +108,11 @@ int main(int argc, char** argv) {
         ss.updateTimeData(vlb.ctx, p.Time, outDir);
         p.nextSaveAvgTime += p.saveTimeStepAvg;
       }
+      // the Nusselt record (clProblem.cpp:110-114)
+      if (vfd.nuOn() && p.Time >= p.nextSaveNuTime) {
+        vfd.updateTimeData(p.Time);
+        p.nextSaveNuTime += p.saveTimeStepNu;
+      }
       if (p.Time >= p.nextDumpStepTime) {
         // clProblem::step's dump (clProblem.cpp:130-168): the save2file text
         // dumps and the held time data, RunParam.yaml with the current time
@@ -111,6 +124,7 @@ int main(int argc, char** argv) {
         if (bin) p.nextDumpBin += int(p.saveBinStep);
         vlb.save2file(outDir);
         vlb.saveTimeData();
+        vfd.saveTimeData();
         p.saveParameters(outDir);
         if (bin) {
           vlb.saveRestartFiles(loadDir);
@@ -120,6 +134,7 @@ int main(int argc, char** argv) {
     }
     vlb.save2file(outDir);
     vlb.saveTimeData();
+    vfd.saveTimeData();
     p.saveParameters(outDir);
     std::printf("done at time %u, Umean %g\n", p.Time, vlb.calcUmean());
   } catch (const std::exception& e) {

@@ -21,6 +21,7 @@
 #include "engine.hpp"
 #include "adjoint.hpp"
 #include "kernels.hpp"
+#include "nusselt.hpp"
 #include "walls.hpp"
 
 namespace kofm {
@@ -164,6 +165,8 @@ class EngineT final : public Engine {
     geo_ = g;
     wallsMoved_ = false;
     tOn_ = false;  // the temperature mask and field follow the clean geometry: init again
+    nuOn_ = false;
+    wallC_ = g.C0;
     gen_++;
     dropGraphs();
     const size_t D = size_t(D_);
@@ -407,6 +410,11 @@ class EngineT final : public Engine {
     geo_.dxSlot.clear();
     geo_.dxLink.clear();
     wallsMoved_ = true;
+    wallC_ = C;
+    if (nuOn_) {
+      const int rc = nusseltLinks(err);
+      if (rc) return rc;
+    }
     // wall nodes and shear coefficients from the updated node types
     int rc = refreshWallDerived(C, geo_, err);
     if (!rc) rc = uploadShear(err);
@@ -897,6 +905,101 @@ class EngineT final : public Engine {
     return 0;
   }
 
+  // clVariablesFD::iniNuCoeffs (clVariablesFD.cpp:362-442): the columns of
+  // the particle domain, TR_X_IND_START = floor(X Min Val) to TR_X_IND_STOP =
+  // floor(X Stop Pos), and the links searched for each, which are the first
+  // min(numBLPerSide rounded up to WORKGROUPSIZE_NU, NU_NUM_NODES) links of
+  // each wall from MIN_BL_BOT / MIN_BL_TOP (updateNuCoeff1's launch and guard,
+  // clVariablesFD.cpp:164-165, tfdKernels.cl:818-819).  The reference
+  // requires the wavy section (Wavy Start Loc, Period Length, Num Periods).
+  int nusseltInit(const NusseltParams& q, std::string& err) override {
+    if (!tOn_) return fail(err, "initialise the temperature solve before the Nusselt number");
+    const int xMin = int(std::floor(geo_.input.xMin)), xStop = int(std::floor(geo_.input.xStop));
+    const int n = xStop - xMin;
+    auto bad = [&err](const char* msg) {
+      err = msg;
+      return KOFM_ERR_ARG;
+    };
+    if (xMin < 1 || xStop > nX_ - 1 || n < 1)
+      return bad("Nusselt number: floor(X Min Val) must be >= 1 and below floor(X Stop Pos), "
+                 "which must be <= nX - 1");
+    if (q.wavyStart < 0 || q.wavyPeriod < 1 || q.wavyPeriods < 0)
+      return bad("Must provide period length and starting location of wavy section when "
+                 "calculating Nusselt number");
+    if (int64_t(q.wavyStart) + int64_t(q.wavyPeriod) * q.wavyPeriods > int64_t(nX_) - 1)
+      return bad("Nusselt number: the last period boundary lies past the channel");
+    if (q.height < 1 || int64_t(xStop - 1) + int64_t(q.height - 1) * nY_ >= D_)
+      return bad("Nusselt number: the bulk-temperature rows (x + j * nY, j < 2 * (int)Pipe "
+                 "Radius) must lie inside the lattice");
+    const int threads = (nY_ / 2 + 31) / 32 * 32;
+    if (threads < 32 || threads > dev::kNuMeanMaxThreads)
+      return bad("Nusselt number: nY / 2 rounded up to 32 must be 32 to 1024");
+    if (!(q.kSoot > 0.) || !(q.kAir >= 0.) || !std::isfinite(q.kSoot) || !std::isfinite(q.kAir))
+      return bad("Nusselt number: need K Soot > 0 and K air >= 0");
+    nuN_ = n;
+    nuXMin_ = xMin;
+    nuXStop_ = xStop;
+    nuPar_ = q;
+    nuLocs_ = q.wavyPeriods + 1;
+    nuThreads_ = threads;
+    HIP_TRY(nuBL_.alloc(geo_.BL.size() * sizeof(int32_t)));
+    HIP_TRY(hipMemcpy(nuBL_.as<void>(), geo_.BL.data(), geo_.BL.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice));
+    HIP_TRY(nuC_.alloc(wallC_.size() * sizeof(double)));
+    HIP_TRY(nuRows_.alloc(2 * size_t(n) * sizeof(int32_t)));
+    HIP_TRY(nuVec_.alloc(4 * size_t(n) * sizeof(double)));
+    HIP_TRY(nuOut_.alloc((5 * size_t(n) + 2 * size_t(nuLocs_)) * sizeof(double)));
+    const int rc = nusseltLinks(err);
+    if (rc) return rc;
+    nuOn_ = true;
+    return 0;
+  }
+
+  int nusseltSizes(int& columns, int& locations, std::string& err) const override {
+    if (!nuOn_) {
+      err = "initialise the Nusselt number first";
+      return KOFM_ERR_STATE;
+    }
+    columns = nuN_;
+    locations = nuLocs_;
+    return 0;
+  }
+
+  // clVariablesFD::updateTimeData (clVariablesFD.cpp:838-869): the nearest
+  // nodes again (the walls may have moved), FD_calc_Nu and getNuMean on the
+  // temperature and the last saved velocity.
+  int nusselt(double* nu, double* mean, int32_t* nodes, double* vec, std::string& err) override {
+    if (!nuOn_) return fail(err, "initialise the Nusselt number first");
+    if (!haveState_) return fail(err, "state not initialised");
+    const int n = nuN_;
+    dev::NuGeom g{nuC_.as<double>(), nuBL_.as<int32_t>(), nuStart_.as<int32_t>(),
+                  nuLinks_.as<int32_t>(), ntype_.as<int16_t>(), n, nuXMin_, nuXStop_, nY_, XF_};
+    double* out = nuOut_.as<double>();
+    hipLaunchKernelGGL(dev::nuNearest, dim3(unsigned((2 * n + 255) / 256)), dim3(256), 0, stream_,
+                       g, nuRows_.as<int32_t>(), nuVec_.as<double>());
+    hipLaunchKernelGGL((dev::nuColumns<R>), dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream_,
+                       (const double*)T_.as<double>(), (const R*)ux_.as<R>(), (const R*)uy_.as<R>(),
+                       (const double*)dX_.as<double>(), (const double*)dX0_.as<double>(),
+                       (const int32_t*)nuRows_.as<int32_t>(), (const double*)nuVec_.as<double>(), n,
+                       nuXMin_, nY_, XF_, D_, nuPar_.height, nuPar_.kSoot, nuPar_.kAir, out);
+    hipLaunchKernelGGL((dev::nuMeans<R>), dim3(unsigned(nuLocs_)), dim3(unsigned(nuThreads_)), 0,
+                       stream_, (const double*)T_.as<double>(), (const R*)ux_.as<R>(),
+                       (const R*)uy_.as<R>(), nuPar_.wavyStart, nuPar_.wavyPeriod, nY_, XF_,
+                       out + 5 * size_t(n));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream_));
+    if (nu) HIP_TRY(hipMemcpy(nu, out, 5 * size_t(n) * sizeof(double), hipMemcpyDeviceToHost));
+    if (mean)
+      HIP_TRY(hipMemcpy(mean, out + 5 * size_t(n), 2 * size_t(nuLocs_) * sizeof(double),
+                        hipMemcpyDeviceToHost));
+    if (nodes)
+      HIP_TRY(hipMemcpy(nodes, nuRows_.as<void>(), 2 * size_t(n) * sizeof(int32_t),
+                        hipMemcpyDeviceToHost));
+    if (vec)
+      HIP_TRY(hipMemcpy(vec, nuVec_.as<void>(), 4 * size_t(n) * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  }
+
   int komegaTimeStep(int action, int& stepsPerLB, double& dt, std::string& err) override {
     if (!ko_) return fail(err, "k-omega is off for this context");
     switch (action) {
@@ -1068,6 +1171,14 @@ class EngineT final : public Engine {
   double tRel_ = 1e-4, tAbs_ = 1e-7, tDt_ = 1.;
   int tMaxIt_ = 1000, tSteps_ = 1, tQuiet_ = 0, tIters_ = 1, tNan_ = 0;
   bool tOn_ = false, tInjectNan_ = false;
+  // Nusselt number: the wall points now (the built ones until a wall
+  // update), the links that reach each (column, wall) slot, the nearest
+  // nodes and the output
+  std::vector<double> wallC_;
+  Buffer nuC_, nuBL_, nuStart_, nuLinks_, nuRows_, nuVec_, nuOut_;
+  NusseltParams nuPar_{};
+  int nuN_ = 0, nuXMin_ = 0, nuXStop_ = 0, nuLocs_ = 0, nuThreads_ = 0;
+  bool nuOn_ = false;
   static constexpr int kDtQuiet = 20;              // kOmega.h:22 timeBtwIncreaseTimeStep
   static constexpr int kMaxStepsPerLB = 1 << 16;   // halvings give up beyond this
   dev::BicgState* hostStatus_ = nullptr;  // pinned copy of the solver state
@@ -2118,6 +2229,46 @@ class EngineT final : public Engine {
     }
     V.prev[0] = V.prev[1] = nullptr;
     return V;
+  }
+
+  // The links whose search box reaches each (column, wall) slot, in link
+  // order (nuFindNearestNode, tfdKernels.cl:759-807: the box is the link's
+  // x range truncated / rounded up, widened by one and clamped to the
+  // particle domain), and the wall points now.
+  int nusseltLinks(std::string& err) {
+    const int n = nuN_, half = int(geo_.BL.size() / 2) / 2;
+    const int perSide = geo_.bounds[1] - 1;  // MAX_BL_BOT - MIN_BL_BOT
+    const int nl = std::min((perSide + 255) / 256 * 256, n);
+    std::vector<std::vector<int32_t>> slot(2 * size_t(n));
+    for (int w = 0; w < 2; w++) {
+      const int base = w ? geo_.bounds[2] : geo_.bounds[0];
+      for (int i = 0; i < nl; i++) {
+        const int l = base + i;
+        if (l >= 2 * half) break;
+        const double xa = wallC_[2 * size_t(geo_.BL[2 * size_t(l)])];
+        const double xb = wallC_[2 * size_t(geo_.BL[2 * size_t(l) + 1])];
+        const int x0 = std::max(int(std::min(xa, xb)) - 1, nuXMin_);
+        const int x1 = std::min(int(std::ceil(std::max(xa, xb))) + 1, nuXStop_ - 1);
+        if (x0 >= nuXStop_ || x1 < nuXMin_) continue;
+        for (int x = x0; x <= x1; x++) slot[size_t(x - nuXMin_ + w * n)].push_back(int32_t(l));
+      }
+    }
+    std::vector<int32_t> start(2 * size_t(n) + 1, 0), links;
+    for (size_t k = 0; k < slot.size(); k++) {
+      start[k] = int32_t(links.size());
+      links.insert(links.end(), slot[k].begin(), slot[k].end());
+    }
+    start.back() = int32_t(links.size());
+    HIP_TRY(nuStart_.alloc(start.size() * sizeof(int32_t)));
+    HIP_TRY(nuLinks_.alloc(std::max<size_t>(links.size(), 1) * sizeof(int32_t)));
+    HIP_TRY(hipMemcpy(nuStart_.as<void>(), start.data(), start.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice));
+    if (!links.empty())
+      HIP_TRY(hipMemcpy(nuLinks_.as<void>(), links.data(), links.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(nuC_.as<void>(), wallC_.data(), wallC_.size() * sizeof(double),
+                      hipMemcpyHostToDevice));
+    return 0;
   }
 
   // Polls the temperature solve until it is done (adding chunks of four

@@ -39,6 +39,16 @@ struct ThermalParams {
   int maxIters = 1000;
 };
 
+// The Nusselt-number output (clVariablesFD::iniNuCoeffs): the rows the bulk
+// temperature sums, the wavy section whose period boundaries get column sums,
+// and the conductivities of soot and air in lattice units.  The columns are
+// the particle domain of the built geometry, floor(X Min Val) to
+// floor(X Stop Pos).
+struct NusseltParams {
+  int height = 0, wavyStart = -1, wavyPeriod = -1, wavyPeriods = -1;
+  double kSoot = 0., kAir = 0.;
+};
+
 // J = mean over the last `window` steps of a run of
 //   pbar(xIn) - pbar(xOut) + shearWeight * sum over shear links of tau_BL * |link|
 // with pbar the mean of rho / 3 over the fluid nodes of a column.
@@ -87,6 +97,14 @@ class Engine {
   virtual int thermalTimeStep(int action, int& stepsPerLB, double& dt, std::string& err) = 0;
   virtual int setTemperature(const double* T, std::string& err) = 0;
   virtual bool thermalOn() const = 0;
+  // clVariablesFD::iniNuCoeffs and ::updateTimeData: per column 5 values
+  // (bulk temperature, then gradient and wall temperature of the bottom and
+  // top walls), per period boundary 2 (sum |u| T, sum |u|); optionally the
+  // nearest node of each (column, wall) slot, (row << 2) | type, and its
+  // vector from the wall.
+  virtual int nusseltInit(const NusseltParams& q, std::string& err) = 0;
+  virtual int nusseltSizes(int& columns, int& locations, std::string& err) const = 0;
+  virtual int nusselt(double* nu, double* mean, int32_t* nodes, double* vec, std::string& err) = 0;
   virtual int64_t fieldSize(int field) const = 0;
   virtual int setLaunch(int block, int npt, int spp, int64_t pad, std::string& err) = 0;
   virtual void setGraphs(bool on) = 0;

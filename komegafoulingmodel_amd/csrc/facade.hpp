@@ -36,11 +36,14 @@ class clProblem {
   unsigned tfdSteps = 1;  // "LB Steps Per T" (clProblem.cpp:506); dTtfd = tfdSteps
   double dTtfd = 1.;
   unsigned saveAvgStartTime = 0, saveTimeStepAvg = 1000, dumpStartTime = 0, dumpTimeStep = 1000;
+  unsigned saveNuStartTime = 0, saveTimeStepNu = 1000;
+  // the wavy section (clProblem.cpp:471-473), -1 when not given
+  int xWavyStart = -1, wavyPeriodLen = -1, numWavyPeriods = -1;
   unsigned saveBinStep = 1;  // dumps per restart-file save ("Save Bin Step", clProblem.cpp:515)
   // The cadence state a restart resumes from (clProblem.cpp:540-550, written
   // by saveParameters :385-403): the next time-data row and dump, the dumps
   // so far and the dump count at which the restart files are next written.
-  unsigned nextSaveAvgTime = 0, nextDumpStepTime = 0;
+  unsigned nextSaveAvgTime = 0, nextSaveNuTime = 0, nextDumpStepTime = 0;
   int currentDumpStep = 0, nextDumpBin = 1;
   RunParams params;
 
@@ -62,11 +65,18 @@ class clProblem {
     if (tfd < 1) throw std::runtime_error("LB Steps Per T must be >= 1");
     tfdSteps = unsigned(tfd);
     dTtfd = double(tfdSteps);
+    xWavyStart = params.getInt(d, "Wavy Start Loc", -1);
+    wavyPeriodLen = params.getInt(d, "Period Length", -1);
+    numWavyPeriods = params.getInt(d, "Num Periods", -1);
     cadence("Save Avgs", 2000, saveAvgStartTime, saveTimeStepAvg);
+    cadence("Save Nu", 2000, saveNuStartTime, saveTimeStepNu);
     cadence("Dump", 1000, dumpStartTime, dumpTimeStep);
     const int bin = params.getInt(d, "Save Bin Step", 1);
     saveBinStep = unsigned(bin > 0 ? bin : 1);
-    nextSaveAvgTime = unsigned(params.getInt(d, "nextSaveAvgTime", int(saveAvgStartTime + saveTimeStepAvg)));
+    nextSaveAvgTime =
+        unsigned(params.getInt(d, "nextSaveAvgTime", int(std::max(saveAvgStartTime, saveTimeStepAvg))));
+    nextSaveNuTime =
+        unsigned(params.getInt(d, "nextSaveNuTime", int(std::max(saveNuStartTime, saveTimeStepNu))));
     nextDumpStepTime = unsigned(params.getInt(d, "nextDumpStepTime", int(dumpStartTime + dumpTimeStep)));
     currentDumpStep = params.getInt(d, "currentDumpStep", 0);
     nextDumpBin = params.getInt(d, "nextDumpBin", int(saveBinStep));
@@ -79,6 +89,7 @@ class clProblem {
     params.setInt(d, "Current Time", int(Time));
     params.setBool(d, "Restart Run", true);
     params.setInt(d, "nextSaveAvgTime", int(nextSaveAvgTime));
+    params.setInt(d, "nextSaveNuTime", int(nextSaveNuTime));
     params.setInt(d, "nextDumpStepTime", int(nextDumpStepTime));
     params.setInt(d, "currentDumpStep", currentDumpStep);
     params.setInt(d, "nextDumpBin", nextDumpBin);
@@ -145,6 +156,45 @@ class Reducer {
   int field_ = 0, op_ = KOFM_RED_SUM;
   int maxLines_ = 128;
   std::string name_, dir_ = ".";
+  std::vector<std::vector<double>> rows_;
+};
+
+// TimeData<double> (Array.h:2702-2904) for rows of several values: the
+// file "<name>.txt" is emptied at the start of a run that is not a restart
+// (iniFile), rows are held and written out "time\tv0\tv1\t...\t" when
+// maxLines are held (setTimeAndIncrement) or on appendData.
+class TimeRows {
+ public:
+  void ini(const std::string& name, const std::string& dir, int maxLines, bool restart) {
+    path_ = dir + "/" + name + ".txt";
+    maxLines_ = maxLines > 0 ? maxLines : 1;
+    rows_.clear();
+    if (!restart) {
+      std::ofstream out(path_, std::ios::trunc);
+      if (!out) throw std::runtime_error("cannot create " + path_);
+    }
+  }
+  void add(unsigned time, std::vector<double> row) {
+    times_.push_back(time);
+    rows_.push_back(std::move(row));
+    if (int(rows_.size()) >= maxLines_) appendData();
+  }
+  void appendData() {
+    if (rows_.empty()) return;
+    std::ofstream out(path_, std::ios::app);
+    for (size_t r = 0; r < rows_.size(); r++) {
+      out << times_[r] << "\t";
+      for (double v : rows_[r]) out << v << "\t";
+      out << "\n";
+    }
+    rows_.clear();
+    times_.clear();
+  }
+
+ private:
+  std::string path_;
+  int maxLines_ = 10;
+  std::vector<unsigned> times_;
   std::vector<std::vector<double>> rows_;
 };
 
@@ -262,6 +312,7 @@ class clVariablesLB {
  public:
   double tau = 1.99, MuVal = 0., Re = 10800., Fval = 0., RhoVal = 1.;
   double nuAir = 4.1e-5;  // "Nu air": DELTA_L^2 / DELTA_T = MuVal / nuAir (clVariablesLB.cpp:608)
+  double rhoAir = 0.675;  // "Rho air" (DENSITY_AIR, ConstDef.h:60): DELTA_M (clVariablesLB.cpp:609)
   bool tauDefined = true, reDefined = true;
   int tlbmIniDumpStep = 100000, tlbmNumIniSteps_LB = 1000000;
   // ConstDef.h:159-192 defaults
@@ -312,6 +363,7 @@ class clVariablesLB {
       throw std::runtime_error("\"Run LBFD First\" (the combined LB and thermal warm-up) is "
                                "not part of this build");
     nuAir = rp.getDouble(d, "Nu air", 4.1e-5);  // VISC_AIR, ConstDef.h:59
+    rhoAir = rp.getDouble(d, "Rho air", 0.675);
     double mu = 0., t = 0.;
     switch (rp.either(d, "Mu", "Tau", mu, t)) {
       case 1: MuVal = mu; tau = 1. / (MuVal * 3. + 0.5); tauDefined = false; break;
@@ -523,7 +575,11 @@ class clVariablesFD {
   int maxOutputLinesNu = 10;
   double Alpha_fluid = 0., Alpha_foul = 0.;
   bool tempLoaded = false;
+  double kSootLattice = 0., kAirLattice = 0.;  // KSOOT, KAIR
+  TimeRows Nu, NuMean;
   kofm_ctx* ctx = nullptr;
+
+  bool nuOn() const { return thermalSolverFlag && calcNuFlag; }
 
   static bool knownKey(const std::string& k) {
     static const char* const keys[] = {
@@ -539,8 +595,9 @@ class clVariablesFD {
 
   // clVariablesFD::loadParams (clVariablesFD.cpp:445-497).  "Pr Number" sets
   // both Pr and the turbulent Pr (the reference reads PrTurbNum from the same
-  // key, :456), which is kept.  "Calculate Nu" is read, but the Nusselt
-  // number is not built; "Solve Steady Temp" is read and, as in the reference
+  // key, :456), which is kept.  "Calculate Nu" (default on) adds the
+  // Nusselt record, which needs the wavy section of the system document, as
+  // in the reference (iniNuCoeffs, :364-366); "Solve Steady Temp" is read and, as in the reference
   // (whose only call of solveSSThermal is commented out,
   // clVariablesLB.cpp:237), changes nothing; "Use CHT Correction" only adds an
   // argument that the active kernel does not use.
@@ -580,6 +637,15 @@ class clVariablesFD {
     if (thermalSolverFlag && !vlb.kOmegaClass.kOmegaSolverFlag)
       throw std::runtime_error("the temperature solve needs k-omega (\"Use Turb Model\"): its rows "
                                "take the eddy viscosity");
+    // KSOOT, KAIR = k * DELTA_F / DELTA_T (clVariablesTR.cpp:1912-1913) with
+    // DELTA_L = 2 R / PIPE_DIAMETER_REAL (clProblem.cpp:484, ConstDef.h:58)
+    // and DELTA_T, DELTA_M, DELTA_F of clVariablesLB.cpp:608-610
+    const double dL = (2. * p.Pipe_radius) / 2.e-3;
+    const double dT = vlb.nuAir * dL * dL / vlb.MuVal;
+    const double dM = vlb.RhoVal * dL * dL * dL / vlb.rhoAir;
+    const double dF = dM * dL / dT / dT;
+    kSootLattice = kSoot * dF / dT;
+    kAirLattice = kAir * dF / dT;
   }
 
   // clVariablesFD::ini (clVariablesFD.cpp:230-283) with testRestartRun
@@ -602,6 +668,42 @@ class clVariablesFD {
     std::vector<double> T;
     tempLoaded = clVariablesLB::readBin(loadDir + "/temp.bin", D, T);
     if (tempLoaded) check(ctx, kofm_set_temperature(ctx, T.data()), "load temp.bin");
+  }
+
+  // clVariablesFD::iniNuCoeffs (clVariablesFD.cpp:362-442), called after
+  // ini as LBLS.cpp:93-96 calls it; the TimeData files Nu.txt and
+  // NuMean.txt start empty unless the run is a restart
+  void iniNuCoeffs(const clProblem& p, bool restart, const std::string& outDir) {
+    if (!nuOn()) return;
+    kofm_nusselt_params q{};
+    q.height = p.Channel_Height;
+    q.wavy_start = p.xWavyStart;
+    q.wavy_period = p.wavyPeriodLen;
+    q.wavy_periods = p.numWavyPeriods;
+    q.k_soot = kSootLattice;
+    q.k_air = kAirLattice;
+    check(ctx, kofm_nusselt_init(ctx, &q), "clVariablesFD::iniNuCoeffs");
+    Nu.ini("Nu", outDir, maxOutputLinesNu, restart);
+    NuMean.ini("NuMean", outDir, maxOutputLinesNu, restart);
+  }
+
+  // clVariablesFD::updateTimeData (clVariablesFD.cpp:844-869): one row of
+  // each record at this time
+  void updateTimeData(unsigned time) {
+    if (!nuOn()) return;
+    int32_t n = 0, locs = 0;
+    check(ctx, kofm_nusselt_sizes(ctx, &n, &locs), "Nu sizes");
+    std::vector<double> nu(5 * size_t(n)), mean(2 * size_t(locs));
+    check(ctx, kofm_nusselt(ctx, nu.data(), mean.data(), nullptr, nullptr), "FD_calc_Nu");
+    Nu.add(time, std::move(nu));
+    NuMean.add(time, std::move(mean));
+  }
+
+  // clVariablesFD::saveTimeData (clVariablesFD.cpp:562-570)
+  void saveTimeData() {
+    if (!nuOn()) return;
+    Nu.appendData();
+    NuMean.appendData();
   }
 
   // clVariablesFD::Solve (clVariablesFD.h:343-358)

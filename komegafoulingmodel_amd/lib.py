@@ -56,6 +56,7 @@ EXPORTS = [
     "kofm_reduce_device", "kofm_komega_time_step", "kofm_ini_poiseuille",
     "kofm_save_restart", "kofm_save_debug", "kofm_update_walls",
     "kofm_thermal_init", "kofm_thermal_step", "kofm_thermal_time_step", "kofm_set_temperature",
+    "kofm_nusselt_init", "kofm_nusselt_sizes", "kofm_nusselt",
 ]
 
 
@@ -127,6 +128,11 @@ class ThermalParams(C.Structure):
                 ("rel_tol", C.c_double), ("abs_tol", C.c_double), ("max_iters", C.c_int32)]
 
 
+class NusseltParams(C.Structure):
+    _fields_ = [("height", C.c_int32), ("wavy_start", C.c_int32), ("wavy_period", C.c_int32),
+                ("wavy_periods", C.c_int32), ("k_soot", C.c_double), ("k_air", C.c_double)]
+
+
 _lib = None
 
 
@@ -164,6 +170,9 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "kofm_thermal_step": (C.c_int, [P, C.c_int32]),
         "kofm_thermal_time_step": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int32), dp]),
         "kofm_set_temperature": (C.c_int, [P, dp]),
+        "kofm_nusselt_init": (C.c_int, [P, C.POINTER(NusseltParams)]),
+        "kofm_nusselt_sizes": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "kofm_nusselt": (C.c_int, [P, dp, dp, C.c_void_p, dp]),
         "kofm_ini_poiseuille": (C.c_int, [P, C.POINTER(IniFlow), dp, dp, dp]),
         "kofm_reduce": (C.c_int, [P, C.c_int32, C.c_int32, dp]),
         "kofm_reduce_device": (C.c_int, [P, C.c_int32, C.c_int64, C.c_int32, P, P]),
@@ -376,6 +385,33 @@ class Lattice:
         T = np.ascontiguousarray(T, dtype=np.float64)
         assert T.size == self.D
         self._check(self.lib.kofm_set_temperature(self.h, _dptr(T)))
+
+    def nusselt_init(self, *, height: int, wavy_start: int, wavy_period: int, wavy_periods: int,
+                     k_soot: float, k_air: float):
+        """kofm_nusselt_init (clVariablesFD::iniNuCoeffs) over the particle
+        domain of the built geometry; needs the temperature solve."""
+        q = NusseltParams(height, wavy_start, wavy_period, wavy_periods, k_soot, k_air)
+        self._check(self.lib.kofm_nusselt_init(self.h, C.byref(q)))
+
+    def nusselt_sizes(self):
+        """(columns, period boundaries) of the Nusselt record."""
+        c, l = C.c_int32(), C.c_int32()
+        self._check(self.lib.kofm_nusselt_sizes(self.h, C.byref(c), C.byref(l)))
+        return c.value, l.value
+
+    def nusselt(self, nodes: bool = False) -> dict:
+        """One Nusselt record: nu [5 columns], mean [2 locations] and, with
+        nodes, the nearest-node slots (nodes [2 columns], vec [4 columns])."""
+        n, locs = self.nusselt_sizes()
+        out = dict(nu=np.zeros(5 * n), mean=np.zeros(2 * locs))
+        if nodes:
+            out["nodes"] = np.zeros(2 * n, np.int32)
+            out["vec"] = np.zeros(4 * n)
+        self._check(self.lib.kofm_nusselt(
+            self.h, _dptr(out["nu"]), _dptr(out["mean"]),
+            out["nodes"].ctypes.data_as(C.c_void_p) if nodes else None,
+            _dptr(out["vec"]) if nodes else None))
+        return out
 
     def reduce(self, field: str, op: int = RED_SUM) -> float:
         out = C.c_double()

@@ -1414,3 +1414,186 @@ int orc_t_step(const orc_par *p, const int16_t *nType, const double *dX0, double
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * Nusselt number (clVariablesFD::iniNuCoeffs / updateTimeData,
+ * clVariablesFD.cpp:362-442, 834-869; tfdKernels.cl:737-1033).
+ * ---------------------------------------------------------------------- */
+
+#define NU_EPSILON 1.11e-16 /* Heading.cl:21 */
+#define NU_NONE (-1000.)
+#define WORKGROUPSIZE_NU 256 /* ConstDef.h:87 */
+
+/* nuFindIntersectionNormal (tfdKernels.cl:737-756). */
+static int nu_foot(double Lx, double Ly, double P0x, double P0y, double P1x, double P1y,
+                   double *dist, double *Cx, double *Cy) {
+  double tx = P1x - P0x, ty = P1y - P0y;
+  double nmx = -ty, nmy = tx;
+  double nl = sqrt(nmx * nmx + nmy * nmy);
+  double nx = nmx / nl, ny = nmy / nl;
+  *dist = (Lx - P0x) * nx + (Ly - P0y) * ny;
+  *Cx = Lx - nx * *dist;
+  *Cy = Ly - ny * *dist;
+  double d0x = *Cx - P0x, d0y = *Cy - P0y, d1x = *Cx - P1x, d1y = *Cy - P1y;
+  double l10 = sqrt(tx * tx + ty * ty);
+  double l0 = sqrt(d0x * d0x + d0y * d0y);
+  double l1 = sqrt(d1x * d1x + d1y * d1y);
+  return fabs(l10 - l0 - l1) < NU_EPSILON;
+}
+
+/* nuFindNearestNode (tfdKernels.cl:759-807, clVariablesFD.cpp:498-536) for
+ * link l: the non-solid nodes of its box closer along its normal than the
+ * slot's best so far.  Rows 0 and nY - 1 are left out (their getTandDT
+ * neighbour would lie outside the lattice). */
+static void nu_nearest(int nY, int XF, const int16_t *nType, const double *C, const int32_t *BL,
+                       int l, int xStart, int xStop, int shift, double *best, double *vec,
+                       int32_t *rows) {
+  double P0x = C[2 * BL[2 * l]], P0y = C[2 * BL[2 * l] + 1];
+  double P1x = C[2 * BL[2 * l + 1]], P1y = C[2 * BL[2 * l + 1] + 1];
+  int x0 = (int)fmin(P0x, P1x) - 1, y0 = (int)fmin(P0y, P1y) - 1;
+  int x1 = (int)ceil(fmax(P0x, P1x)) + 1, y1 = (int)ceil(fmax(P0y, P1y)) + 1;
+  if (x0 < xStart) x0 = xStart;
+  if (y0 < 1) y0 = 1;
+  if (x1 > xStop - 1) x1 = xStop - 1;
+  if (y1 > nY - 2) y1 = nY - 2;
+  if (x0 >= xStop || x1 < xStart) return;
+  for (int ii = x0; ii <= x1; ii++)
+    for (int jj = y0; jj <= y1; jj++) {
+      if (nType[ii + (long)XF * jj] & M_SOLID_NODE) continue;
+      double d, cx, cy;
+      if (nu_foot(ii, jj, P0x, P0y, P1x, P1y, &d, &cx, &cy) && d < best[ii - shift]) {
+        best[ii - shift] = d;
+        vec[2 * (ii - shift)] = ii - cx;
+        vec[2 * (ii - shift) + 1] = jj - cy;
+        rows[ii - shift] = jj;
+      }
+    }
+}
+
+/* getTandDT (tfdKernels.cl:878-917). */
+static double nu_wall(const double *T, const double *dX, const double *dX0, long lin, int type,
+                      double vx, double vy, int XF, long D, double kSoot, double kAir, double *dT) {
+  double Tn = T[lin];
+  long nb = (type & 1) ? 1 : -1;
+  long dir = ((type & 1) ? 0 : D) + lin;
+  double dx = dX[dir], dx0 = dX0[dir];
+  double gx = kSoot * (Tn - T[lin + nb]) / (kAir * (dx0 - dx) + kSoot * dx);
+  nb = (type & 2) ? XF : -XF;
+  dir = ((type & 2) ? 2 * D : 3 * D) + lin;
+  dx = dX[dir];
+  dx0 = dX0[dir];
+  double gy = kSoot * (Tn - T[lin + nb]) / (kAir * (dx0 - dx) + kSoot * dx);
+  double ax = fabs(vx), ay = fabs(vy), al = sqrt(ax * ax + ay * ay);
+  *dT = gx * (ax / al) + gy * (ay / al);
+  return Tn - *dT * sqrt(vx * vx + vy * vy);
+}
+
+/* The Nusselt record.  MIN_BL_BOT / MAX_BL_BOT from the walls the geometry
+ * was built from, C0 (clVariablesTR.cpp:497-530); the search over the walls
+ * now, C, and the node types now; updateNuCoeff2's type bits
+ * (tfdKernels.cl:832-856); FD_calc_Nu (:919-982) with getTbm (:858-876,
+ * rows x + j * nY, j < height); getNuMean's OpenCL 1.2 form (:984-1033) at
+ * wavyPeriods + 1 period boundaries, its halving tree over
+ * roundup(nY / 2, 32) partial sums.  Outputs as kofm_nusselt.  Returns -1
+ * when no link pair brackets the domain. */
+int orc_nusselt(int nX, int XF, int nY, const int16_t *nType, const double *C0, const double *C,
+                const int32_t *BL, int nBL, double xMinVal, double xStopPos, int height,
+                int wavyStart, int wavyPeriod, int wavyPeriods, double kSoot, double kAir,
+                const double *T, const double *ux, const double *uy, const double *dX,
+                const double *dX0, double *nu, double *mean, int32_t *nodes, double *vec) {
+  (void)nX;
+  long D = (long)XF * nY;
+  int xStart = (int)floor(xMinVal), xStop = (int)floor(xStopPos);
+  int n = xStop - xStart, half = nBL / 2;
+  int lo = -1, hi = -1;
+  for (int i = 0; i < half; i++) {
+    double mid = (int)((C0[2 * BL[2 * i]] + C0[2 * BL[2 * i + 1]]) / 2.);
+    if (mid >= xStart - 1) {
+      lo = i;
+      break;
+    }
+  }
+  for (int i = lo + 1; lo >= 0 && i < half; i++) {
+    int mid = (int)((C0[2 * BL[2 * i]] + C0[2 * BL[2 * i + 1]]) / 2.);
+    if (mid > xStop + 1) {
+      hi = i;
+      break;
+    }
+  }
+  if (lo < 0 || hi < 0 || n < 1) return -1;
+  int perSide = hi - lo;
+  int nl = (perSide + WORKGROUPSIZE_NU - 1) / WORKGROUPSIZE_NU * WORKGROUPSIZE_NU;
+  if (nl > n) nl = n;
+  double *best = (double *)malloc(sizeof(double) * 2 * n);
+  for (int s = 0; s < 2 * n; s++) {
+    best[s] = 100.;
+    nodes[s] = -1;
+    vec[2 * s] = vec[2 * s + 1] = 0.;
+  }
+  for (int i = 0; i < nl; i++) {
+    if (lo + i < nBL)
+      nu_nearest(nY, XF, nType, C, BL, lo + i, xStart, xStop, xStart, best, vec, nodes);
+    if (half + lo + i < nBL)
+      nu_nearest(nY, XF, nType, C, BL, half + lo + i, xStart, xStop, xStart - n, best, vec, nodes);
+  }
+  free(best);
+  for (int s = 0; s < 2 * n; s++) {
+    int type = 0;
+    if (nodes[s] != -1) {
+      if (vec[2 * s] < 0.) type |= 1;
+      if (vec[2 * s + 1] < 0.) type |= 2;
+    }
+    nodes[s] = (int32_t)((uint32_t)nodes[s] << 2) | type;
+  }
+  for (int ix = 0; ix < n; ix++) {
+    int ii = ix + xStart;
+    double Um = 0., Tm = 0.;
+    for (int j = 0; j < height; j++) {
+      long l = ii + (long)j * nY;
+      double U = sqrt(ux[l] * ux[l] + uy[l] * uy[l]);
+      Um += U;
+      Tm += U * T[l];
+    }
+    nu[5 * ix] = Tm / Um;
+    for (int w = 0; w < 2; w++) {
+      int s = ix + w * n;
+      int jj = nodes[s] >> 2;
+      double dT = 0., Ts = 0.;
+      if (jj != -1)
+        Ts = nu_wall(T, dX, dX0, ii + (long)XF * jj, nodes[s] & 3, vec[2 * s], vec[2 * s + 1], XF,
+                     D, kSoot, kAir, &dT);
+      nu[5 * ix + 1 + 2 * w] = jj > -1 ? dT : NU_NONE;
+      nu[5 * ix + 2 + 2 * w] = jj > -1 ? Ts : NU_NONE;
+    }
+  }
+  int L = (nY / 2 + 31) / 32 * 32;
+  double *ut = (double *)malloc(sizeof(double) * L), *us = (double *)malloc(sizeof(double) * L);
+  for (int g = 0; g <= wavyPeriods; g++) {
+    int xloc = wavyStart + g * wavyPeriod;
+    for (int t = 0; t < L; t++) {
+      double U1 = 0., U2 = 0., T1 = 0., T2 = 0.;
+      if (2 * t < nY) {
+        long l = xloc + (long)(2 * t) * XF;
+        U1 = sqrt(ux[l] * ux[l] + uy[l] * uy[l]);
+        T1 = T[l];
+      }
+      if (2 * t + 1 < nY) {
+        long l = xloc + (long)(2 * t + 1) * XF;
+        U2 = sqrt(ux[l] * ux[l] + uy[l] * uy[l]);
+        T2 = T[l];
+      }
+      ut[t] = U1 * T1 + U2 * T2;
+      us[t] = U1 + U2;
+    }
+    for (int s = L >> 1; s > 0; s >>= 1)
+      for (int t = 0; t < s; t++) {
+        ut[t] += ut[t + s];
+        us[t] += us[t + s];
+      }
+    mean[2 * g] = ut[0];
+    mean[2 * g + 1] = us[0];
+  }
+  free(ut);
+  free(us);
+  return 0;
+}

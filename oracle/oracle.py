@@ -117,6 +117,35 @@ def geometry(nX, nY, C0, BL, *, eps=1e-5, min_dx=0.001, x_release_pos=10.0, x_st
     return g
 
 
+def nusselt(nX, nY, nType, C0, C_now, BL, *, x_min_val, x_stop_pos, height, wavy_start, wavy_period,
+            wavy_periods, k_soot, k_air, T, ux, uy, dX, dX0):
+    """orc_nusselt: one Nusselt record from the given fields (FP64); C0 are
+    the walls the geometry was built from, C_now the walls now.
+
+    Returns dict(nu [5 n], mean [2 (wavy_periods + 1)], nodes [2 n], vec [4 n]).
+    """
+    lib = load()
+    XF = ((nX + 127) // 128) * 128
+    n = int(np.floor(x_stop_pos)) - int(np.floor(x_min_val))
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    T, ux, uy, dX, dX0 = f64(T), f64(ux), f64(uy), f64(dX), f64(dX0)
+    C0, C_now = f64(C0).ravel(), f64(C_now).ravel()
+    BL = np.ascontiguousarray(BL, dtype=np.int32).ravel()
+    nType = np.ascontiguousarray(nType, dtype=np.int16)
+    out = dict(nu=np.zeros(5 * n), mean=np.zeros(2 * (wavy_periods + 1)),
+               nodes=np.zeros(2 * n, np.int32), vec=np.zeros(4 * n))
+    lib.orc_nusselt.argtypes = ([C.c_int] * 3 + [C.c_void_p] * 4 + [C.c_int, C.c_double,
+                                C.c_double] + [C.c_int] * 4 + [C.c_double] * 2
+                                + [C.c_void_p] * 9)
+    rc = lib.orc_nusselt(nX, XF, nY, p(nType), p(C0), p(C_now), p(BL), len(BL) // 2,
+                         float(x_min_val), float(x_stop_pos), height, wavy_start, wavy_period,
+                         wavy_periods, float(k_soot), float(k_air), p(T), p(ux), p(uy), p(dX),
+                         p(dX0), p(out["nu"]), p(out["mean"]), p(out["nodes"]), p(out["vec"]))
+    if rc != 0:
+        raise ValueError("orc_nusselt: no boundary links bracket the column range")
+    return out
+
+
 def feq(rho, ux, uy):
     lib = load()
     out = np.zeros(9)

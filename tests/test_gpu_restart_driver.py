@@ -62,6 +62,10 @@ def test_save2file_writes_the_reference_layout(tmp_path):
 
 
 def test_driver_runs_the_reference_sequence(tmp_path):
+    """The laminar sequence.  A thermal document holding only "Dummy", as the
+    reference's RunParam.yaml has, leaves "Thermal Solver" at its default, on
+    (ConstDef.h:222), and the temperature rows read nu_t (tfdKernels.cl:
+    230-233), so with k-omega off the solve is switched off explicitly."""
     w = _mini()
     load, out = tmp_path / "load", tmp_path / "out"
     load.mkdir()
@@ -93,6 +97,7 @@ load Vel Text: False
 --- # Thermal Parameters
 ParamsName: 3
 Dummy: !!int 0
+Thermal Solver: False
 
 --- # Particle Parameters
 ParamsName: 6
@@ -290,20 +295,36 @@ def test_driver_runs_the_temperature_solve_and_restarts_it(tmp_path):
     the first restarted step sees none (on this slow channel the limiter
     does not act and the runs agree bit for bit).  The check is that the
     restarted temperature stays within 1 % of the change the first 12 steps
-    made; restarted from the initial field it would be far off."""
+    made; restarted from the initial field it would be far off.
+
+    "Calculate Nu" is on by default, with the wavy section given: Nu.txt and
+    NuMean.txt get one row every "Save Nu Timestep" steps (time, then 5
+    values per column of the particle domain, 2 per period boundary), held
+    "Max Nu Output Lines" at a time; the restarted run resumes the cadence
+    from RunParam.yaml's nextSaveNuTime."""
     w = _mini()
     fl = ("Use Turb Model: True\nKappa Max Rel Tol: !!float 1e-12\n"
           "Kappa Max Abs Tol: !!float 0.0\nKappa Max Iterations: !!int 400\n")
     th = ("--- # Thermal Parameters\nParamsName: 3\nThermal Solver: True\n"
           "Tinitial: !!float 0.5\nTinlet Sim: !!float 2.0\nTemp Max Rel Tol: !!float 1e-12\n"
-          "Temp Max Abs Tol: !!float 0.0\n")
-    sys_extra = "Dump Timestep: !!int 6\nSave Avgs Timestep: !!int 6\nSave Bin Step: !!int 2\n"
+          "Temp Max Abs Tol: !!float 0.0\nMax Nu Output Lines: !!int 2\n")
+    sys_extra = ("Dump Timestep: !!int 6\nSave Avgs Timestep: !!int 6\nSave Bin Step: !!int 2\n"
+                 "Wavy Start Loc: !!int 0\nPeriod Length: !!int 100\nNum Periods: !!int 2\n"
+                 "Save Nu Timestep: !!int 4\n")
     la, oa = tmp_path / "la", tmp_path / "oa"
     la.mkdir(), oa.mkdir()
     channel.write_load_files(str(la), w.C0, w.BL)
     (la / "RunParam.yaml").write_text(_driver_yaml(w, 24, sys_extra, fl, th))
     res = _run_driver(la, oa)
-    assert "temperature on" in res.stdout
+    assert "temperature on Nu on" in res.stdout
+    n = w.nX - 12 - 8  # columns floor(X Min Val) = 8 to floor(X Stop Pos) - 1
+    nu = np.loadtxt(oa / "Nu.txt", ndmin=2)
+    mean = np.loadtxt(oa / "NuMean.txt", ndmin=2)
+    assert nu.shape == (6, 1 + 5 * n) and mean.shape == (6, 1 + 2 * 3)
+    assert np.array_equal(nu[:, 0], [4, 8, 12, 16, 20, 24])
+    assert np.array_equal(mean[:, 0], nu[:, 0])
+    assert np.all(np.isfinite(nu[:, 1::5])) and np.all(nu[:, 1::5] > 0.0)
+    assert np.all(mean[:, 2::2] > 0.0)
     T = np.loadtxt(oa / "temp.txt")
     assert T.shape == (w.nX, w.nY)
     g = oracle.geometry(w.nX, w.nY, w.C0, w.BL, **w.geom_params())
@@ -323,6 +344,7 @@ def test_driver_runs_the_temperature_solve_and_restarts_it(tmp_path):
     ob2 = tmp_path / "ob2"
     ob2.mkdir()
     _run_driver(lb, ob2)
+    assert np.array_equal(np.loadtxt(ob2 / "Nu.txt", ndmin=2)[:, 0], [16, 20, 24])
     T2 = np.loadtxt(ob2 / "temp.txt")
     gap = np.max(np.abs(T2[rows] - T[rows])) / np.max(np.abs(T[rows]))
     T0 = np.loadtxt(ob / "temp.txt")

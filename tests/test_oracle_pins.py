@@ -848,3 +848,127 @@ def test_thermal_outlet_column_has_zero_gradient():
     dev = np.max(np.abs(T1[rows, 16:] - expect[:, None])) / t_in
     print(f"columns 16..{nX - 1} against the 1-D step: {dev:.2e}")
     assert dev < 1e-12
+
+
+def _nu_case(layer, x_release=13.5, x_stop=245.0):
+    """A flat channel (walls at rows 7.5 and 63.5 of 72) with a uniform layer
+    on both walls: node types and spacings built from the fouled walls,
+    clean spacings from the clean ones; seeded |u| on the fluid nodes.  The
+    particle domain is columns floor(x_release - 1.25) to x_stop - 1."""
+    w = channel.wavy_channel(256, 72, H=56.0, amp=0.0, period=128.0, nu=0.01, Re=500.0,
+                             Fval=1e-6, seed=7)
+    w.geom = {"x_release_pos": x_release, "x_stop_pos": x_stop}
+    nb = len(w.C0) // 2
+    C = w.C0.copy()
+    C[:nb, 1] += layer
+    C[nb:, 1] -= layer
+    g0 = oracle.geometry(w.nX, w.nY, w.C0, w.BL, **w.geom_params())
+    g = oracle.geometry(w.nX, w.nY, C, w.BL, **w.geom_params())
+    fluid = (g["nType"].astype(np.int32) & 1) == 0
+    rng = np.random.Generator(np.random.PCG64(5))
+    ux = np.where(fluid, rng.uniform(0.01, 0.1, w.D), 0.0)
+    uy = np.where(fluid, rng.uniform(-0.01, 0.01, w.D), 0.0)
+    return w, C, g0, g, ux, uy
+
+
+def _nu(w, C, g0, g, ux, uy, T, kS=0.57, kA=0.42):
+    return oracle.nusselt(w.nX, w.nY, g["nType"], w.C0, C, w.BL,
+                          x_min_val=w.geom["x_release_pos"] - 1.25,
+                          x_stop_pos=w.geom["x_stop_pos"], height=56, wavy_start=0, wavy_period=128,
+                          wavy_periods=1, k_soot=kS, k_air=kA, T=T, ux=ux, uy=uy, dX=g["dX"],
+                          dX0=g0["dX"])
+
+
+@pytest.mark.parametrize("layer", [0.0, 0.3])
+def test_nusselt_record_closed_forms_on_flat_walls(layer):
+    """orc_nusselt (FD_calc_Nu, getTandDT, getTbm, getNuMean) against
+    closed forms: the nearest node is the first fluid row off each wall
+    (found by scanning node types) at distance 0.5 - layer; the gradient is
+    the series conduction (T_node - T_beyond) / (d + s k_air / k_soot) through
+    the air gap d and the soot layer s; the wall temperature is
+    T_node - d * gradient; T = index mod nY makes the bulk temperature of
+    column x equal x mod nY (getTbm's rows x + j nY); the period-boundary sums
+    are plain column sums (36 partial sums in a tree of 64)."""
+    w, C, g0, g, ux, uy = _nu_case(layer)
+    ys, xs = np.mgrid[0:w.nY, 0:w.XF]
+    T = (1.0 + 0.03 * ys + 0.001 * xs + 0.0002 * ys * ys).ravel()
+    r = _nu(w, C, g0, g, ux, uy, T)
+    cols = np.arange(12, 245)
+    solid = ((g["nType"].astype(np.int32) & 1) != 0).reshape(w.nY, w.XF)
+    bot = np.array([next(j for j in range(w.nY) if j > 7.5 + layer and not solid[j, x])
+                    for x in cols])
+    top = np.array([next(j for j in range(w.nY - 1, -1, -1) if j < 63.5 - layer
+                         and not solid[j, x]) for x in cols])
+    n = len(cols)
+    assert np.array_equal(r["nodes"][:n] >> 2, bot) and np.array_equal(r["nodes"][n:] >> 2, top)
+    assert np.array_equal(r["nodes"][:n] & 3, np.zeros(n)) and np.all(r["nodes"][n:] & 3 == 2)
+    T2 = T.reshape(w.nY, w.XF)
+    nu = r["nu"].reshape(-1, 5)
+    d = 0.5 - layer
+    for k, rows, beyond in ((0, bot, bot - 1), (1, top, top + 1)):
+        grad = (T2[rows, cols] - T2[beyond, cols]) / (d + layer * 0.42 / 0.57)
+        assert np.allclose(nu[:, 1 + 2 * k], grad, rtol=1e-11, atol=1e-14)
+        assert np.allclose(nu[:, 2 + 2 * k], T2[rows, cols] - d * grad, rtol=1e-12, atol=1e-14)
+    U = np.sqrt(ux * ux + uy * uy).reshape(w.nY, w.XF)
+    for k, x in enumerate((0, 128)):
+        assert np.isclose(r["mean"][2 * k], np.sum(U[:, x] * T2[:, x]), rtol=1e-13)
+        assert np.isclose(r["mean"][2 * k + 1], np.sum(U[:, x]), rtol=1e-13)
+    r = _nu(w, C, g0, g, ux, uy, (np.arange(w.D) % w.nY).astype(np.float64))
+    assert np.allclose(r["nu"][0::5], cols % w.nY, rtol=1e-14, atol=0)
+    # swapping the conductivities changes the gradient exactly when there is
+    # a layer (the soot one multiplies the air gap)
+    same = np.allclose(_nu(w, C, g0, g, ux, uy, T, kS=0.42, kA=0.57)["nu"][1::5], nu[:, 1],
+                       rtol=1e-12)
+    assert same == (layer == 0.0)
+
+
+def test_nusselt_top_wall_links_follow_the_bottom_index_range():
+    """MIN_BL_TOP = nBL / 2 + MIN_BL_BOT (clVariablesTR.cpp:493-508) takes the
+    top links in the bottom links' x order, but the channels list the top
+    wall right to left (MakeChannel_Test.m:91-106): with points at
+    x = k - 0.75 the top search reaches the feet of columns
+    nX + 1 - TR_X_IND_STOP to nX - TR_X_IND_START (all of the domain when
+    TR_X_IND_STOP = nX + 1 - TR_X_IND_START, as in the other tests here).  With the domain 8 to
+    243 on nX = 256 the first top columns get no node (-1000) and every
+    other slot has one."""
+    w, C, g0, g, ux, uy = _nu_case(0.0, x_release=10.0, x_stop=244.0)
+    T = np.ones(w.D)
+    r = _nu(w, C, g0, g, ux, uy, T)
+    n = 236
+    top = r["nodes"][n:] >> 2
+    missing = np.flatnonzero(top == -1)
+    assert np.all((r["nodes"][:n] >> 2) == 8)
+    assert 0 < len(missing) <= 8 and np.array_equal(missing, np.arange(len(missing)))
+    assert np.all(top[len(missing):] == 63)
+    nu = r["nu"].reshape(-1, 5)
+    assert np.all(nu[missing, 3:] == -1000.0) and np.all(nu[len(missing):, 3:] != -1000.0)
+
+
+def test_nusselt_mean_tree_drops_partials_when_not_a_power_of_two():
+    """getNuMean's halving tree over nY / 2 rounded up to 32 partial sums: at
+    nY = 192 that is 96, and the tree adds 96 -> 48 -> 24 -> 12 -> 6 -> 3 -> 1,
+    so at the step from 3 the partial sum of thread 2 is never added.  With
+    |u| = 1 on every row the kept count is 192 minus the rows of the threads
+    left out (brute force over the tree's index pattern); with T constant the
+    ratio stays T."""
+    nX, nY = 256, 192
+    XF, D = 256, 256 * 192
+    C0, BL = channel.wall_points(nX, lambda x: np.full_like(x, 10.5),
+                                 lambda x: np.full_like(x, nY - 11.5))
+    g = oracle.geometry(nX, nY, C0, BL, x_release_pos=10.0, x_stop_pos=float(nX - 12),
+                        pipe_radius=(nY - 22) / 2.0)
+    ux, uy = np.ones(D), np.zeros(D)
+    r = oracle.nusselt(nX, nY, g["nType"], C0, C0, BL, x_min_val=8.75,
+                       x_stop_pos=float(nX - 12), height=nY - 22, wavy_start=5, wavy_period=100,
+                       wavy_periods=2, k_soot=1.0, k_air=1.0, T=np.full(D, 3.0), ux=ux, uy=uy,
+                       dX=g["dX"], dX0=g["dX"])
+    L = 96
+    vals = np.ones(L) * 2.0  # two rows per thread, |u| = 1
+    s = L >> 1
+    while s > 0:
+        vals[:s] += vals[s:2 * s]
+        s >>= 1
+    kept = vals[0]
+    assert kept < 192
+    assert np.all(r["mean"][1::2] == kept)
+    assert np.all(r["mean"][0::2] == 3.0 * kept)
