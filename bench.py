@@ -284,7 +284,8 @@ def measure_thermal(calls=50):
     """The temperature solve (clVariablesFD::Solve) on the 4096x512 k-omega
     channel after 200 k-omega steps: the time of one call with the flow
     held (every call assembles the rows and solves to the reference's
-    tolerances) and of the reference's step, LB + k-omega + temperature."""
+    tolerances), of the reference's step, LB + k-omega + temperature, and of
+    one Nusselt record read back to the host."""
     from komegafoulingmodel_amd import channel, lib
 
     wk = channel.config("c2")
@@ -310,6 +311,16 @@ def measure_thermal(calls=50):
         L.thermal_step(1)
     L.synchronize()
     ms_step = (time.perf_counter() - t0) * 1e3 / calls
+    # the Nusselt record (clVariablesFD::updateTimeData) at the period
+    # boundaries x = 0, 1024, 2048, 3072, copied to the host each call
+    L.nusselt_init(height=2 * int(wk.pipe_radius), wavy_start=0, wavy_period=1024,
+                   wavy_periods=3, k_soot=0.057, k_air=0.042)
+    cols = L.nusselt_sizes()[0]
+    L.nusselt()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        L.nusselt()
+    ms_nu = (time.perf_counter() - t0) * 1e3 / calls
     L.close()
     achieved = B_LU["t"] * rows / (ms_t * 1e-3) / 1e9
     return {"rows": rows, "ms_per_call": ms_t, "mrows_per_s": rows / ms_t / 1e3,
@@ -317,7 +328,8 @@ def measure_thermal(calls=50):
             "iterations_last": st.temp_iters,
             "residual_last": st.temp_res / st.temp_res0 if st.temp_res0 else 0.0,
             "ms_per_step_lb_komega_temperature": ms_step,
-            "mlups_with_temperature": n / ms_step / 1e3}
+            "mlups_with_temperature": n / ms_step / 1e3,
+            "nusselt_columns": cols, "ms_per_nusselt_record": ms_nu}
 
 
 def kernel_name(launch, storage="float"):
