@@ -424,8 +424,8 @@ int kofm_nusselt_sizes(kofm_ctx* ctx, int32_t* columns, int32_t* locations);
 /* One Nusselt record (clVariablesFD::updateTimeData, clVariablesFD.cpp:
  * 838-869) from the current T and the last saved velocity, all in FP64:
  * the nearest node of each column to each wall along the wall normal
- * (updateNuCoeff1/2, tfdKernels.cl:759-856; ties go to the lower link, then
- * the lower row), then FD_calc_Nu (:919-982) into nu[5 * columns]: per
+ * (updateNuCoeff1/2, tfdKernels.cl:759-856; ties go to the link earlier in
+ * the particle tracer's order, then the lower row), then FD_calc_Nu (:919-982) into nu[5 * columns]: per
  * column the bulk temperature (getTbm, :858-876, whose rows x + j * nY are
  * kept), then the wall-normal gradient and the wall temperature (getTandDT,
  * :878-917) of the bottom and of the top wall, -1000 where a wall has no

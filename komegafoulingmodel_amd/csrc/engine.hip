@@ -2231,20 +2231,24 @@ class EngineT final : public Engine {
     return V;
   }
 
-  // The links whose search box reaches each (column, wall) slot, in link
-  // order (nuFindNearestNode, tfdKernels.cl:759-807: the box is the link's
-  // x range truncated / rounded up, widened by one and clamped to the
-  // particle domain), and the wall points now.
+  // The links whose search box reaches each (column, wall) slot, in the
+  // particle tracer's link order (nuFindNearestNode, tfdKernels.cl:759-807:
+  // the box is the link's x range truncated / rounded up, widened by one and
+  // clamped to the particle domain), and the wall points now.  The tracer
+  // keeps the top wall's links in reverse (iniBLinksTop, clVariablesTR.cpp:
+  // 600-603): its link t is the wall link t below nBL / 2 and
+  // nBL - 1 - t + nBL / 2 above, so that both walls run left to right.
   int nusseltLinks(std::string& err) {
-    const int n = nuN_, half = int(geo_.BL.size() / 2) / 2;
+    const int n = nuN_, nBL = int(geo_.BL.size() / 2), half = nBL / 2;
     const int perSide = geo_.bounds[1] - 1;  // MAX_BL_BOT - MIN_BL_BOT
     const int nl = std::min((perSide + 255) / 256 * 256, n);
     std::vector<std::vector<int32_t>> slot(2 * size_t(n));
     for (int w = 0; w < 2; w++) {
       const int base = w ? geo_.bounds[2] : geo_.bounds[0];
       for (int i = 0; i < nl; i++) {
-        const int l = base + i;
-        if (l >= 2 * half) break;
+        const int t = base + i;
+        if (t >= 2 * half) break;
+        const int l = t < half ? t : nBL - 1 - t + half;
         const double xa = wallC_[2 * size_t(geo_.BL[2 * size_t(l)])];
         const double xb = wallC_[2 * size_t(geo_.BL[2 * size_t(l) + 1])];
         const int x0 = std::max(int(std::min(xa, xb)) - 1, nuXMin_);

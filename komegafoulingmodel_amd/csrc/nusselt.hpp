@@ -10,8 +10,8 @@
 // atomic minimum on the distance, then writes the node's row and vector
 // outside the atomic, so a tie or two close updates can leave a row and
 // vector from different links.  Here one thread per (column, wall) visits
-// the links that reach its column in increasing link order and keeps the
-// first strict minimum: the order of the reference's host search
+// the links that reach its column in the particle tracer's link order and
+// keeps the first strict minimum: the order of the reference's host search
 // (clVariablesFD::nuFindNearestNode, clVariablesFD.cpp:498-536).
 #pragma once
 #include <hip/hip_runtime.h>

@@ -1489,7 +1489,8 @@ static double nu_wall(const double *T, const double *dX, const double *dX0, long
 }
 
 /* The Nusselt record.  MIN_BL_BOT / MAX_BL_BOT from the walls the geometry
- * was built from, C0 (clVariablesTR.cpp:497-530); the search over the walls
+ * was built from, C0 (clVariablesTR.cpp:497-530); the search (in the
+ * tracer's link order, MIN_BL_TOP = MIN_BL_BOT + nBL / 2) over the walls
  * now, C, and the node types now; updateNuCoeff2's type bits
  * (tfdKernels.cl:832-856); FD_calc_Nu (:919-982) with getTbm (:858-876,
  * rows x + j * nY, j < height); getNuMean's OpenCL 1.2 form (:984-1033) at
@@ -1530,11 +1531,18 @@ int orc_nusselt(int nX, int XF, int nY, const int16_t *nType, const double *C0, 
     nodes[s] = -1;
     vec[2 * s] = vec[2 * s + 1] = 0.;
   }
+  /* the tracer's link t (vtr.BL) is the wall link t on the bottom half and
+   * nBL - 1 - t + nBL / 2 on the top half (iniBLinksTop,
+   * clVariablesTR.cpp:600-603) */
   for (int i = 0; i < nl; i++) {
-    if (lo + i < nBL)
-      nu_nearest(nY, XF, nType, C, BL, lo + i, xStart, xStop, xStart, best, vec, nodes);
-    if (half + lo + i < nBL)
-      nu_nearest(nY, XF, nType, C, BL, half + lo + i, xStart, xStop, xStart - n, best, vec, nodes);
+    int t = lo + i;
+    if (t < 2 * half)
+      nu_nearest(nY, XF, nType, C, BL, t < half ? t : nBL - 1 - t + half, xStart, xStop, xStart,
+                 best, vec, nodes);
+    t = half + lo + i;
+    if (t < 2 * half)
+      nu_nearest(nY, XF, nType, C, BL, t < half ? t : nBL - 1 - t + half, xStart, xStop,
+                 xStart - n, best, vec, nodes);
   }
   free(best);
   for (int s = 0; s < 2 * n; s++) {

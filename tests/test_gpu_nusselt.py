@@ -45,14 +45,12 @@ def _nu_params(w, **kw):
 
 
 def _workload(amp):
-    """The particle domain is columns 12 to 244, which the top wall's link
-    range (MIN_BL_TOP = nBL / 2 + MIN_BL_BOT over wall points listed right to
-    left) covers whole: TR_X_IND_STOP = nX + 1 - TR_X_IND_START."""
+    """The particle domain is columns 8 to 243 (X Release Pos 10, X Stop Pos
+    nX - 12): not symmetric, so the top wall's links are found through the
+    tracer's reversed top order."""
     nu, Re = 0.01, 500.0
-    w = channel.wavy_channel(256, 72, H=H, amp=amp, period=128.0, nu=nu, Re=Re,
-                             Fval=channel.turbulent_forcing(H, nu, Re), seed=7)
-    w.geom = {"x_release_pos": 13.5, "x_stop_pos": 245.0}
-    return w
+    return channel.wavy_channel(256, 72, H=H, amp=amp, period=128.0, nu=nu, Re=Re,
+                                Fval=channel.turbulent_forcing(H, nu, Re), seed=7)
 
 
 def _lattice(w, storage, steps=20):
@@ -96,7 +94,7 @@ def test_nusselt_matches_oracle(storage, amp):
     dX0 = L.geometry("dX")
     q = _nu_params(w)
     L.nusselt_init(**q)
-    assert L.nusselt_sizes() == (233, 2)
+    assert L.nusselt_sizes() == (236, 2)
     got = L.nusselt(nodes=True)
     ref = _oracle_record(L, w, w.C0, dX0, q)
     found = _compare(got, ref, f"{storage} amp {amp}")
@@ -111,7 +109,7 @@ def _flat_expectations(L, w, y_bot, y_top):
     """Rows by scanning the node types: the first non-solid row above y_bot
     and below y_top in each column of the particle domain."""
     nT = L.geometry("nType").astype(np.int32).reshape(w.nY, w.XF)
-    cols = np.arange(12, 245)
+    cols = np.arange(8, 244)
     solid = (nT & 1) != 0
     bot = np.array([next(j for j in range(w.nY) if j > y_bot and not solid[j, x]) for x in cols])
     top = np.array([next(j for j in range(w.nY - 1, -1, -1) if j < y_top and not solid[j, x])
@@ -200,5 +198,5 @@ def test_nusselt_argument_checks():
             L.nusselt_init(**_nu_params(w, **bad))
         assert e.value.code == -1, bad
     L.nusselt_init(**_nu_params(w))
-    assert L.nusselt()["nu"].size == 5 * 233
+    assert L.nusselt()["nu"].size == 5 * 236
     L.close()

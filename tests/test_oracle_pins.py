@@ -922,26 +922,17 @@ def test_nusselt_record_closed_forms_on_flat_walls(layer):
     assert same == (layer == 0.0)
 
 
-def test_nusselt_top_wall_links_follow_the_bottom_index_range():
-    """MIN_BL_TOP = nBL / 2 + MIN_BL_BOT (clVariablesTR.cpp:493-508) takes the
-    top links in the bottom links' x order, but the channels list the top
-    wall right to left (MakeChannel_Test.m:91-106): with points at
-    x = k - 0.75 the top search reaches the feet of columns
-    nX + 1 - TR_X_IND_STOP to nX - TR_X_IND_START (all of the domain when
-    TR_X_IND_STOP = nX + 1 - TR_X_IND_START, as in the other tests here).  With the domain 8 to
-    243 on nX = 256 the first top columns get no node (-1000) and every
-    other slot has one."""
+def test_nusselt_top_wall_follows_the_tracer_link_order():
+    """The tracer keeps the top wall's links in reverse (iniBLinksTop,
+    clVariablesTR.cpp:600-603), so its top range MIN_BL_TOP = MIN_BL_BOT +
+    nBL / 2 (clVariablesTR.cpp:493-508) starts at the x of the bottom range:
+    on an asymmetric particle domain (columns 8 to 243 of 256) every column
+    finds its node on both walls, rows 8 and 63, and no value is -1000."""
     w, C, g0, g, ux, uy = _nu_case(0.0, x_release=10.0, x_stop=244.0)
-    T = np.ones(w.D)
-    r = _nu(w, C, g0, g, ux, uy, T)
+    r = _nu(w, C, g0, g, ux, uy, np.ones(w.D))
     n = 236
-    top = r["nodes"][n:] >> 2
-    missing = np.flatnonzero(top == -1)
-    assert np.all((r["nodes"][:n] >> 2) == 8)
-    assert 0 < len(missing) <= 8 and np.array_equal(missing, np.arange(len(missing)))
-    assert np.all(top[len(missing):] == 63)
-    nu = r["nu"].reshape(-1, 5)
-    assert np.all(nu[missing, 3:] == -1000.0) and np.all(nu[len(missing):, 3:] != -1000.0)
+    assert np.all((r["nodes"][:n] >> 2) == 8) and np.all((r["nodes"][n:] >> 2) == 63)
+    assert np.all(r["nu"] != -1000.0)
 
 
 def test_nusselt_mean_tree_drops_partials_when_not_a_power_of_two():
