@@ -182,6 +182,34 @@ typedef struct kofm_nusselt_params {
   double k_soot, k_air;
 } kofm_nusselt_params;
 
+/* TR_shear_removal's constants: particleProperties' per-size arrays
+ * (host pointers; particleProperties.cpp:264-330) and the tracer's defines
+ * (clVariablesTR.cpp:1900-1942). */
+#define KOFM_MAX_PAR_SIZES 32
+typedef struct kofm_removal_params {
+  int32_t n_sizes;          /* NUM_PAR_SIZES: 1 to KOFM_MAX_PAR_SIZES, at most nBL */
+  const double* tau_crit;   /* 2 * n_sizes: paramTauCrit, per size clean then fouled surface */
+  const double* drag_coeff; /* n_sizes: paramDcoeff */
+  const double* lift_coeff; /* n_sizes: paramLcoeff */
+  const double* mass;       /* n_sizes: paramMp */
+  double dt_wall;           /* DTTR_WALL: "LB Steps Per TR Wall" */
+  double foul_size_switch;  /* FOUL_SIZE_SWITCH_SOOT2: "Foul Size Switch Soot" */
+  int32_t loc_stride;       /* FULLSIZEX_TR_PADDED: the particle location's row length */
+  int32_t timer_start;      /* PAR_TIMER_START: min("Time Before Re-release", INT_MAX - 1) */
+} kofm_removal_params;
+
+/* The tracer's particle arrays (vtr.P), device pointers on the context's
+ * GPU. */
+typedef struct kofm_particles {
+  double* pos;              /* 2 per particle, x y */
+  const uint32_t* num_rep;  /* P_Num_rep */
+  const int16_t* type;      /* P_type: the size index */
+  int32_t* dep_flag;        /* P_Dep_Flag: >= 0 the link a particle sits on; -1 free; -2 gone */
+  uint16_t* dep_timer;      /* P_Dep_timer */
+  int32_t* loc;             /* P_loc */
+  int32_t* timer;           /* P_timer */
+} kofm_particles;
+
 /* --- lifecycle ------------------------------------------------------------ */
 
 /* Allocates device storage for an nX x nY lattice on the current HIP device.
@@ -435,6 +463,38 @@ int kofm_nusselt_sizes(kofm_ctx* ctx, int32_t* columns, int32_t* locations);
  * (row << 2) | type (bottom wall first; row -1 when none) and
  * vec[4 * columns] its vector from the wall (x y interleaved). */
 int kofm_nusselt(kofm_ctx* ctx, double* nu, double* mean, int32_t* nodes, double* vec);
+
+/* --- particle re-entrainment by the wall shear ------------------------------ */
+
+/* TR_shear_removal (trKernelsShear.cl:13-121) on particles [offset, offset +
+ * count) of the tracer's arrays, as the reference calls it after TR_shear_2
+ * at the start of a tracer step (LBLS.cpp:236-246, shearStress.cpp:327-335):
+ * a particle sitting on link b (dep_flag = b, the tracer's link index)
+ * counts its deposit timer down; at zero it deposits for good
+ * (bl_dep[b + n_sizes * size] += num_rep, the reference's index; flag and
+ * location -2). Otherwise, when |tau_b| exceeds tau_crit of its size and the
+ * link's surface type, it re-enters the flow: drag along the link and lift
+ * L sqrt(tau) along its normal, moved tau / m * dt^2 / 2 from the link's
+ * midpoint 0.001 off the wall, with flag -1, the new location (-2 outside
+ * [X Release Pos - 0.5, X Stop Pos)) and timer timer_start. tau_b is the
+ * link shear of the last kofm_wall_shear (0 on links outside its range);
+ * the link's point, normal, length and surface type are the tracer's BL
+ * arrays for the walls now (updateBoundaryNodes, lsKernels.cl:246-258;
+ * kofm_removal_links). bl_dep is a device array of nBL * n_sizes counters.
+ * Enqueued on the context's stream (kofm_synchronize waits for it); a
+ * particle whose link or size index is out of range is left unchanged.
+ * KOFM_ERR_ARG for missing arrays or constants. */
+int kofm_shear_removal(kofm_ctx* ctx, const kofm_removal_params* params,
+                       const kofm_particles* particles, int64_t offset, int64_t count,
+                       uint32_t* bl_dep);
+
+/* The tracer's link arrays kofm_shear_removal uses, for the walls now and
+ * the surface-type threshold foul_size_switch, in the tracer's link order
+ * (its top wall runs left to right, iniBLinksTop, clVariablesTR.cpp:
+ * 600-603): first point p0[2 * nBL], unit normal vN[2 * nBL], length
+ * len[nBL], surface type surf[nBL]; any may be NULL.  Host arrays. */
+int kofm_removal_links(kofm_ctx* ctx, double foul_size_switch, double* p0, double* vN, double* len,
+                       int16_t* surf);
 
 /* Copies one field (KOFM_FIELD_*) to host FP64 in the reference layout. */
 int kofm_download(kofm_ctx* ctx, int32_t field, double* host);

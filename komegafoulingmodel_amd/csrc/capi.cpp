@@ -380,6 +380,42 @@ int kofm_nusselt(kofm_ctx* ctx, double* nu, double* mean, int32_t* nodes, double
   return rc ? setErr(ctx, rc, err) : KOFM_OK;
 }
 
+int kofm_shear_removal(kofm_ctx* ctx, const kofm_removal_params* q, const kofm_particles* par,
+                       int64_t offset, int64_t count, uint32_t* bl_dep) {
+  if (!ctx || !q || !par) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  kofm::RemovalParams p;
+  p.nSizes = q->n_sizes;
+  p.tauCrit = q->tau_crit;
+  p.dcoeff = q->drag_coeff;
+  p.lcoeff = q->lift_coeff;
+  p.mp = q->mass;
+  p.dtWall = q->dt_wall;
+  p.foulSwitch = q->foul_size_switch;
+  p.locStride = q->loc_stride;
+  p.timerStart = q->timer_start;
+  kofm::ParticleArrays P;
+  P.pos = par->pos;
+  P.numRep = par->num_rep;
+  P.type = par->type;
+  P.depFlag = par->dep_flag;
+  P.depTimer = par->dep_timer;
+  P.loc = par->loc;
+  P.timer = par->timer;
+  std::string err;
+  int rc = ctx->engine->shearRemoval(p, P, offset, count, bl_dep, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
+int kofm_removal_links(kofm_ctx* ctx, double foul_size_switch, double* p0, double* vN, double* len,
+                       int16_t* surf) {
+  if (!ctx) return KOFM_ERR_ARG;
+  NEED_ENGINE(ctx);
+  std::string err;
+  int rc = ctx->engine->removalLinks(foul_size_switch, p0, vN, len, surf, err);
+  return rc ? setErr(ctx, rc, err) : KOFM_OK;
+}
+
 int kofm_get_solver_stats(kofm_ctx* ctx, kofm_solver_stats* out) {
   if (!ctx || !out) return KOFM_ERR_ARG;
   NEED_ENGINE(ctx);

@@ -22,6 +22,7 @@
 #include "adjoint.hpp"
 #include "kernels.hpp"
 #include "nusselt.hpp"
+#include "removal.hpp"
 #include "walls.hpp"
 
 namespace kofm {
@@ -167,6 +168,7 @@ class EngineT final : public Engine {
     tOn_ = false;  // the temperature mask and field follow the clean geometry: init again
     nuOn_ = false;
     wallC_ = g.C0;
+    remValid_ = false;
     gen_++;
     dropGraphs();
     const size_t D = size_t(D_);
@@ -411,6 +413,7 @@ class EngineT final : public Engine {
     geo_.dxLink.clear();
     wallsMoved_ = true;
     wallC_ = C;
+    remValid_ = false;
     if (nuOn_) {
       const int rc = nusseltLinks(err);
       if (rc) return rc;
@@ -1000,6 +1003,68 @@ class EngineT final : public Engine {
     return 0;
   }
 
+  // TR_shear_removal (trKernelsShear.cl:13-121), called by the reference at
+  // the start of a tracer step after TR_shear_2 (LBLS.cpp:236-246) over the
+  // particles sorted as temporarily deposited, [Ploc(1).x, Ploc(1).y)
+  // (shearStress::updateParRemArgs, shearStress.cpp:327-335).
+  int shearRemoval(const RemovalParams& q, const ParticleArrays& P, int64_t offset, int64_t count,
+                   uint32_t* blDep, std::string& err) override {
+    if (!haveGeo_) return fail(err, "build the geometry first");
+    const int nBL = int(geo_.BL.size() / 2);
+    auto bad = [&err](const char* msg) {
+      err = msg;
+      return KOFM_ERR_ARG;
+    };
+    if (q.nSizes < 1 || q.nSizes > dev::kMaxParSizes || q.nSizes > nBL)
+      return bad("particle sizes: need 1 to 32, and no more than the boundary links");
+    if (!q.tauCrit || !q.dcoeff || !q.lcoeff || !q.mp) return bad("particle size constants missing");
+    if (offset < 0 || count < 0) return bad("particle range: need offset, count >= 0");
+    if (count == 0) return 0;
+    if (!P.pos || !P.numRep || !P.type || !P.depFlag || !P.depTimer || !P.loc || !P.timer || !blDep)
+      return bad("particle arrays and deposit counters are required");
+    if (!remValid_ || remSwitch_ != q.foulSwitch) {
+      int rc = uploadRemovalLinks(q.foulSwitch, err);
+      if (rc) return rc;
+    }
+    dev::RemovalParams p{};
+    for (int s = 0; s < q.nSizes; s++) {
+      p.tauCrit[2 * s] = q.tauCrit[2 * s];
+      p.tauCrit[2 * s + 1] = q.tauCrit[2 * s + 1];
+      p.dcoeff[s] = q.dcoeff[s];
+      p.lcoeff[s] = q.lcoeff[s];
+      p.mp[s] = q.mp[s];
+    }
+    p.nSizes = q.nSizes;
+    p.dtWall = q.dtWall;
+    p.xMinVal = geo_.input.xRelease - 0.5;
+    p.xMaxVal = geo_.input.xStop;
+    p.xIndStart = int(std::floor(geo_.input.xMin));
+    p.locStride = q.locStride;
+    p.timerStart = q.timerStart;
+    p.nBL = nBL;
+    dev::RemovalLinks L{remP0_.as<double>(), remVN_.as<double>(), remLen_.as<double>(),
+                        remSurf_.as<int16_t>(), remAt_.as<int32_t>()};
+    dev::Particles dp{P.pos, P.numRep, P.type, P.depFlag, P.depTimer, P.loc, P.timer};
+    hipLaunchKernelGGL((dev::shearRemoval<R>), dim3(unsigned((count + 255) / 256)), dim3(256), 0,
+                       stream_, p, L, (const R*)tauBL_.as<R>(), dp, blDep, offset, count);
+    HIP_TRY(hipGetLastError());
+    return 0;
+  }
+
+  int removalLinks(double foulSwitch, double* p0, double* vN, double* len, int16_t* surf,
+                   std::string& err) override {
+    if (!haveGeo_) return fail(err, "build the geometry first");
+    std::vector<double> a, b, c;
+    std::vector<int16_t> d;
+    std::vector<int32_t> at;
+    removalLinkArrays(foulSwitch, a, b, c, d, at);
+    if (p0) std::copy(a.begin(), a.end(), p0);
+    if (vN) std::copy(b.begin(), b.end(), vN);
+    if (len) std::copy(c.begin(), c.end(), len);
+    if (surf) std::copy(d.begin(), d.end(), surf);
+    return 0;
+  }
+
   int komegaTimeStep(int action, int& stepsPerLB, double& dt, std::string& err) override {
     if (!ko_) return fail(err, "k-omega is off for this context");
     switch (action) {
@@ -1179,6 +1244,10 @@ class EngineT final : public Engine {
   NusseltParams nuPar_{};
   int nuN_ = 0, nuXMin_ = 0, nuXStop_ = 0, nuLocs_ = 0, nuThreads_ = 0;
   bool nuOn_ = false;
+  // particle re-entrainment: the tracer's link arrays for the walls now
+  Buffer remP0_, remVN_, remLen_, remSurf_, remAt_;
+  double remSwitch_ = 0.;
+  bool remValid_ = false;
   static constexpr int kDtQuiet = 20;              // kOmega.h:22 timeBtwIncreaseTimeStep
   static constexpr int kMaxStepsPerLB = 1 << 16;   // halvings give up beyond this
   dev::BicgState* hostStatus_ = nullptr;  // pinned copy of the solver state
@@ -2272,6 +2341,67 @@ class EngineT final : public Engine {
                         hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(nuC_.as<void>(), wallC_.data(), wallC_.size() * sizeof(double),
                       hipMemcpyHostToDevice));
+    return 0;
+  }
+
+  // The tracer's link arrays for the walls now (updateBoundaryNodes,
+  // lsKernels.cl:246-258): the tracer's link t is the wall link t below
+  // nBL / 2 and nBL - 1 - t + nBL / 2 above (iniBLinksTop,
+  // clVariablesTR.cpp:600-603); its surface type is 1 where the link's
+  // centre has moved FOUL_SIZE_SWITCH_SOOT2 or more from the clean wall's;
+  // its shear is entry k of the link shear output where TR_shear_2 writes
+  // one (GET_SHEAR_BL_INDEX), none elsewhere.
+  void removalLinkArrays(double sw, std::vector<double>& p0, std::vector<double>& vN,
+                         std::vector<double>& len, std::vector<int16_t>& surf,
+                         std::vector<int32_t>& at) const {
+    const int nBL = int(geo_.BL.size() / 2), half = nBL / 2;
+    p0.assign(2 * size_t(nBL), 0.);
+    vN.assign(2 * size_t(nBL), 0.);
+    len.assign(size_t(nBL), 0.);
+    surf.assign(size_t(nBL), 0);
+    at.assign(size_t(nBL), -1);
+    for (int t = 0; t < nBL; t++) {
+      const int i = t < half ? t : nBL - 1 - t + half;
+      const size_t a = size_t(geo_.BL[2 * size_t(i)]), b = size_t(geo_.BL[2 * size_t(i) + 1]);
+      const double c0x = wallC_[2 * a], c0y = wallC_[2 * a + 1];
+      const double c1x = wallC_[2 * b], c1y = wallC_[2 * b + 1];
+      const double n0x = wallC0_[2 * a], n0y = wallC0_[2 * a + 1];
+      const double n1x = wallC0_[2 * b], n1y = wallC0_[2 * b + 1];
+      double tx = c1x - c0x, ty = c1y - c0y;
+      const double ux = n1x - n0x, uy = n1y - n0y;
+      const double dx = (c0x + tx / 2.) - (n0x + ux / 2.), dy = (c0y + ty / 2.) - (n0y + uy / 2.);
+      surf[size_t(t)] = std::sqrt(dx * dx + dy * dy) >= sw ? 1 : 0;
+      const double l = std::sqrt(tx * tx + ty * ty);
+      len[size_t(t)] = l;
+      tx /= l;
+      ty /= l;
+      vN[2 * size_t(t)] = -ty;
+      vN[2 * size_t(t) + 1] = tx;
+      p0[2 * size_t(t)] = c0x;
+      p0[2 * size_t(t) + 1] = c0y;
+    }
+    for (int k = 0; k < geo_.bounds[1]; k++) at[size_t(geo_.bounds[0] + k)] = k;
+    for (int k = 0; k < geo_.bounds[3]; k++) at[size_t(geo_.bounds[2] + k)] = geo_.bounds[1] + k;
+  }
+
+  int uploadRemovalLinks(double sw, std::string& err) {
+    std::vector<double> p0, vN, len;
+    std::vector<int16_t> surf;
+    std::vector<int32_t> at;
+    removalLinkArrays(sw, p0, vN, len, surf, at);
+    HIP_TRY(remP0_.alloc(p0.size() * sizeof(double)));
+    HIP_TRY(remVN_.alloc(vN.size() * sizeof(double)));
+    HIP_TRY(remLen_.alloc(len.size() * sizeof(double)));
+    HIP_TRY(upload<double>(remP0_, p0));
+    HIP_TRY(upload<double>(remVN_, vN));
+    HIP_TRY(upload<double>(remLen_, len));
+    HIP_TRY(remSurf_.alloc(surf.size() * sizeof(int16_t)));
+    HIP_TRY(hipMemcpy(remSurf_.as<void>(), surf.data(), surf.size() * sizeof(int16_t),
+                      hipMemcpyHostToDevice));
+    HIP_TRY(remAt_.alloc(at.size() * sizeof(int32_t)));
+    HIP_TRY(hipMemcpy(remAt_.as<void>(), at.data(), at.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    remSwitch_ = sw;
+    remValid_ = true;
     return 0;
   }
 

@@ -49,6 +49,24 @@ struct NusseltParams {
   double kSoot = 0., kAir = 0.;
 };
 
+// TR_shear_removal's constants (particleProperties' arrays and the
+// tracer's defines) and the tracer's particle arrays (device pointers).
+struct RemovalParams {
+  int nSizes = 0;
+  const double *tauCrit = nullptr, *dcoeff = nullptr, *lcoeff = nullptr, *mp = nullptr;
+  double dtWall = 3., foulSwitch = 0.5;
+  int locStride = 0, timerStart = 0;
+};
+struct ParticleArrays {
+  double* pos = nullptr;
+  const uint32_t* numRep = nullptr;
+  const int16_t* type = nullptr;
+  int32_t* depFlag = nullptr;
+  uint16_t* depTimer = nullptr;
+  int32_t* loc = nullptr;
+  int32_t* timer = nullptr;
+};
+
 // J = mean over the last `window` steps of a run of
 //   pbar(xIn) - pbar(xOut) + shearWeight * sum over shear links of tau_BL * |link|
 // with pbar the mean of rho / 3 over the fluid nodes of a column.
@@ -105,6 +123,14 @@ class Engine {
   virtual int nusseltInit(const NusseltParams& q, std::string& err) = 0;
   virtual int nusseltSizes(int& columns, int& locations, std::string& err) const = 0;
   virtual int nusselt(double* nu, double* mean, int32_t* nodes, double* vec, std::string& err) = 0;
+  // TR_shear_removal over particles [offset, offset + count), enqueued on
+  // the context stream with the last computed link shear
+  virtual int shearRemoval(const RemovalParams& q, const ParticleArrays& P, int64_t offset,
+                           int64_t count, uint32_t* blDep, std::string& err) = 0;
+  // the tracer's link arrays the removal uses (first point, unit normal,
+  // length, surface type), nBL each, for the walls now
+  virtual int removalLinks(double foulSwitch, double* p0, double* vN, double* len, int16_t* surf,
+                           std::string& err) = 0;
   virtual int64_t fieldSize(int field) const = 0;
   virtual int setLaunch(int block, int npt, int spp, int64_t pad, std::string& err) = 0;
   virtual void setGraphs(bool on) = 0;

@@ -57,6 +57,7 @@ EXPORTS = [
     "kofm_save_restart", "kofm_save_debug", "kofm_update_walls",
     "kofm_thermal_init", "kofm_thermal_step", "kofm_thermal_time_step", "kofm_set_temperature",
     "kofm_nusselt_init", "kofm_nusselt_sizes", "kofm_nusselt",
+    "kofm_shear_removal", "kofm_removal_links",
 ]
 
 
@@ -133,6 +134,20 @@ class NusseltParams(C.Structure):
                 ("wavy_periods", C.c_int32), ("k_soot", C.c_double), ("k_air", C.c_double)]
 
 
+class RemovalParams(C.Structure):
+    _fields_ = [("n_sizes", C.c_int32), ("tau_crit", C.c_void_p), ("drag_coeff", C.c_void_p),
+                ("lift_coeff", C.c_void_p), ("mass", C.c_void_p), ("dt_wall", C.c_double),
+                ("foul_size_switch", C.c_double), ("loc_stride", C.c_int32),
+                ("timer_start", C.c_int32)]
+
+
+class Particles(C.Structure):
+    """Device addresses of the tracer's particle arrays."""
+    _fields_ = [("pos", C.c_void_p), ("num_rep", C.c_void_p), ("type", C.c_void_p),
+                ("dep_flag", C.c_void_p), ("dep_timer", C.c_void_p), ("loc", C.c_void_p),
+                ("timer", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -173,6 +188,9 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "kofm_nusselt_init": (C.c_int, [P, C.POINTER(NusseltParams)]),
         "kofm_nusselt_sizes": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "kofm_nusselt": (C.c_int, [P, dp, dp, C.c_void_p, dp]),
+        "kofm_shear_removal": (C.c_int, [P, C.POINTER(RemovalParams), C.POINTER(Particles),
+                                         C.c_int64, C.c_int64, C.c_void_p]),
+        "kofm_removal_links": (C.c_int, [P, C.c_double, dp, dp, dp, C.c_void_p]),
         "kofm_ini_poiseuille": (C.c_int, [P, C.POINTER(IniFlow), dp, dp, dp]),
         "kofm_reduce": (C.c_int, [P, C.c_int32, C.c_int32, dp]),
         "kofm_reduce_device": (C.c_int, [P, C.c_int32, C.c_int64, C.c_int32, P, P]),
@@ -285,6 +303,7 @@ class Lattice:
             self.h, _dptr(C0), C0.shape[0], BL.ctypes.data_as(C.POINTER(C.c_int32)), BL.shape[0],
             C.byref(p)))
         self._nN = C0.shape[0]
+        self._nBL = BL.shape[0]
 
     def update_walls(self, C: np.ndarray):
         """Moves the walls to the points C [nN, 2] (a fouling layer grows
@@ -411,6 +430,32 @@ class Lattice:
             self.h, _dptr(out["nu"]), _dptr(out["mean"]),
             out["nodes"].ctypes.data_as(C.c_void_p) if nodes else None,
             _dptr(out["vec"]) if nodes else None))
+        return out
+
+    def shear_removal(self, particles: dict, bl_dep: int, offset: int, count: int, *,
+                      tau_crit, drag_coeff, lift_coeff, mass, dt_wall: float = 3.0,
+                      foul_size_switch: float = 0.5, loc_stride: int, timer_start: int):
+        """kofm_shear_removal (TR_shear_removal) on the particle arrays at the
+        device addresses in `particles` (pos, num_rep, type, dep_flag,
+        dep_timer, loc, timer) and the deposit counters at bl_dep; enqueued on
+        the context's stream."""
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        tc, dc, lc, mp = f64(tau_crit), f64(drag_coeff), f64(lift_coeff), f64(mass)
+        q = RemovalParams(len(mp), tc.ctypes.data, dc.ctypes.data, lc.ctypes.data, mp.ctypes.data,
+                          dt_wall, foul_size_switch, loc_stride, timer_start)
+        par = Particles(*(particles[k] for k in ("pos", "num_rep", "type", "dep_flag",
+                                                 "dep_timer", "loc", "timer")))
+        self._check(self.lib.kofm_shear_removal(self.h, C.byref(q), C.byref(par), offset, count,
+                                                bl_dep))
+
+    def removal_links(self, foul_size_switch: float = 0.5) -> dict:
+        """kofm_removal_links: the tracer's link arrays for the walls now."""
+        nBL = self._nBL
+        out = dict(p0=np.zeros(2 * nBL), vN=np.zeros(2 * nBL), len=np.zeros(nBL),
+                   surf=np.zeros(nBL, np.int16))
+        self._check(self.lib.kofm_removal_links(self.h, foul_size_switch, _dptr(out["p0"]),
+                                                _dptr(out["vN"]), _dptr(out["len"]),
+                                                out["surf"].ctypes.data_as(C.c_void_p)))
         return out
 
     def reduce(self, field: str, op: int = RED_SUM) -> float:

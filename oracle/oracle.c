@@ -1605,3 +1605,82 @@ int orc_nusselt(int nX, int XF, int nY, const int16_t *nType, const double *C0, 
   free(us);
   return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * Particle re-entrainment by the wall shear (TR_shear_removal,
+ * trKernelsShear.cl:13-121).
+ * ---------------------------------------------------------------------- */
+
+/* The tracer's link arrays for walls C (clean walls C0):
+ * updateBoundaryNodes (lsKernels.cl:246-258) on the tracer's links, whose
+ * link t is the wall link t below nBL / 2 and nBL - 1 - t + nBL / 2 above
+ * (iniBLinksTop, clVariablesTR.cpp:600-603). */
+void orc_removal_links(int nBL, const double *C, const double *C0, const int32_t *BL, double sw,
+                       double *p0, double *vN, double *len, int16_t *surf) {
+  int half = nBL / 2;
+  for (int t = 0; t < nBL; t++) {
+    int i = t < half ? t : nBL - 1 - t + half;
+    int a = BL[2 * i], b = BL[2 * i + 1];
+    double vTx = C[2 * b] - C[2 * a], vTy = C[2 * b + 1] - C[2 * a + 1];
+    double vNx = C0[2 * b] - C0[2 * a], vNy = C0[2 * b + 1] - C0[2 * a + 1];
+    double cdx = (C[2 * a] + vTx / 2.) - (C0[2 * a] + vNx / 2.);
+    double cdy = (C[2 * a + 1] + vTy / 2.) - (C0[2 * a + 1] + vNy / 2.);
+    surf[t] = sqrt(cdx * cdx + cdy * cdy) >= sw ? 1 : 0;
+    double l = sqrt(vTx * vTx + vTy * vTy);
+    len[t] = l;
+    vTx /= l;
+    vTy /= l;
+    vN[2 * t] = -vTy;
+    vN[2 * t + 1] = vTx;
+    p0[2 * t] = C[2 * a];
+    p0[2 * t + 1] = C[2 * a + 1];
+  }
+}
+
+/* TR_shear_removal over particles [offset, offset + count), tau per tracer
+ * link (0 where TR_shear_2 writes none).  A particle whose link or size
+ * index lies outside the arrays is skipped. */
+void orc_shear_removal(int nSizes, const double *tauCrit, const double *dcoeff,
+                       const double *lcoeff, const double *mp, double dt, double xMinVal,
+                       double xMaxVal, int xIndStart, int locStride, int timerStart, int nBL,
+                       const double *p0, const double *vN, const double *len, const int16_t *surf,
+                       const double *tau, double *pos, const uint32_t *numRep,
+                       const int16_t *type, int32_t *depFlag, uint16_t *depTimer, int32_t *loc,
+                       int32_t *timer, uint32_t *blDep, long offset, long count) {
+  for (long k = 0; k < count; k++) {
+    long i = k + offset;
+    int bl = depFlag[i];
+    if (bl < 0) continue;
+    int sz = type[i];
+    if (bl >= nBL || sz < 0 || sz >= nSizes) continue;
+    uint16_t dtim = (uint16_t)(depTimer[i] - 1);
+    if (dtim == 0) {
+      blDep[bl + nSizes * sz] += numRep[i];
+      depFlag[i] = -2;
+      loc[i] = -2;
+      continue;
+    }
+    double shear = tau[bl];
+    if (fabs(shear) <= tauCrit[sz * 2 + surf[bl]]) {
+      depTimer[i] = dtim;
+      continue;
+    }
+    double fd = dcoeff[sz], fl = lcoeff[sz] * sqrt(shear);
+    double nx = vN[2 * bl], ny = vN[2 * bl + 1];
+    double dx = fd * ny, dy = fd * -nx;
+    double h = len[bl] * 0.5;
+    double cx = p0[2 * bl] + ny * h + nx * 0.001;
+    double cy = p0[2 * bl + 1] + -nx * h + ny * 0.001;
+    double rx = nx * fl + dx, ry = ny * fl + dy;
+    cx += rx * shear / mp[sz] * dt * dt / 2.;
+    cy += ry * shear / mp[sz] * dt * dt / 2.;
+    int ix = cx == cx ? (int)cx : 0, iy = cy == cy ? (int)cy : 0;
+    int l = (ix - xIndStart) + iy * locStride;
+    if (cx < xMinVal || cx >= xMaxVal) l = -2;
+    depFlag[i] = -1;
+    loc[i] = l;
+    pos[2 * i] = cx;
+    pos[2 * i + 1] = cy;
+    timer[i] = timerStart;
+  }
+}

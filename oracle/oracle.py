@@ -146,6 +146,41 @@ def nusselt(nX, nY, nType, C0, C_now, BL, *, x_min_val, x_stop_pos, height, wavy
     return out
 
 
+def removal_links(C_now, C0, BL, foul_size_switch):
+    """orc_removal_links: the tracer's link arrays (p0 [nBL, 2], vN [nBL, 2],
+    len [nBL], surf [nBL]) for the walls C_now built from C0."""
+    lib = load()
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64).ravel()
+    C_now, C0 = f64(C_now), f64(C0)
+    BL = np.ascontiguousarray(BL, dtype=np.int32).ravel()
+    nBL = len(BL) // 2
+    out = dict(p0=np.zeros(2 * nBL), vN=np.zeros(2 * nBL), len=np.zeros(nBL),
+               surf=np.zeros(nBL, np.int16))
+    lib.orc_removal_links.argtypes = [C.c_int] + [C.c_void_p] * 3 + [C.c_double] + [C.c_void_p] * 4
+    lib.orc_removal_links(nBL, p(C_now), p(C0), p(BL), float(foul_size_switch), p(out["p0"]),
+                          p(out["vN"]), p(out["len"]), p(out["surf"]))
+    return out
+
+
+def shear_removal(links, tau_link, par, *, tau_crit, drag_coeff, lift_coeff, mass, dt_wall,
+                  x_min_val, x_max_val, x_ind_start, loc_stride, timer_start, bl_dep, offset,
+                  count):
+    """orc_shear_removal on host copies: `par` holds pos [2 n], num_rep, type,
+    dep_flag, dep_timer, loc, timer (updated in place), bl_dep likewise."""
+    lib = load()
+    n_sizes = len(mass)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    tc, dc, lc, mp, tau = f64(tau_crit), f64(drag_coeff), f64(lift_coeff), f64(mass), f64(tau_link)
+    lib.orc_shear_removal.argtypes = ([C.c_int] + [C.c_void_p] * 4 + [C.c_double] * 3
+                                      + [C.c_int] * 4 + [C.c_void_p] * 13 + [C.c_long] * 2)
+    lib.orc_shear_removal(n_sizes, p(tc), p(dc), p(lc), p(mp), float(dt_wall), float(x_min_val),
+                          float(x_max_val), x_ind_start, loc_stride, timer_start,
+                          len(links["len"]), p(links["p0"]), p(links["vN"]), p(links["len"]),
+                          p(links["surf"]), p(tau), p(par["pos"]), p(par["num_rep"]),
+                          p(par["type"]), p(par["dep_flag"]), p(par["dep_timer"]), p(par["loc"]),
+                          p(par["timer"]), p(bl_dep), int(offset), int(count))
+
+
 def feq(rho, ux, uy):
     lib = load()
     out = np.zeros(9)

@@ -963,3 +963,90 @@ def test_nusselt_mean_tree_drops_partials_when_not_a_power_of_two():
     assert kept < 192
     assert np.all(r["mean"][1::2] == kept)
     assert np.all(r["mean"][0::2] == 3.0 * kept)
+
+
+def test_shear_removal_closed_forms_on_flat_walls():
+    """orc_removal_links / orc_shear_removal (TR_shear_removal,
+    trKernelsShear.cl:13-121) on flat walls at rows 7.5 and 40.5 with link
+    shears set by hand.  A particle on a bottom link above its critical
+    shear leaves from the link midpoint 0.001 off the wall, moved by drag
+    D tau dt^2 / (2 m) along +x and lift L sqrt(tau) tau dt^2 / (2 m) along +y;
+    on a top link (the tracer's top links run left to right, normal -y) the
+    drag acts along -x and the lift along -y.  Below it the deposit timer
+    counts down (0 wraps to 65535, uint16); at 1 the particle deposits for
+    good (counter link + n_sizes * size).  The surface type is 1 only where
+    a link's centre moved 0.5 or more (points 30-32 raised by 0.6), and then
+    the fouled critical shear applies.  A particle landing at x >= X_MAX_VAL
+    gets location -2 (flag -1); a negative shear beyond critical gives a NaN
+    position with location -TR_X_IND_START."""
+    nX = 64
+    C0, BL = channel.wall_points(nX, lambda x: np.full_like(x, 7.5), lambda x: np.full_like(x, 40.5))
+    nb = len(C0) // 2
+    C = C0.copy()
+    C[30:33, 1] += 0.6
+    L = oracle.removal_links(C, C0, BL, 0.5)
+    nBL, half = len(BL), len(BL) // 2
+    surf = L["surf"]
+    assert np.array_equal(np.flatnonzero(surf), [30, 31])
+    p0 = L["p0"].reshape(-1, 2)
+    vN = L["vN"].reshape(-1, 2)
+    ks = np.arange(half)
+    flat = (ks < 29) | (ks > 32)  # links 29 and 32 slope up to the raised points
+    assert np.allclose(p0[:half, 0], -0.75 + ks) and np.all(vN[:half][flat] == [0.0, 1.0])
+    assert np.allclose(p0[half:, 0], 0.25 + ks) and np.all(p0[half:, 1] == 40.5)
+    assert np.all(vN[half:] == [0.0, -1.0]) and np.allclose(L["len"][:half][flat], 1.0)
+    assert np.allclose(L["len"][half:], 1.0)
+    del nb
+    D, Lc, M = np.array([0.3, 0.7]), np.array([0.2, 0.05]), np.array([2.0, 5.0])
+    crit = np.array([1.0, 1.0, 10.0, 0.1])  # size 0 clean / fouled, size 1 clean / fouled
+    dt, x_min, x_max, xs, stride, t0 = 3.0, 9.5, 28.0, 8, 64, 777
+    tau = np.zeros(nBL)
+    cases = [  # (link, size, tau, dep timer)
+        (20, 0, 2.0, 5), (21, 1, 0.5, 5), (22, 1, 2.0, 1), (23, 0, 2.0, 0), (half + 20, 0, 2.0, 5),
+        (25, 0, -3.0, 5), (27, 0, 4.0, 5), (30, 1, 1.0, 5), (24, 1, 1.0, 5)]
+    n = len(cases) + 2
+    par = dict(pos=np.zeros(2 * n), num_rep=np.arange(1, n + 1, dtype=np.uint32),
+               type=np.zeros(n, np.int16), dep_flag=np.full(n, -1, np.int32),
+               dep_timer=np.zeros(n, np.uint16), loc=np.full(n, 5, np.int32),
+               timer=np.zeros(n, np.int32))
+    for i, (b, s, t, tm) in enumerate(cases):
+        tau[b] = t
+        par["dep_flag"][i], par["type"][i], par["dep_timer"][i] = b, s, tm
+    par["dep_flag"][-1] = -2
+    before = {k: v.copy() for k, v in par.items()}
+    bl_dep = np.zeros(nBL * 2, np.uint32)
+    oracle.shear_removal(L, tau, par, tau_crit=crit, drag_coeff=D, lift_coeff=Lc, mass=M,
+                         dt_wall=dt, x_min_val=x_min, x_max_val=x_max, x_ind_start=xs,
+                         loc_stride=stride, timer_start=t0, bl_dep=bl_dep, offset=0, count=n)
+    pos = par["pos"].reshape(-1, 2)
+
+    def moved(i, x_mid, y_wall, sign, s, t):
+        dx = sign * D[s] * t * dt * dt / (2 * M[s])
+        dy = sign * Lc[s] * np.sqrt(t) * t * dt * dt / (2 * M[s])
+        want = (x_mid + dx, y_wall + sign * 0.001 + dy)
+        assert np.allclose(pos[i], want, rtol=0, atol=1e-12), (i, pos[i], want)
+        assert par["dep_flag"][i] == -1 and par["timer"][i] == t0
+        loc = (int(want[0]) - xs) + int(want[1]) * stride
+        assert par["loc"][i] == (loc if x_min <= want[0] < x_max else -2), i
+
+    moved(0, 20.25 - 0.5, 7.5, 1, 0, 2.0)
+    assert par["dep_timer"][0] == 5
+    # below the critical shear: the timer counts down, nothing else moves
+    assert par["dep_timer"][1] == 4 and par["dep_flag"][1] == 21 and par["loc"][1] == 5
+    # timer 1: deposited for good, counter 22 + n_sizes * 1
+    assert par["dep_flag"][2] == -2 and par["loc"][2] == -2
+    assert bl_dep[22 + 2 * 1] == 3 and bl_dep.sum() == 3
+    # timer 0 wraps to 65535 and the particle is removed
+    moved(3, 23.25 - 0.5, 7.5, 1, 0, 2.0)
+    # top link half + 20: from x 20.25 to 19.25, drag and lift reversed
+    moved(4, 20.25 - 0.5, 40.5, -1, 0, 2.0)
+    # negative shear beyond critical: NaN position, location -TR_X_IND_START
+    assert np.all(np.isnan(pos[5])) and par["loc"][5] == -xs and par["dep_flag"][5] == -1
+    # landing at x >= X_MAX_VAL: location -2, flag -1
+    moved(6, 27.25 - 0.5, 7.5, 1, 0, 4.0)
+    assert par["loc"][6] == -2
+    # size 1: removed from the fouled link 30 (crit 0.1), kept on clean link 24 (crit 10)
+    assert par["dep_flag"][7] == -1 and par["dep_timer"][8] == 4 and par["dep_flag"][8] == 24
+    # a free particle and one deposited for good are not touched
+    for k in par:
+        assert np.array_equal(par[k].reshape(n, -1)[-2:], before[k].reshape(n, -1)[-2:]), k
