@@ -332,6 +332,95 @@ def measure_thermal(calls=50):
             "nusselt_columns": cols, "ms_per_nusselt_record": ms_nu}
 
 
+def measure_removal(n=1 << 26, calls=50):
+    """TR_shear_removal over n particles (the tracer's device arrays) sitting
+    on random links of the 8192x1024 fouled channel's shear range, with the
+    link shear of its flow after 100 steps, in link order (the order the
+    tracer's sort by location leaves them in).  Two passes: every particle
+    below its critical shear, so each call reads the flag, size and deposit
+    timer and writes the timer back (10 B per particle; the steady state of
+    particles on a wall), timed over `calls` calls; and every particle
+    re-entrained (critical shear 0) from fresh arrays, reading 8 B and
+    writing position, flag, location and timer (36 B per particle), the
+    fastest of 3 single calls, each bracketed by synchronisation.  The
+    countdown pass is also timed with the particles in random order."""
+    import ctypes as C
+
+    from komegafoulingmodel_amd import channel, lib
+
+    w = channel.config("c3")
+    L, _ = setup(w, "f32", tune=False)
+    L.step(100, lib.OPTION_SAVE_MACRO_FIELDS)
+    L.wall_shear()
+    b = L.geometry("bounds")
+    tau = L.download("tau_bl")
+    links = np.r_[np.arange(b[0], b[0] + b[1]), np.arange(b[2], b[2] + b[3])]
+    links = links[np.abs(tau) > 0]
+    rng = np.random.Generator(np.random.PCG64(3))
+    host = dict(pos=np.zeros(2 * n), num_rep=np.ones(n, np.uint32),
+                type=rng.integers(0, 4, n).astype(np.int16),
+                dep_flag=np.sort(rng.choice(links, n)).astype(np.int32),
+                dep_timer=np.full(n, 60000, np.uint16), loc=np.zeros(n, np.int32),
+                timer=np.zeros(n, np.int32))
+    h = C.CDLL(lib.load()._name)
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipFree.argtypes = [C.c_void_p]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    dev = {}
+    for k, a in host.items():
+        ptr = C.c_void_p()
+        assert h.hipMalloc(C.byref(ptr), a.nbytes) == 0
+        dev[k] = ptr
+    dep = C.c_void_p()
+    assert h.hipMalloc(C.byref(dep), 4 * 4 * len(w.BL)) == 0
+
+    def upload(keys):
+        for k in keys:
+            assert h.hipMemcpy(dev[k], host[k].ctypes.data_as(C.c_void_p), host[k].nbytes, 1) == 0
+
+    upload(host)
+    addr = {k: v.value for k, v in dev.items()}
+    k = dict(drag_coeff=np.full(4, 1e3), lift_coeff=np.full(4, 1e2), mass=np.ones(4),
+             loc_stride=8192, timer_start=1000)
+    stay = np.full(8, 1e30)
+    for _ in range(2):
+        L.shear_removal(addr, dep.value, 0, n, tau_crit=stay, **k)
+    L.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        L.shear_removal(addr, dep.value, 0, n, tau_crit=stay, **k)
+    L.synchronize()
+    ms_stay = (time.perf_counter() - t0) * 1e3 / calls
+    sorted_flags = host["dep_flag"]
+    host["dep_flag"] = rng.permutation(sorted_flags)
+    upload(("dep_flag",))
+    L.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        L.shear_removal(addr, dep.value, 0, n, tau_crit=stay, **k)
+    L.synchronize()
+    ms_random = (time.perf_counter() - t0) * 1e3 / calls
+    host["dep_flag"] = sorted_flags
+    ms_leave = float("inf")
+    for _ in range(3):
+        upload(("dep_flag", "dep_timer"))
+        L.synchronize()
+        t0 = time.perf_counter()
+        L.shear_removal(addr, dep.value, 0, n, tau_crit=np.zeros(8), **k)
+        L.synchronize()
+        ms_leave = min(ms_leave, (time.perf_counter() - t0) * 1e3)
+    for v in list(dev.values()) + [dep]:
+        h.hipFree(v)
+    L.close()
+    g_stay = 10.0 * n / (ms_stay * 1e-3) / 1e9
+    g_leave = 36.0 * n / (ms_leave * 1e-3) / 1e9
+    return {"particles": n, "links": int(len(links)), "ms_per_call_countdown": ms_stay,
+            "achieved_gbs_countdown": g_stay, "frac_countdown": g_stay / HBM_PEAK_GBS,
+            "ms_per_call_countdown_random_order": ms_random,
+            "ms_per_call_reentrain": ms_leave, "achieved_gbs_reentrain": g_leave,
+            "frac_reentrain": g_leave / HBM_PEAK_GBS, "bytes_per_particle": [10, 36]}
+
+
 def kernel_name(launch, storage="float"):
     if launch.get("steps_per_pass", 1) == 2:
         tile = {512: (64, 8), 640: (128, 5), 832: (64, 13), 1856: (64, 29), 1024: (64, 16),
@@ -419,7 +508,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--forms", default="p32_ref,f32x,f16,bf16,prec,k32,k16,k32_c3,t32,adj32,adj32_exact,adj32_exact_smooth,adj32_c3,ens32,ens16,ens32ko,ens16ko")
+    ap.add_argument("--forms", default="p32_ref,f32x,f16,bf16,prec,k32,k16,k32_c3,t32,rem32,adj32,adj32_exact,adj32_exact_smooth,adj32_c3,ens32,ens16,ens32ko,ens16ko")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true")
@@ -495,6 +584,8 @@ def main():
             Lk.close()
         elif form == "t32":
             forms["t32"] = measure_thermal()
+        elif form == "rem32":
+            forms["rem32"] = measure_removal()
         elif form in ("ens32", "ens16"):
             forms[form] = measure_ensemble("f32" if form == "ens32" else "f16")
         elif form in ("ens32ko", "ens16ko"):

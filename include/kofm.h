@@ -201,7 +201,7 @@ typedef struct kofm_removal_params {
 /* The tracer's particle arrays (vtr.P), device pointers on the context's
  * GPU. */
 typedef struct kofm_particles {
-  double* pos;              /* 2 per particle, x y */
+  double* pos;              /* 2 per particle, x y (16-byte aligned, as the double2 P_pos) */
   const uint32_t* num_rep;  /* P_Num_rep */
   const int16_t* type;      /* P_type: the size index */
   int32_t* dep_flag;        /* P_Dep_Flag: >= 0 the link a particle sits on; -1 free; -2 gone */

@@ -39,6 +39,9 @@ static int printParams(const std::string& loadDir) {
               "NumPeriods %d\nSaveNuTimestep %u\nnextSaveNuTime %u\nnextSaveAvgTime %u\n",
               vfd.nuOn() ? 1 : 0, vfd.kSootLattice, vfd.kAirLattice, p.xWavyStart, p.wavyPeriodLen,
               p.numWavyPeriods, p.saveTimeStepNu, p.nextSaveNuTime, p.nextSaveAvgTime);
+  const kofm_removal_params rq = shearStress::removalParams(p);
+  std::printf("RemovalDtWall %.17g\nFoulSizeSwitch %.17g\nParTimerStart %d\nParLocStride %d\n",
+              rq.dt_wall, rq.foul_size_switch, rq.timer_start, rq.loc_stride);
   std::printf("IniPoiseuille %d\nIniTurbVelocity %d\nPerturbVelField %d\nSaveAvgInfo %d\n"
               "LoadVelText %d\nUseTurbVelBC %d\nGeomPenaltyFactor %.17g\nlbOutMaxLines %d\n",
               vlb.iniPoiseuille, vlb.kOmegaClass.iniTurbVelocity, vlb.kOmegaClass.perturbVelField,

@@ -1022,6 +1022,8 @@ class EngineT final : public Engine {
     if (count == 0) return 0;
     if (!P.pos || !P.numRep || !P.type || !P.depFlag || !P.depTimer || !P.loc || !P.timer || !blDep)
       return bad("particle arrays and deposit counters are required");
+    if (reinterpret_cast<uintptr_t>(P.pos) % 16)
+      return bad("particle positions must be 16-byte aligned (double2, as P_pos)");
     if (!remValid_ || remSwitch_ != q.foulSwitch) {
       int rc = uploadRemovalLinks(q.foulSwitch, err);
       if (rc) return rc;
@@ -1042,11 +1044,15 @@ class EngineT final : public Engine {
     p.locStride = q.locStride;
     p.timerStart = q.timerStart;
     p.nBL = nBL;
-    dev::RemovalLinks L{remP0_.as<double>(), remVN_.as<double>(), remLen_.as<double>(),
-                        remSurf_.as<int16_t>(), remAt_.as<int32_t>()};
+    hipLaunchKernelGGL((dev::removalLinkShear<R>), dim3(unsigned((nBL + 255) / 256)), dim3(256), 0,
+                       stream_, (const R*)tauBL_.as<R>(), (const int32_t*)remAt_.as<int32_t>(),
+                       (const int16_t*)remSurf_.as<int16_t>(), nBL, remTS_.as<double2>());
+    dev::RemovalLinks L{remTS_.as<double2>(), remGeo_.as<double>()};
     dev::Particles dp{P.pos, P.numRep, P.type, P.depFlag, P.depTimer, P.loc, P.timer};
-    hipLaunchKernelGGL((dev::shearRemoval<R>), dim3(unsigned((count + 255) / 256)), dim3(256), 0,
-                       stream_, p, L, (const R*)tauBL_.as<R>(), dp, blDep, offset, count);
+    const int64_t per = 256 * dev::kRemovalPer;
+    if ((count + per - 1) / per > int64_t(UINT32_MAX)) return bad("particle range too large");
+    hipLaunchKernelGGL(dev::shearRemoval, dim3(unsigned((count + per - 1) / per)), dim3(256), 0,
+                       stream_, p, L, dp, blDep, offset, count);
     HIP_TRY(hipGetLastError());
     return 0;
   }
@@ -1245,7 +1251,7 @@ class EngineT final : public Engine {
   int nuN_ = 0, nuXMin_ = 0, nuXStop_ = 0, nuLocs_ = 0, nuThreads_ = 0;
   bool nuOn_ = false;
   // particle re-entrainment: the tracer's link arrays for the walls now
-  Buffer remP0_, remVN_, remLen_, remSurf_, remAt_;
+  Buffer remGeo_, remTS_, remSurf_, remAt_;
   double remSwitch_ = 0.;
   bool remValid_ = false;
   static constexpr int kDtQuiet = 20;              // kOmega.h:22 timeBtwIncreaseTimeStep
@@ -2389,12 +2395,20 @@ class EngineT final : public Engine {
     std::vector<int16_t> surf;
     std::vector<int32_t> at;
     removalLinkArrays(sw, p0, vN, len, surf, at);
-    HIP_TRY(remP0_.alloc(p0.size() * sizeof(double)));
-    HIP_TRY(remVN_.alloc(vN.size() * sizeof(double)));
-    HIP_TRY(remLen_.alloc(len.size() * sizeof(double)));
-    HIP_TRY(upload<double>(remP0_, p0));
-    HIP_TRY(upload<double>(remVN_, vN));
-    HIP_TRY(upload<double>(remLen_, len));
+    // the re-entry start point and normal per link, with TR_shear_removal's
+    // operations: first point + (ny, -nx) (len * 0.5) + (nx, ny) 0.001
+    const size_t nBL = len.size();
+    std::vector<double> geo(4 * nBL);
+    for (size_t t = 0; t < nBL; t++) {
+      const double nx = vN[2 * t], ny = vN[2 * t + 1], h = len[t] * 0.5;
+      geo[4 * t] = p0[2 * t] + ny * h + nx * 0.001;
+      geo[4 * t + 1] = p0[2 * t + 1] + -nx * h + ny * 0.001;
+      geo[4 * t + 2] = nx;
+      geo[4 * t + 3] = ny;
+    }
+    HIP_TRY(remGeo_.alloc(geo.size() * sizeof(double)));
+    HIP_TRY(upload<double>(remGeo_, geo));
+    HIP_TRY(remTS_.alloc(2 * nBL * sizeof(double)));
     HIP_TRY(remSurf_.alloc(surf.size() * sizeof(int16_t)));
     HIP_TRY(hipMemcpy(remSurf_.as<void>(), surf.data(), surf.size() * sizeof(int16_t),
                       hipMemcpyHostToDevice));

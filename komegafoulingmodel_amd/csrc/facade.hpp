@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <climits>
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
@@ -730,6 +731,36 @@ class shearStress {
     out << time;
     for (double v : blTau(ctx)) out << "\t" << v;
     out << "\n";
+  }
+
+  // The tracer's defines TR_shear_removal takes (clVariablesTR.cpp:
+  // 1328-1373, 1900-1942) from the particle document: the surface-type
+  // threshold, the re-release timer and the particle location's row length
+  // (the particle domain floor(X Min Val) to floor(X Stop Pos) rounded up to
+  // WORKGROUPSIZE_TR = 64).  The per-size arrays are particleProperties' and
+  // stay with the tracer.
+  static kofm_removal_params removalParams(const clProblem& p) {
+    const RunParams& rp = p.params;
+    kofm_removal_params q{};
+    q.dt_wall = double(p.trSteps_wall);
+    q.foul_size_switch = rp.getDouble(RunParams::kParticles, "Foul Size Switch Soot", 0.5);
+    // TIME_BEFORE_RERELEASE = 200000 / LB_STEPS_PER_TR_WALL (ConstDef.h:150, 290)
+    const int before = rp.getInt(RunParams::kParticles, "Time Before Re-release", 200000 / 3);
+    q.timer_start = std::min(before, INT_MAX - 1);
+    const double release = rp.getDouble(RunParams::kParticles, "X Release Pos", 90.);
+    const double xMin = rp.getDouble(RunParams::kParticles, "X Min Val", release - 1.25);
+    const double xStop = rp.getDouble(RunParams::kParticles, "X Stop Pos", 1900.);
+    const int width = int(std::floor(xStop)) - int(std::floor(xMin));
+    q.loc_stride = (width + 63) / 64 * 64;
+    return q;
+  }
+
+  // shearStress::shearRemoval with updateParRemArgs' range (shearStress.cpp:
+  // 327-335): the particles sorted as temporarily deposited,
+  // [Ploc(1).x, Ploc(1).y), after calculateShear on the same stream.
+  void shearRemoval(kofm_ctx* ctx, const kofm_removal_params& q, const kofm_particles& P,
+                    int64_t first, int64_t end, uint32_t* blDep) {
+    check(ctx, kofm_shear_removal(ctx, &q, &P, first, end - first, blDep), "TR_shear_removal");
   }
 };
 

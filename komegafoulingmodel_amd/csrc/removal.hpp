@@ -31,17 +31,18 @@ struct RemovalParams {
   int nBL;
 };
 
-// the tracer's link arrays, indexed by the tracer's link index
+// Per tracer link: (shear, surface type) for the call, and the start
+// point and unit normal of a re-entrained particle, (cx0, cy0, nx, ny) with
+// (cx0, cy0) = first point + (ny, -nx) len / 2 + (nx, ny) 0.001, the link's
+// midpoint moved 0.001 into the flow (computed on the host with the same
+// operations when the walls change).
 struct RemovalLinks {
-  const double* p0;      // 2 * nBL: lsc[P01ind.x]
-  const double* vN;      // 2 * nBL: BL_vNvec
-  const double* len;     // nBL: BL_blLen
-  const int16_t* surf;   // nBL: BL_Surf_Type (int_type)
-  const int32_t* tauAt;  // nBL: entry of the link shear output, -1 where none is computed
+  double2* ts;          // nBL: written by removalLinkShear each call
+  const double* geo;    // 4 * nBL
 };
 
 struct Particles {
-  double* pos;             // 2 per particle
+  double* pos;             // 2 per particle (P_pos, double2: 16-byte aligned)
   const uint32_t* numRep;  // P_Num_rep
   const int16_t* type;     // P_type (size index)
   int32_t* depFlag;        // P_Dep_Flag: link index while temporarily deposited
@@ -50,61 +51,106 @@ struct Particles {
   int32_t* timer;          // P_timer
 };
 
-// One thread per particle offset + i, i < count.  A particle whose link
-// index or size index lies outside the arrays is left as it is (the
-// reference would read past them).
+// (shear, surface type) of every tracer link: the link shear output entry
+// tauAt[t], 0 where TR_shear_2 writes none.
 template <class R>
-__global__ __launch_bounds__(256) void shearRemoval(RemovalParams p, RemovalLinks L,
-                                                    const R* __restrict__ tau, Particles P,
+__global__ __launch_bounds__(256) void removalLinkShear(const R* __restrict__ tau,
+                                                        const int32_t* __restrict__ tauAt,
+                                                        const int16_t* __restrict__ surf, int nBL,
+                                                        double2* __restrict__ ts) {
+  const int t = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= nBL) return;
+  const int at = tauAt[t];
+  ts[t] = make_double2(at >= 0 ? double(tau[at]) : 0., double(surf[t]));
+}
+
+// Particles per thread: the loads of all of them are issued before any is
+// used, so that a wave keeps several memory round trips in flight.
+constexpr int kRemovalPer = 8;
+
+// Particle k of the call is offset + k, k < count; a block of 256 threads
+// takes 256 * kRemovalPer of them, thread t the ones t + 256 j.  A particle
+// whose link index or size index lies outside the arrays is left as it is
+// (the reference would read past them).  The per-size constants are staged
+// in LDS; a particle reads one 16-byte link record, and a second of 32 bytes
+// when it re-enters the flow.
+__global__ __launch_bounds__(256) void shearRemoval(RemovalParams p, RemovalLinks L, Particles P,
                                                     uint32_t* __restrict__ blDep, int64_t offset,
                                                     int64_t count) {
-  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k >= count) return;
-  const int64_t i = k + offset;
-  const int bl = P.depFlag[i];
-  if (bl < 0) return;  // not deposited on a link
-  const int sz = P.type[i];
-  if (bl >= p.nBL || sz < 0 || sz >= p.nSizes) return;
-  const uint16_t depTimer = uint16_t(P.depTimer[i] - 1);
-  // the deposit timer ran out: the particle stays for good
-  if (depTimer == 0) {
-    // the reference's index, link + NUM_PAR_SIZES * size
-    atomicAdd(&blDep[bl + p.nSizes * sz], P.numRep[i]);
-    P.depFlag[i] = -2;
-    P.loc[i] = -2;
-    return;
+  __shared__ double crit[2 * kMaxParSizes], dco[kMaxParSizes], lco[kMaxParSizes],
+      mass[kMaxParSizes];
+  const int tid = int(threadIdx.x);
+  if (tid < 2 * kMaxParSizes) crit[tid] = p.tauCrit[tid];
+  if (tid < kMaxParSizes) {
+    dco[tid] = p.dcoeff[tid];
+    lco[tid] = p.lcoeff[tid];
+    mass[tid] = p.mp[tid];
   }
-  const int surf = sz * 2 + L.surf[bl];
-  const int at = L.tauAt[bl];
-  const double shear = at >= 0 ? double(tau[at]) : 0.;
-  if (fabs(shear) <= p.tauCrit[surf]) {
-    P.depTimer[i] = depTimer;
-    return;
+  __syncthreads();
+  const int64_t base = int64_t(blockIdx.x) * (256 * kRemovalPer) + tid;
+  int bl[kRemovalPer], sz[kRemovalPer];
+  uint16_t tm[kRemovalPer];
+#pragma unroll
+  for (int j = 0; j < kRemovalPer; j++) {
+    const int64_t k = base + 256 * j;
+    bl[j] = -1;
+    sz[j] = 0;
+    tm[j] = 0;
+    if (k < count) {
+      bl[j] = P.depFlag[k + offset];
+      sz[j] = P.type[k + offset];
+      tm[j] = P.depTimer[k + offset];
+    }
   }
-  // re-entrained: drag along the link, lift along its normal, from the
-  // link's midpoint moved 0.001 into the flow
-  const double fd = p.dcoeff[sz];
-  const double fl = p.lcoeff[sz] * sqrt(shear);
-  const double nx = L.vN[2 * bl], ny = L.vN[2 * bl + 1];
-  const double dx = fd * ny, dy = fd * -nx;
-  const double h = L.len[bl] * 0.5;
-  double cx = L.p0[2 * bl] + ny * h + nx * 0.001;
-  double cy = L.p0[2 * bl + 1] + -nx * h + ny * 0.001;
-  const double rx = nx * fl + dx, ry = ny * fl + dy;
-  cx += rx * shear / p.mp[sz] * p.dtWall * p.dtWall / 2.;
-  cy += ry * shear / p.mp[sz] * p.dtWall * p.dtWall / 2.;
-  // convert_int2 (toward zero); a negative shear beyond the critical value
-  // gives sqrt of it and a NaN position, whose index is taken as 0
-  const int ix = cx == cx ? int(cx) : 0, iy = cy == cy ? int(cy) : 0;
-  int loc = (ix - p.xIndStart) + iy * p.locStride;
-  // outside the tracer's range the location is -2; the reference's flag for
-  // that case is overwritten by the -1 below, and so it is here
-  if (cx < p.xMinVal || cx >= p.xMaxVal) loc = -2;
-  P.depFlag[i] = -1;
-  P.loc[i] = loc;
-  P.pos[2 * i] = cx;
-  P.pos[2 * i + 1] = cy;
-  P.timer[i] = p.timerStart;
+  double2 ts[kRemovalPer];
+#pragma unroll
+  for (int j = 0; j < kRemovalPer; j++) {
+    // not deposited on a link (flag < 0), or indices outside the arrays
+    if (bl[j] < 0 || bl[j] >= p.nBL || sz[j] < 0 || sz[j] >= p.nSizes) bl[j] = -1;
+    ts[j] = make_double2(0., 0.);
+    if (bl[j] >= 0 && uint16_t(tm[j] - 1) != 0) ts[j] = L.ts[bl[j]];
+  }
+#pragma unroll
+  for (int j = 0; j < kRemovalPer; j++) {
+    if (bl[j] < 0) continue;
+    const int64_t i = base + 256 * j + offset;
+    const int b = bl[j], s = sz[j];
+    const uint16_t depTimer = uint16_t(tm[j] - 1);
+    // the deposit timer ran out: the particle stays for good
+    if (depTimer == 0) {
+      // the reference's index, link + NUM_PAR_SIZES * size
+      atomicAdd(&blDep[b + p.nSizes * s], P.numRep[i]);
+      P.depFlag[i] = -2;
+      P.loc[i] = -2;
+      continue;
+    }
+    const double shear = ts[j].x;
+    if (fabs(shear) <= crit[s * 2 + int(ts[j].y)]) {
+      P.depTimer[i] = depTimer;
+      continue;
+    }
+    // re-entrained: drag along the link, lift along its normal
+    const double4 g = reinterpret_cast<const double4*>(L.geo)[b];
+    const double nx = g.z, ny = g.w;
+    const double fd = dco[s];
+    const double fl = lco[s] * sqrt(shear);
+    const double dx = fd * ny, dy = fd * -nx;
+    const double rx = nx * fl + dx, ry = ny * fl + dy;
+    const double m = mass[s];
+    const double cx = g.x + rx * shear / m * p.dtWall * p.dtWall / 2.;
+    const double cy = g.y + ry * shear / m * p.dtWall * p.dtWall / 2.;
+    // convert_int2 (toward zero); a negative shear beyond the critical value
+    // gives sqrt of it and a NaN position, whose index is taken as 0
+    const int ix = cx == cx ? int(cx) : 0, iy = cy == cy ? int(cy) : 0;
+    int loc = (ix - p.xIndStart) + iy * p.locStride;
+    // outside the tracer's range the location is -2; the reference's flag
+    // for that case is overwritten by the -1 below, and so it is here
+    if (cx < p.xMinVal || cx >= p.xMaxVal) loc = -2;
+    P.depFlag[i] = -1;
+    P.loc[i] = loc;
+    reinterpret_cast<double2*>(P.pos)[i] = make_double2(cx, cy);
+    P.timer[i] = p.timerStart;
+  }
 }
 
 }  // namespace dev

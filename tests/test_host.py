@@ -211,7 +211,7 @@ def test_ini_poiseuille_perturbation_uses_the_reference_generator():
     assert np.any(v1 != 0.0)
 
 
-def _kofm_params(tmp_path, fluid_extra, thermal=None, system_extra=""):
+def _kofm_params(tmp_path, fluid_extra, thermal=None, system_extra="", docs=""):
     load = tmp_path / "load"
     load.mkdir(exist_ok=True)
     thermal_doc = f"--- # Thermal Parameters\nParamsName: 3\n{thermal}\n" if thermal is not None else ""
@@ -229,7 +229,7 @@ ParamsName: 2
 Tau: !!float 1.25
 Fval: !!float 1e-6
 {fluid_extra}
-""")
+{docs}""")
     exe = os.path.join(ROOT, "komegafoulingmodel_amd", "kofm_run")
     import subprocess
     return subprocess.run([exe, "--params", str(load)], capture_output=True, text=True)
@@ -299,3 +299,42 @@ def test_runparam_thermal_rejections(tmp_path):
     assert res.returncode != 0 and "unsupported thermal parameter" in res.stderr
     res = _kofm_params(tmp_path, "Use Turb Model: False", thermal="Thermal Solver: True")
     assert res.returncode != 0 and "needs k-omega" in res.stderr
+
+
+def _params(res):
+    assert res.returncode == 0, res.stderr
+    return dict(line.split(" ", 1) for line in res.stdout.strip().splitlines())
+
+
+def test_runparam_removal_and_nusselt_parameters(tmp_path):
+    """The tracer's defines TR_shear_removal takes (clVariablesTR.cpp:
+    1328-1373, 1900-1942): DTTR_WALL = "LB Steps Per TR Wall", the soot
+    switch, PAR_TIMER_START = "Time Before Re-release" (default 200000 / 3)
+    and FULLSIZEX_TR_PADDED = floor(X Stop Pos) - floor(X Min Val) rounded up
+    to 64; and KSOOT, KAIR = k DELTA_F / DELTA_T with DELTA_L = 2 R / 2e-3,
+    DELTA_T = nu_air DELTA_L^2 / nu, DELTA_M = rho DELTA_L^3 / rho_air,
+    DELTA_F = DELTA_M DELTA_L / DELTA_T^2 (clVariablesLB.cpp:608-610)."""
+    q = _params(_kofm_params(tmp_path, "Use Turb Model: True"))
+    assert float(q["RemovalDtWall"]) == 3.0 and float(q["FoulSizeSwitch"]) == 0.5
+    assert int(q["ParTimerStart"]) == 66666 and int(q["ParLocStride"]) == 1856
+    assert int(q["CalcNu"]) == 0
+    docs = ("--- # Particle Parameters\nParamsName: 6\nX Release Pos: !!float 10.0\n"
+            "X Stop Pos: !!float 200.0\nFoul Size Switch Soot: !!float 0.7\n"
+            "Time Before Re-release: !!int 500\n")
+    res = _kofm_params(tmp_path, "Use Turb Model: True\nRho air: !!float 0.8",
+                       thermal="K Soot: !!float 0.06\nK air: !!float 0.04",
+                       system_extra="LB Steps Per TR Wall: !!int 4\nWavy Start Loc: !!int 20\n"
+                                    "Period Length: !!int 50\nNum Periods: !!int 3\n"
+                                    "Save Nu Timestep: !!int 25",
+                       docs=docs)
+    q = _params(res)
+    assert float(q["RemovalDtWall"]) == 4.0 and float(q["FoulSizeSwitch"]) == 0.7
+    assert int(q["ParTimerStart"]) == 500 and int(q["ParLocStride"]) == 192
+    assert int(q["CalcNu"]) == 1 and int(q["WavyStart"]) == 20 and int(q["PeriodLength"]) == 50
+    assert int(q["NumPeriods"]) == 3 and int(q["SaveNuTimestep"]) == 25
+    assert int(q["nextSaveNuTime"]) == 25
+    nu, dl = (1.0 / 1.25 - 0.5) / 3.0, 2.0 * 19.0 / 2e-3
+    dt = 4.1e-5 * dl * dl / nu
+    df = (1.0 * dl ** 3 / 0.8) * dl / dt / dt
+    assert math.isclose(float(q["KSootLattice"]), 0.06 * df / dt, rel_tol=1e-14)
+    assert math.isclose(float(q["KAirLattice"]), 0.04 * df / dt, rel_tol=1e-14)
