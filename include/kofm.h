@@ -617,8 +617,12 @@ int kofm_geometry_field_gradient(const kofm_ctx* ctx, const double* wall_d_bar,
  * k-omega each step (the launch and every member's implicit solves) does (see
  * kofm_ensemble_set_cache_budget for the grouping).  Members keep their own arrays and are read
  * with kofm_download as usual; calls on a member are ordered after the
- * ensemble's steps.  The reference runs one channel per process; this has no
- * counterpart there. */
+ * ensemble's steps.  Members may change between ensemble steps (new
+ * parameters, geometry, launch shape or padding): the next step rebuilds the
+ * ensemble's tables and graphs from them.  A member destroyed with
+ * kofm_destroy makes every later kofm_ensemble_step fail with KOFM_ERR_STATE;
+ * destroy the ensemble before or after its members.  The reference runs one
+ * channel per process; this has no counterpart there. */
 typedef struct kofm_ensemble kofm_ensemble;
 
 int kofm_ensemble_create(kofm_ctx* const* members, int32_t n, kofm_ensemble** out);
