@@ -69,7 +69,8 @@ __global__ __launch_bounds__(256) void removalLinkShear(const R* __restrict__ ta
 constexpr int kRemovalPer = 8;
 
 // Particle k of the call is offset + k, k < count; a block of 256 threads
-// takes 256 * kRemovalPer of them, thread t the ones t + 256 j.  A particle
+// takes 256 * kRemovalPer of them, thread t the ones t + 256 j (so that the
+// lanes of a wave load and store neighbouring particles).  A particle
 // whose link index or size index lies outside the arrays is left as it is
 // (the reference would read past them).  The per-size constants are staged
 // in LDS; a particle reads one 16-byte link record, and a second of 32 bytes
@@ -80,13 +81,6 @@ __global__ __launch_bounds__(256) void shearRemoval(RemovalParams p, RemovalLink
   __shared__ double crit[2 * kMaxParSizes], dco[kMaxParSizes], lco[kMaxParSizes],
       mass[kMaxParSizes];
   const int tid = int(threadIdx.x);
-  if (tid < 2 * kMaxParSizes) crit[tid] = p.tauCrit[tid];
-  if (tid < kMaxParSizes) {
-    dco[tid] = p.dcoeff[tid];
-    lco[tid] = p.lcoeff[tid];
-    mass[tid] = p.mp[tid];
-  }
-  __syncthreads();
   const int64_t base = int64_t(blockIdx.x) * (256 * kRemovalPer) + tid;
   int bl[kRemovalPer], sz[kRemovalPer];
   uint16_t tm[kRemovalPer];
@@ -110,6 +104,15 @@ __global__ __launch_bounds__(256) void shearRemoval(RemovalParams p, RemovalLink
     ts[j] = make_double2(0., 0.);
     if (bl[j] >= 0 && uint16_t(tm[j] - 1) != 0) ts[j] = L.ts[bl[j]];
   }
+  // the constants go to LDS while the particle loads and gathers above are
+  // in flight
+  if (tid < 2 * kMaxParSizes) crit[tid] = p.tauCrit[tid];
+  if (tid < kMaxParSizes) {
+    dco[tid] = p.dcoeff[tid];
+    lco[tid] = p.lcoeff[tid];
+    mass[tid] = p.mp[tid];
+  }
+  __syncthreads();
 #pragma unroll
   for (int j = 0; j < kRemovalPer; j++) {
     if (bl[j] < 0) continue;

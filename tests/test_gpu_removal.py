@@ -7,8 +7,9 @@ tracer's do; 20,000 seeded particles sit on random links (both walls, links
 inside and outside the range the wall shear covers, a few indices past the
 arrays), with random sizes (a few invalid), deposit timers (0, 1, 2, 3, 100),
 weights and stale positions, and 15 % free and 15 % gone for good.  The call
-covers particles [1000, 19000): the rest must stay as they were.  The
-critical shears are quantiles of the measured |tau|, so some particles stay,
+covers particles [1000, 19000) (and, in one test, the range [1003, 18998),
+whose ends are not multiples of the kernel's particle blocks): the rest must
+stay as they were.  The critical shears are quantiles of the measured |tau|, so some particles stay,
 some leave and some deposit for good.  Every array after the call is
 bit-identical to orc_shear_removal given the same link shears, and the link
 arrays kofm_removal_links reports are bit-identical to orc_removal_links; both
@@ -65,7 +66,7 @@ def _constants(tau):
                 dt_wall=3.0, loc_stride=256, timer_start=1234)
 
 
-def _check(h, L, w, C, seed, label):
+def _check(h, L, w, C, seed, label, offset=OFFSET, count=COUNT):
     nBL = len(w.BL)
     links = L.removal_links(0.5)
     ref_links = oracle.removal_links(C, w.C0, w.BL, 0.5)
@@ -76,7 +77,7 @@ def _check(h, L, w, C, seed, label):
     host = _particles(nBL, seed)
     dev = {key: DeviceArray(h, host[key]) for key in KEYS}
     dep = DeviceArray(h, np.zeros(nBL * NSIZES, np.uint32))
-    L.shear_removal({key: dev[key].addr for key in KEYS}, dep.addr, OFFSET, COUNT, **k,
+    L.shear_removal({key: dev[key].addr for key in KEYS}, dep.addr, offset, count, **k,
                     foul_size_switch=0.5)
     L.synchronize()
     got = {key: dev[key].get() for key in KEYS}
@@ -86,7 +87,7 @@ def _check(h, L, w, C, seed, label):
     oracle.shear_removal(ref_links, tau, ref, **k, x_min_val=w.geom_params()["x_release_pos"] - 0.5,
                          x_max_val=w.geom_params()["x_stop_pos"],
                          x_ind_start=int(np.floor(w.geom_params()["x_release_pos"] - 1.25)),
-                         bl_dep=ref_dep, offset=OFFSET, count=COUNT)
+                         bl_dep=ref_dep, offset=offset, count=count)
     left = int(np.sum((host["dep_flag"] >= 0) & (ref["dep_flag"] == -1)))
     stayed = int(np.sum((ref["dep_timer"] != host["dep_timer"]) & (ref["dep_flag"] >= 0)))
     gone = int(np.sum((host["dep_flag"] >= 0) & (ref["dep_flag"] == -2)))
@@ -96,7 +97,7 @@ def _check(h, L, w, C, seed, label):
     assert left > 100 and stayed > 100 and gone > 100
     for key in KEYS:
         assert np.array_equal(got[key], ref[key], equal_nan=key == "pos"), (label, key)
-        outside = np.r_[0:OFFSET, OFFSET + COUNT:N]
+        outside = np.r_[0:offset, offset + count:N]
         width = 2 if key == "pos" else 1
         g, h0 = got[key].reshape(N, width), host[key].reshape(N, width)
         assert np.array_equal(g[outside], h0[outside]), (label, key)
@@ -121,6 +122,18 @@ def test_shear_removal_matches_oracle(storage):
     L.step(20, lib.OPTION_SAVE_MACRO_FIELDS)
     L.wall_shear()
     assert _check(h, L, w, C, 12, f"{storage} fouled walls") > 30
+    L.close()
+
+
+def test_shear_removal_on_an_unaligned_range_matches_oracle():
+    """A range that starts and ends off the kernel's blocks of 2048
+    particles; the results are the same."""
+    h = hip()
+    w = channel.fouled_channel(256, 64, Fval=1e-5, seed=5, amp=(0.0, 4.0), width=(10.0, 60.0))
+    L, _ = make_pair(w, "f32")
+    L.step(200, lib.OPTION_SAVE_MACRO_FIELDS)
+    L.wall_shear()
+    _check(h, L, w, w.C0, 13, "f32 offset 1003", offset=1003, count=17995)
     L.close()
 
 
