@@ -266,6 +266,34 @@ def test_komega_strain_limiter_bradshaw_and_free_stream_limits():
     assert np.allclose(N[wl.nY // 2, :64], k0 / w0, rtol=1e-13)
 
 
+def test_komega_f2_blend_is_bounded_and_monotone_off_the_wall():
+    """Between the two limits of nu_t = a1 k / max(a1 omega, F2 |S|): with
+    uniform k, omega and strain, F2 = tanh(arg2^2) falls as the wall distance
+    grows (both terms of arg2 do), so nu_t must rise monotonically from
+    Bradshaw's a1 k / |S| at the wall to k / omega in the core, staying
+    between them, and the rows in between must cover the blend (values
+    strictly inside the bounds)."""
+    k0, w0, sxy = 1e-4, 1e-2, 0.01
+    wl = _komega_channel(nY=200, k0=k0, w0=w0)
+    g, orc = _lattice(wl)
+    D = g["D"]
+    orc.init_state(np.ones(D), np.zeros(D), np.zeros(D))
+    orc.init_komega(k0, w0)
+    fluid = (g["nType"].astype(np.int32) & 1) == 0
+    orc.S[D:2 * D] = np.where(fluid, sxy, 0.0)
+    orc.step(1)
+    N = orc.nut.reshape(wl.nY, wl.XF)
+    lo, hi = 0.31 * k0 / (2.0 * sxy), k0 / w0
+    for x in (5, 40):
+        col = N[1:wl.nY // 2 + 1, x]
+        assert np.all(col >= lo * (1 - 1e-12)) and np.all(col <= hi * (1 + 1e-12)), x
+        assert np.all(np.diff(col) >= -1e-15 * hi), x
+        inside = (col > lo * (1 + 1e-6)) & (col < hi * (1 - 1e-6))
+        assert inside.sum() >= 3, (x, inside.sum())
+        # and the lower half mirrors the upper one (the wall distance is symmetric)
+        assert np.allclose(N[wl.nY - 2:wl.nY // 2 - 1:-1, x], col[:wl.nY // 2 - 1], rtol=1e-12)
+
+
 @pytest.mark.parametrize("n", [2 ** 20, 2 ** 20 - 54, 2 ** 20 - 256 + 10, 2 ** 20 - 108, 2 ** 20 + 4343])
 def test_reduction_known_answers(n):
     """ReduceTests.h:20-143 cases as assertions: sums of ones count the
